@@ -1,0 +1,164 @@
+/* C ABI of libseiswg_hip.so: the MI355X (gfx950) device module of the
+ * seiswg staggered-grid time loop.
+ *
+ * It replaces the device half of the reference's src/solver.rs: wgpu
+ * initialisation, buffer allocation, pipelines and dispatch (Solver::new,
+ * solver.rs:578-933), the per-step dispatch lists (solver.rs:999-1030,
+ * 1389-1407, 1442-1498), clears/readbacks/uploads (solver.rs:1126-1242) and
+ * the smoothing driver (solver.rs:1249-1274).  The host half (config, file
+ * formats, workflow drivers) sits above this ABI; see seiswg_host.h.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative code on error;
+ *     sw_last_error() then returns a thread-local message.
+ *   - Host pointers are borrowed for the duration of the call.  Device memory
+ *     belongs to the context.
+ *   - One context = one HIP device + one HIP stream; a context must not be
+ *     used from two threads at once.  There are no internal host threads.
+ *   - All field data are FP32, grid-major with k = i*nz + j (z contiguous),
+ *     exactly the reference layout (fd2d.wgsl:140-145).
+ */
+#ifndef SEISWG_HIP_H
+#define SEISWG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SW_API __attribute__((visibility("default")))
+
+typedef struct sw_ctx sw_ctx;
+
+/* Grid and solver switches: the reference's Params uniform (fd2d.wgsl:18-49,
+ * solver.rs:92-101) minus the per-dispatch fields (it, isrc). */
+typedef struct sw_grid {
+  int32_t nx, nz, nt;
+  float dx, dz, dt;
+  int32_t sh, psv, spin;
+  int32_t abs_left, abs_right, abs_bottom, abs_top;
+  int32_t abs_width;
+  float abs_alpha;
+} sw_grid;
+
+/* Model kinds of sw_set_model (solver.rs:614-700). */
+#define SW_MODEL_VPVS 0 /* fields = {rho, vp, vs}; converted by vps2lm (fd2d.wgsl:188-201) */
+#define SW_MODEL_LAME 1 /* fields = {rho, lambda, mu, nu, j, lambda_c, mu_c, nu_c} (spin files) */
+
+/* Field ids of sw_read_field / sw_write_field: the reference's F_* slice ids
+ * (fd2d.wgsl:53-105).  Only the materialised ones are accepted: state fields
+ * of the enabled physics, model fields, the taper and the kernels. */
+#define SW_F_VY 0
+#define SW_F_SXY 2
+#define SW_F_SZY 3
+#define SW_F_VX 7
+#define SW_F_VZ 8
+#define SW_F_SXX 11
+#define SW_F_SZZ 12
+#define SW_F_SXZ 13
+#define SW_F_VY_C 20
+#define SW_F_SYX_C 22
+#define SW_F_SYY_C 23
+#define SW_F_SYZ_C 24
+#define SW_F_LAM 30
+#define SW_F_MU 31
+#define SW_F_NU 32
+#define SW_F_J 33
+#define SW_F_LAM_C 34
+#define SW_F_MU_C 35
+#define SW_F_NU_C 36
+#define SW_F_RHO 37
+#define SW_F_BOUND 38
+#define SW_F_K_LAM 39
+#define SW_F_K_MU 40
+#define SW_F_K_RHO 41
+
+/* Trace components (fd2d.wgsl:110-114). */
+#define SW_OBS_X 0
+#define SW_OBS_Y 1
+#define SW_OBS_Z 2
+#define SW_OBS_YI 3
+#define SW_OBS_YC 4
+
+/* Snapshot callback: called after step `it` (it > 0, it % snap_every == 0,
+ * solver.rs:1035) once per field written by the reference's save_snapshot
+ * (solver.rs:1144-1176: vy | vx, vz | vy_c) with a host copy of the field.
+ * A non-zero return aborts the run. */
+typedef int (*sw_snapshot_fn)(void* user, int32_t shot, int32_t it, int32_t field, const float* data,
+                              int64_t npt);
+
+SW_API const char* sw_last_error(void);
+
+/* Solver::new, device part (solver.rs:578-612, 677-700, 799-809). */
+SW_API int sw_create(const sw_grid* grid, int32_t hip_device, sw_ctx** out);
+SW_API void sw_destroy(sw_ctx* ctx);
+
+/* Model upload + init pass (solver.rs:677-700, 898-915): copies the fields,
+ * runs vps2lm for SW_MODEL_VPVS and builds the absorbing taper (set_bound,
+ * fd2d.wgsl:205-235). */
+SW_API int sw_set_model(sw_ctx* ctx, int32_t kind, const float* const* fields);
+
+/* Sources: grid indices k = ix*nz + iz (solver.rs:716-718) and the source
+ * time functions [3][nsrc][nt] (x, y, z; solver.rs:721-739). */
+SW_API int sw_set_sources(sw_ctx* ctx, int32_t nsrc, const int32_t* grid_index, const float* stf);
+/* Receivers: grid indices (solver.rs:745-751). */
+SW_API int sw_set_receivers(sw_ctx* ctx, int32_t nrec, const int32_t* grid_index);
+
+/* One forward run (solver.rs:976-1046): clear wavefields and traces, run nt
+ * steps with source isrc active (-1 = all sources, combine_sources). */
+SW_API int sw_forward(sw_ctx* ctx, int32_t isrc, int32_t snap_every, sw_snapshot_fn cb, void* user);
+/* nshots forward runs advanced together, one launch per time step for all
+ * shots (the reference runs them one after another, solver.rs:964-970).
+ * Shot s fires source isrc[s] (-1 = all). */
+SW_API int sw_forward_batch(sw_ctx* ctx, int32_t nshots, const int32_t* isrc, int32_t snap_every,
+                            sw_snapshot_fn cb, void* user);
+/* Traces of shot `shot` of the last run, component comp, [nrec][nt]
+ * (readback_obs + slice_obs, solver.rs:1180-1195). */
+SW_API int sw_read_obs(sw_ctx* ctx, int32_t shot, int32_t comp, float* out);
+/* readback_field (solver.rs:1205-1211) / write_field_slice (1239-1242). */
+SW_API int sw_read_field(sw_ctx* ctx, int32_t shot, int32_t field, float* out);
+SW_API int sw_write_field(sw_ctx* ctx, int32_t shot, int32_t field, const float* in);
+
+/* Low-level stepping for tests: prepare `nshots` shots (zeroed state and
+ * traces, step clock at 0) without running them ... */
+SW_API int sw_prepare(sw_ctx* ctx, int32_t nshots, const int32_t* isrc);
+/* ... then advance all prepared shots by `nsteps` time steps. */
+SW_API int sw_advance(sw_ctx* ctx, int32_t nsteps);
+
+/* run_adjoint (solver.rs:1298-1529), SH, reference imaging condition:
+ * nruns runs (run r fires isrc[r]); obs_ref = observed uy traces
+ * [nruns][nrec][nt]; K_mu is accumulated over runs and then smoothed with
+ * sigma = smooth.  misfit[r] receives the run's misfit (may be NULL).
+ * Read the result with sw_read_field(ctx, 0, SW_F_K_MU, ...). */
+SW_API int sw_adjoint_plain(sw_ctx* ctx, int32_t nruns, const int32_t* isrc, const float* obs_ref,
+                            int32_t adj_interval, float smooth, double* misfit);
+
+/* Gaussian smoothing of a kernel field in place (smooth_gradient,
+ * solver.rs:1249-1274; init_gsum, gaussian_x, gaussian_z). */
+SW_API int sw_smooth(sw_ctx* ctx, int32_t field, float sigma);
+
+/* Tile shape of the fused step kernels: threads per workgroup (64, 128 or
+ * 256; a workgroup owns threads-6 z-columns) and rows marched per workgroup. */
+SW_API int sw_set_tiling(sw_ctx* ctx, int32_t threads, int32_t seglen);
+SW_API int sw_get_tiling(const sw_ctx* ctx, int32_t* threads, int32_t* seglen);
+
+/* Timing of the last run's time loop, from HIP events on the context stream. */
+typedef struct sw_timing {
+  double loop_ms;        /* events around all time steps of the last run */
+  int64_t steps;         /* time steps in that loop */
+  int64_t step_launches; /* fused step-kernel launches in that loop */
+  int64_t cell_updates;  /* nx*nz*steps*shots */
+} sw_timing;
+SW_API int sw_get_timing(const sw_ctx* ctx, sw_timing* out);
+
+/* Measurement aid with no counterpart in the reference: `iters` launches of
+ * a plain float4 copy kernel moving nbytes (read) + nbytes (write) on
+ * `hip_device`; average kernel time in ms from HIP events.  Used to calibrate
+ * the HBM read/write counters against a kernel of known traffic. */
+SW_API int sw_bench_copy(int32_t hip_device, int64_t nbytes, int32_t iters, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,788 @@
+// C ABI of the device module (include/seiswg_hip.h).
+//
+// Owns the HIP device state of one problem: model fields, 1-D taper tables,
+// source time functions, receiver lists, ping-pong state buffers for a batch
+// of shots, per-receiver displacement accumulators, traces and the device
+// step clock.  Builds the per-tile injection/receiver lists the fused step
+// kernels consume and drives the time loop (solver.rs:941-1119, 1298-1529).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/seiswg_hip.h"
+#include "kernels.h"
+
+using namespace sw;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return fail(-3, "%s: %s", #expr, hipGetErrorString(e_));     \
+  } while (0)
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t ensure(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+};
+
+}  // namespace
+
+namespace sw {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}
+
+struct sw_ctx {
+  sw_grid g{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t npt = 0;
+  float h8x = 0, h8z = 0;
+  bool model_set = false;
+
+  // model
+  DBuf<float> rho, lam, mu, nu, jj, lam_c, mu_c, nu_c;
+  DBuf<float> bx, bzb, bzt;
+  std::vector<float> h_bx, h_bzb, h_bzt;
+
+  // sources / receivers
+  int nsrc = 0, nrec = 0;
+  std::vector<int32_t> src_k, rec_k;
+  DBuf<float> stf;   // [3][nsrc][nt]
+
+  // tiling
+  Tiling tile{256, 64, 0, 0};
+
+  // batch
+  int nshots = 0;
+  std::vector<int32_t> shot_isrc;
+  int slot_index[N_SLOTS];
+  int nslots = 0;
+  DBuf<float> state[2];
+  int cur = 0;                // buffer holding the current state
+  DBuf<float> uacc, obs;
+  DBuf<int> clock;
+  DBuf<int> inj_off, rec_off;
+  DBuf<int4> inj, rec;
+  int64_t steps_done = 0;
+  // injection values of the current batch (forward: stf; adjoint: adjoint source)
+  const float* val_x = nullptr;
+  const float* val_y = nullptr;
+  const float* val_z = nullptr;
+  int nt_val = 0;
+  bool record = true;
+  bool sh_only = false;       // the adjoint workflow dispatches the SH kernels only
+
+  // plain adjoint
+  DBuf<float> kmu, k_lam, k_rho, gsum, gtmp, wtab, sumx, sumz, fwd_store, adj_src, ref, taper;
+  DBuf<double> partial, misfit_d;
+
+  // timing
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  sw_timing timing{};
+  int64_t launches = 0;
+};
+
+namespace {
+
+bool uses_sh(const sw_ctx* c) { return c->g.sh != 0; }
+bool uses_psv(const sw_ctx* c) { return c->g.psv != 0; }
+bool uses_spin(const sw_ctx* c) { return c->g.psv != 0 && c->g.spin != 0; }
+
+void compute_slots(sw_ctx* c) {
+  for (int s = 0; s < N_SLOTS; ++s) c->slot_index[s] = -1;
+  int n = 0;
+  auto add = [&](int s) { c->slot_index[s] = n++; };
+  if (uses_sh(c)) { add(S_VY); add(S_SXY); add(S_SZY); }
+  if (uses_psv(c)) { add(S_VX); add(S_VZ); add(S_SXX); add(S_SZZ); add(S_SXZ); }
+  if (uses_spin(c)) { add(S_VYC); add(S_SYXC); add(S_SYYC); add(S_SYZC); }
+  c->nslots = n;
+}
+
+// set_bound (fd2d.wgsl:205-235) factored as bx(i) * bz_bottom(j) * bz_top(j)
+// in the reference's multiplication order; edge tests keep u32 arithmetic.
+void build_taper(sw_ctx* c) {
+  const uint32_t nx = uint32_t(c->g.nx), nz = uint32_t(c->g.nz), w = uint32_t(c->g.abs_width);
+  const float alpha = c->g.abs_alpha;
+  c->h_bx.assign(nx, 1.0f);
+  c->h_bzb.assign(nz, 1.0f);
+  c->h_bzt.assign(nz, 1.0f);
+  for (uint32_t i = 0; i < nx; ++i) {
+    float b = 1.0f;
+    if (c->g.abs_left && (i + 1u) < w) { float aw = alpha * float(w - i - 1u); b *= std::exp(-aw * aw); }
+    if (c->g.abs_right && i > (nx - w)) { float aw = alpha * float(w + i - nx); b *= std::exp(-aw * aw); }
+    c->h_bx[i] = b;
+  }
+  for (uint32_t j = 0; j < nz; ++j) {
+    if (c->g.abs_bottom && j > (nz - w)) { float aw = alpha * float(w + j - nz); c->h_bzb[j] = std::exp(-aw * aw); }
+    if (c->g.abs_top && (j + 1u) < w) { float aw = alpha * float(w - j - 1u); c->h_bzt[j] = std::exp(-aw * aw); }
+  }
+}
+
+int check_tiling(sw_ctx* c) {
+  Tiling& t = c->tile;
+  if (!tiling_supported(t.nt_threads)) return fail(-2, "unsupported tile width %d", t.nt_threads);
+  if (t.seglen < 1) return fail(-2, "seglen must be >= 1");
+  const int W = t.nt_threads - 6;
+  t.nstrips = (c->g.nz + W - 1) / W;
+  t.nsegs = (c->g.nx + t.seglen - 1) / t.seglen;
+  return 0;
+}
+
+// Per-tile CSR lists.  Injection entries of a tile: every point whose new
+// velocity the tile computes (rows [i0-1, i1+2), columns [c0, c0+NT)).
+// Receiver entries: receivers whose cell the tile owns.  Both sorted by row,
+// stable, so duplicates of one cell stay in index order.
+struct Entry { int i, j, q; };
+
+void tile_bounds(const sw_ctx* c, int tile, int& i0, int& i1, int& c0) {
+  const Tiling& t = c->tile;
+  const int strip = tile % t.nstrips, seg = tile / t.nstrips;
+  i0 = seg * t.seglen;
+  i1 = std::min(i0 + t.seglen, c->g.nx);
+  c0 = strip * (t.nt_threads - 6) - 3;
+}
+
+int build_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj) {
+  const int ntile = c->tile.nstrips * c->tile.nsegs;
+  const int NT = c->tile.nt_threads;
+  std::vector<int> off(size_t(c->nshots) * ntile + 1, 0);
+  std::vector<int4> data;
+  for (int s = 0; s < c->nshots; ++s) {
+    std::vector<Entry> pts = shot_inj[s];
+    std::stable_sort(pts.begin(), pts.end(), [](const Entry& a, const Entry& b) { return a.i < b.i; });
+    for (int t = 0; t < ntile; ++t) {
+      int i0, i1, c0;
+      tile_bounds(c, t, i0, i1, c0);
+      off[size_t(s) * ntile + t] = int(data.size());
+      for (const Entry& e : pts)
+        if (e.i >= i0 - 1 && e.i < i1 + 2 && e.j >= c0 && e.j < c0 + NT) data.push_back(make_int4(e.i, e.j, e.q, 0));
+    }
+  }
+  off.back() = int(data.size());
+  // receivers
+  std::vector<Entry> recs;
+  for (int r = 0; r < c->nrec; ++r) {
+    const uint32_t k = uint32_t(c->rec_k[r]);
+    if (int64_t(k) >= c->npt) continue;   // outside the grid: trace stays zero
+    recs.push_back({int(k / uint32_t(c->g.nz)), int(k % uint32_t(c->g.nz)), r});
+  }
+  std::stable_sort(recs.begin(), recs.end(), [](const Entry& a, const Entry& b) { return a.i < b.i; });
+  std::vector<int> roff(ntile + 1, 0);
+  std::vector<int4> rdata;
+  for (int t = 0; t < ntile; ++t) {
+    int i0, i1, c0;
+    tile_bounds(c, t, i0, i1, c0);
+    roff[t] = int(rdata.size());
+    for (const Entry& e : recs)
+      if (e.i >= i0 && e.i < i1 && e.j >= c0 + 3 && e.j < c0 + NT - 3 && e.j < c->g.nz)
+        rdata.push_back(make_int4(e.i, e.j, e.q, 0));
+  }
+  roff[ntile] = int(rdata.size());
+  HIPCHK(c->inj_off.ensure(off.size()));
+  HIPCHK(hipMemcpyAsync(c->inj_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c->inj.ensure(std::max<size_t>(data.size(), 1)));
+  if (!data.empty())
+    HIPCHK(hipMemcpyAsync(c->inj.p, data.data(), data.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c->rec_off.ensure(roff.size()));
+  HIPCHK(hipMemcpyAsync(c->rec_off.p, roff.data(), roff.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c->rec.ensure(std::max<size_t>(rdata.size(), 1)));
+  if (!rdata.empty())
+    HIPCHK(hipMemcpyAsync(c->rec.p, rdata.data(), rdata.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));   // host vectors go out of scope
+  return 0;
+}
+
+std::vector<Entry> source_entries(const sw_ctx* c, int isrc) {
+  std::vector<Entry> v;
+  for (int q = 0; q < c->nsrc; ++q) {
+    if (isrc >= 0 && q != isrc) continue;
+    const uint32_t k = uint32_t(c->src_k[q]);
+    if (int64_t(k) >= c->npt) continue;
+    v.push_back({int(k / uint32_t(c->g.nz)), int(k % uint32_t(c->g.nz)), q});
+  }
+  return v;
+}
+
+// Allocate/zero the batch state for `nshots` shots with the given injection
+// lists (solver.rs:978-981: clear dynamic fields and traces).
+int prepare_batch(sw_ctx* c, int nshots, const std::vector<std::vector<Entry>>& inj, bool record) {
+  if (!c->model_set) return fail(-2, "model not set");
+  if (nshots < 1) return fail(-2, "nshots must be >= 1");
+  int rc = check_tiling(c);
+  if (rc) return rc;
+  c->nshots = nshots;
+  const size_t slot_elems = size_t(nshots) * size_t(c->npt);
+  for (int p = 0; p < 2; ++p) {
+    HIPCHK(c->state[p].ensure(std::max<size_t>(size_t(c->nslots) * slot_elems, 1)));
+    HIPCHK(launch_fill(c->state[p].p, 0.0f, int64_t(c->nslots) * int64_t(slot_elems), c->stream));
+  }
+  const size_t nu = size_t(nshots) * std::max(c->nrec, 1) * N_UACC;
+  HIPCHK(c->uacc.ensure(nu));
+  HIPCHK(launch_fill(c->uacc.p, 0.0f, int64_t(nu), c->stream));
+  const size_t no = size_t(nshots) * 5 * std::max(c->nrec, 1) * size_t(c->g.nt);
+  HIPCHK(c->obs.ensure(no));
+  HIPCHK(launch_fill(c->obs.p, 0.0f, int64_t(no), c->stream));
+  HIPCHK(c->clock.ensure(2));
+  const int zero2[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(c->clock.p, zero2, sizeof zero2, hipMemcpyHostToDevice, c->stream));
+  rc = build_lists(c, inj);
+  if (rc) return rc;
+  c->cur = 0;
+  c->steps_done = 0;
+  c->record = record;
+  return 0;
+}
+
+int launch_steps(sw_ctx* c, int nsteps) {
+  StepArgs a{};
+  a.shot_stride = c->npt;
+  a.rho = c->rho.p; a.lam = c->lam.p; a.mu = c->mu.p;
+  a.nu = c->nu.p; a.jj = c->jj.p; a.muc = c->mu_c.p; a.nuc = c->nu_c.p;
+  a.bx = c->bx.p; a.bzb = c->bzb.p; a.bzt = c->bzt.p;
+  a.nx = c->g.nx; a.nz = c->g.nz; a.nshots = c->nshots;
+  a.nstrips = c->tile.nstrips; a.nsegs = c->tile.nsegs; a.seglen = c->tile.seglen;
+  a.dt = c->g.dt; a.h8x = c->h8x; a.h8z = c->h8z;
+  a.inj_off = c->inj_off.p; a.inj = c->inj.p;
+  a.val_x = c->val_x; a.val_y = c->val_y; a.val_z = c->val_z; a.nt_val = c->nt_val;
+  a.rec_off = c->rec_off.p; a.rec = c->rec.p;
+  a.uacc = c->uacc.p; a.obs = c->obs.p; a.nrec = c->nrec; a.nt = c->g.nt;
+  a.record = c->record ? 1 : 0;
+  a.clock = c->clock.p;
+  const size_t slot_elems = size_t(c->nshots) * size_t(c->npt);
+  const bool sh = uses_sh(c), psv = uses_psv(c) && !c->sh_only, spin = uses_spin(c);
+  for (int s = 0; s < nsteps; ++s) {
+    for (int q = 0; q < N_SLOTS; ++q) {
+      const int si = c->slot_index[q];
+      a.in[q] = si < 0 ? nullptr : c->state[c->cur].p + size_t(si) * slot_elems;
+      a.out[q] = si < 0 ? nullptr : c->state[c->cur ^ 1].p + size_t(si) * slot_elems;
+    }
+    if (sh) {
+      a.advance = psv ? 0 : 1;
+      HIPCHK(launch_step_sh(a, c->tile, c->stream));
+      ++c->launches;
+    }
+    if (psv) {
+      a.advance = 1;
+      HIPCHK(launch_step_psv(a, c->tile, spin, c->stream));
+      ++c->launches;
+    }
+    if (!sh && !psv) {
+      // nothing to integrate; keep the clock in step with the host
+      const int v[2] = {int(c->steps_done + 1), 0};
+      HIPCHK(hipMemcpyAsync(c->clock.p, v, sizeof v, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    c->cur ^= 1;
+    ++c->steps_done;
+  }
+  return 0;
+}
+
+float* slot_ptr(sw_ctx* c, int slot, int shot) {
+  const int si = c->slot_index[slot];
+  if (si < 0) return nullptr;
+  return c->state[c->cur].p + (size_t(si) * c->nshots + shot) * size_t(c->npt);
+}
+
+int field_slot(int field) {
+  switch (field) {
+    case SW_F_VY: return S_VY;
+    case SW_F_SXY: return S_SXY;
+    case SW_F_SZY: return S_SZY;
+    case SW_F_VX: return S_VX;
+    case SW_F_VZ: return S_VZ;
+    case SW_F_SXX: return S_SXX;
+    case SW_F_SZZ: return S_SZZ;
+    case SW_F_SXZ: return S_SXZ;
+    case SW_F_VY_C: return S_VYC;
+    case SW_F_SYX_C: return S_SYXC;
+    case SW_F_SYY_C: return S_SYYC;
+    case SW_F_SYZ_C: return S_SYZC;
+  }
+  return -1;
+}
+
+float* model_ptr(sw_ctx* c, int field) {
+  switch (field) {
+    case SW_F_LAM: return c->lam.p;
+    case SW_F_MU: return c->mu.p;
+    case SW_F_NU: return c->nu.p;
+    case SW_F_J: return c->jj.p;
+    case SW_F_LAM_C: return c->lam_c.p;
+    case SW_F_MU_C: return c->mu_c.p;
+    case SW_F_NU_C: return c->nu_c.p;
+    case SW_F_RHO: return c->rho.p;
+    case SW_F_K_MU: return c->kmu.p;
+    case SW_F_K_LAM: return c->k_lam.p;
+    case SW_F_K_RHO: return c->k_rho.p;
+  }
+  return nullptr;
+}
+
+int run_with_snapshots(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user) {
+  const int nt = c->g.nt;
+  std::vector<float> host;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  const int64_t l0 = c->launches;
+  int it = 0;
+  while (it < nt) {
+    int stop = nt - 1;
+    if (snap > 0) {
+      int next = ((it + snap - 1) / snap) * snap;   // first multiple of snap >= it
+      if (next == 0) next = snap;
+      if (next < nt) stop = next;
+    }
+    int rc = launch_steps(c, stop - it + 1);
+    if (rc) return rc;
+    if (snap > 0 && stop > 0 && stop % snap == 0 && cb) {
+      // save_snapshot (solver.rs:1144-1176)
+      const int fields[4] = {SW_F_VY, SW_F_VX, SW_F_VZ, SW_F_VY_C};
+      const bool on[4] = {uses_sh(c), uses_psv(c), uses_psv(c), uses_spin(c)};
+      host.resize(size_t(c->npt));
+      for (int s = 0; s < c->nshots; ++s)
+        for (int q = 0; q < 4; ++q) {
+          if (!on[q]) continue;
+          HIPCHK(hipMemcpyAsync(host.data(), slot_ptr(c, field_slot(fields[q]), s), size_t(c->npt) * 4,
+                                hipMemcpyDeviceToHost, c->stream));
+          HIPCHK(hipStreamSynchronize(c->stream));
+          if (cb(user, s, stop, fields[q], host.data(), c->npt) != 0) return fail(-4, "snapshot callback failed");
+        }
+    }
+    it = stop + 1;
+  }
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->timing.loop_ms = ms;
+  c->timing.steps = nt;
+  c->timing.step_launches = c->launches - l0;
+  c->timing.cell_updates = int64_t(c->npt) * nt * c->nshots;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+SW_API const char* sw_last_error(void) { return g_err.c_str(); }
+
+SW_API int sw_create(const sw_grid* g, int32_t device, sw_ctx** out) {
+  if (!g || !out) return fail(-1, "null argument");
+  if (g->nx < 4 || g->nz < 4) return fail(-2, "grid must be at least 4x4 (got %dx%d)", g->nx, g->nz);
+  if (g->nt < 0) return fail(-2, "nt must be >= 0");
+  if (int64_t(g->nx) * g->nz > (int64_t(1) << 31) - 1) return fail(-2, "grid too large");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(-2, "no HIP device %d (found %d)", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  sw_ctx* c = new sw_ctx;
+  c->g = *g;
+  c->device = device;
+  c->npt = int64_t(g->nx) * g->nz;
+  c->h8x = 8.0f * g->dx;
+  c->h8z = 8.0f * g->dz;
+  compute_slots(c);
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    sw_destroy(c);
+    return fail(-3, "stream/event creation failed: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return 0;
+}
+
+SW_API void sw_destroy(sw_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+SW_API int sw_set_model(sw_ctx* c, int32_t kind, const float* const* f) {
+  if (!c || !f) return fail(-1, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t n = size_t(c->npt), bytes = n * 4;
+  auto up = [&](DBuf<float>& d, const float* h) -> int {
+    HIPCHK(d.ensure(n));
+    if (h) HIPCHK(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(launch_fill(d.p, 0.0f, int64_t(n), c->stream));
+    return 0;
+  };
+  int rc = 0;
+  if (kind == SW_MODEL_VPVS) {
+    if (!f[0] || !f[1] || !f[2]) return fail(-1, "rho, vp and vs are required");
+    if ((rc = up(c->rho, f[0])) || (rc = up(c->lam, f[1])) || (rc = up(c->mu, f[2]))) return rc;
+    for (DBuf<float>* d : {&c->nu, &c->jj, &c->lam_c, &c->mu_c, &c->nu_c})
+      if ((rc = up(*d, nullptr))) return rc;
+    HIPCHK(launch_vps2lm(c->rho.p, c->lam.p, c->mu.p, c->npt, c->stream));
+  } else if (kind == SW_MODEL_LAME) {
+    DBuf<float>* d[8] = {&c->rho, &c->lam, &c->mu, &c->nu, &c->jj, &c->lam_c, &c->mu_c, &c->nu_c};
+    for (int q = 0; q < 8; ++q) {
+      if (!f[q]) return fail(-1, "model field %d missing", q);
+      if ((rc = up(*d[q], f[q]))) return rc;
+    }
+  } else {
+    return fail(-2, "unknown model kind %d", kind);
+  }
+  build_taper(c);
+  HIPCHK(c->bx.ensure(size_t(c->g.nx)));
+  HIPCHK(c->bzb.ensure(size_t(c->g.nz)));
+  HIPCHK(c->bzt.ensure(size_t(c->g.nz)));
+  HIPCHK(hipMemcpyAsync(c->bx.p, c->h_bx.data(), c->h_bx.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->bzb.p, c->h_bzb.data(), c->h_bzb.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->bzt.p, c->h_bzt.data(), c->h_bzt.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->model_set = true;
+  return 0;
+}
+
+SW_API int sw_set_sources(sw_ctx* c, int32_t nsrc, const int32_t* idx, const float* stf) {
+  if (!c || nsrc < 0 || (nsrc > 0 && (!idx || !stf))) return fail(-1, "bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  c->nsrc = nsrc;
+  c->src_k.assign(idx, idx + nsrc);
+  const size_t n = size_t(3) * nsrc * size_t(c->g.nt);
+  HIPCHK(c->stf.ensure(std::max<size_t>(n, 1)));
+  if (n) HIPCHK(hipMemcpy(c->stf.p, stf, n * 4, hipMemcpyHostToDevice));
+  return 0;
+}
+
+SW_API int sw_set_receivers(sw_ctx* c, int32_t nrec, const int32_t* idx) {
+  if (!c || nrec < 0 || (nrec > 0 && !idx)) return fail(-1, "bad arguments");
+  c->nrec = nrec;
+  c->rec_k.assign(idx, idx + nrec);
+  return 0;
+}
+
+SW_API int sw_prepare(sw_ctx* c, int32_t nshots, const int32_t* isrc) {
+  if (!c || nshots < 1 || !isrc) return fail(-1, "bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<std::vector<Entry>> inj(nshots);
+  c->shot_isrc.assign(isrc, isrc + nshots);
+  for (int s = 0; s < nshots; ++s) {
+    if (isrc[s] >= c->nsrc) return fail(-2, "source %d out of range (nsrc=%d)", isrc[s], c->nsrc);
+    inj[s] = source_entries(c, isrc[s]);
+  }
+  const size_t sstride = size_t(c->nsrc) * c->g.nt;
+  c->val_x = c->stf.p;
+  c->val_y = c->stf.p + sstride;
+  c->val_z = c->stf.p + 2 * sstride;
+  c->nt_val = c->g.nt;
+  return prepare_batch(c, nshots, inj, true);
+}
+
+SW_API int sw_advance(sw_ctx* c, int32_t nsteps) {
+  if (!c || nsteps < 0) return fail(-1, "bad arguments");
+  if (c->nshots < 1) return fail(-2, "no prepared batch");
+  if (c->steps_done + nsteps > c->g.nt)
+    return fail(-2, "cannot advance %d steps: %lld of nt=%d done", nsteps, (long long)c->steps_done, c->g.nt);
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t l0 = c->launches;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  int rc = launch_steps(c, nsteps);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->timing.loop_ms = ms;
+  c->timing.steps = nsteps;
+  c->timing.step_launches = c->launches - l0;
+  c->timing.cell_updates = int64_t(c->npt) * nsteps * c->nshots;
+  return 0;
+}
+
+SW_API int sw_forward_batch(sw_ctx* c, int32_t nshots, const int32_t* isrc, int32_t snap, sw_snapshot_fn cb,
+                            void* user) {
+  int rc = sw_prepare(c, nshots, isrc);
+  if (rc) return rc;
+  return run_with_snapshots(c, snap, cb, user);
+}
+
+SW_API int sw_forward(sw_ctx* c, int32_t isrc, int32_t snap, sw_snapshot_fn cb, void* user) {
+  return sw_forward_batch(c, 1, &isrc, snap, cb, user);
+}
+
+SW_API int sw_read_obs(sw_ctx* c, int32_t shot, int32_t comp, float* out) {
+  if (!c || !out || comp < 0 || comp > 4) return fail(-1, "bad arguments");
+  if (shot < 0 || shot >= c->nshots) return fail(-2, "shot %d not in last run", shot);
+  HIPCHK(hipSetDevice(c->device));
+  const size_t n = size_t(c->nrec) * c->g.nt;
+  if (n == 0) return 0;
+  HIPCHK(hipMemcpyAsync(out, c->obs.p + (size_t(shot) * 5 + comp) * n, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+SW_API int sw_read_field(sw_ctx* c, int32_t shot, int32_t field, float* out) {
+  if (!c || !out) return fail(-1, "bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t bytes = size_t(c->npt) * 4;
+  if (field == SW_F_BOUND) {
+    if (!c->model_set) return fail(-2, "model not set");
+    for (int i = 0; i < c->g.nx; ++i)
+      for (int j = 0; j < c->g.nz; ++j)
+        out[size_t(i) * c->g.nz + j] = (c->h_bx[i] * c->h_bzb[j]) * c->h_bzt[j];
+    return 0;
+  }
+  const int slot = field_slot(field);
+  const float* src = nullptr;
+  if (slot >= 0) {
+    if (shot < 0 || shot >= c->nshots) return fail(-2, "shot %d not prepared", shot);
+    src = slot_ptr(c, slot, shot);
+  } else {
+    src = model_ptr(c, field);
+  }
+  if (!src) return fail(-2, "field %d is not materialised for this configuration", field);
+  HIPCHK(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+SW_API int sw_write_field(sw_ctx* c, int32_t shot, int32_t field, const float* in) {
+  if (!c || !in) return fail(-1, "bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const int slot = field_slot(field);
+  float* dst = nullptr;
+  if (slot >= 0) {
+    if (shot < 0 || shot >= c->nshots) return fail(-2, "shot %d not prepared", shot);
+    dst = slot_ptr(c, slot, shot);
+  } else {
+    if (field == SW_F_K_MU) HIPCHK(c->kmu.ensure(size_t(c->npt)));
+    if (field == SW_F_K_LAM) HIPCHK(c->k_lam.ensure(size_t(c->npt)));
+    if (field == SW_F_K_RHO) HIPCHK(c->k_rho.ensure(size_t(c->npt)));
+    dst = model_ptr(c, field);
+  }
+  if (!dst) return fail(-2, "field %d is not materialised for this configuration", field);
+  HIPCHK(hipMemcpyAsync(dst, in, size_t(c->npt) * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+SW_API int sw_set_tiling(sw_ctx* c, int32_t threads, int32_t seglen) {
+  if (!c) return fail(-1, "null ctx");
+  if (!tiling_supported(threads)) return fail(-2, "threads must be 64, 128 or 256");
+  if (seglen < 1) return fail(-2, "seglen must be >= 1");
+  c->tile.nt_threads = threads;
+  c->tile.seglen = seglen;
+  c->nshots = 0;   // lists must be rebuilt
+  return 0;
+}
+
+SW_API int sw_get_tiling(const sw_ctx* c, int32_t* threads, int32_t* seglen) {
+  if (!c || !threads || !seglen) return fail(-1, "null argument");
+  *threads = c->tile.nt_threads;
+  *seglen = c->tile.seglen;
+  return 0;
+}
+
+SW_API int sw_bench_copy(int32_t device, int64_t nbytes, int32_t iters, double* avg_ms) {
+  if (nbytes <= 0 || nbytes % 16 || iters < 1 || !avg_ms) return fail(-1, "bad arguments");
+  HIPCHK(hipSetDevice(device));
+  float *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipMalloc(&a, size_t(nbytes)));
+  HIPCHK(hipMalloc(&b, size_t(nbytes)));
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(launch_fill(a, 1.0f, nbytes / 4, s));
+  HIPCHK(launch_copy(a, b, nbytes / 4, s));
+  HIPCHK(hipEventRecord(e0, s));
+  for (int q = 0; q < iters; ++q) HIPCHK(launch_copy(a, b, nbytes / 4, s));
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = double(ms) / iters;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return 0;
+}
+
+SW_API int sw_get_timing(const sw_ctx* c, sw_timing* out) {
+  if (!c || !out) return fail(-1, "null argument");
+  *out = c->timing;
+  return 0;
+}
+
+SW_API int sw_smooth(sw_ctx* c, int32_t field, float sigma) {
+  if (!c) return fail(-1, "null ctx");
+  HIPCHK(hipSetDevice(c->device));
+  float* k = model_ptr(c, field);
+  if (field != SW_F_K_MU && field != SW_F_K_LAM && field != SW_F_K_RHO) return fail(-2, "only kernel fields smooth");
+  if (!k) return fail(-2, "kernel field %d not computed", field);
+  const int nx = c->g.nx, nz = c->g.nz, nmax = std::max(nx, nz);
+  // w[d] = exp(-d*d / (2*sigma*sigma)) in f32, the reference's expression order
+  std::vector<float> w(nmax), sx(nx, 0.0f), sz(nz, 0.0f);
+  for (int d = 0; d < nmax; ++d) {
+    const float fd = float(d);
+    w[d] = std::exp(-fd * fd / (2.0f * sigma * sigma));
+  }
+  for (int i = 0; i < nx; ++i) {
+    float s = 0.0f;
+    for (int n = 0; n < nx; ++n) s = s + w[std::abs(i - n)];
+    sx[i] = s;
+  }
+  for (int j = 0; j < nz; ++j) {
+    float s = 0.0f;
+    for (int n = 0; n < nz; ++n) s = s + w[std::abs(j - n)];
+    sz[j] = s;
+  }
+  HIPCHK(c->wtab.ensure(nmax));
+  HIPCHK(c->sumx.ensure(nx));
+  HIPCHK(c->sumz.ensure(nz));
+  HIPCHK(c->gsum.ensure(size_t(c->npt)));
+  HIPCHK(c->gtmp.ensure(size_t(c->npt)));
+  HIPCHK(hipMemcpyAsync(c->wtab.p, w.data(), w.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->sumx.p, sx.data(), sx.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->sumz.p, sz.data(), sz.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_gsum(c->sumx.p, c->sumz.p, c->gsum.p, nx, nz, c->stream));
+  HIPCHK(launch_gauss_x(c->wtab.p, k, c->gtmp.p, nx, nz, c->stream));
+  HIPCHK(launch_gauss_z(c->wtab.p, c->gtmp.p, c->gsum.p, k, nx, nz, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+SW_API int sw_adjoint_plain(sw_ctx* c, int32_t nruns, const int32_t* isrc, const float* obs_ref,
+                            int32_t adj_interval, float smooth, double* misfit) {
+  if (!c || nruns < 1 || !isrc || !obs_ref) return fail(-1, "bad arguments");
+  if (adj_interval <= 0) return fail(-2, "adj_interval must be > 0");
+  const int nt = c->g.nt, nrec = c->nrec;
+  const int nsa = nt / adj_interval;
+  if (nsa == 0) return fail(-2, "adj_interval (%d) >= nt (%d)", adj_interval, nt);   // solver.rs:1311-1314
+  HIPCHK(hipSetDevice(c->device));
+  const size_t npt = size_t(c->npt);
+  HIPCHK(c->kmu.ensure(npt));
+  HIPCHK(launch_fill(c->kmu.p, 0.0f, int64_t(npt), c->stream));
+  HIPCHK(c->fwd_store.ensure(size_t(nsa) * npt));
+  const size_t nobs = size_t(nrec) * nt;
+  HIPCHK(c->adj_src.ensure(std::max<size_t>(nobs, 1)));
+  HIPCHK(c->ref.ensure(std::max<size_t>(nobs, 1)));
+  // misfit taper (solver.rs:441-455) tabulated in f32 on the host
+  std::vector<float> taper(nt);
+  {
+    const float PI_F = 3.14159265358979323846f;
+    const float dt = c->g.dt;
+    const float t_end = float(nt - 1) * dt, tw = t_end / 10.0f, t_min = tw, t_max = t_end - tw;
+    for (int it = 0; it < nt; ++it) {
+      const float t = float(it) * dt;
+      if (t <= t_min) taper[it] = 0.5f + 0.5f * std::cos(PI_F * (t_min - t) / tw);
+      else if (t >= t_max) taper[it] = 0.5f + 0.5f * std::cos(PI_F * (t_max - t) / tw);
+      else taper[it] = 1.0f;
+    }
+  }
+  HIPCHK(c->taper.ensure(std::max(nt, 1)));
+  HIPCHK(hipMemcpyAsync(c->taper.p, taper.data(), size_t(nt) * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c->partial.ensure(std::max<size_t>((nobs + 255) / 256, 1)));
+  HIPCHK(c->misfit_d.ensure(1));
+  const float ndt = float(adj_interval) * c->g.dt;
+  const size_t sstride = size_t(c->nsrc) * nt;
+  const bool sh = uses_sh(c);
+  struct ShOnly {
+    sw_ctx* c;
+    explicit ShOnly(sw_ctx* x) : c(x) { c->sh_only = true; }
+    ~ShOnly() { c->sh_only = false; }
+  } sh_only_scope(c);
+  for (int r = 0; r < nruns; ++r) {
+    if (isrc[r] >= c->nsrc) return fail(-2, "source %d out of range", isrc[r]);
+    // forward run, keeping vy after every adj_interval-th step (solver.rs:1381-1414)
+    std::vector<std::vector<Entry>> inj(1, source_entries(c, isrc[r]));
+    c->val_x = c->stf.p; c->val_y = c->stf.p + sstride; c->val_z = c->stf.p + 2 * sstride; c->nt_val = nt;
+    int rc = prepare_batch(c, 1, inj, true);
+    if (rc) return rc;
+    HIPCHK(launch_fill(c->fwd_store.p, 0.0f, int64_t(nsa) * int64_t(npt), c->stream));
+    for (int it = 0; it < nt; ++it) {
+      if ((rc = launch_steps(c, 1))) return rc;
+      if ((it + 1) % adj_interval == 0 && sh) {
+        const int isa = nsa - (it + 1) / adj_interval;
+        HIPCHK(hipMemcpyAsync(c->fwd_store.p + size_t(isa) * npt, slot_ptr(c, S_VY, 0), npt * 4,
+                              hipMemcpyDeviceToDevice, c->stream));
+      }
+    }
+    // misfit + adjoint source (solver.rs:1425-1439)
+    double mis = 0.0;
+    if (sh && nobs) {
+      HIPCHK(hipMemcpyAsync(c->ref.p, obs_ref + size_t(r) * nobs, nobs * 4, hipMemcpyHostToDevice, c->stream));
+      int nb = 0;
+      HIPCHK(launch_misfit(c->obs.p + 1 * nobs, c->ref.p, c->taper.p, c->adj_src.p, c->partial.p, nrec, nt,
+                           c->stream, &nb));
+      HIPCHK(launch_reduce_sum(c->partial.p, nb, c->misfit_d.p, c->stream));
+      HIPCHK(hipMemcpyAsync(&mis, c->misfit_d.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      mis = std::sqrt(mis);
+    } else if (nobs) {
+      HIPCHK(launch_fill(c->adj_src.p, 0.0f, int64_t(nobs), c->stream));
+    }
+    if (misfit) misfit[r] = mis;
+    // adjoint run: adjoint source injected at the receivers (adj_dsy,
+    // fd2d.wgsl:562-568), imaging every adj_interval steps (solver.rs:1442-1498)
+    std::vector<Entry> rin;
+    for (int q = 0; q < nrec; ++q) {
+      const uint32_t k = uint32_t(c->rec_k[q]);
+      if (int64_t(k) >= c->npt) continue;
+      rin.push_back({int(k / uint32_t(c->g.nz)), int(k % uint32_t(c->g.nz)), q});
+    }
+    std::vector<std::vector<Entry>> ainj(1, rin);
+    c->val_x = c->adj_src.p; c->val_y = c->adj_src.p; c->val_z = c->adj_src.p; c->nt_val = nt;
+    if ((rc = prepare_batch(c, 1, ainj, false))) return rc;
+    for (int it = 0; it < nt; ++it) {
+      if ((rc = launch_steps(c, 1))) return rc;
+      if (it % adj_interval == 0 && sh) {
+        const int isae = it / adj_interval;
+        if (isae < nsa)
+          HIPCHK(launch_imaging_sh(slot_ptr(c, S_VY, 0), c->fwd_store.p + size_t(isae) * npt, c->kmu.p, c->g.nx,
+                                   c->g.nz, c->h8x, c->h8z, ndt, c->stream));
+      }
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return sw_smooth(c, SW_F_K_MU, smooth);
+}
+
+}  // extern "C"

@@ -1,0 +1,644 @@
+// Host side of the `seiswg <config.ini>` workflow (include/seiswg_host.h).
+//
+// Restates config.rs (INI parsing rules), main.rs (workflow dispatch) and the
+// host parts of solver.rs (file formats, grid set-up, Ricker table, forward
+// and adjoint drivers, writers) over the device ABI of seiswg_hip.h.  All
+// shots of a forward workflow advance together in one batched launch per time
+// step; outputs are written exactly as the reference writes them.
+#include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/seiswg_host.h"
+
+namespace fs = std::filesystem;
+
+namespace {
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+std::string fmt(const char* f, ...) {
+  char buf[2048];
+  va_list ap;
+  va_start(ap, f);
+  vsnprintf(buf, sizeof buf, f, ap);
+  va_end(ap);
+  return buf;
+}
+
+// env_logger-style info lines on stderr; RUST_LOG=error|warn|off silences them
+// (main.rs:34-36, integration.rs:83).
+int log_level() {
+  static int lvl = [] {
+    const char* e = std::getenv("RUST_LOG");
+    if (!e) return 3;
+    std::string s(e);
+    for (auto& ch : s) ch = char(std::tolower(ch));
+    if (s == "off") return 0;
+    if (s == "error") return 1;
+    if (s == "warn") return 2;
+    return 3;
+  }();
+  return lvl;
+}
+void log_info(const std::string& m) {
+  if (log_level() >= 3) fprintf(stderr, "[INFO  seiswg] %s\n", m.c_str());
+}
+void log_warn(const std::string& m) {
+  if (log_level() >= 2) fprintf(stderr, "[WARN  seiswg] %s\n", m.c_str());
+}
+
+std::string trim(const std::string& s) {
+  size_t a = 0, b = s.size();
+  while (a < b && std::isspace(static_cast<unsigned char>(s[a]))) ++a;
+  while (b > a && std::isspace(static_cast<unsigned char>(s[b - 1]))) --b;
+  return s.substr(a, b - a);
+}
+std::string lower(std::string s) {
+  for (auto& ch : s) ch = char(std::tolower(static_cast<unsigned char>(ch)));
+  return s;
+}
+
+// Grammar of Rust's f32/f64 FromStr: [+-]? (inf | infinity | nan | digits[.digits][exp] | .digits[exp])
+bool rust_float_syntax(const std::string& s) {
+  size_t p = 0, n = s.size();
+  if (p < n && (s[p] == '+' || s[p] == '-')) ++p;
+  std::string rest = lower(s.substr(p));
+  if (rest == "inf" || rest == "infinity" || rest == "nan") return true;
+  size_t d0 = p;
+  while (p < n && std::isdigit(static_cast<unsigned char>(s[p]))) ++p;
+  size_t ni = p - d0, nf = 0;
+  if (p < n && s[p] == '.') {
+    ++p;
+    size_t f0 = p;
+    while (p < n && std::isdigit(static_cast<unsigned char>(s[p]))) ++p;
+    nf = p - f0;
+  }
+  if (ni + nf == 0) return false;
+  if (p < n && (s[p] == 'e' || s[p] == 'E')) {
+    ++p;
+    if (p < n && (s[p] == '+' || s[p] == '-')) ++p;
+    size_t e0 = p;
+    while (p < n && std::isdigit(static_cast<unsigned char>(s[p]))) ++p;
+    if (p == e0) return false;
+  }
+  return p == n;
+}
+bool parse_f64(const std::string& s, double& out) {
+  if (!rust_float_syntax(s)) return false;
+  out = std::strtod(s.c_str(), nullptr);
+  return true;
+}
+bool parse_f32(const std::string& s, float& out) {
+  if (!rust_float_syntax(s)) return false;
+  out = std::strtof(s.c_str(), nullptr);
+  return true;
+}
+bool parse_u32(const std::string& s, uint32_t& out) {
+  size_t p = 0;
+  if (!s.empty() && s[0] == '+') p = 1;
+  if (p >= s.size()) return false;
+  uint64_t v = 0;
+  for (; p < s.size(); ++p) {
+    if (!std::isdigit(static_cast<unsigned char>(s[p]))) return false;
+    v = v * 10 + uint64_t(s[p] - '0');
+    if (v > 0xFFFFFFFFull) return false;
+  }
+  out = uint32_t(v);
+  return true;
+}
+
+// ---------------------------------------------------------------- INI
+// configparser 3.1.0 rules as used by config.rs: sections and keys are
+// lower-cased, '=' or ':' delimit, ';' and '#' start comments, a key without
+// a delimiter has no value.
+struct Ini {
+  std::map<std::string, std::map<std::string, std::pair<bool, std::string>>> sec;
+  bool get(const std::string& s, const std::string& k, std::string& v) const {
+    auto a = sec.find(s);
+    if (a == sec.end()) return false;
+    auto b = a->second.find(k);
+    if (b == a->second.end() || !b->second.first) return false;
+    v = b->second.second;
+    return true;
+  }
+};
+
+Ini parse_ini_text(const std::string& text) {
+  Ini ini;
+  std::string section = "default";
+  std::istringstream in(text);
+  std::string raw;
+  int num = 0;
+  while (std::getline(in, raw)) {
+    if (!raw.empty() && raw.back() == '\r') raw.pop_back();
+    size_t c = raw.find_first_of(";#");
+    std::string line = trim(c == std::string::npos ? raw : raw.substr(0, c));
+    ++num;
+    if (line.empty()) continue;
+    if (line[0] == '[') {
+      size_t e = line.rfind(']');
+      if (e == std::string::npos)
+        throw Error(fmt("line %d: Found opening bracket for section name but no closing bracket", num - 1));
+      section = lower(trim(line.substr(1, e - 1)));
+      ini.sec[section];
+      continue;
+    }
+    size_t d = line.find_first_of("=:");
+    if (d != std::string::npos) {
+      std::string key = lower(trim(line.substr(0, d)));
+      if (key.empty()) throw Error(fmt("line %d:%zu: Key cannot be empty", num - 1, d));
+      ini.sec[section][key] = {true, trim(line.substr(d + 1))};
+    } else {
+      ini.sec[section][lower(line)] = {false, ""};
+    }
+  }
+  return ini;
+}
+
+struct Config {
+  std::string workflow;
+  uint32_t nt = 0;
+  float dt = 0;
+  bool left = false, right = false, bottom = false, top = false;
+  uint32_t width = 20;
+  float alpha = 0.015f;
+  bool sh = false, psv = true, spin = false;
+  uint32_t threads_per_block = 128;
+  bool combine = false;
+  uint32_t save_snapshot = 0, adj_interval = 10;
+  float smooth = 5.0f;
+  fs::path output, output_traces, model_true, model_init, sources, stations, traces;
+  bool has_output_traces = false, has_model_true = false, has_model_init = false, has_traces = false;
+};
+
+// config.rs:62-192
+Config load_config(const fs::path& path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw Error(fmt("Cannot open %s: %s", path.string().c_str(), std::strerror(errno)));
+  std::stringstream ss;
+  ss << f.rdbuf();
+  Ini ini = parse_ini_text(ss.str());
+  fs::path base = path.parent_path();
+  auto required = [&](const char* s, const char* k) {
+    std::string v;
+    if (!ini.get(s, k, v)) throw Error(fmt("[%s] %s is required", s, k));
+    return v;
+  };
+  auto get_bool = [&](const char* k, bool d) {
+    std::string v;
+    return ini.get("solver", k, v) ? lower(trim(v)) == "yes" : d;
+  };
+  auto get_u32_or = [&](const char* k, uint32_t d) {
+    std::string v;
+    uint32_t x;
+    return (ini.get("solver", k, v) && parse_u32(trim(v), x)) ? x : d;
+  };
+  auto get_f32_or = [&](const char* k, float d) {
+    std::string v;
+    float x;
+    return (ini.get("solver", k, v) && parse_f32(trim(v), x)) ? x : d;
+  };
+  auto resolve = [&](const std::string& raw) {
+    fs::path p(trim(raw));
+    return p.is_absolute() ? p : base / p;
+  };
+  Config c;
+  c.workflow = lower(trim(required("workflow", "mode")));
+  if (!parse_u32(trim(required("solver", "nt")), c.nt)) throw Error("[solver] nt must be an integer");
+  if (!parse_f32(trim(required("solver", "dt")), c.dt)) throw Error("[solver] dt must be a float");
+  c.left = get_bool("abs_left", false);
+  c.right = get_bool("abs_right", false);
+  c.bottom = get_bool("abs_bottom", false);
+  c.top = get_bool("abs_top", false);
+  c.width = get_u32_or("abs_width", 20);
+  c.alpha = get_f32_or("abs_alpha", 0.015f);
+  c.sh = get_bool("sh", false);
+  c.psv = get_bool("psv", true);
+  c.spin = get_bool("spin", false);
+  c.threads_per_block = get_u32_or("threads_per_block", 128);
+  c.combine = get_bool("combine_sources", false);
+  c.save_snapshot = get_u32_or("save_snapshot", 0);
+  c.adj_interval = get_u32_or("adj_interval", 10);
+  c.smooth = get_f32_or("smooth", 5.0f);
+  std::string v;
+  c.output = resolve(required("path", "output"));
+  if (ini.get("path", "output_traces", v)) { c.output_traces = resolve(v); c.has_output_traces = true; }
+  if (ini.get("path", "model_true", v)) { c.model_true = resolve(v); c.has_model_true = true; }
+  if (ini.get("path", "model_init", v)) { c.model_init = resolve(v); c.has_model_init = true; }
+  c.sources = resolve(required("path", "sources"));
+  c.stations = resolve(required("path", "stations"));
+  if (ini.get("path", "traces", v)) { c.traces = resolve(v); c.has_traces = true; }
+  return c;
+}
+
+// ---------------------------------------------------------------- files
+std::vector<char> read_all(const fs::path& p, bool& ok) {
+  std::ifstream f(p, std::ios::binary);
+  ok = bool(f);
+  if (!ok) return {};
+  return std::vector<char>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+}
+
+// load_model_field (solver.rs:300-325): i32 npt, then npt f32; extra bytes ignored
+std::vector<float> load_model_field(const fs::path& p) {
+  bool ok;
+  std::vector<char> b = read_all(p, ok);
+  if (!ok) throw Error(fmt("Cannot read model file %s", p.string().c_str()));
+  if (b.size() < 4) throw Error(fmt("Model file too small: %s", p.string().c_str()));
+  int32_t n;
+  std::memcpy(&n, b.data(), 4);
+  const size_t expected = 4 + size_t(uint32_t(n)) * 4;
+  if (n < 0 || b.size() < expected)
+    throw Error(fmt("Model file %s too short (expected %zu bytes, got %zu)", p.string().c_str(), expected, b.size()));
+  std::vector<float> v(static_cast<size_t>(n));
+  if (n) std::memcpy(v.data(), b.data() + 4, size_t(n) * 4);
+  return v;
+}
+
+// load_text_table (solver.rs:328-346)
+std::vector<std::vector<double>> load_table(const fs::path& p) {
+  bool ok;
+  std::vector<char> b = read_all(p, ok);
+  if (!ok) throw Error(fmt("Cannot read %s", p.string().c_str()));
+  std::vector<std::vector<double>> rows;
+  std::istringstream in(std::string(b.begin(), b.end()));
+  std::string line;
+  while (std::getline(in, line)) {
+    std::string t = trim(line);
+    if (t.empty() || t[0] == '#') continue;
+    std::vector<double> row;
+    std::istringstream ls(line);
+    std::string tok;
+    while (ls >> tok) {
+      double x;
+      row.push_back(parse_f64(tok, x) ? x : 0.0);
+    }
+    rows.push_back(row);
+  }
+  return rows;
+}
+
+// write_npy_f32 / make_npy_bytes (solver.rs:267-285, 361-389)
+void write_npy(const fs::path& p, const float* data, size_t nrows, size_t ncols) {
+  std::string dict = fmt("{'descr': '<f4', 'fortran_order': False, 'shape': (%zu, %zu), }", nrows, ncols);
+  const size_t hl = ((dict.size() + 1 + 63) / 64) * 64;
+  std::string header = dict;
+  while (header.size() < hl - 1) header.push_back(' ');
+  header.push_back('\n');
+  std::ofstream f(p, std::ios::binary | std::ios::trunc);
+  if (!f) throw Error(fmt("Cannot create %s", p.string().c_str()));
+  f.write("\x93NUMPY\x01\x00", 8);
+  const uint16_t h16 = uint16_t(hl);
+  f.write(reinterpret_cast<const char*>(&h16), 2);
+  f.write(header.data(), std::streamsize(header.size()));
+  f.write(reinterpret_cast<const char*>(data), std::streamsize(nrows * ncols * 4));
+  if (!f) throw Error(fmt("Cannot write %s", p.string().c_str()));
+}
+
+void write_raw(const fs::path& p, const float* data, size_t n, const int32_t* header = nullptr) {
+  std::ofstream f(p, std::ios::binary | std::ios::trunc);
+  if (!f) throw Error(fmt("Cannot write %s", p.string().c_str()));
+  if (header) f.write(reinterpret_cast<const char*>(header), 4);
+  f.write(reinterpret_cast<const char*>(data), std::streamsize(n * 4));
+  if (!f) throw Error(fmt("Cannot write %s", p.string().c_str()));
+}
+
+// load_npy_f32 (solver.rs:401-428)
+std::vector<float> load_npy(const fs::path& p) {
+  bool ok;
+  std::vector<char> b = read_all(p, ok);
+  if (!ok) throw Error(fmt("Cannot load observed traces %s", p.string().c_str()));
+  if (b.size() < 10 || std::memcmp(b.data(), "\x93NUMPY", 6) != 0)
+    throw Error(fmt("Cannot load observed traces %s: Not a NumPy file", p.string().c_str()));
+  const size_t hl = size_t(uint8_t(b[8])) | (size_t(uint8_t(b[9])) << 8);
+  const size_t start = 10 + hl;
+  if (b.size() < start) throw Error(fmt("Truncated .npy header in %s", p.string().c_str()));
+  if ((b.size() - start) % 4) throw Error(fmt("Payload length not a multiple of 4 in %s", p.string().c_str()));
+  std::vector<float> v((b.size() - start) / 4);
+  if (!v.empty()) std::memcpy(v.data(), b.data() + start, v.size() * 4);
+  return v;
+}
+
+uint32_t sat_u32(float v) {   // Rust `as u32`: NaN -> 0, saturating
+  if (!(v == v) || v <= 0.0f) return 0;
+  if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+  return uint32_t(v);
+}
+
+void check(int rc) {
+  if (rc != 0) throw Error(sw_last_error());
+}
+
+struct Problem {
+  Config cfg;
+  int32_t nx = 0, nz = 0;
+  float dx = 0, dz = 0;
+  int64_t npt = 0;
+  std::vector<uint32_t> src_k, rec_k;
+  sw_ctx* ctx = nullptr;
+  ~Problem() { sw_destroy(ctx); }
+};
+
+// Solver::new (solver.rs:578-933)
+void solver_new(Problem& P) {
+  const Config& c = P.cfg;
+  fs::path model_dir;
+  if (c.has_model_init) model_dir = c.model_init;
+  else if (c.has_model_true) model_dir = c.model_true;
+  else throw Error("No model path in config");
+  const bool spin = c.spin;
+  std::vector<float> rho = load_model_field(model_dir / "proc000000_rho.bin");
+  const int64_t npt = int64_t(rho.size());
+  std::vector<float> x = load_model_field(model_dir / "proc000000_x.bin");
+  std::vector<float> z = load_model_field(model_dir / "proc000000_z.bin");
+  if (int64_t(x.size()) != npt || int64_t(z.size()) != npt)
+    throw Error(fmt("coordinate files hold %zu/%zu points, expected %lld", x.size(), z.size(), (long long)npt));
+  int32_t fb = 0;
+  check(seiswg_infer_grid(npt, x.data(), z.data(), &P.nx, &P.nz, &P.dx, &P.dz, &fb));
+  if (fb)
+    log_warn(fmt("grid rule round(sqrt(npt*lx/lz)) does not factor npt=%lld; using the exact shape %dx%d taken "
+                 "from the x coordinates",
+                 (long long)npt, P.nx, P.nz));
+  log_info(fmt("Grid: nx=%d nz=%d npt=%lld dx=%.1f dz=%.1f", P.nx, P.nz, (long long)npt, P.dx, P.dz));
+  P.npt = npt;
+  std::vector<std::vector<float>> fields;
+  const char* names_vs[] = {"rho", "vp", "vs"};
+  const char* names_sp[] = {"rho", "lambda", "mu", "nu", "j", "lambda_c", "mu_c", "nu_c"};
+  fields.push_back(std::move(rho));
+  const int nf = spin ? 8 : 3;
+  for (int q = 1; q < nf; ++q) {
+    const char* nm = spin ? names_sp[q] : names_vs[q];
+    fields.push_back(load_model_field(model_dir / (std::string("proc000000_") + nm + ".bin")));
+    if (int64_t(fields.back().size()) < npt)
+      throw Error(fmt("Model file proc000000_%s.bin holds %zu values, expected %lld", nm, fields.back().size(),
+                      (long long)npt));
+  }
+  // sources (solver.rs:702-739)
+  auto src_table = load_table(c.sources);
+  const int nsrc = int(src_table.size());
+  std::vector<float> sp(size_t(nsrc) * 4);
+  for (int s = 0; s < nsrc; ++s) {
+    const auto& r = src_table[s];
+    auto g = [&](size_t n, double d) { return n < r.size() ? r[n] : d; };
+    const float sx = float(g(0, 0.0)), sz = float(g(1, 0.0));
+    sp[4 * s + 0] = float(g(3, 10.0));
+    sp[4 * s + 1] = float(g(4, 0.1));
+    sp[4 * s + 2] = float(g(5, 0.0));
+    sp[4 * s + 3] = float(g(6, 1.0e10));
+    P.src_k.push_back(seiswg_grid_index(sx, sz, P.dx, P.dz, uint32_t(P.nz)));
+  }
+  std::vector<float> stf(size_t(3) * nsrc * c.nt);
+  check(seiswg_ricker_table(nsrc, int32_t(c.nt), c.dt, sp.data(), stf.data()));
+  // stations (solver.rs:741-751)
+  auto rec_table = load_table(c.stations);
+  for (const auto& r : rec_table) {
+    const float rx = float(r.size() > 0 ? r[0] : 0.0), rz = float(r.size() > 1 ? r[1] : 0.0);
+    P.rec_k.push_back(seiswg_grid_index(rx, rz, P.dx, P.dz, uint32_t(P.nz)));
+  }
+  // device module
+  sw_grid g{};
+  g.nx = P.nx; g.nz = P.nz; g.nt = int32_t(c.nt);
+  g.dx = P.dx; g.dz = P.dz; g.dt = c.dt;
+  g.sh = c.sh; g.psv = c.psv; g.spin = c.spin;
+  g.abs_left = c.left; g.abs_right = c.right; g.abs_bottom = c.bottom; g.abs_top = c.top;
+  g.abs_width = int32_t(c.width); g.abs_alpha = c.alpha;
+  check(sw_create(&g, 0, &P.ctx));
+  std::vector<const float*> fp;
+  for (auto& f : fields) fp.push_back(f.data());
+  check(sw_set_model(P.ctx, spin ? SW_MODEL_LAME : SW_MODEL_VPVS, fp.data()));
+  std::vector<int32_t> si(P.src_k.begin(), P.src_k.end()), ri(P.rec_k.begin(), P.rec_k.end());
+  check(sw_set_sources(P.ctx, nsrc, si.data(), stf.data()));
+  check(sw_set_receivers(P.ctx, int32_t(ri.size()), ri.data()));
+}
+
+struct SnapWriter {
+  fs::path out_dir;
+  int last_shot;
+  std::string err;
+};
+
+int snap_cb(void* user, int32_t shot, int32_t it, int32_t field, const float* data, int64_t npt) {
+  auto* w = static_cast<SnapWriter*>(user);
+  if (shot != w->last_shot) return 0;   // later runs overwrite the same files (solver.rs:1144-1176)
+  const char* nm = field == SW_F_VY ? "vy" : field == SW_F_VX ? "vx" : field == SW_F_VZ ? "vz" : "ry";
+  try {
+    write_raw(w->out_dir / fmt("proc%06d_%s.bin", it, nm), data, size_t(npt));
+  } catch (const std::exception& e) {
+    w->err = e.what();
+    return 1;
+  }
+  return 0;
+}
+
+// run_forward (solver.rs:941-1119)
+void run_forward(Problem& P) {
+  const Config& c = P.cfg;
+  const int nsrc = int(P.src_k.size()), nrec = int(P.rec_k.size()), nt = int(c.nt);
+  fs::path trace_dir = c.has_output_traces ? c.output_traces : c.output / "traces";
+  std::error_code ec;
+  fs::create_directories(trace_dir, ec);
+  if (ec) throw Error(fmt("Cannot create %s: %s", trace_dir.string().c_str(), ec.message().c_str()));
+  if (c.save_snapshot > 0) {
+    fs::create_directories(c.output, ec);
+    if (ec) throw Error(fmt("Cannot create %s: %s", c.output.string().c_str(), ec.message().c_str()));
+  }
+  std::vector<int32_t> runs;
+  if (c.combine) runs.push_back(-1);
+  else for (int s = 0; s < nsrc; ++s) runs.push_back(s);
+  if (runs.empty()) return;
+  for (int32_t r : runs)
+    log_info(fmt("Forward run  isrc=%s  (%d time steps)", r < 0 ? "all" : std::to_string(r).c_str(), nt));
+  SnapWriter w{c.output, int(runs.size()) - 1, ""};
+  const int rc = sw_forward_batch(P.ctx, int32_t(runs.size()), runs.data(), int32_t(c.save_snapshot), snap_cb, &w);
+  if (rc) throw Error(w.err.empty() ? std::string(sw_last_error()) : w.err);
+  sw_timing tm{};
+  sw_get_timing(P.ctx, &tm);
+  log_info(fmt("time loop: %lld steps x %zu shots in %.1f ms (%.3g cell-updates/s)", (long long)tm.steps,
+               runs.size(), tm.loop_ms, tm.loop_ms > 0 ? double(tm.cell_updates) / (tm.loop_ms * 1e-3) : 0.0));
+  std::vector<float> comp(size_t(nrec) * nt);
+  for (size_t s = 0; s < runs.size(); ++s) {
+    const int idx = runs[s] < 0 ? 0 : runs[s];
+    auto get = [&](int cc) { check(sw_read_obs(P.ctx, int32_t(s), cc, comp.data())); };
+    if (c.sh) {
+      get(SW_OBS_Y);
+      write_npy(trace_dir / fmt("uy_%06d.npy", idx), comp.data(), nrec, nt);
+      write_raw(trace_dir / fmt("vy_%06d.npy", idx), comp.data(), comp.size());
+    }
+    if (c.psv) {
+      get(SW_OBS_X);
+      write_npy(trace_dir / fmt("ux_%06d.npy", idx), comp.data(), nrec, nt);
+      write_raw(trace_dir / fmt("vx_%06d.npy", idx), comp.data(), comp.size());
+      get(SW_OBS_Z);
+      write_npy(trace_dir / fmt("uz_%06d.npy", idx), comp.data(), nrec, nt);
+      write_raw(trace_dir / fmt("vz_%06d.npy", idx), comp.data(), comp.size());
+      if (c.spin) {
+        get(SW_OBS_Y);
+        write_npy(trace_dir / fmt("ry_%06d.npy", idx), comp.data(), nrec, nt);
+        get(SW_OBS_YI);
+        write_npy(trace_dir / fmt("yi_%06d.npy", idx), comp.data(), nrec, nt);
+      }
+    }
+  }
+  log_info("  → traces saved to " + trace_dir.string());
+}
+
+// run_adjoint (solver.rs:1298-1529)
+void run_adjoint(Problem& P) {
+  const Config& c = P.cfg;
+  const int nsrc = int(P.src_k.size()), nrec = int(P.rec_k.size()), nt = int(c.nt);
+  if (c.adj_interval == 0) throw Error("adj_interval must be > 0");
+  const uint32_t nsa = c.nt / c.adj_interval;
+  if (nsa == 0) throw Error(fmt("adj_interval (%u) >= nt (%u)", c.adj_interval, c.nt));
+  std::error_code ec;
+  fs::create_directories(c.output, ec);
+  if (ec) throw Error(fmt("Cannot create %s: %s", c.output.string().c_str(), ec.message().c_str()));
+  if (!c.has_traces) throw Error("config [path] traces is required for the adjoint workflow");
+  std::vector<int32_t> runs;
+  std::vector<int> ref_idx;
+  if (c.combine) { runs.push_back(-1); ref_idx.push_back(0); }
+  else for (int s = 0; s < nsrc; ++s) { runs.push_back(s); ref_idx.push_back(s); }
+  const size_t nobs = size_t(nrec) * nt;
+  std::vector<float> ref(runs.size() * nobs, 0.0f);
+  if (c.sh) {
+    for (size_t r = 0; r < runs.size(); ++r) {
+      const int i = c.combine ? 0 : ref_idx[r];
+      fs::path p = c.traces / fmt("uy_%06d.npy", i);
+      std::vector<float> v = load_npy(p);
+      if (v.size() < nobs)
+        throw Error(fmt("observed traces %s hold %zu values, expected %zu", p.string().c_str(), v.size(), nobs));
+      std::copy(v.begin(), v.begin() + nobs, ref.begin() + r * nobs);
+    }
+  }
+  for (size_t r = 0; r < runs.size(); ++r)
+    log_info(fmt("Adjoint: source %zu/%zu (%d steps, %u checkpoints)…", r + 1, runs.size(), nt, nsa));
+  std::vector<double> mis(std::max<size_t>(runs.size(), 1), 0.0);
+  check(sw_adjoint_plain(P.ctx, int32_t(runs.size()), runs.data(), ref.data(), int32_t(c.adj_interval), c.smooth,
+                         mis.data()));
+  double total = 0.0;
+  for (size_t r = 0; r < runs.size(); ++r) {
+    log_info(fmt("Adjoint: source %zu misfit = %.6e", r + 1, mis[r]));
+    total += mis[r];
+  }
+  log_info(fmt("Adjoint: total misfit = %.6e", total));
+  log_info(fmt("Adjoint: smoothing kernel (sigma=%.1f)…", c.smooth));
+  std::vector<float> kmu(size_t(P.npt));
+  check(sw_read_field(P.ctx, 0, SW_F_K_MU, kmu.data()));
+  const int32_t hdr = int32_t(P.npt);
+  fs::path kp = c.output / "proc000000_kmu.bin";
+  write_raw(kp, kmu.data(), kmu.size());
+  write_raw(c.output / "gradient_kmu.bin", kmu.data(), kmu.size(), &hdr);
+  log_info("Adjoint: gradient saved to " + kp.string());
+}
+
+}  // namespace
+
+namespace sw {
+void set_last_error(const std::string& msg);   // capi.cpp
+}
+
+extern "C" {
+
+SW_API int seiswg_ricker_table(int32_t nsrc, int32_t nt, float dt, const float* params, float* out) {
+  if (nsrc < 0 || nt < 0 || (nsrc > 0 && (!params || !out))) return -1;
+  const float PI_F = 3.14159265358979323846f;
+  const float RAD = PI_F / 180.0f;   // f32::to_radians
+  const size_t stride = size_t(nsrc) * nt;
+  for (int32_t s = 0; s < nsrc; ++s) {
+    const float f0 = params[4 * s], t0 = params[4 * s + 1], ang = params[4 * s + 2], amp = params[4 * s + 3];
+    const float pf = PI_F * f0;
+    const float a = pf * pf;           // powi(2)
+    const float ar = ang * RAD;
+    const float sa = std::sin(ar), ca = std::cos(ar);
+    for (int32_t it = 0; it < nt; ++it) {
+      const float t = float(it) * dt;
+      const float d = t - t0;
+      const float st = -amp * (1.0f - 2.0f * a * d * d) * std::exp(-a * d * d);
+      out[0 * stride + size_t(s) * nt + it] = st * sa;
+      out[1 * stride + size_t(s) * nt + it] = st;
+      out[2 * stride + size_t(s) * nt + it] = -st * ca;
+    }
+  }
+  return 0;
+}
+
+SW_API int seiswg_infer_grid(int64_t npt, const float* x, const float* z, int32_t* nx, int32_t* nz, float* dx,
+                             float* dz, int32_t* fallback) {
+  if (npt < 4 || !x || !z || !nx || !nz || !dx || !dz) return -1;
+  float xmax = -INFINITY, xmin = INFINITY, zmax = -INFINITY, zmin = INFINITY;
+  for (int64_t k = 0; k < npt; ++k) {
+    xmax = std::fmax(xmax, x[k]); xmin = std::fmin(xmin, x[k]);
+    zmax = std::fmax(zmax, z[k]); zmin = std::fmin(zmin, z[k]);
+  }
+  const float lx = xmax - xmin, lz = zmax - zmin;
+  auto u32r = [](double v) -> uint32_t { return (v != v || v <= 0) ? 0u : v >= 4294967295.0 ? 0xFFFFFFFFu : uint32_t(v); };
+  uint32_t ax = u32r(std::round(std::sqrt(double(npt) * double(lx) / double(lz))));
+  uint32_t az = u32r(std::round(std::sqrt(double(npt) * double(lz) / double(lx))));
+  int32_t fb = 0;
+  if (int64_t(ax) * int64_t(az) != npt) {
+    // exact shape: z is the fast axis, so the first column is the run of equal x values
+    int64_t run = 1;
+    while (run < npt && x[run] == x[0]) ++run;
+    if (npt % run != 0 || run < 2 || npt / run < 2) return -2;
+    az = uint32_t(run);
+    ax = uint32_t(npt / run);
+    fb = 1;
+  }
+  if (ax < 2 || az < 2) return -2;
+  *nx = int32_t(ax);
+  *nz = int32_t(az);
+  *dx = lx / float(ax - 1);
+  *dz = lz / float(az - 1);
+  if (fallback) *fallback = fb;
+  return 0;
+}
+
+SW_API uint32_t seiswg_grid_index(float x, float z, float dx, float dz, uint32_t nz) {
+  const uint32_t ix = sat_u32(std::round(x / dx));
+  const uint32_t iz = sat_u32(std::round(z / dz));
+  return ix * nz + iz;
+}
+
+SW_API int seiswg_run(const char* config_path) {
+  try {
+    if (!config_path) throw Error("no config path");
+    log_info(fmt("Loading config from %s", config_path));
+    Problem P;
+    try {
+      P.cfg = load_config(config_path);
+    } catch (const Error& e) {
+      throw Error(fmt("Failed to load %s: %s", config_path, e.what()));
+    }
+    log_info("Workflow: " + P.cfg.workflow);
+    try {
+      solver_new(P);
+    } catch (const Error& e) {
+      throw Error(std::string("Failed to initialise solver: ") + e.what());
+    }
+    if (P.cfg.workflow == "forward") run_forward(P);
+    else if (P.cfg.workflow == "adjoint") run_adjoint(P);
+    else throw Error(fmt("Workflow '%s' is not yet implemented in seiswg", P.cfg.workflow.c_str()));
+    log_info("Done.");
+    return 0;
+  } catch (const std::exception& e) {
+    sw::set_last_error(e.what());
+    fprintf(stderr, "Error: %s\n", e.what());
+    return 1;
+  }
+}
+
+}  // extern "C"

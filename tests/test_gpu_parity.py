@@ -1,0 +1,232 @@
+"""GPU parity: the fused gfx950 kernels against the CPU oracle (same inputs,
+same FP32 operation order) and against the reference's golden fixtures.
+
+Bar: state fields and traces equal to the FP32 oracle within 1 ulp-level
+relative error (rel-L2 <= 1e-6 per field; typically bit-identical), and
+reference fixtures within the reference's own tolerance (max-rel <= 1e-4,
+tests/integration.rs:40) and rel-L2 <= 1e-5.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import examples as E
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+STATE = {"sh": ["VY", "SXY", "SZY"], "psv": ["VX", "VZ", "SXX", "SZZ", "SXZ"],
+         "spin": ["VY_C", "SYX_C", "SYY_C", "SYZ_C"]}
+
+
+def random_problem(nx, nz, nt, sh, psv, spin, seed, nsrc=4, nrec=7):
+    rng = np.random.default_rng(seed)
+    npt = nx * nz
+    if spin:
+        names = ["rho", "lambda", "mu", "nu", "j", "lambda_c", "mu_c", "nu_c"]
+        model = {n: rng.uniform(0.5, 1.5, npt).astype(np.float32) for n in names}
+    else:
+        model = {"rho": rng.uniform(1.0, 2.0, npt).astype(np.float32),
+                 "vp": rng.uniform(1.5, 2.5, npt).astype(np.float32),
+                 "vs": rng.uniform(0.5, 1.2, npt).astype(np.float32)}
+        model["vp"][:7] = 0.3      # vp <= vs branch of vps2lm
+    # sources: interior, near edges and a duplicated cell (index order kept)
+    cells = [(nx // 2, nz // 2), (1, 2), (nx - 2, nz - 1), (nx // 2, nz // 2), (3, nz - 3)][:nsrc]
+    src_idx = [i * nz + j for i, j in cells]
+    rcells = [(nx // 3, nz // 4), (0, 0), (nx - 1, nz - 2), (2, 1), (nx // 3, nz // 4), (nx - 3, 5), (7, nz - 3)]
+    rec_idx = [i * nz + j for i, j in rcells[:nrec]]
+    p = O.Problem(nx=nx, nz=nz, nt=nt, dx=float(np.float32(1.7)), dz=float(np.float32(2.3)),
+                  dt=float(np.float32(0.05)), sh=sh, psv=psv, spin=spin, abs_left=True, abs_right=True,
+                  abs_bottom=True, abs_top=True, abs_width=6, abs_alpha=float(np.float32(0.1)), model=model,
+                  src_rows=[], src_idx=src_idx, rec_idx=rec_idx)
+    stf = rng.standard_normal((3, len(src_idx), nt)).astype(np.float32)
+    return p, stf
+
+
+def gpu_context(p, stf, seiswg):
+    c = seiswg.Context.from_problem(p)
+    c.set_sources(p.src_idx, stf)
+    c.set_receivers(p.rec_idx)
+    return c
+
+
+def compare(got, want, label, tol=1e-6):
+    got = np.asarray(got, np.float32)
+    want = np.asarray(want, np.float64).astype(np.float32)
+    same = np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    err = O.rel_l2(got, want)
+    assert err <= tol, f"{label}: rel-L2 {err:.3e}"
+    return same
+
+
+@pytest.mark.parametrize("phys", ["sh", "psv", "spin", "all"])
+@pytest.mark.parametrize("tiling", [(64, 7), (128, 1), (256, 64), (64, 200)])
+def test_steps_from_random_state_match_oracle(seiswg, phys, tiling):
+    sh = phys in ("sh", "all")
+    psv = phys in ("psv", "spin", "all")
+    spin = phys in ("spin", "all")
+    nx, nz, nsteps = 45, 131, 4
+    p, stf = random_problem(nx, nz, 6, sh, psv, spin, seed=11)
+    print(f"case phys={phys} tiling={tiling}", flush=True)
+    c = gpu_context(p, stf, seiswg)
+    c.set_tiling(*tiling)
+    o = O.Oracle(p, 4, stf=stf.astype(np.float64))
+    o.clear()
+    rng = np.random.default_rng(5)
+    fields = (STATE["sh"] if sh else []) + (STATE["psv"] if psv else []) + (STATE["spin"] if spin else [])
+    c.prepare([-1])
+    for name in fields:
+        v = rng.standard_normal(nx * nz).astype(np.float32)
+        c.write_field(getattr(seiswg.abi, "F_" + name), v)
+        o.set_field(O.F[name], v)
+    c.advance(nsteps)
+    for it in range(nsteps):
+        o.step(it, -1)
+    exact = []
+    for name in fields:
+        exact.append(compare(c.field(getattr(seiswg.abi, "F_" + name)), o.field(O.F[name]), name))
+    obs = o.obs()
+    comps = ([O.OBS_Y] if sh or spin else []) + ([O.OBS_X, O.OBS_Z] if psv else []) + \
+            ([O.OBS_YI, O.OBS_YC] if spin else [])
+    for comp in comps:
+        exact.append(compare(c.obs(0, comp)[:, :nsteps], obs[comp][:, :nsteps], f"obs{comp}"))
+    print(f"bit-identical: {sum(exact)}/{len(exact)}")
+
+
+@pytest.mark.parametrize("threads", [64, 256])
+def test_single_source_runs_and_batched_shots_agree(seiswg, threads):
+    nx, nz, nt = 60, 70, 40
+    p, stf = random_problem(nx, nz, nt, False, True, False, seed=3)
+    c = gpu_context(p, stf, seiswg)
+    c.set_tiling(threads, 16)
+    singles = []
+    for s in range(4):
+        c.forward(s)
+        singles.append((c.obs(0, O.OBS_X).copy(), c.field(seiswg.abi.F_VZ).copy()))
+    c.forward_batch([0, 1, 2, 3])
+    for s in range(4):
+        assert np.array_equal(c.obs(s, O.OBS_X), singles[s][0])
+        assert np.array_equal(c.field(seiswg.abi.F_VZ, shot=s), singles[s][1])
+    # per-shot oracle check for one shot
+    o = O.Oracle(p, 4, stf=stf.astype(np.float64))
+    obs, _ = o.forward(2, 0)
+    compare(c.obs(2, O.OBS_X), obs[O.OBS_X], "batched shot 2 ux")
+
+
+def load_example(tmp_path, ex, ini_name="config.ini"):
+    d = E.materialise(ex, str(tmp_path))
+    ini = E.ini_values(os.path.join(d, ini_name))
+    mdir = ini["paths"].get("model_init") or ini["paths"].get("model_true")
+    p = O.load_problem(os.path.join(d, mdir), os.path.join(d, ini["paths"]["sources"]),
+                       os.path.join(d, ini["paths"]["stations"]), ini)
+    return d, p, ini
+
+
+@pytest.mark.parametrize("ex,comp,fid,snap_file,trace_file",
+                         [("forward", O.OBS_Y, 0, "proc000100_vy.bin", "uy_000000.npy"),
+                          ("spin", O.OBS_Y, 7, "proc000200_vx.bin", "ry_000000.npy")])
+def test_examples_match_fixtures_and_oracle(seiswg, tmp_path, ex, comp, fid, snap_file, trace_file):
+    d, p, ini = load_example(tmp_path, ex)
+    c = seiswg.Context.from_problem(p)
+    stf = seiswg.ricker_table([[r[3], r[4], r[5], r[6]] for r in p.src_rows], p.nt, p.dt)
+    c.set_sources(p.src_idx, stf)
+    c.set_receivers(p.rec_idx)
+    snaps = {}
+    c.forward(-1, ini["save_snapshot"], snaps)
+    step = int(snap_file[4:10])
+    snap = snaps[(0, step, fid)]
+    tr = c.obs(0, comp)
+    ref_sn, ref_tr = E.fixture(ex, snap_file), E.fixture(ex, trace_file)
+    print(f"{ex}: trace rl2 {O.rel_l2(tr, ref_tr):.2e} max {O.max_rel_err(tr, ref_tr):.2e}; "
+          f"snap rl2 {O.rel_l2(snap, ref_sn):.2e}")
+    assert O.max_rel_err(tr, ref_tr) <= 1e-4 and O.rel_l2(tr, ref_tr) <= 1e-5
+    assert O.max_rel_err(snap, ref_sn) <= 1e-4 and O.rel_l2(snap, ref_sn) <= 1e-5
+    o = O.Oracle(p, 4)
+    oobs, osnaps = o.forward(-1, ini["save_snapshot"])
+    q = {0: 0, 7: 1}[fid]
+    compare(tr, oobs[comp], "trace vs oracle")
+    compare(snap, osnaps[step // ini["save_snapshot"] - 1][q], "snapshot vs oracle")
+
+
+def run_cli(ini):
+    import seiswg_amd
+    r = subprocess.run([seiswg_amd.CLI_PATH, ini], capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, RUST_LOG="error"))
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def test_cli_forward_and_spin_examples_reproduce_fixtures(seiswg, tmp_path):
+    for ex, files in [("forward", [("proc000100_vy.bin", "proc000100_vy.bin"),
+                                   ("traces/uy_000000.npy", "uy_000000.npy")]),
+                      ("spin", [("proc000200_vx.bin", "proc000200_vx.bin"), ("ry_000000.npy", "ry_000000.npy")])]:
+        d = E.materialise(ex, str(tmp_path))
+        run_cli(os.path.join(d, "config.ini"))
+        for got_name, fx in files:
+            gp = os.path.join(d, "output", got_name)
+            fp = os.path.join(E.GOLDEN, "fixtures", ex, fx)
+            if fx.endswith(".npy"):
+                gb, fb = open(gp, "rb").read(), open(fp, "rb").read()
+                hl = 10 + int.from_bytes(fb[8:10], "little")
+                assert gb[:hl] == fb[:hl], "NPY header differs from the reference's"
+                got, ref = E.read_npy_payload(gp), E.read_npy_payload(fp)
+            else:
+                got, ref = np.fromfile(gp, "<f4"), np.fromfile(fp, "<f4")
+            assert got.size == ref.size
+            assert O.max_rel_err(got, ref) <= 1e-4 and O.rel_l2(got, ref) <= 1e-5, (ex, got_name)
+    # the forward example writes 11 snapshots (it = 100..1100) and raw vy traces
+    d = os.path.join(str(tmp_path), "forward", "output")
+    snaps = sorted(f for f in os.listdir(d) if f.startswith("proc"))
+    assert snaps == [f"proc{it:06d}_vy.bin" for it in range(100, 1200, 100)]
+    raw = np.fromfile(os.path.join(d, "traces", "vy_000000.npy"), "<f4")
+    assert np.array_equal(raw, E.read_npy_payload(os.path.join(d, "traces", "uy_000000.npy")))
+
+
+def test_cli_adjoint_example_reproduces_kmu_fixture(seiswg, tmp_path):
+    """run_example.bash's adjoint flow: config_true forward writes the observed
+    traces, then the adjoint workflow; K_mu against the reference fixture."""
+    d = E.materialise("adjoint", str(tmp_path))
+    run_cli(os.path.join(d, "config_true.ini"))
+    sub = E.obs_traces_every16()
+    for s in (0, 6, 11):
+        got = np.load(os.path.join(d, "obs_traces", f"uy_{s:06d}.npy"))[:, ::16]
+        assert O.max_rel_err(got, sub[f"uy_{s:06d}"]) <= 1e-4 and O.rel_l2(got, sub[f"uy_{s:06d}"]) <= 1e-5
+    run_cli(os.path.join(d, "config.ini"))
+    got = np.fromfile(os.path.join(d, "output", "proc000000_kmu.bin"), "<f4")
+    ref = E.fixture("adjoint", "proc000000_kmu.bin")
+    print(f"kmu rl2 {O.rel_l2(got, ref):.2e} max {O.max_rel_err(got, ref):.2e}")
+    assert O.max_rel_err(got, ref) <= 1e-4 and O.rel_l2(got, ref) <= 1e-5
+    hdr = np.fromfile(os.path.join(d, "output", "gradient_kmu.bin"), "<i4", count=1)[0]
+    assert hdr == got.size
+
+
+def test_adjoint_plain_matches_oracle(seiswg):
+    nx, nz, nt = 50, 46, 300
+    p, _ = random_problem(nx, nz, nt, True, False, False, seed=8, nsrc=3, nrec=6)
+    p.src_rows = [[0, 0, 0, 3.0, 0.3, 0.0, 1.0], [0, 0, 0, 4.0, 0.25, 0.0, 1.0], [0, 0, 0, 5.0, 0.2, 0.0, 1.0]]
+    stf = O.ricker_table(4, p.src_rows, nt, p.dt).astype(np.float32)
+    rng = np.random.default_rng(2)
+    ref = rng.standard_normal((3, len(p.rec_idx), nt)).astype(np.float32) * 1e-3
+    c = gpu_context(p, stf, seiswg)
+    kmu, mis = c.adjoint_plain([0, 1, 2], ref, 10, 2.5)
+    o = O.Oracle(p, 4, stf=stf.astype(np.float64))
+    kmu_o, mis_o = o.adjoint_plain([0, 1, 2], ref, 10, 2.5)
+    compare(kmu, kmu_o, "kmu", tol=1e-6)
+    np.testing.assert_allclose(mis, mis_o, rtol=1e-9)
+
+
+def test_smoothing_matches_oracle(seiswg):
+    nx, nz = 37, 53
+    p, stf = random_problem(nx, nz, 4, True, False, False, seed=1)
+    c = gpu_context(p, stf, seiswg)
+    rng = np.random.default_rng(4)
+    k = rng.standard_normal(nx * nz).astype(np.float32)
+    c.adjoint_plain([0], np.zeros((1, len(p.rec_idx), 4), np.float32), 1, 3.0)   # allocates K_mu
+    c.write_field(seiswg.abi.F_K_MU, k)
+    c.smooth(seiswg.abi.F_K_MU, 3.0)
+    o = O.Oracle(p, 4, stf=stf.astype(np.float64))
+    o.set_field(O.F["K_MU"], k)
+    o.smooth(3.0)
+    compare(c.field(seiswg.abi.F_K_MU), o.field(O.F["K_MU"]), "smoothed", tol=1e-6)

@@ -134,6 +134,19 @@ SW_API int sw_advance(sw_ctx* ctx, int32_t nsteps);
 SW_API int sw_adjoint_plain(sw_ctx* ctx, int32_t nruns, const int32_t* isrc, const float* obs_ref,
                             int32_t adj_interval, float smooth, double* misfit);
 
+/* Differentiable gradient (no counterpart in the reference, whose only
+ * gradient is the SH imaging kernel of run_adjoint): exact discrete adjoint of
+ * the SH or P-SV scheme (spin unsupported) for the objective
+ *   J = sum_shots sum_comp sum_rec sum_n (w_n (d^n - obs^n))^2,
+ * w_n the misfit taper of solver.rs:441-455, d the recorded traces (SH: uy;
+ * P-SV: ux, uz).  obs_ref is [nshots][ncomp][nrec][nt].  The forward is
+ * checkpointed every ckpt_every steps and recomputed segment by segment.
+ * shots_per_launch shots advance together (0 = all).  The gradients w.r.t.
+ * the stored rho, mu, lambda are summed over shots into SW_F_K_RHO,
+ * SW_F_K_MU, SW_F_K_LAM (read with sw_read_field); misfit[s] = J of shot s. */
+SW_API int sw_gradient(sw_ctx* ctx, int32_t nshots, const int32_t* isrc, const float* obs_ref, int32_t ckpt_every,
+                       int32_t shots_per_launch, double* misfit);
+
 /* Gaussian smoothing of a kernel field in place (smooth_gradient,
  * solver.rs:1249-1274; init_gsum, gaussian_x, gaussian_z). */
 SW_API int sw_smooth(sw_ctx* ctx, int32_t field, float sigma);

@@ -457,6 +457,229 @@ int adjoint_plain(Oracle<T>& o, int32_t nruns, const int32_t* isrc_list, const T
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Exact discrete adjoint of the forward scheme (no counterpart in the
+// reference, whose only gradient is the SH cross-correlation imaging kernel of
+// run_adjoint).  Objective, per run r and traced component c:
+//   J = sum_{r,c,rec,n} ( w_n * (d_{r,c,rec}^n - obs_{r,c,rec}^n) )^2
+// with w_n the cosine time taper of compute_waveform_misfit (solver.rs:441-455)
+// and d^n = u^{n+1} at the receiver (u = sum v*dt, recorded after step n).
+// SH traces uy; P-SV traces ux and uz.  Gradients are with respect to the
+// stored rho, lambda, mu (after vps2lm).  All forward states are kept (tiny
+// grids only); the sweep runs the transposed step operators in reverse order.
+template <typename T>
+struct Adjoint {
+  Oracle<T>& o;
+  explicit Adjoint(Oracle<T>& x) : o(x) {}
+  uint32_t nx() const { return o.nx; }
+  uint32_t nz() const { return o.nz; }
+
+  // transposes of the D operators (scatter form over the active output cells)
+  void dxm_t(const std::vector<T>& w, std::vector<T>& out) const {   // (Dx-)^T
+    const uint32_t nz = o.nz;
+    for (uint32_t i = 2; i + 2 < o.nx; ++i)
+      for (uint32_t j = 0; j < nz; ++j) {
+        const uint32_t k = i * nz + j;
+        const T c = w[k] / (T(8) * o.dx);
+        out[k] += T(9) * c; out[k - nz] -= T(9) * c; out[k + nz] -= c; out[k - 2 * nz] += c;
+      }
+  }
+  void dzm_t(const std::vector<T>& w, std::vector<T>& out) const {   // (Dz-)^T
+    const uint32_t nz = o.nz;
+    for (uint32_t i = 0; i < o.nx; ++i)
+      for (uint32_t j = 2; j + 2 < nz; ++j) {
+        const uint32_t k = i * nz + j;
+        const T c = w[k] / (T(8) * o.dz);
+        out[k] += T(9) * c; out[k - 1] -= T(9) * c; out[k + 1] -= c; out[k - 2] += c;
+      }
+  }
+  void dxp_t(const std::vector<T>& w, std::vector<T>& out) const {   // (Dx+)^T
+    const uint32_t nz = o.nz;
+    for (uint32_t i = 1; i + 2 < o.nx; ++i)
+      for (uint32_t j = 0; j < nz; ++j) {
+        const uint32_t k = i * nz + j;
+        const T c = w[k] / (T(8) * o.dx);
+        out[k + nz] += T(9) * c; out[k] -= T(9) * c; out[k + 2 * nz] -= c; out[k - nz] += c;
+      }
+  }
+  void dzp_t(const std::vector<T>& w, std::vector<T>& out) const {   // (Dz+)^T
+    const uint32_t nz = o.nz;
+    for (uint32_t i = 0; i < o.nx; ++i)
+      for (uint32_t j = 1; j + 2 < nz; ++j) {
+        const uint32_t k = i * nz + j;
+        const T c = w[k] / (T(8) * o.dz);
+        out[k + 1] += T(9) * c; out[k] -= T(9) * c; out[k + 2] -= c; out[k - 1] += c;
+      }
+  }
+  // forward operators on a standalone field (same edge rules as fd2d.wgsl)
+  T dxm(const std::vector<T>& f, uint32_t k) const {
+    uint32_t i = k / o.nz; const uint32_t nz = o.nz;
+    if (i < 2u || i >= o.nx - 2u) return T(0);
+    return (T(9) * (f[k] - f[k - nz]) - (f[k + nz] - f[k - 2 * nz])) / (T(8) * o.dx);
+  }
+  T dzm(const std::vector<T>& f, uint32_t k) const {
+    uint32_t j = k % o.nz;
+    if (j < 2u || j >= o.nz - 2u) return T(0);
+    return (T(9) * (f[k] - f[k - 1]) - (f[k + 1] - f[k - 2])) / (T(8) * o.dz);
+  }
+  T dxp(const std::vector<T>& f, uint32_t k) const {
+    uint32_t i = k / o.nz; const uint32_t nz = o.nz;
+    if (i < 1u || i >= o.nx - 2u) return T(0);
+    return (T(9) * (f[k + nz] - f[k]) - (f[k + 2 * nz] - f[k - nz])) / (T(8) * o.dx);
+  }
+  T dzp(const std::vector<T>& f, uint32_t k) const {
+    uint32_t j = k % o.nz;
+    if (j < 1u || j >= o.nz - 2u) return T(0);
+    return (T(9) * (f[k + 1] - f[k]) - (f[k + 2] - f[k - 1])) / (T(8) * o.dz);
+  }
+
+  std::vector<T> taper() const {
+    const T PI_T = T(3.14159265358979323846);
+    std::vector<T> w(o.nt);
+    T t_end = T(o.nt - 1) * o.dt, tw = t_end / T(10), t_min = tw, t_max = t_end - tw;
+    for (uint32_t n = 0; n < o.nt; ++n) {
+      T t = T(n) * o.dt;
+      if (t <= t_min) w[n] = T(0.5) + T(0.5) * std::cos(PI_T * (t_min - t) / tw);
+      else if (t >= t_max) w[n] = T(0.5) + T(0.5) * std::cos(PI_T * (t_max - t) / tw);
+      else w[n] = T(1);
+    }
+    return w;
+  }
+
+  // state field ids per physics
+  std::vector<uint32_t> state_ids() const {
+    if (o.sh) return {F_VY, F_SXY, F_SZY};
+    return {F_VX, F_VZ, F_SXX, F_SZZ, F_SXZ};
+  }
+  std::vector<uint32_t> comps() const {
+    if (o.sh) return {OBS_Y};
+    return {OBS_X, OBS_Z};
+  }
+
+  // J for run isrc against obs [ncomp][nrec][nt]; optionally keeps all states
+  double forward(int32_t isrc, const T* obs, std::vector<std::vector<T>>* states) {
+    o.clear_dyn(); o.clear_obs();
+    const auto ids = state_ids();
+    const size_t npt = o.npt;
+    if (states) {
+      states->assign(size_t(o.nt) + 1, std::vector<T>(ids.size() * npt, T(0)));
+    }
+    for (uint32_t n = 0; n < o.nt; ++n) {
+      o.step(n, isrc);
+      if (states)
+        for (size_t q = 0; q < ids.size(); ++q)
+          std::copy(o.fields.begin() + size_t(ids[q]) * npt, o.fields.begin() + size_t(ids[q] + 1) * npt,
+                    (*states)[n + 1].begin() + q * npt);
+    }
+    const auto w = taper();
+    const auto cs = comps();
+    double J = 0.0;
+    for (size_t c = 0; c < cs.size(); ++c)
+      for (uint32_t r = 0; r < o.nrec; ++r)
+        for (uint32_t n = 0; n < o.nt; ++n) {
+          const T d = o.obs[(size_t(cs[c]) * o.nrec + r) * o.nt + n] - obs[(c * o.nrec + r) * o.nt + n];
+          const T e = w[n] * d;
+          J += double(e) * double(e);
+        }
+    return J;
+  }
+
+  // adds this run's gradient into kr, km, kl; returns J
+  double gradient(int32_t isrc, const T* obs, std::vector<T>& kr, std::vector<T>& km, std::vector<T>& kl) {
+    std::vector<std::vector<T>> st;
+    const double J = forward(isrc, obs, &st);
+    const size_t npt = o.npt;
+    const uint32_t nt = o.nt, nrec = o.nrec, nsrc = o.nsrc;
+    const auto w = taper();
+    const auto cs = comps();
+    // reverse-accumulated adjoint sources A^n = sum_{k>=n} 2 w_k^2 (d^k - obs^k)
+    std::vector<T> A(cs.size() * nrec * nt, T(0));
+    for (size_t c = 0; c < cs.size(); ++c)
+      for (uint32_t r = 0; r < nrec; ++r) {
+        T acc = T(0);
+        for (int64_t n = int64_t(nt) - 1; n >= 0; --n) {
+          const size_t q = (c * nrec + r) * nt + size_t(n);
+          const T d = o.obs[(size_t(cs[c]) * nrec + r) * nt + size_t(n)] - obs[q];
+          acc += T(2) * w[n] * w[n] * d;
+          A[q] = acc;
+        }
+      }
+    const T dt = o.dt;
+    auto P = [&](uint32_t f, uint32_t k) { return o.fields[size_t(f) * npt + k]; };
+    if (o.sh) {
+      std::vector<T> vb(npt, 0), sbx(npt, 0), sbz(npt, 0), tx(npt), tz(npt), q(npt), ex(npt), ez(npt);
+      for (int64_t n = int64_t(nt) - 1; n >= 0; --n) {
+        const T* s1 = st[size_t(n) + 1].data();   // after step n: vy, sxy, szy
+        const T* s0 = st[size_t(n)].data();       // before step n
+        std::vector<T> vy1(s1, s1 + npt), sxy0(s0 + npt, s0 + 2 * npt), szy0(s0 + 2 * npt, s0 + 3 * npt);
+        for (uint32_t r = 0; r < nrec; ++r) vb[o.rec_id(r)] += dt * A[size_t(r) * nt + size_t(n)];
+        for (size_t k = 0; k < npt; ++k) {
+          const T mu = P(F_MU, uint32_t(k));
+          tx[k] = dt * mu * sbx[k];
+          tz[k] = dt * mu * sbz[k];
+          km[k] += dt * (sbx[k] * dxp(vy1, uint32_t(k)) + sbz[k] * dzp(vy1, uint32_t(k)));
+        }
+        dxp_t(tx, vb); dzp_t(tz, vb);
+        for (size_t k = 0; k < npt; ++k) {
+          const T b = P(F_BOUND, uint32_t(k)), rho = P(F_RHO, uint32_t(k));
+          const T g = b * vb[k];
+          T ds = dxm(sxy0, uint32_t(k)) + dzm(szy0, uint32_t(k));
+          for (uint32_t s = 0; s < nsrc; ++s)
+            if ((isrc < 0 || int32_t(s) == isrc) && o.src_id(s) == uint32_t(k)) ds += o.stf[(size_t(STF_Y) * nsrc + s) * nt + size_t(n)];
+          kr[k] += -dt * g * ds / (rho * rho);
+          q[k] = dt * g / rho;
+          vb[k] = g;
+        }
+        dxm_t(q, sbx); dzm_t(q, sbz);
+      }
+    } else {
+      std::vector<T> vbx(npt, 0), vbz(npt, 0), sbxx(npt, 0), sbzz(npt, 0), sbxz(npt, 0);
+      std::vector<T> txx(npt), tzz(npt), txz(npt), qx(npt), qz(npt);
+      for (int64_t n = int64_t(nt) - 1; n >= 0; --n) {
+        const T* s1 = st[size_t(n) + 1].data();   // vx, vz, sxx, szz, sxz after step n
+        const T* s0 = st[size_t(n)].data();
+        std::vector<T> vx1(s1, s1 + npt), vz1(s1 + npt, s1 + 2 * npt);
+        std::vector<T> sxx0(s0 + 2 * npt, s0 + 3 * npt), szz0(s0 + 3 * npt, s0 + 4 * npt), sxz0(s0 + 4 * npt, s0 + 5 * npt);
+        for (uint32_t r = 0; r < nrec; ++r) {
+          vbx[o.rec_id(r)] += dt * A[(0 * size_t(nrec) + r) * nt + size_t(n)];
+          vbz[o.rec_id(r)] += dt * A[(1 * size_t(nrec) + r) * nt + size_t(n)];
+        }
+        for (size_t k = 0; k < npt; ++k) {
+          const uint32_t kk = uint32_t(k);
+          const T lam = P(F_LAM, kk), mu = P(F_MU, kk);
+          txx[k] = dt * ((lam + T(2) * mu) * sbxx[k] + lam * sbzz[k]);
+          tzz[k] = dt * (lam * sbxx[k] + (lam + T(2) * mu) * sbzz[k]);
+          txz[k] = dt * mu * sbxz[k];
+          const T exx = dxp(vx1, kk), ezz = dzp(vz1, kk), exz = dzp(vx1, kk) + dxp(vz1, kk);
+          kl[k] += dt * (sbxx[k] + sbzz[k]) * (exx + ezz);
+          km[k] += dt * (T(2) * sbxx[k] * exx + T(2) * sbzz[k] * ezz + sbxz[k] * exz);
+        }
+        dxp_t(txx, vbx); dzp_t(txz, vbx);
+        dzp_t(tzz, vbz); dxp_t(txz, vbz);
+        for (size_t k = 0; k < npt; ++k) {
+          const uint32_t kk = uint32_t(k);
+          const T b = P(F_BOUND, kk), rho = P(F_RHO, kk);
+          const T gx = b * vbx[k], gz = b * vbz[k];
+          T dsx = dxm(sxx0, kk) + dzm(sxz0, kk), dsz = dxm(sxz0, kk) + dzm(szz0, kk);
+          for (uint32_t s = 0; s < nsrc; ++s)
+            if ((isrc < 0 || int32_t(s) == isrc) && o.src_id(s) == kk) {
+              dsx += o.stf[(size_t(STF_X) * nsrc + s) * nt + size_t(n)];
+              dsz += o.stf[(size_t(STF_Z) * nsrc + s) * nt + size_t(n)];
+            }
+          kr[k] += -dt * (gx * dsx + gz * dsz) / (rho * rho);
+          qx[k] = dt * gx / rho;
+          qz[k] = dt * gz / rho;
+          vbx[k] = gx;
+          vbz[k] = gz;
+        }
+        dxm_t(qx, sbxx); dzm_t(qx, sbxz);
+        dxm_t(qz, sbxz); dzm_t(qz, sbzz);
+      }
+    }
+    return J;
+  }
+};
+
 }  // namespace
 
 // ------------------------------------------------------------------------
@@ -549,6 +772,41 @@ int32_t or_adjoint_plain(or_handle* h, int32_t nruns, const int32_t* isrc_list, 
     rc = adjoint_plain<T>(o, nruns, isrc_list, r.data(), uint32_t(adj_interval), T(smooth), misfits);
   });
   return rc;
+}
+
+// Misfit J (sum over runs) of the forward scheme against obs
+// [nruns][ncomp][nrec][nt] (ncomp = 1 for SH: uy; 2 for P-SV: ux, uz).
+double or_objective(or_handle* h, int32_t nruns, const int32_t* isrc, const double* obs) {
+  double J = 0.0;
+  DISPATCH(h, {
+    using T = decltype(o.dt);
+    Adjoint<T> ad(o);
+    const size_t per = ad.comps().size() * size_t(o.nrec) * o.nt;
+    std::vector<T> ob(per);
+    for (int32_t r = 0; r < nruns; ++r) {
+      for (size_t q = 0; q < per; ++q) ob[q] = T(obs[size_t(r) * per + q]);
+      J += ad.forward(isrc[r], ob.data(), nullptr);
+    }
+  });
+  return J;
+}
+
+// Exact adjoint gradient of or_objective with respect to rho, mu, lambda.
+double or_gradient(or_handle* h, int32_t nruns, const int32_t* isrc, const double* obs, double* k_rho, double* k_mu,
+                   double* k_lam) {
+  double J = 0.0;
+  DISPATCH(h, {
+    using T = decltype(o.dt);
+    Adjoint<T> ad(o);
+    const size_t per = ad.comps().size() * size_t(o.nrec) * o.nt;
+    std::vector<T> ob(per), kr(o.npt, T(0)), km(o.npt, T(0)), kl(o.npt, T(0));
+    for (int32_t r = 0; r < nruns; ++r) {
+      for (size_t q = 0; q < per; ++q) ob[q] = T(obs[size_t(r) * per + q]);
+      J += ad.gradient(isrc[r], ob.data(), kr, km, kl);
+    }
+    for (uint32_t k = 0; k < o.npt; ++k) { k_rho[k] = double(kr[k]); k_mu[k] = double(km[k]); k_lam[k] = double(kl[k]); }
+  });
+  return J;
 }
 
 // Ricker table (solver.rs:350-356, 721-739) in precision prec.

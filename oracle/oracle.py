@@ -76,6 +76,10 @@ def lib():
         L.or_ricker_table.argtypes = [C.c_int, C.c_int32, C.c_int32, C.c_float, fp, dp]
         L.or_infer_grid.argtypes = [C.c_int32, fp, fp, ip, ip, fp, fp]
         L.or_num_threads.restype = C.c_int32
+        L.or_objective.restype = C.c_double
+        L.or_objective.argtypes = [C.c_void_p, C.c_int32, ip, dp]
+        L.or_gradient.restype = C.c_double
+        L.or_gradient.argtypes = [C.c_void_p, C.c_int32, ip, dp, dp, dp, dp]
         L.or_misfit.restype = C.c_double
         L.or_misfit.argtypes = [C.c_int, dp, dp, C.c_int32, C.c_int32, C.c_float, dp]
         _lib = L
@@ -296,6 +300,22 @@ class Oracle:
         if rc != 0:
             raise ValueError(f"adjoint failed rc={rc}")
         return self.field(F["K_MU"]), mis
+
+    def objective(self, isrc_list, obs) -> float:
+        """J = sum (taper * (traces - obs))^2 over runs, components and time;
+        obs: [nruns][ncomp][nrec][nt], ncomp 1 (SH: uy) or 2 (P-SV: ux, uz)."""
+        isrc = np.ascontiguousarray(isrc_list, np.int32)
+        obs, po = _d(obs)
+        return self.L.or_objective(self.h, len(isrc), isrc.ctypes.data_as(C.POINTER(C.c_int32)), po)
+
+    def gradient(self, isrc_list, obs):
+        """(J, K_rho, K_mu, K_lambda): exact adjoint gradient of objective()."""
+        isrc = np.ascontiguousarray(isrc_list, np.int32)
+        obs, po = _d(obs)
+        ks = [np.zeros(self.p.npt) for _ in range(3)]
+        J = self.L.or_gradient(self.h, len(isrc), isrc.ctypes.data_as(C.POINTER(C.c_int32)), po,
+                               *[k.ctypes.data_as(C.POINTER(C.c_double)) for k in ks])
+        return J, ks[0], ks[1], ks[2]
 
     def smooth(self, sigma: float):
         self.L.or_smooth(self.h, sigma)

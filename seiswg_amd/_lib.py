@@ -48,7 +48,7 @@ SNAP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.PO
 EXPORTS = ["sw_last_error", "sw_create", "sw_destroy", "sw_set_model", "sw_set_sources", "sw_set_receivers",
            "sw_forward", "sw_forward_batch", "sw_read_obs", "sw_read_field", "sw_write_field", "sw_prepare",
            "sw_advance", "sw_adjoint_plain", "sw_smooth", "sw_set_tiling", "sw_get_tiling", "sw_get_timing",
-           "sw_bench_copy",
+           "sw_bench_copy", "sw_gradient",
            "seiswg_run", "seiswg_ricker_table", "seiswg_infer_grid", "seiswg_grid_index"]
 
 _lib = None
@@ -81,6 +81,7 @@ def load():
     L.sw_set_tiling.argtypes = [vp, C.c_int32, C.c_int32]
     L.sw_get_tiling.argtypes = [vp, ip, ip]
     L.sw_get_timing.argtypes = [vp, C.POINTER(sw_timing)]
+    L.sw_gradient.argtypes = [vp, C.c_int32, ip, fp, C.c_int32, C.c_int32, dp]
     L.sw_bench_copy.argtypes = [C.c_int32, C.c_int64, C.c_int32, dp]
     L.seiswg_run.argtypes = [C.c_char_p]
     L.seiswg_ricker_table.argtypes = [C.c_int32, C.c_int32, C.c_float, fp, fp]
@@ -259,6 +260,15 @@ class Context:
         _check(self.L.sw_adjoint_plain(self.h, len(isrc), _iptr(isrc), _fptr(ref), adj_interval, C.c_float(smooth),
                                        mis.ctypes.data_as(C.POINTER(C.c_double))))
         return self.field(F_K_MU), mis[:len(isrc)]
+
+    def gradient(self, isrc_list, obs_ref, ckpt_every: int = 50, shots_per_launch: int = 0):
+        """Adjoint gradient; returns (misfit per shot, K_rho, K_mu, K_lambda)."""
+        isrc = np.ascontiguousarray(isrc_list, np.int32)
+        ref = _f32(obs_ref)
+        mis = np.zeros(len(isrc), np.float64)
+        _check(self.L.sw_gradient(self.h, len(isrc), _iptr(isrc), _fptr(ref), ckpt_every, shots_per_launch,
+                                  mis.ctypes.data_as(C.POINTER(C.c_double))))
+        return mis, self.field(F_K_RHO), self.field(F_K_MU), self.field(F_K_LAM)
 
     def smooth(self, fid: int, sigma: float):
         _check(self.L.sw_smooth(self.h, fid, C.c_float(sigma)))

@@ -74,8 +74,9 @@ struct sw_ctx {
   float h8x = 0, h8z = 0;
   bool model_set = false;
 
-  // model
-  DBuf<float> rho, lam, mu, nu, jj, lam_c, mu_c, nu_c;
+  // model fields [MF_N][npt]: rho, lambda, mu, nu, j, lambda_c, mu_c, nu_c
+  DBuf<float> model;
+  float* mf(int f) { return model.p ? model.p + size_t(f) * size_t(npt) : nullptr; }
   DBuf<float> bx, bzb, bzt;
   std::vector<float> h_bx, h_bzb, h_bzt;
 
@@ -92,7 +93,7 @@ struct sw_ctx {
   std::vector<int32_t> shot_isrc;
   int slot_index[N_SLOTS];
   int nslots = 0;
-  DBuf<float> state[2];
+  DBuf<float> state[2];       // [shot][slot][npt]
   int cur = 0;                // buffer holding the current state
   DBuf<float> uacc, obs;
   DBuf<int> clock;
@@ -106,10 +107,23 @@ struct sw_ctx {
   int nt_val = 0;
   bool record = true;
   bool sh_only = false;       // the adjoint workflow dispatches the SH kernels only
+  int aux_base = 0;           // step index of aux slot 0 (gradient recompute)
 
   // plain adjoint
   DBuf<float> kmu, k_lam, k_rho, gsum, gtmp, wtab, sumx, sumz, fwd_store, adj_src, ref, taper;
   DBuf<double> partial, misfit_d;
+
+  // differentiable gradient: adjoint state ([shot][slot][npt] ping-pong),
+  // adjoint clock, receiver-injection lists, checkpoints, per-step aux
+  // fields of the recomputed segment and per-shot kernels
+  DBuf<float> adj[2];
+  int acur = 0;
+  DBuf<int> aclock;
+  DBuf<int> ainj_off;
+  DBuf<int4> ainj;
+  DBuf<float> ckpt, aux, kshot, adj_val, gref, gtaper;
+  DBuf<double> gmis;
+  DBuf<int> gcomps;
 
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -175,6 +189,32 @@ void tile_bounds(const sw_ctx* c, int tile, int& i0, int& i1, int& c0) {
   i0 = seg * t.seglen;
   i1 = std::min(i0 + t.seglen, c->g.nx);
   c0 = strip * (t.nt_threads - 6) - 3;
+}
+
+int upload_inj(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj, DBuf<int>& off_d, DBuf<int4>& data_d) {
+  const int ntile = c->tile.nstrips * c->tile.nsegs;
+  const int NT = c->tile.nt_threads;
+  std::vector<int> off(size_t(c->nshots) * ntile + 1, 0);
+  std::vector<int4> data;
+  for (int s = 0; s < c->nshots; ++s) {
+    std::vector<Entry> pts = shot_inj[s];
+    std::stable_sort(pts.begin(), pts.end(), [](const Entry& a, const Entry& b) { return a.i < b.i; });
+    for (int t = 0; t < ntile; ++t) {
+      int i0, i1, c0;
+      tile_bounds(c, t, i0, i1, c0);
+      off[size_t(s) * ntile + t] = int(data.size());
+      for (const Entry& e : pts)
+        if (e.i >= i0 - 1 && e.i < i1 + 2 && e.j >= c0 && e.j < c0 + NT) data.push_back(make_int4(e.i, e.j, e.q, 0));
+    }
+  }
+  off.back() = int(data.size());
+  HIPCHK(off_d.ensure(off.size()));
+  HIPCHK(hipMemcpyAsync(off_d.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(data_d.ensure(std::max<size_t>(data.size(), 1)));
+  if (!data.empty())
+    HIPCHK(hipMemcpyAsync(data_d.p, data.data(), data.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 int build_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj) {
@@ -268,37 +308,58 @@ int prepare_batch(sw_ctx* c, int nshots, const std::vector<std::vector<Entry>>& 
   return 0;
 }
 
-int launch_steps(sw_ctx* c, int nsteps) {
+enum StepMode { M_FWD = 0, M_FWD_AUX, M_ADJ };
+
+int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   StepArgs a{};
-  a.shot_stride = c->npt;
-  a.rho = c->rho.p; a.lam = c->lam.p; a.mu = c->mu.p;
-  a.nu = c->nu.p; a.jj = c->jj.p; a.muc = c->mu_c.p; a.nuc = c->nu_c.p;
+  for (int q = 0; q < N_SLOTS; ++q) a.slot_off[q] = c->slot_index[q] < 0 ? -1 : int(c->slot_index[q] * c->npt);
+  a.shot_floats = int(c->nslots * c->npt);
+  a.model = c->model.p;
+  a.npt = int(c->npt);
   a.bx = c->bx.p; a.bzb = c->bzb.p; a.bzt = c->bzt.p;
   a.nx = c->g.nx; a.nz = c->g.nz; a.nshots = c->nshots;
   a.nstrips = c->tile.nstrips; a.nsegs = c->tile.nsegs; a.seglen = c->tile.seglen;
   a.dt = c->g.dt; a.h8x = c->h8x; a.h8z = c->h8z;
+  a.r8x = 1.0f / c->h8x; a.r8z = 1.0f / c->h8z;
   a.inj_off = c->inj_off.p; a.inj = c->inj.p;
   a.val_x = c->val_x; a.val_y = c->val_y; a.val_z = c->val_z; a.nt_val = c->nt_val;
   a.rec_off = c->rec_off.p; a.rec = c->rec.p;
   a.uacc = c->uacc.p; a.obs = c->obs.p; a.nrec = c->nrec; a.nt = c->g.nt;
-  a.record = c->record ? 1 : 0;
+  a.record = (c->record && mode == M_FWD) ? 1 : 0;
   a.clock = c->clock.p;
-  const size_t slot_elems = size_t(c->nshots) * size_t(c->npt);
+  a.aux = c->aux.p;
+  a.aux_base = c->aux_base;
+  a.kgrad = c->kshot.p;
   const bool sh = uses_sh(c), psv = uses_psv(c) && !c->sh_only, spin = uses_spin(c);
-  for (int s = 0; s < nsteps; ++s) {
-    for (int q = 0; q < N_SLOTS; ++q) {
-      const int si = c->slot_index[q];
-      a.in[q] = si < 0 ? nullptr : c->state[c->cur].p + size_t(si) * slot_elems;
-      a.out[q] = si < 0 ? nullptr : c->state[c->cur ^ 1].p + size_t(si) * slot_elems;
+  if (mode == M_ADJ) {
+    a.inj_off = c->ainj_off.p; a.inj = c->ainj.p;
+    a.val_x = c->adj_val.p;
+    a.val_y = c->adj_val.p;
+    a.val_z = c->adj_val.p + size_t(c->nshots) * c->nrec * c->g.nt;
+    a.nt_val = c->g.nt;
+    a.clock = c->aclock.p;
+    a.advance = 1;
+    for (int s = 0; s < nsteps; ++s) {
+      a.state_in = c->adj[c->acur].p;
+      a.state_out = c->adj[c->acur ^ 1].p;
+      HIPCHK(sh ? launch_adj_sh(a, c->tile, c->stream) : launch_adj_psv(a, c->tile, c->stream));
+      ++c->launches;
+      c->acur ^= 1;
     }
+    return 0;
+  }
+  const bool aux = mode == M_FWD_AUX;
+  for (int s = 0; s < nsteps; ++s) {
+    a.state_in = c->state[c->cur].p;
+    a.state_out = c->state[c->cur ^ 1].p;
     if (sh) {
       a.advance = psv ? 0 : 1;
-      HIPCHK(launch_step_sh(a, c->tile, c->stream));
+      HIPCHK(launch_step_sh(a, c->tile, c->stream, aux));
       ++c->launches;
     }
     if (psv) {
       a.advance = 1;
-      HIPCHK(launch_step_psv(a, c->tile, spin, c->stream));
+      HIPCHK(launch_step_psv(a, c->tile, spin, c->stream, aux));
       ++c->launches;
     }
     if (!sh && !psv) {
@@ -316,7 +377,7 @@ int launch_steps(sw_ctx* c, int nsteps) {
 float* slot_ptr(sw_ctx* c, int slot, int shot) {
   const int si = c->slot_index[slot];
   if (si < 0) return nullptr;
-  return c->state[c->cur].p + (size_t(si) * c->nshots + shot) * size_t(c->npt);
+  return c->state[c->cur].p + (size_t(shot) * c->nslots + si) * size_t(c->npt);
 }
 
 int field_slot(int field) {
@@ -339,14 +400,14 @@ int field_slot(int field) {
 
 float* model_ptr(sw_ctx* c, int field) {
   switch (field) {
-    case SW_F_LAM: return c->lam.p;
-    case SW_F_MU: return c->mu.p;
-    case SW_F_NU: return c->nu.p;
-    case SW_F_J: return c->jj.p;
-    case SW_F_LAM_C: return c->lam_c.p;
-    case SW_F_MU_C: return c->mu_c.p;
-    case SW_F_NU_C: return c->nu_c.p;
-    case SW_F_RHO: return c->rho.p;
+    case SW_F_LAM: return c->mf(MF_LAM);
+    case SW_F_MU: return c->mf(MF_MU);
+    case SW_F_NU: return c->mf(MF_NU);
+    case SW_F_J: return c->mf(MF_J);
+    case SW_F_LAM_C: return c->mf(MF_LAM_C);
+    case SW_F_MU_C: return c->mf(MF_MU_C);
+    case SW_F_NU_C: return c->mf(MF_NU_C);
+    case SW_F_RHO: return c->mf(MF_RHO);
     case SW_F_K_MU: return c->kmu.p;
     case SW_F_K_LAM: return c->k_lam.p;
     case SW_F_K_RHO: return c->k_rho.p;
@@ -406,7 +467,10 @@ SW_API int sw_create(const sw_grid* g, int32_t device, sw_ctx** out) {
   if (!g || !out) return fail(-1, "null argument");
   if (g->nx < 4 || g->nz < 4) return fail(-2, "grid must be at least 4x4 (got %dx%d)", g->nx, g->nz);
   if (g->nt < 0) return fail(-2, "nt must be >= 0");
-  if (int64_t(g->nx) * g->nz > (int64_t(1) << 31) - 1) return fail(-2, "grid too large");
+  // byte offsets inside one shot's state ([slot][npt]) and the model buffer are 32-bit
+  if (int64_t(g->nx) * g->nz * int64_t(N_SLOTS) * 4 >= (int64_t(1) << 32))
+    return fail(-2, "grid too large (%d x %d): at most %lld cells", g->nx, g->nz,
+                (long long)(((int64_t(1) << 32) - 1) / (4 * N_SLOTS)));
   int ndev = 0;
   HIPCHK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(-2, "no HIP device %d (found %d)", device, ndev);
@@ -443,24 +507,22 @@ SW_API int sw_set_model(sw_ctx* c, int32_t kind, const float* const* f) {
   if (!c || !f) return fail(-1, "null argument");
   HIPCHK(hipSetDevice(c->device));
   const size_t n = size_t(c->npt), bytes = n * 4;
-  auto up = [&](DBuf<float>& d, const float* h) -> int {
-    HIPCHK(d.ensure(n));
-    if (h) HIPCHK(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c->stream));
-    else HIPCHK(launch_fill(d.p, 0.0f, int64_t(n), c->stream));
+  HIPCHK(c->model.ensure(size_t(MF_N) * n));
+  HIPCHK(launch_fill(c->model.p, 0.0f, int64_t(MF_N) * int64_t(n), c->stream));
+  auto up = [&](int mf, const float* h) -> int {
+    HIPCHK(hipMemcpyAsync(c->mf(mf), h, bytes, hipMemcpyHostToDevice, c->stream));
     return 0;
   };
   int rc = 0;
   if (kind == SW_MODEL_VPVS) {
     if (!f[0] || !f[1] || !f[2]) return fail(-1, "rho, vp and vs are required");
-    if ((rc = up(c->rho, f[0])) || (rc = up(c->lam, f[1])) || (rc = up(c->mu, f[2]))) return rc;
-    for (DBuf<float>* d : {&c->nu, &c->jj, &c->lam_c, &c->mu_c, &c->nu_c})
-      if ((rc = up(*d, nullptr))) return rc;
-    HIPCHK(launch_vps2lm(c->rho.p, c->lam.p, c->mu.p, c->npt, c->stream));
+    if ((rc = up(MF_RHO, f[0])) || (rc = up(MF_LAM, f[1])) || (rc = up(MF_MU, f[2]))) return rc;
+    HIPCHK(launch_vps2lm(c->mf(MF_RHO), c->mf(MF_LAM), c->mf(MF_MU), c->npt, c->stream));
   } else if (kind == SW_MODEL_LAME) {
-    DBuf<float>* d[8] = {&c->rho, &c->lam, &c->mu, &c->nu, &c->jj, &c->lam_c, &c->mu_c, &c->nu_c};
+    const int order[8] = {MF_RHO, MF_LAM, MF_MU, MF_NU, MF_J, MF_LAM_C, MF_MU_C, MF_NU_C};
     for (int q = 0; q < 8; ++q) {
       if (!f[q]) return fail(-1, "model field %d missing", q);
-      if ((rc = up(*d[q], f[q]))) return rc;
+      if ((rc = up(order[q], f[q]))) return rc;
     }
   } else {
     return fail(-2, "unknown model kind %d", kind);
@@ -686,6 +748,120 @@ SW_API int sw_smooth(sw_ctx* c, int32_t field, float sigma) {
   HIPCHK(launch_gauss_x(c->wtab.p, k, c->gtmp.p, nx, nz, c->stream));
   HIPCHK(launch_gauss_z(c->wtab.p, c->gtmp.p, c->gsum.p, k, nx, nz, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+
+SW_API int sw_gradient(sw_ctx* c, int32_t nshots, const int32_t* isrc, const float* obs_ref, int32_t ckpt_every,
+                       int32_t shots_per_launch, double* misfit) {
+  if (!c || nshots < 1 || !isrc || !obs_ref) return fail(-1, "bad arguments");
+  if (uses_spin(c) || (uses_sh(c) == uses_psv(c)))
+    return fail(-2, "the gradient needs exactly one of sh / psv, without spin");
+  if (ckpt_every < 1) return fail(-2, "ckpt_every must be >= 1");
+  if (!c->model_set) return fail(-2, "model not set");
+  HIPCHK(hipSetDevice(c->device));
+  const bool sh = uses_sh(c);
+  const int nt = c->g.nt, nrec = c->nrec, ncomp = sh ? 1 : 2;
+  const int naux = sh ? NAUX_SH : NAUX_PSV;
+  const size_t npt = size_t(c->npt);
+  const int K = std::min(ckpt_every, std::max(nt, 1));
+  const int nck = (nt + K - 1) / K;
+  const int S = (shots_per_launch <= 0 || shots_per_launch > nshots) ? nshots : shots_per_launch;
+  for (int s = 0; s < nshots; ++s)
+    if (isrc[s] >= c->nsrc) return fail(-2, "source %d out of range (nsrc=%d)", isrc[s], c->nsrc);
+  // kernels summed over all shots
+  for (DBuf<float>* k : {&c->k_rho, &c->kmu, &c->k_lam}) {
+    HIPCHK(k->ensure(npt));
+    HIPCHK(launch_fill(k->p, 0.0f, int64_t(npt), c->stream));
+  }
+  // misfit time taper (solver.rs:441-455), f32 on the host
+  std::vector<float> taper(std::max(nt, 1));
+  {
+    const float PI_F = 3.14159265358979323846f, dt = c->g.dt;
+    const float t_end = float(nt - 1) * dt, tw = t_end / 10.0f, t_min = tw, t_max = t_end - tw;
+    for (int n = 0; n < nt; ++n) {
+      const float t = float(n) * dt;
+      taper[n] = t <= t_min ? 0.5f + 0.5f * std::cos(PI_F * (t_min - t) / tw)
+                 : t >= t_max ? 0.5f + 0.5f * std::cos(PI_F * (t_max - t) / tw) : 1.0f;
+    }
+  }
+  HIPCHK(c->gtaper.ensure(taper.size()));
+  HIPCHK(hipMemcpyAsync(c->gtaper.p, taper.data(), taper.size() * 4, hipMemcpyHostToDevice, c->stream));
+  const int comps_h[2] = {sh ? SW_OBS_Y : SW_OBS_X, SW_OBS_Z};
+  HIPCHK(c->gcomps.ensure(2));
+  HIPCHK(hipMemcpyAsync(c->gcomps.p, comps_h, sizeof comps_h, hipMemcpyHostToDevice, c->stream));
+  const size_t per_shot_obs = size_t(ncomp) * nrec * nt;
+  const size_t sstride = size_t(c->nsrc) * nt;
+  std::vector<double> mis_h(S);
+  for (int b0 = 0; b0 < nshots; b0 += S) {
+    const int nb = std::min(S, nshots - b0);
+    // ---- forward with checkpoints (traces recorded)
+    std::vector<std::vector<Entry>> inj(nb), ainj(nb);
+    for (int s = 0; s < nb; ++s) {
+      inj[s] = source_entries(c, isrc[b0 + s]);
+      for (int r = 0; r < nrec; ++r) {
+        const uint32_t k = uint32_t(c->rec_k[r]);
+        if (int64_t(k) >= c->npt) continue;
+        ainj[s].push_back({int(k / uint32_t(c->g.nz)), int(k % uint32_t(c->g.nz)), s * nrec + r});
+      }
+    }
+    c->val_x = c->stf.p; c->val_y = c->stf.p + sstride; c->val_z = c->stf.p + 2 * sstride; c->nt_val = nt;
+    int rc = prepare_batch(c, nb, inj, true);
+    if (rc) return rc;
+    if ((rc = upload_inj(c, ainj, c->ainj_off, c->ainj))) return rc;
+    const size_t state_floats = size_t(nb) * c->nslots * npt;
+    HIPCHK(c->ckpt.ensure(std::max<size_t>(size_t(nck) * state_floats, 1)));
+    for (int q = 0; q < nck; ++q) {
+      if (q > 0)
+        HIPCHK(hipMemcpyAsync(c->ckpt.p + size_t(q) * state_floats, c->state[c->cur].p, state_floats * 4,
+                              hipMemcpyDeviceToDevice, c->stream));
+      if ((rc = launch_steps(c, std::min(K, nt - q * K), M_FWD))) return rc;
+    }
+    // ---- adjoint sources and misfit
+    HIPCHK(c->gref.ensure(std::max<size_t>(size_t(nb) * per_shot_obs, 1)));
+    HIPCHK(hipMemcpyAsync(c->gref.p, obs_ref + size_t(b0) * per_shot_obs, size_t(nb) * per_shot_obs * 4,
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c->adj_val.ensure(std::max<size_t>(size_t(ncomp) * nb * nrec * nt, 1)));
+    HIPCHK(c->gmis.ensure(size_t(nb) + size_t(nb) * ncomp * std::max(nrec, 1)));
+    HIPCHK(launch_adjsrc(c->obs.p, c->gref.p, c->gtaper.p, c->adj_val.p, c->gmis.p, nb, ncomp, c->gcomps.p, nrec, nt,
+                         c->g.dt, c->stream));
+    HIPCHK(hipMemcpyAsync(mis_h.data(), c->gmis.p, size_t(nb) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // ---- backward: per segment, recompute the forward steps (aux kept), then the adjoint steps
+    HIPCHK(c->aux.ensure(std::max<size_t>(size_t(K) * nb * naux * npt, 1)));
+    HIPCHK(c->kshot.ensure(size_t(nb) * NK * npt));
+    HIPCHK(launch_fill(c->kshot.p, 0.0f, int64_t(nb) * NK * int64_t(npt), c->stream));
+    for (int p = 0; p < 2; ++p) {
+      HIPCHK(c->adj[p].ensure(std::max<size_t>(state_floats, 1)));
+      HIPCHK(launch_fill(c->adj[p].p, 0.0f, int64_t(state_floats), c->stream));
+    }
+    c->acur = 0;
+    HIPCHK(c->aclock.ensure(2));
+    for (int q = nck - 1; q >= 0; --q) {
+      const int n0 = q * K, n1 = std::min(nt, n0 + K);
+      c->cur = 0;
+      if (q == 0) HIPCHK(launch_fill(c->state[0].p, 0.0f, int64_t(state_floats), c->stream));
+      else
+        HIPCHK(hipMemcpyAsync(c->state[0].p, c->ckpt.p + size_t(q) * state_floats, state_floats * 4,
+                              hipMemcpyDeviceToDevice, c->stream));
+      const int fclk[2] = {n0, 0}, aclk[2] = {n1 - 1, 0};
+      HIPCHK(hipMemcpyAsync(c->clock.p, fclk, sizeof fclk, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->aclock.p, aclk, sizeof aclk, hipMemcpyHostToDevice, c->stream));
+      c->aux_base = n0;
+      if ((rc = launch_steps(c, n1 - n0, M_FWD_AUX))) return rc;
+      if ((rc = launch_steps(c, n1 - n0, M_ADJ))) return rc;
+      HIPCHK(hipStreamSynchronize(c->stream));   // the clock values above live on the host stack
+    }
+    // ---- sum the shots' kernels into the context's K fields (fixed order)
+    HIPCHK(launch_reduce_shots(c->kshot.p + size_t(K_RHO) * npt, c->k_rho.p, nb, int64_t(npt), NK * int64_t(npt),
+                               c->stream));
+    HIPCHK(launch_reduce_shots(c->kshot.p + size_t(K_MU) * npt, c->kmu.p, nb, int64_t(npt), NK * int64_t(npt),
+                               c->stream));
+    HIPCHK(launch_reduce_shots(c->kshot.p + size_t(K_LAM) * npt, c->k_lam.p, nb, int64_t(npt), NK * int64_t(npt),
+                               c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (misfit)
+      for (int s = 0; s < nb; ++s) misfit[b0 + s] = mis_h[s];
+  }
   return 0;
 }
 

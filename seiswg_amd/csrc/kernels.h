@@ -30,17 +30,22 @@ struct Tiling {
   int nstrips, nsegs;
 };
 
+// Rows of the model buffer [MF_N][npt].
+enum ModelField : int { MF_RHO = 0, MF_LAM, MF_MU, MF_NU, MF_J, MF_LAM_C, MF_MU_C, MF_NU_C, MF_N };
+
 struct StepArgs {
-  const float* in[N_SLOTS];   // shot-0 base of each state slot (read)
-  float* out[N_SLOTS];        // shot-0 base of each state slot (write)
-  int64_t shot_stride;        // floats between consecutive shots of one slot
-  // model (shared by all shots)
-  const float* rho; const float* lam; const float* mu;
-  const float* nu; const float* jj; const float* muc; const float* nuc;
+  // state buffers laid out [shot][slot][npt]
+  const float* state_in;
+  float* state_out;
+  int slot_off[N_SLOTS];      // float offset of each slot inside one shot (-1: not stored)
+  int shot_floats;            // floats per shot
+  const float* model;         // [MF_N][npt], shared by all shots
+  int npt;
   const float* bx; const float* bzb; const float* bzt;   // 1-D taper factors
   int nx, nz, nshots;
   int nstrips, nsegs, seglen;
   float dt, h8x, h8z;         // h8 = 8*h (fd2d.wgsl:156)
+  float r8x, r8z;             // correctly rounded 1/h8
   // body-force injections: CSR over (shot, tile), entries (i, j, q) sorted by i
   const int* inj_off; const int4* inj;
   const float* val_x; const float* val_y; const float* val_z;   // [q][nt_val]
@@ -52,12 +57,30 @@ struct StepArgs {
   int nrec, nt;
   int record;                 // 1: accumulate u and write traces
   int* clock;                 // [0] = current step it, [1] = finished-block counter
-  int advance;                // 1: the last block advances clock[0]
+  int advance;                // forward: last block sets clock[0] = it+1; adjoint: it-1
+  // gradient support: per-step auxiliary fields (forward recompute writes,
+  // adjoint reads) laid out [step - aux_base][shot][naux][npt], and the
+  // per-shot kernels [shot][3][npt] (rho, mu, lambda)
+  float* aux;
+  int aux_base;
+  float* kgrad;
 };
 
+// auxiliary fields kept per recomputed forward step
+enum AuxSh : int { A_DSY = 0, A_EXY, A_EZY, NAUX_SH };
+enum AuxPsv : int { A_DSX = 0, A_DSZ, A_EXX, A_EZZ, A_EXZ, NAUX_PSV };
+enum KField : int { K_RHO = 0, K_MU, K_LAM, NK };
+
 // launches (kernels.hip)
-hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s);
-hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStream_t s);
+hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false);
+hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStream_t s, bool aux = false);
+hipError_t launch_adj_sh(const StepArgs& a, const Tiling& t, hipStream_t s);
+hipError_t launch_adj_psv(const StepArgs& a, const Tiling& t, hipStream_t s);
+// adjoint sources dt * sum_{k>=n} 2 w_k^2 (d^k - obs^k) and per-shot misfit
+hipError_t launch_adjsrc(const float* obs_syn, const float* obs_ref, const float* taper, float* adj, double* misfit,
+                         int nshots, int ncomp, const int* comps, int nrec, int nt, float dt, hipStream_t s);
+hipError_t launch_reduce_shots(const float* kshot, float* kout, int nshots, int64_t n, int64_t shot_stride,
+                               hipStream_t s);
 bool tiling_supported(int nt_threads);
 
 hipError_t launch_imaging_sh(const float* vadj, const float* vfw, float* kmu, int nx, int nz,

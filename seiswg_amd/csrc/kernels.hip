@@ -3,16 +3,20 @@
 // One time step of one physics is ONE launch.  Each workgroup owns a z-strip
 // of W = NT-6 columns (z is the contiguous axis, k = i*nz + j, fd2d.wgsl:140)
 // and marches a segment of rows along x.  x-neighbours stay in registers (a
-// 4-row window per stress field, a 4-row window of new velocities), z-
-// neighbours are exchanged through LDS.  The velocity update of row m and the
-// stress update of row m-2 are fused in the same iteration, so every state
-// value is read once and written once per step; the 3-column/3-row halos are
+// 4-row ring per stress field, a 4-row ring of new velocities), z-neighbours
+// are exchanged through LDS.  The velocity update of row m and the stress
+// update of row m-2 are fused in the same iteration, so every state value is
+// read once and written once per step; the 3-column/3-row halos are
 // recomputed, never exchanged.  Reads and writes go to different (ping-pong)
-// state buffers, so tiles never race on their halos.
+// state buffers, so tiles never race on their halos.  All field traffic uses
+// buffer loads/stores: the per-thread column offset is a constant VGPR and the
+// row/field offset a scalar, so no vector address arithmetic is spent per load.
 //
 // Arithmetic follows fd2d.wgsl operation by operation (no FMA contraction,
-// IEEE division), so results equal the reference kernel sequence of
+// IEEE-exact divisions), so results equal the reference kernel sequence of
 // solver.rs:999-1030 evaluated in FP32.
+#include <type_traits>
+
 #include "kernels.h"
 
 #pragma clang fp contract(off)
@@ -20,13 +24,25 @@
 namespace sw {
 namespace {
 
-// fd2d.wgsl:151-164 (backward) and 166-180 (forward) differences, with the
-// reference's 1/8 weight on the 3-cell difference.
-__device__ __forceinline__ float dminus(float fm2, float fm1, float f0, float fp1, float h8) {
-  return (9.0f * (f0 - fm1) - (fp1 - fm2)) / h8;
+// x / y for a divisor y that is the same for the whole launch, with r the
+// correctly rounded 1/y: q0 = x*r is within 1 ulp of x/y, the remainder
+// x - q0*y is exact in one FMA and q0 + rem*r (one FMA) is the correctly
+// rounded quotient (Markstein's correction), i.e. bit-identical to IEEE
+// division for normal results, at 3 VALU instead of the ~9 of the generic
+// division sequence.
+__device__ __forceinline__ float div_const(float x, float y, float r) {
+  const float q0 = x * r;
+  const float rem = __builtin_fmaf(-q0, y, x);
+  return __builtin_fmaf(rem, r, q0);
 }
-__device__ __forceinline__ float dplus(float fm1, float f0, float fp1, float fp2, float h8) {
-  return (9.0f * (fp1 - f0) - (fp2 - fm1)) / h8;
+
+// fd2d.wgsl:151-164 (backward) and 166-180 (forward) differences, with the
+// reference's 1/8 weight on the 3-cell difference; h8 = 8*h, r8 = 1/h8.
+__device__ __forceinline__ float dminus(float fm2, float fm1, float f0, float fp1, float h8, float r8) {
+  return div_const(9.0f * (f0 - fm1) - (fp1 - fm2), h8, r8);
+}
+__device__ __forceinline__ float dplus(float fm1, float f0, float fp1, float fp2, float h8, float r8) {
+  return div_const(9.0f * (fp1 - f0) - (fp2 - fm1), h8, r8);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Map them so that
@@ -36,95 +52,133 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   return (x < r) ? x * (q + 1) + p : r * (q + 1) + (x - r) * q + p;
 }
 
-__device__ __forceinline__ float ldr(const float* __restrict__ p, int m, int nx, int nz, int j, bool jin,
-                                     float dflt = 0.0f) {
-  return (jin && m >= 0 && m < nx) ? p[m * nz + j] : dflt;
+// Raw buffer resource over `bytes` bytes (gfx9 dword3: 32-bit raw access).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const float* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, int(bytes), 0x00020000);
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 0));
+}
+__device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, int(voff), int(soff), 0);
 }
 
 // Advance the step clock once every workgroup of this launch has read it.
-__device__ __forceinline__ void advance_clock(int* clock, int it) {
+__device__ __forceinline__ void advance_clock(int* clock, int it, int dir = 1) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
     const int done = atomicAdd(&clock[1], 1);
     if (done == int(gridDim.x) - 1) {
       atomicExch(&clock[1], 0);
-      atomicExch(&clock[0], it + 1);
+      atomicExch(&clock[0], it + dir);
     }
   }
 }
 
-// ---------------------------------------------------------------- SH step
-// div_sy, stf_dsy, add_vy, div_vy, add_sy_sh, save_obs_y_sh (fd2d.wgsl:242-301)
-template <int NT>
-__global__ __launch_bounds__(NT) void k_step_sh(const StepArgs a) {
-  constexpr int W = NT - 6;
-  __shared__ float lds_s[2][NT];
-  __shared__ float lds_v[2][NT];
+template <int R>
+using Phase = std::integral_constant<int, R>;
 
+// Geometry shared by the step kernels.  Rows are clamped to the grid and the
+// column to [0, nz): loads never branch and never leave the slice; values read
+// for rows/columns outside the grid only feed halo lanes whose results are
+// discarded or zeroed by the edge rules of the FD operators.
+struct TileGeom {
+  int shot, tile, t, j, i0, i1;
+  bool jin, jst, zm_ok, zp_ok;
+  uint32_t jc4, rowb;
+  int nx;
+  __device__ __forceinline__ uint32_t row(int m) const { return uint32_t(min(max(m, 0), nx - 1)) * rowb; }
+};
+
+template <int NT>
+__device__ __forceinline__ TileGeom tile_geom(const StepArgs& a) {
+  constexpr int W = NT - 6;
+  TileGeom g;
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
   const int tps = a.nstrips * a.nsegs;
-  const int shot = logical / tps;
-  const int tile = logical - shot * tps;
-  const int strip = tile % a.nstrips, seg = tile / a.nstrips;
-  const int t = threadIdx.x;
-  const int nx = a.nx, nz = a.nz;
-  const int j = strip * W - 3 + t;
-  const int i0 = seg * a.seglen, i1 = min(i0 + a.seglen, nx);
-  const bool jin = j >= 0 && j < nz;
-  const bool jst = jin && t >= 3 && t < NT - 3;
-  const bool zm_ok = j >= 2 && j < nz - 2;
-  const bool zp_ok = j >= 1 && j < nz - 2;
+  g.shot = logical / tps;
+  g.tile = logical - g.shot * tps;
+  const int strip = g.tile % a.nstrips, seg = g.tile / a.nstrips;
+  g.t = threadIdx.x;
+  g.nx = a.nx;
+  g.j = strip * W - 3 + g.t;
+  g.i0 = seg * a.seglen;
+  g.i1 = min(g.i0 + a.seglen, a.nx);
+  g.jin = g.j >= 0 && g.j < a.nz;
+  g.jst = g.jin && g.t >= 3 && g.t < NT - 3;
+  g.zm_ok = g.j >= 2 && g.j < a.nz - 2;
+  g.zp_ok = g.j >= 1 && g.j < a.nz - 2;
+  g.jc4 = 4u * uint32_t(min(max(g.j, 0), a.nz - 1));
+  g.rowb = 4u * uint32_t(a.nz);
+  return g;
+}
+
+// ---------------------------------------------------------------- SH step
+// div_sy, stf_dsy, add_vy, div_vy, add_sy_sh, save_obs_y_sh (fd2d.wgsl:242-301)
+template <int NT, bool AUX>
+__global__ __launch_bounds__(NT) void k_step_sh(const StepArgs a) {
+  __shared__ float lds_s[2][NT];
+  __shared__ float lds_v[2][NT];
+  const TileGeom g = tile_geom<NT>(a);
+  const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const int64_t so = int64_t(shot) * a.shot_stride;
-  const float* __restrict__ ivy = a.in[S_VY] + so;
-  const float* __restrict__ isxy = a.in[S_SXY] + so;
-  const float* __restrict__ iszy = a.in[S_SZY] + so;
-  float* __restrict__ ovy = a.out[S_VY] + so;
-  float* __restrict__ osxy = a.out[S_SXY] + so;
-  float* __restrict__ oszy = a.out[S_SZY] + so;
-  const float bzb = jin ? a.bzb[j] : 1.0f, bzt = jin ? a.bzt[j] : 1.0f;
-  const float dt = a.dt, h8x = a.h8x, h8z = a.h8z;
+  const int64_t so = int64_t(g.shot) * a.shot_floats;
+  const uint32_t sbytes = 4u * uint32_t(a.shot_floats);
+  const auto rin = make_rsrc(a.state_in + so, sbytes);
+  const auto rout = make_rsrc(a.state_out + so, sbytes);
+  const uint32_t npt4 = 4u * uint32_t(a.npt);
+  const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
+  const uint32_t o_vy = 4u * a.slot_off[S_VY], o_sxy = 4u * a.slot_off[S_SXY], o_szy = 4u * a.slot_off[S_SZY];
+  const uint32_t m_rho = MF_RHO * npt4, m_mu = MF_MU * npt4;
+  const uint32_t jc4 = g.jc4;
+  const int jc = int(jc4 >> 2);
+  const float bzb = a.bzb[jc], bzt = a.bzt[jc];
+  const float dt = a.dt, h8x = a.h8x, h8z = a.h8z, r8x = a.r8x, r8z = a.r8z;
   const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
-
-  int ip = a.inj_off[shot * tps + tile];
-  const int ie = a.inj_off[shot * tps + tile + 1];
+  const int tps = a.nstrips * a.nsegs;
+  int ip = a.inj_off[g.shot * tps + g.tile];
+  const int ie = a.inj_off[g.shot * tps + g.tile + 1];
   int rp = 0, re = 0;
-  if (a.record) { rp = a.rec_off[tile]; re = a.rec_off[tile + 1]; }
-  float* uacc = a.uacc + int64_t(shot) * a.nrec * N_UACC;
-  float* obs = a.obs + int64_t(shot) * 5 * a.nrec * a.nt;
+  if (a.record) { rp = a.rec_off[g.tile]; re = a.rec_off[g.tile + 1]; }
+  float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
+  float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
+  const auto raux =
+      make_rsrc(AUX ? a.aux + (int64_t(it - a.aux_base) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt) : a.model,
+                AUX ? uint32_t(NAUX_SH) * npt4 : 0u);
 
-  // stress windows hold rows m-2..m+1; prefetch registers hold row m+2
-  float sxy[4] = {0, 0, 0, 0}, szy[4] = {0, 0, 0, 0};
-  for (int q = 1; q < 4; ++q) {
-    sxy[q] = ldr(isxy, i0 - 4 + q, nx, nz, j, jin);
-    szy[q] = ldr(iszy, i0 - 4 + q, nx, nz, j, jin);
+  // rings: stress row r in slot (r - (i0-3)) & 3; new-velocity row r in slot
+  // (r - (i0-4)) & 3; d+z of new vy, row r, in slot (r - (i0-3)) & 3
+  float sxy[4], szy[4], vy[4] = {0, 0, 0, 0}, dvydz[4] = {0, 0, 0, 0};
+  for (int q = 0; q < 3; ++q) {
+    sxy[q] = bload(rin, jc4, o_sxy + g.row(i0 - 3 + q));
+    szy[q] = bload(rin, jc4, o_szy + g.row(i0 - 3 + q));
   }
-  float pf_sxy = ldr(isxy, i0, nx, nz, j, jin), pf_szy = ldr(iszy, i0, nx, nz, j, jin);
-  float pf_vy = ldr(ivy, i0 - 1, nx, nz, j, jin);
-  float pf_rho = ldr(a.rho, i0 - 1, nx, nz, j, jin, 1.0f);
-  float pf_mu = ldr(a.mu, i0 - 3, nx, nz, j, jin);
-  float vy[4] = {0, 0, 0, 0};        // new velocity, rows m-3..m
-  float dvydz[3] = {0, 0, 0};        // d+z of new vy, rows m-2..m
+  sxy[3] = szy[3] = 0.0f;
+  float pf_sxy = bload(rin, jc4, o_sxy + g.row(i0)), pf_szy = bload(rin, jc4, o_szy + g.row(i0));
+  float pf_vy = bload(rin, jc4, o_vy + g.row(i0 - 1));
+  float pf_rho = bload(rmod, jc4, m_rho + g.row(i0 - 1));
+  float pf_mu = bload(rmod, jc4, m_mu + g.row(i0 - 3));
 
-  int par = 0;
-  for (int m = i0 - 1; m <= i1 + 1; ++m) {
-    sxy[0] = sxy[1]; sxy[1] = sxy[2]; sxy[2] = sxy[3]; sxy[3] = pf_sxy;
-    szy[0] = szy[1]; szy[1] = szy[2]; szy[2] = szy[3]; szy[3] = pf_szy;
+  auto body = [&](auto phase, int m) __attribute__((always_inline)) {
+    constexpr int R = decltype(phase)::value;
+    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
+    sxy[s3] = pf_sxy;          // row m+1
+    szy[s3] = pf_szy;
     const float vy_o = pf_vy, rho = pf_rho, mu = pf_mu;
-    pf_sxy = ldr(isxy, m + 2, nx, nz, j, jin);
-    pf_szy = ldr(iszy, m + 2, nx, nz, j, jin);
-    pf_vy = ldr(ivy, m + 1, nx, nz, j, jin);
-    pf_rho = ldr(a.rho, m + 1, nx, nz, j, jin, 1.0f);
-    pf_mu = ldr(a.mu, m - 1, nx, nz, j, jin);
-    const float bxm = (m >= 0 && m < nx) ? a.bx[m] : 1.0f;
+    pf_sxy = bload(rin, jc4, o_sxy + g.row(m + 2));
+    pf_szy = bload(rin, jc4, o_szy + g.row(m + 2));
+    pf_vy = bload(rin, jc4, o_vy + g.row(m + 1));
+    pf_rho = bload(rmod, jc4, m_rho + g.row(m + 1));
+    pf_mu = bload(rmod, jc4, m_mu + g.row(m - 1));
+    const float bxm = a.bx[min(max(m, 0), nx - 1)];
 
     // velocity half-step at row m
-    lds_s[par][t] = szy[2];
+    lds_s[par][t] = szy[s2];
     __syncthreads();
     const bool xm_ok = m >= 2 && m < nx - 2;
-    float dsy = (xm_ok ? dminus(sxy[0], sxy[1], sxy[2], sxy[3], h8x) : 0.0f) +
-                (zm_ok ? dminus(lds_s[par][tm2], lds_s[par][tm1], szy[2], lds_s[par][tp1], h8z) : 0.0f);
+    float dsy = (xm_ok ? dminus(sxy[s0], sxy[s1], sxy[s2], sxy[s3], h8x, r8x) : 0.0f) +
+                (g.zm_ok ? dminus(lds_s[par][tm2], lds_s[par][tm1], szy[s2], lds_s[par][tp1], h8z, r8z) : 0.0f);
     while (ip < ie) {
       const int4 e = a.inj[ip];
       if (e.x > m) break;
@@ -133,12 +187,14 @@ __global__ __launch_bounds__(NT) void k_step_sh(const StepArgs a) {
     }
     const float b = (bxm * bzb) * bzt;
     const float vyn = b * (vy_o + dt * dsy / rho);
-    if (jst && m >= i0 && m < i1) ovy[m * nz + j] = vyn;
-    vy[0] = vy[1]; vy[1] = vy[2]; vy[2] = vy[3]; vy[3] = vyn;
+    if (g.jst && m >= i0 && m < i1) {
+      bstore(vyn, rout, jc4, o_vy + uint32_t(m) * g.rowb);
+      if constexpr (AUX) bstore(dsy, raux, jc4, A_DSY * npt4 + uint32_t(m) * g.rowb);
+    }
+    vy[s3] = vyn;              // new-velocity ring: rows m-3..m in slots s0..s3
     lds_v[par][t] = vyn;
     __syncthreads();
-    dvydz[0] = dvydz[1]; dvydz[1] = dvydz[2];
-    dvydz[2] = zp_ok ? dplus(lds_v[par][tm1], vyn, lds_v[par][tp1], lds_v[par][tp2], h8z) : 0.0f;
+    dvydz[s2] = g.zp_ok ? dplus(lds_v[par][tm1], vyn, lds_v[par][tp1], lds_v[par][tp2], h8z, r8z) : 0.0f;
 
     // receivers on row m: uy += vy*dt, trace = uy
     while (rp < re) {
@@ -158,12 +214,30 @@ __global__ __launch_bounds__(NT) void k_step_sh(const StepArgs a) {
     if (i >= i0) {
       const bool xp_ok = i >= 1 && i < nx - 2;
       const float mu_dt = mu * dt;
-      const float dvydx = xp_ok ? dplus(vy[0], vy[1], vy[2], vy[3], h8x) : 0.0f;
-      const float nsxy = sxy[0] + mu_dt * dvydx;
-      const float nszy = szy[0] + mu_dt * dvydz[0];
-      if (jst) { osxy[i * nz + j] = nsxy; oszy[i * nz + j] = nszy; }
+      const float dvydx = xp_ok ? dplus(vy[s0], vy[s1], vy[s2], vy[s3], h8x, r8x) : 0.0f;
+      const float nsxy = sxy[s0] + mu_dt * dvydx;
+      const float nszy = szy[s0] + mu_dt * dvydz[s0];
+      if (g.jst) {
+        bstore(nsxy, rout, jc4, o_sxy + uint32_t(i) * g.rowb);
+        bstore(nszy, rout, jc4, o_szy + uint32_t(i) * g.rowb);
+        if constexpr (AUX) {
+          bstore(dvydx, raux, jc4, A_EXY * npt4 + uint32_t(i) * g.rowb);
+          bstore(dvydz[s0], raux, jc4, A_EZY * npt4 + uint32_t(i) * g.rowb);
+        }
+      }
     }
-    par ^= 1;
+  };
+
+  const int mlast = i1 + 1;
+  for (int m = i0 - 1;;) {
+    body(Phase<0>{}, m);
+    if (++m > mlast) break;
+    body(Phase<1>{}, m);
+    if (++m > mlast) break;
+    body(Phase<2>{}, m);
+    if (++m > mlast) break;
+    body(Phase<3>{}, m);
+    if (++m > mlast) break;
   }
   if (a.advance) advance_clock(a.clock, it);
 }
@@ -172,144 +246,133 @@ __global__ __launch_bounds__(NT) void k_step_sh(const StepArgs a) {
 // div_sxz, div_sy_c, div_sxyz_c, stf_dsxz, add_vxz, div_vxz, add_sxz,
 // add_vy_c, div_vy_c, add_sy_c, save_obs_x/z, compute_du_grad, save_obs_ry
 // (fd2d.wgsl:308-492)
-template <int NT, bool SPIN>
+template <int NT, bool SPIN, bool AUX>
 __global__ __launch_bounds__(NT) void k_step_psv(const StepArgs a) {
-  constexpr int W = NT - 6;
   constexpr int NSL = SPIN ? 4 : 2;   // sxz, szz, syz_c, syy_c
   constexpr int NVL = SPIN ? 3 : 2;   // vx, vz, vy_c
   __shared__ float lds_s[2][NSL][NT];
   __shared__ float lds_v[2][NVL][NT];
-
-  const int logical = xcd_remap(blockIdx.x, gridDim.x);
-  const int tps = a.nstrips * a.nsegs;
-  const int shot = logical / tps;
-  const int tile = logical - shot * tps;
-  const int strip = tile % a.nstrips, seg = tile / a.nstrips;
-  const int t = threadIdx.x;
-  const int nx = a.nx, nz = a.nz;
-  const int j = strip * W - 3 + t;
-  const int i0 = seg * a.seglen, i1 = min(i0 + a.seglen, nx);
-  const bool jin = j >= 0 && j < nz;
-  const bool jst = jin && t >= 3 && t < NT - 3;
-  const bool zm_ok = j >= 2 && j < nz - 2;
-  const bool zp_ok = j >= 1 && j < nz - 2;
+  const TileGeom g = tile_geom<NT>(a);
+  const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const int64_t so = int64_t(shot) * a.shot_stride;
-  const float* __restrict__ ivx = a.in[S_VX] + so;
-  const float* __restrict__ ivz = a.in[S_VZ] + so;
-  const float* __restrict__ isxx = a.in[S_SXX] + so;
-  const float* __restrict__ iszz = a.in[S_SZZ] + so;
-  const float* __restrict__ isxz = a.in[S_SXZ] + so;
-  float* __restrict__ ovx = a.out[S_VX] + so;
-  float* __restrict__ ovz = a.out[S_VZ] + so;
-  float* __restrict__ osxx = a.out[S_SXX] + so;
-  float* __restrict__ oszz = a.out[S_SZZ] + so;
-  float* __restrict__ osxz = a.out[S_SXZ] + so;
-  const float* __restrict__ ivyc = nullptr;
-  const float* __restrict__ isyx = nullptr;
-  const float* __restrict__ isyy = nullptr;
-  const float* __restrict__ isyz = nullptr;
-  float* __restrict__ ovyc = nullptr;
-  float* __restrict__ osyx = nullptr;
-  float* __restrict__ osyy = nullptr;
-  float* __restrict__ osyz = nullptr;
+  const int64_t so = int64_t(g.shot) * a.shot_floats;
+  const uint32_t sbytes = 4u * uint32_t(a.shot_floats);
+  const auto rin = make_rsrc(a.state_in + so, sbytes);
+  const auto rout = make_rsrc(a.state_out + so, sbytes);
+  const uint32_t npt4 = 4u * uint32_t(a.npt);
+  const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
+  const uint32_t o_vx = 4u * a.slot_off[S_VX], o_vz = 4u * a.slot_off[S_VZ];
+  const uint32_t o_sxx = 4u * a.slot_off[S_SXX], o_szz = 4u * a.slot_off[S_SZZ], o_sxz = 4u * a.slot_off[S_SXZ];
+  uint32_t o_vyc = 0, o_syx = 0, o_syy = 0, o_syz = 0;
   if constexpr (SPIN) {
-    ivyc = a.in[S_VYC] + so; isyx = a.in[S_SYXC] + so; isyy = a.in[S_SYYC] + so; isyz = a.in[S_SYZC] + so;
-    ovyc = a.out[S_VYC] + so; osyx = a.out[S_SYXC] + so; osyy = a.out[S_SYYC] + so; osyz = a.out[S_SYZC] + so;
+    o_vyc = 4u * a.slot_off[S_VYC]; o_syx = 4u * a.slot_off[S_SYXC];
+    o_syy = 4u * a.slot_off[S_SYYC]; o_syz = 4u * a.slot_off[S_SYZC];
   }
-  const float bzb = jin ? a.bzb[j] : 1.0f, bzt = jin ? a.bzt[j] : 1.0f;
-  const float dt = a.dt, h8x = a.h8x, h8z = a.h8z;
+  const uint32_t m_rho = MF_RHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4;
+  const uint32_t m_nu = MF_NU * npt4, m_j = MF_J * npt4, m_muc = MF_MU_C * npt4, m_nuc = MF_NU_C * npt4;
+  const uint32_t jc4 = g.jc4;
+  const int jc = int(jc4 >> 2);
+  const float bzb = a.bzb[jc], bzt = a.bzt[jc];
+  const float dt = a.dt, h8x = a.h8x, h8z = a.h8z, r8x = a.r8x, r8z = a.r8z;
   const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
-
-  int ip = a.inj_off[shot * tps + tile];
-  const int ie = a.inj_off[shot * tps + tile + 1];
+  const int tps = a.nstrips * a.nsegs;
+  int ip = a.inj_off[g.shot * tps + g.tile];
+  const int ie = a.inj_off[g.shot * tps + g.tile + 1];
   int rp = 0, re = 0;
-  if (a.record) { rp = a.rec_off[tile]; re = a.rec_off[tile + 1]; }
-  float* uacc = a.uacc + int64_t(shot) * a.nrec * N_UACC;
-  float* obs = a.obs + int64_t(shot) * 5 * a.nrec * a.nt;
+  if (a.record) { rp = a.rec_off[g.tile]; re = a.rec_off[g.tile + 1]; }
+  float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
+  float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const int nobs = a.nrec * a.nt;
+  const auto raux =
+      make_rsrc(AUX ? a.aux + (int64_t(it - a.aux_base) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt) : a.model,
+                AUX ? uint32_t(NAUX_PSV) * npt4 : 0u);
 
-  float sxx[4] = {0, 0, 0, 0}, sxz[4] = {0, 0, 0, 0}, szz[4] = {0, 0, 0, 0};
-  float syx[4] = {0, 0, 0, 0}, syy[4] = {0, 0, 0, 0}, syz[4] = {0, 0, 0, 0};
-  for (int q = 1; q < 4; ++q) {
-    const int r = i0 - 4 + q;
-    sxx[q] = ldr(isxx, r, nx, nz, j, jin);
-    sxz[q] = ldr(isxz, r, nx, nz, j, jin);
-    szz[q] = ldr(iszz, r, nx, nz, j, jin);
+  // rings as in the SH kernel
+  float sxx[4], sxz[4], szz[4], syx[4], syy[4], syz[4];
+  float vx[4] = {0, 0, 0, 0}, vz[4] = {0, 0, 0, 0}, vyc[4] = {0, 0, 0, 0};
+  float dvxdz[4] = {0, 0, 0, 0}, dvzdz[4] = {0, 0, 0, 0}, dvycdz[4] = {0, 0, 0, 0};
+  for (int q = 0; q < 3; ++q) {
+    const uint32_t r = g.row(i0 - 3 + q);
+    sxx[q] = bload(rin, jc4, o_sxx + r);
+    sxz[q] = bload(rin, jc4, o_sxz + r);
+    szz[q] = bload(rin, jc4, o_szz + r);
     if constexpr (SPIN) {
-      syx[q] = ldr(isyx, r, nx, nz, j, jin);
-      syy[q] = ldr(isyy, r, nx, nz, j, jin);
-      syz[q] = ldr(isyz, r, nx, nz, j, jin);
+      syx[q] = bload(rin, jc4, o_syx + r);
+      syy[q] = bload(rin, jc4, o_syy + r);
+      syz[q] = bload(rin, jc4, o_syz + r);
     }
   }
-  float pf_sxx = ldr(isxx, i0, nx, nz, j, jin), pf_sxz = ldr(isxz, i0, nx, nz, j, jin),
-        pf_szz = ldr(iszz, i0, nx, nz, j, jin);
+  sxx[3] = sxz[3] = szz[3] = syx[3] = syy[3] = syz[3] = 0.0f;
+  if constexpr (!SPIN) syx[0] = syx[1] = syx[2] = syy[0] = syy[1] = syy[2] = syz[0] = syz[1] = syz[2] = 0.0f;
+  float pf_sxx = bload(rin, jc4, o_sxx + g.row(i0)), pf_sxz = bload(rin, jc4, o_sxz + g.row(i0)),
+        pf_szz = bload(rin, jc4, o_szz + g.row(i0));
   float pf_syx = 0, pf_syy = 0, pf_syz = 0, pf_vyc = 0, pf_jj = 1.0f, pf_nu = 0, pf_muc = 0, pf_nuc = 0;
   if constexpr (SPIN) {
-    pf_syx = ldr(isyx, i0, nx, nz, j, jin);
-    pf_syy = ldr(isyy, i0, nx, nz, j, jin);
-    pf_syz = ldr(isyz, i0, nx, nz, j, jin);
-    pf_vyc = ldr(ivyc, i0 - 1, nx, nz, j, jin);
-    pf_jj = ldr(a.jj, i0 - 1, nx, nz, j, jin, 1.0f);
-    pf_nu = ldr(a.nu, i0 - 3, nx, nz, j, jin);
-    pf_muc = ldr(a.muc, i0 - 3, nx, nz, j, jin);
-    pf_nuc = ldr(a.nuc, i0 - 3, nx, nz, j, jin);
+    pf_syx = bload(rin, jc4, o_syx + g.row(i0));
+    pf_syy = bload(rin, jc4, o_syy + g.row(i0));
+    pf_syz = bload(rin, jc4, o_syz + g.row(i0));
+    pf_vyc = bload(rin, jc4, o_vyc + g.row(i0 - 1));
+    pf_jj = bload(rmod, jc4, m_j + g.row(i0 - 1));
+    pf_nu = bload(rmod, jc4, m_nu + g.row(i0 - 3));
+    pf_muc = bload(rmod, jc4, m_muc + g.row(i0 - 3));
+    pf_nuc = bload(rmod, jc4, m_nuc + g.row(i0 - 3));
   }
-  float pf_vx = ldr(ivx, i0 - 1, nx, nz, j, jin), pf_vz = ldr(ivz, i0 - 1, nx, nz, j, jin);
-  float pf_rho = ldr(a.rho, i0 - 1, nx, nz, j, jin, 1.0f);
-  float pf_lam = ldr(a.lam, i0 - 3, nx, nz, j, jin), pf_mu = ldr(a.mu, i0 - 3, nx, nz, j, jin);
-  float vx[4] = {0, 0, 0, 0}, vz[4] = {0, 0, 0, 0}, vyc[4] = {0, 0, 0, 0};   // rows m-3..m
-  float dvxdz[3] = {0, 0, 0}, dvzdz[3] = {0, 0, 0}, dvycdz[3] = {0, 0, 0};   // rows m-2..m
+  float pf_vx = bload(rin, jc4, o_vx + g.row(i0 - 1)), pf_vz = bload(rin, jc4, o_vz + g.row(i0 - 1));
+  float pf_rho = bload(rmod, jc4, m_rho + g.row(i0 - 1));
+  float pf_lam = bload(rmod, jc4, m_lam + g.row(i0 - 3)), pf_mu = bload(rmod, jc4, m_mu + g.row(i0 - 3));
 
-  int par = 0;
-  for (int m = i0 - 1; m <= i1 + 1; ++m) {
-    sxx[0] = sxx[1]; sxx[1] = sxx[2]; sxx[2] = sxx[3]; sxx[3] = pf_sxx;
-    sxz[0] = sxz[1]; sxz[1] = sxz[2]; sxz[2] = sxz[3]; sxz[3] = pf_sxz;
-    szz[0] = szz[1]; szz[1] = szz[2]; szz[2] = szz[3]; szz[3] = pf_szz;
-    if constexpr (SPIN) {
-      syx[0] = syx[1]; syx[1] = syx[2]; syx[2] = syx[3]; syx[3] = pf_syx;
-      syy[0] = syy[1]; syy[1] = syy[2]; syy[2] = syy[3]; syy[3] = pf_syy;
-      syz[0] = syz[1]; syz[1] = syz[2]; syz[2] = syz[3]; syz[3] = pf_syz;
-    }
+  auto body = [&](auto phase, int m) __attribute__((always_inline)) {
+    constexpr int R = decltype(phase)::value;
+    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
+    sxx[s3] = pf_sxx; sxz[s3] = pf_sxz; szz[s3] = pf_szz;     // row m+1
+    if constexpr (SPIN) { syx[s3] = pf_syx; syy[s3] = pf_syy; syz[s3] = pf_syz; }
     const float vx_o = pf_vx, vz_o = pf_vz, rho = pf_rho, lam = pf_lam, mu = pf_mu;
     const float vyc_o = pf_vyc, jj = pf_jj, nu = pf_nu, muc = pf_muc, nuc = pf_nuc;
-    pf_sxx = ldr(isxx, m + 2, nx, nz, j, jin);
-    pf_sxz = ldr(isxz, m + 2, nx, nz, j, jin);
-    pf_szz = ldr(iszz, m + 2, nx, nz, j, jin);
-    pf_vx = ldr(ivx, m + 1, nx, nz, j, jin);
-    pf_vz = ldr(ivz, m + 1, nx, nz, j, jin);
-    pf_rho = ldr(a.rho, m + 1, nx, nz, j, jin, 1.0f);
-    pf_lam = ldr(a.lam, m - 1, nx, nz, j, jin);
-    pf_mu = ldr(a.mu, m - 1, nx, nz, j, jin);
-    if constexpr (SPIN) {
-      pf_syx = ldr(isyx, m + 2, nx, nz, j, jin);
-      pf_syy = ldr(isyy, m + 2, nx, nz, j, jin);
-      pf_syz = ldr(isyz, m + 2, nx, nz, j, jin);
-      pf_vyc = ldr(ivyc, m + 1, nx, nz, j, jin);
-      pf_jj = ldr(a.jj, m + 1, nx, nz, j, jin, 1.0f);
-      pf_nu = ldr(a.nu, m - 1, nx, nz, j, jin);
-      pf_muc = ldr(a.muc, m - 1, nx, nz, j, jin);
-      pf_nuc = ldr(a.nuc, m - 1, nx, nz, j, jin);
+    {
+      const uint32_t r2 = g.row(m + 2), r1 = g.row(m + 1), rm1 = g.row(m - 1);
+      pf_sxx = bload(rin, jc4, o_sxx + r2);
+      pf_sxz = bload(rin, jc4, o_sxz + r2);
+      pf_szz = bload(rin, jc4, o_szz + r2);
+      pf_vx = bload(rin, jc4, o_vx + r1);
+      pf_vz = bload(rin, jc4, o_vz + r1);
+      pf_rho = bload(rmod, jc4, m_rho + r1);
+      pf_lam = bload(rmod, jc4, m_lam + rm1);
+      pf_mu = bload(rmod, jc4, m_mu + rm1);
+      if constexpr (SPIN) {
+        pf_syx = bload(rin, jc4, o_syx + r2);
+        pf_syy = bload(rin, jc4, o_syy + r2);
+        pf_syz = bload(rin, jc4, o_syz + r2);
+        pf_vyc = bload(rin, jc4, o_vyc + r1);
+        pf_jj = bload(rmod, jc4, m_j + r1);
+        pf_nu = bload(rmod, jc4, m_nu + rm1);
+        pf_muc = bload(rmod, jc4, m_muc + rm1);
+        pf_nuc = bload(rmod, jc4, m_nuc + rm1);
+      }
     }
-    const float bxm = (m >= 0 && m < nx) ? a.bx[m] : 1.0f;
+    const float bxm = a.bx[min(max(m, 0), nx - 1)];
 
     // ---- velocity half-step at row m
-    lds_s[par][0][t] = sxz[2];
-    lds_s[par][1][t] = szz[2];
-    if constexpr (SPIN) { lds_s[par][2][t] = syz[2]; lds_s[par][3][t] = syy[2]; }
+    lds_s[par][0][t] = sxz[s2];
+    lds_s[par][1][t] = szz[s2];
+    if constexpr (SPIN) { lds_s[par][2][t] = syz[s2]; lds_s[par][3][t] = syy[s2]; }
     __syncthreads();
     const bool xm_ok = m >= 2 && m < nx - 2;
-    float dsx = (xm_ok ? dminus(sxx[0], sxx[1], sxx[2], sxx[3], h8x) : 0.0f) +
-                (zm_ok ? dminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[2], lds_s[par][0][tp1], h8z) : 0.0f);
-    float dsz = (xm_ok ? dminus(sxz[0], sxz[1], sxz[2], sxz[3], h8x) : 0.0f) +
-                (zm_ok ? dminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[2], lds_s[par][1][tp1], h8z) : 0.0f);
+    const bool zm_ok = g.zm_ok;
+    float dsx = (xm_ok ? dminus(sxx[s0], sxx[s1], sxx[s2], sxx[s3], h8x, r8x) : 0.0f) +
+                (zm_ok ? dminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[s2], lds_s[par][0][tp1], h8z, r8z)
+                       : 0.0f);
+    float dsz = (xm_ok ? dminus(sxz[s0], sxz[s1], sxz[s2], sxz[s3], h8x, r8x) : 0.0f) +
+                (zm_ok ? dminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[s2], lds_s[par][1][tp1], h8z, r8z)
+                       : 0.0f);
     float dsyc = 0.0f;
     if constexpr (SPIN) {
-      dsyc = (xm_ok ? dminus(syx[0], syx[1], syx[2], syx[3], h8x) : 0.0f) +
-             (zm_ok ? dminus(lds_s[par][2][tm2], lds_s[par][2][tm1], syz[2], lds_s[par][2][tp1], h8z) : 0.0f);
-      dsx = dsx + (zm_ok ? dminus(lds_s[par][3][tm2], lds_s[par][3][tm1], syy[2], lds_s[par][3][tp1], h8z) : 0.0f);
-      dsz = dsz + (-(xm_ok ? dminus(syy[0], syy[1], syy[2], syy[3], h8x) : 0.0f));
-      dsyc = dsyc + 2.0f * syy[2];
+      dsyc = (xm_ok ? dminus(syx[s0], syx[s1], syx[s2], syx[s3], h8x, r8x) : 0.0f) +
+             (zm_ok ? dminus(lds_s[par][2][tm2], lds_s[par][2][tm1], syz[s2], lds_s[par][2][tp1], h8z, r8z)
+                    : 0.0f);
+      dsx = dsx + (zm_ok ? dminus(lds_s[par][3][tm2], lds_s[par][3][tm1], syy[s2], lds_s[par][3][tp1], h8z, r8z)
+                         : 0.0f);
+      dsz = dsz + (-(xm_ok ? dminus(syy[s0], syy[s1], syy[s2], syy[s3], h8x, r8x) : 0.0f));
+      dsyc = dsyc + 2.0f * syy[s2];
     }
     while (ip < ie) {
       const int4 e = a.inj[ip];
@@ -325,26 +388,28 @@ __global__ __launch_bounds__(NT) void k_step_psv(const StepArgs a) {
     const float vzn = b * (vz_o + dt * dsz / rho);
     float vycn = 0.0f;
     if constexpr (SPIN) vycn = b * (vyc_o + dt * dsyc / jj);
-    if (jst && m >= i0 && m < i1) {
-      ovx[m * nz + j] = vxn;
-      ovz[m * nz + j] = vzn;
-      if constexpr (SPIN) ovyc[m * nz + j] = vycn;
+    if (g.jst && m >= i0 && m < i1) {
+      const uint32_t ro = uint32_t(m) * g.rowb;
+      bstore(vxn, rout, jc4, o_vx + ro);
+      bstore(vzn, rout, jc4, o_vz + ro);
+      if constexpr (SPIN) bstore(vycn, rout, jc4, o_vyc + ro);
+      if constexpr (AUX) {
+        bstore(dsx, raux, jc4, A_DSX * npt4 + ro);
+        bstore(dsz, raux, jc4, A_DSZ * npt4 + ro);
+      }
     }
-    vx[0] = vx[1]; vx[1] = vx[2]; vx[2] = vx[3]; vx[3] = vxn;
-    vz[0] = vz[1]; vz[1] = vz[2]; vz[2] = vz[3]; vz[3] = vzn;
-    if constexpr (SPIN) { vyc[0] = vyc[1]; vyc[1] = vyc[2]; vyc[2] = vyc[3]; vyc[3] = vycn; }
+    vx[s3] = vxn;
+    vz[s3] = vzn;
+    if constexpr (SPIN) vyc[s3] = vycn;
     lds_v[par][0][t] = vxn;
     lds_v[par][1][t] = vzn;
     if constexpr (SPIN) lds_v[par][2][t] = vycn;
     __syncthreads();
-    dvxdz[0] = dvxdz[1]; dvxdz[1] = dvxdz[2];
-    dvzdz[0] = dvzdz[1]; dvzdz[1] = dvzdz[2];
-    dvxdz[2] = zp_ok ? dplus(lds_v[par][0][tm1], vxn, lds_v[par][0][tp1], lds_v[par][0][tp2], h8z) : 0.0f;
-    dvzdz[2] = zp_ok ? dplus(lds_v[par][1][tm1], vzn, lds_v[par][1][tp1], lds_v[par][1][tp2], h8z) : 0.0f;
-    if constexpr (SPIN) {
-      dvycdz[0] = dvycdz[1]; dvycdz[1] = dvycdz[2];
-      dvycdz[2] = zp_ok ? dplus(lds_v[par][2][tm1], vycn, lds_v[par][2][tp1], lds_v[par][2][tp2], h8z) : 0.0f;
-    }
+    const bool zp_ok = g.zp_ok;
+    dvxdz[s2] = zp_ok ? dplus(lds_v[par][0][tm1], vxn, lds_v[par][0][tp1], lds_v[par][0][tp2], h8z, r8z) : 0.0f;
+    dvzdz[s2] = zp_ok ? dplus(lds_v[par][1][tm1], vzn, lds_v[par][1][tp1], lds_v[par][1][tp2], h8z, r8z) : 0.0f;
+    if constexpr (SPIN)
+      dvycdz[s2] = zp_ok ? dplus(lds_v[par][2][tm1], vycn, lds_v[par][2][tp1], lds_v[par][2][tp2], h8z, r8z) : 0.0f;
 
     // ---- receivers: u = sum v*dt at the cells the traces need
     if (a.record) {
@@ -355,7 +420,8 @@ __global__ __launch_bounds__(NT) void k_step_psv(const StepArgs a) {
           if (e.x == m && e.y == j) {
             float* u = uacc + e.z * N_UACC;
             const float ux = u[U_UX] + vxn * dt, uz = u[U_UZ] + vzn * dt;
-            u[U_UX] = ux; u[U_UZ] = uz;
+            u[U_UX] = ux;
+            u[U_UZ] = uz;
             obs[0 * nobs + e.z * a.nt + it] = ux;
             obs[2 * nobs + e.z * a.nt + it] = uz;
           }
@@ -387,8 +453,8 @@ __global__ __launch_bounds__(NT) void k_step_psv(const StepArgs a) {
               const float ux = u[U_UX], uz = u[U_UZ], uyc = u[U_UYC];
               const bool xp = e.x >= 1 && e.x < nx - 2;
               const bool zp = e.y >= 1 && e.y < nz - 2;
-              const float duzdx = xp ? dplus(u[U_UZ_M1], uz, u[U_UZ_P1], uz_p2, h8x) : 0.0f;
-              const float duxdz = zp ? dplus(u[U_UX_M1], ux, u[U_UX_P1], u[U_UX_P2], h8z) : 0.0f;
+              const float duzdx = xp ? dplus(u[U_UZ_M1], uz, u[U_UZ_P1], uz_p2, h8x, r8x) : 0.0f;
+              const float duxdz = zp ? dplus(u[U_UX_M1], ux, u[U_UX_P1], u[U_UX_P2], h8z, r8z) : 0.0f;
               const float curl = -(duzdx - duxdz);
               const float iso = uyc - 0.5f * curl;
               const int kt = e.z * a.nt + it;
@@ -407,39 +473,409 @@ __global__ __launch_bounds__(NT) void k_step_psv(const StepArgs a) {
     const int i = m - 2;
     if (i >= i0) {
       const bool xp_ok = i >= 1 && i < nx - 2;
-      const float dvxdx = xp_ok ? dplus(vx[0], vx[1], vx[2], vx[3], h8x) : 0.0f;
-      const float dvzdx = xp_ok ? dplus(vz[0], vz[1], vz[2], vz[3], h8x) : 0.0f;
+      const float dvxdx = xp_ok ? dplus(vx[s0], vx[s1], vx[s2], vx[s3], h8x, r8x) : 0.0f;
+      const float dvzdx = xp_ok ? dplus(vz[s0], vz[s1], vz[s2], vz[s3], h8x, r8x) : 0.0f;
       const float l2m = lam + 2.0f * mu;
-      const float nsxx = sxx[0] + dt * (l2m * dvxdx + lam * dvzdz[0]);
-      const float nszz = szz[0] + dt * (l2m * dvzdz[0] + lam * dvxdx);
-      const float nsxz = sxz[0] + dt * mu * (dvxdz[0] + dvzdx);
-      if (jst) {
-        osxx[i * nz + j] = nsxx;
-        oszz[i * nz + j] = nszz;
-        osxz[i * nz + j] = nsxz;
+      const float nsxx = sxx[s0] + dt * (l2m * dvxdx + lam * dvzdz[s0]);
+      const float nszz = szz[s0] + dt * (l2m * dvzdz[s0] + lam * dvxdx);
+      const float nsxz = sxz[s0] + dt * mu * (dvxdz[s0] + dvzdx);
+      const uint32_t ro = uint32_t(i) * g.rowb;
+      if (g.jst) {
+        bstore(nsxx, rout, jc4, o_sxx + ro);
+        bstore(nszz, rout, jc4, o_szz + ro);
+        bstore(nsxz, rout, jc4, o_sxz + ro);
+        if constexpr (AUX) {
+          bstore(dvxdx, raux, jc4, A_EXX * npt4 + ro);
+          bstore(dvzdz[s0], raux, jc4, A_EZZ * npt4 + ro);
+          bstore(dvxdz[s0] + dvzdx, raux, jc4, A_EXZ * npt4 + ro);
+        }
       }
       if constexpr (SPIN) {
-        const float dvycdx = xp_ok ? dplus(vyc[0], vyc[1], vyc[2], vyc[3], h8x) : 0.0f;
-        const float nsyy = syy[0] + 2.0f * dt * nu * (-vyc[1] - 0.5f * (dvzdx - dvxdz[0]));
+        const float dvycdx = xp_ok ? dplus(vyc[s0], vyc[s1], vyc[s2], vyc[s3], h8x, r8x) : 0.0f;
+        const float nsyy = syy[s0] + 2.0f * dt * nu * (-vyc[s1] - 0.5f * (dvzdx - dvxdz[s0]));
         const float cc = dt * (muc + nuc);
-        const float nsyx = syx[0] + cc * dvycdx;
-        const float nsyz = syz[0] + cc * dvycdz[0];
-        if (jst) {
-          osyy[i * nz + j] = nsyy;
-          osyx[i * nz + j] = nsyx;
-          osyz[i * nz + j] = nsyz;
+        const float nsyx = syx[s0] + cc * dvycdx;
+        const float nsyz = syz[s0] + cc * dvycdz[s0];
+        if (g.jst) {
+          bstore(nsyy, rout, jc4, o_syy + ro);
+          bstore(nsyx, rout, jc4, o_syx + ro);
+          bstore(nsyz, rout, jc4, o_syz + ro);
         }
       }
     }
-    par ^= 1;
+  };
+
+  const int mlast = i1 + 1;
+  for (int m = i0 - 1;;) {
+    body(Phase<0>{}, m);
+    if (++m > mlast) break;
+    body(Phase<1>{}, m);
+    if (++m > mlast) break;
+    body(Phase<2>{}, m);
+    if (++m > mlast) break;
+    body(Phase<3>{}, m);
+    if (++m > mlast) break;
   }
   if (a.advance) advance_clock(a.clock, it);
+}
+
+// ------------------------------------------------------ adjoint time steps
+// Exact transpose of one forward step (SH or P-SV), run in reverse time.
+// With the forward step
+//   v' = B (v + dt (D- sigma + f) / rho),   sigma' = sigma + dt C(lambda, mu) D+ v'
+// and the traces d^n = u^{n+1} = u^n + dt v' at the receivers, the adjoint
+// step maps (vbar', sigbar') to (vbar, sigbar):
+//   vbar''  = vbar' + dt A^n (receivers) + (D+)^T t,     t = dt C sigbar'
+//   g       = B vbar'',   vbar = g,   q = dt g / rho
+//   sigbar  = sigbar' + (D-)^T q
+//   K_rho  += -dt g . ds / rho^2             (ds = D- sigma + f of the forward step)
+//   K_mu, K_lambda += dt sigbar' . dC/dm . strain(v')
+// with (D+)^T w = -D-(w~) and (D-)^T w = -D+(w^), where w~ / w^ are w zeroed
+// outside the rows/columns on which the forward operator is active (D+: 1 <=
+// i < n-2, D-: 2 <= i < n-2).  ds and the strains come from the aux fields
+// of the recomputed forward step.  Same marching/tiling as the forward step.
+__device__ __forceinline__ float ndm(float fm2, float fm1, float f0, float fp1, float h8, float r8) {
+  return -div_const(9.0f * (f0 - fm1) - (fp1 - fm2), h8, r8);
+}
+__device__ __forceinline__ float ndp(float fm1, float f0, float fp1, float fp2, float h8, float r8) {
+  return -div_const(9.0f * (fp1 - f0) - (fp2 - fm1), h8, r8);
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_adj_sh(const StepArgs a) {
+  __shared__ float lds_t[2][NT];
+  __shared__ float lds_q[2][NT];
+  const TileGeom g = tile_geom<NT>(a);
+  const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
+  const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int64_t so = int64_t(g.shot) * a.shot_floats;
+  const uint32_t sbytes = 4u * uint32_t(a.shot_floats);
+  const auto rin = make_rsrc(a.state_in + so, sbytes);
+  const auto rout = make_rsrc(a.state_out + so, sbytes);
+  const uint32_t npt4 = 4u * uint32_t(a.npt);
+  const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
+  const auto raux =
+      make_rsrc(a.aux + (int64_t(it - a.aux_base) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt), NAUX_SH * npt4);
+  const auto rk = make_rsrc(a.kgrad + int64_t(g.shot) * NK * int64_t(a.npt), NK * npt4);
+  const uint32_t o_vy = 4u * a.slot_off[S_VY], o_sxy = 4u * a.slot_off[S_SXY], o_szy = 4u * a.slot_off[S_SZY];
+  const uint32_t m_rho = MF_RHO * npt4, m_mu = MF_MU * npt4;
+  const uint32_t jc4 = g.jc4;
+  const int jc = int(jc4 >> 2);
+  const float bzb = a.bzb[jc], bzt = a.bzt[jc];
+  const float dt = a.dt, h8x = a.h8x, h8z = a.h8z, r8x = a.r8x, r8z = a.r8z;
+  const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
+  const bool zt = j >= 1 && j < nz - 2, zq = j >= 2 && j < nz - 2;
+  auto xt = [&](int r) { return r >= 1 && r < nx - 2; };
+  auto xq = [&](int r) { return r >= 2 && r < nx - 2; };
+  const int tps = a.nstrips * a.nsegs;
+  int ip = a.inj_off[g.shot * tps + g.tile];
+  const int ie = a.inj_off[g.shot * tps + g.tile + 1];
+
+  // rings (as in the forward): sigbar rows m-2..m+1 and their x-masked t;
+  // q (x-masked) rows m-3..m; -D+z(q^) rows m-2..m
+  float sbx[4], sbz[4], tx[4], q[4] = {0, 0, 0, 0}, dzq[4] = {0, 0, 0, 0};
+  float tz_cur = 0.0f;    // z-masked t_z of row m
+  for (int k = 0; k < 3; ++k) {
+    const int r = i0 - 3 + k;
+    const uint32_t ro = g.row(r);
+    sbx[k] = bload(rin, jc4, o_sxy + ro);
+    sbz[k] = bload(rin, jc4, o_szy + ro);
+    const float mu = bload(rmod, jc4, m_mu + ro);
+    tx[k] = xt(r) ? dt * mu * sbx[k] : 0.0f;
+    if (k == 2) tz_cur = zt ? dt * mu * sbz[k] : 0.0f;
+  }
+  sbx[3] = sbz[3] = tx[3] = 0.0f;
+  float pf_sbx = bload(rin, jc4, o_sxy + g.row(i0)), pf_sbz = bload(rin, jc4, o_szy + g.row(i0));
+  float pf_mu = bload(rmod, jc4, m_mu + g.row(i0));
+  float pf_vb = bload(rin, jc4, o_vy + g.row(i0 - 1));
+  float pf_rho = bload(rmod, jc4, m_rho + g.row(i0 - 1));
+  float pf_ds = bload(raux, jc4, A_DSY * npt4 + g.row(i0 - 1));
+  float pf_ex = bload(raux, jc4, A_EXY * npt4 + g.row(i0 - 3));
+  float pf_ez = bload(raux, jc4, A_EZY * npt4 + g.row(i0 - 3));
+
+  auto body = [&](auto phase, int m) __attribute__((always_inline)) {
+    constexpr int R = decltype(phase)::value;
+    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
+    // row m+1 enters the rings
+    sbx[s3] = pf_sbx;
+    sbz[s3] = pf_sbz;
+    const float mu1 = pf_mu;
+    tx[s3] = xt(m + 1) ? dt * mu1 * pf_sbx : 0.0f;
+    const float tz_next = zt ? dt * mu1 * pf_sbz : 0.0f;
+    const float vb_o = pf_vb, rho = pf_rho, ds = pf_ds, ex = pf_ex, ez = pf_ez;
+    {
+      const uint32_t r2 = g.row(m + 2), r1 = g.row(m + 1), rm1 = g.row(m - 1);
+      pf_sbx = bload(rin, jc4, o_sxy + r2);
+      pf_sbz = bload(rin, jc4, o_szy + r2);
+      pf_mu = bload(rmod, jc4, m_mu + r2);
+      pf_vb = bload(rin, jc4, o_vy + r1);
+      pf_rho = bload(rmod, jc4, m_rho + r1);
+      pf_ds = bload(raux, jc4, A_DSY * npt4 + r1);
+      pf_ex = bload(raux, jc4, A_EXY * npt4 + rm1);
+      pf_ez = bload(raux, jc4, A_EZY * npt4 + rm1);
+    }
+    const float bxm = a.bx[min(max(m, 0), nx - 1)];
+
+    // adjoint "velocity" at row m
+    lds_t[par][t] = tz_cur;
+    __syncthreads();
+    float vb = vb_o;
+    while (ip < ie) {
+      const int4 e = a.inj[ip];
+      if (e.x > m) break;
+      if (e.x == m && e.y == j) vb = vb + a.val_y[int64_t(e.z) * a.nt_val + it];
+      ++ip;
+    }
+    vb = vb + (ndm(tx[s0], tx[s1], tx[s2], tx[s3], h8x, r8x) +
+               ndm(lds_t[par][tm2], lds_t[par][tm1], tz_cur, lds_t[par][tp1], h8z, r8z));
+    const float b = (bxm * bzb) * bzt;
+    const float gv = b * vb;
+    const bool own_m = g.jst && m >= i0 && m < i1;
+    if (own_m) {
+      const uint32_t ro = uint32_t(m) * g.rowb;
+      bstore(gv, rout, jc4, o_vy + ro);
+      const uint32_t ko = K_RHO * npt4 + ro;
+      bstore(bload(rk, jc4, ko) + (-dt * gv * ds / (rho * rho)), rk, jc4, ko);
+    }
+    const float qv = dt * gv / rho;
+    q[s3] = xq(m) ? qv : 0.0f;
+    const float qz = zq ? qv : 0.0f;
+    lds_q[par][t] = qz;
+    __syncthreads();
+    dzq[s2] = ndp(lds_q[par][tm1], qz, lds_q[par][tp1], lds_q[par][tp2], h8z, r8z);
+    tz_cur = tz_next;
+
+    // adjoint "stress" at row i = m-2
+    const int i = m - 2;
+    if (i >= i0 && g.jst) {
+      const uint32_t ro = uint32_t(i) * g.rowb;
+      const float nsx = sbx[s0] + ndp(q[s0], q[s1], q[s2], q[s3], h8x, r8x);
+      const float nsz = sbz[s0] + dzq[s0];
+      bstore(nsx, rout, jc4, o_sxy + ro);
+      bstore(nsz, rout, jc4, o_szy + ro);
+      const uint32_t ko = K_MU * npt4 + ro;
+      bstore(bload(rk, jc4, ko) + dt * (sbx[s0] * ex + sbz[s0] * ez), rk, jc4, ko);
+    }
+  };
+
+  const int mlast = i1 + 1;
+  for (int m = i0 - 1;;) {
+    body(Phase<0>{}, m);
+    if (++m > mlast) break;
+    body(Phase<1>{}, m);
+    if (++m > mlast) break;
+    body(Phase<2>{}, m);
+    if (++m > mlast) break;
+    body(Phase<3>{}, m);
+    if (++m > mlast) break;
+  }
+  if (a.advance) advance_clock(a.clock, it, -1);
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_adj_psv(const StepArgs a) {
+  __shared__ float lds_t[2][2][NT];   // z-masked t_xz, t_zz of row m
+  __shared__ float lds_q[2][2][NT];   // z-masked qx, qz of row m
+  const TileGeom g = tile_geom<NT>(a);
+  const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
+  const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int64_t so = int64_t(g.shot) * a.shot_floats;
+  const uint32_t sbytes = 4u * uint32_t(a.shot_floats);
+  const auto rin = make_rsrc(a.state_in + so, sbytes);
+  const auto rout = make_rsrc(a.state_out + so, sbytes);
+  const uint32_t npt4 = 4u * uint32_t(a.npt);
+  const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
+  const auto raux = make_rsrc(a.aux + (int64_t(it - a.aux_base) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt),
+                              NAUX_PSV * npt4);
+  const auto rk = make_rsrc(a.kgrad + int64_t(g.shot) * NK * int64_t(a.npt), NK * npt4);
+  const uint32_t o_vx = 4u * a.slot_off[S_VX], o_vz = 4u * a.slot_off[S_VZ];
+  const uint32_t o_sxx = 4u * a.slot_off[S_SXX], o_szz = 4u * a.slot_off[S_SZZ], o_sxz = 4u * a.slot_off[S_SXZ];
+  const uint32_t m_rho = MF_RHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4;
+  const uint32_t jc4 = g.jc4;
+  const int jc = int(jc4 >> 2);
+  const float bzb = a.bzb[jc], bzt = a.bzt[jc];
+  const float dt = a.dt, h8x = a.h8x, h8z = a.h8z, r8x = a.r8x, r8z = a.r8z;
+  const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
+  const bool zt = j >= 1 && j < nz - 2, zq = j >= 2 && j < nz - 2;
+  auto xt = [&](int r) { return r >= 1 && r < nx - 2; };
+  auto xq = [&](int r) { return r >= 2 && r < nx - 2; };
+  const int tps = a.nstrips * a.nsegs;
+  int ip = a.inj_off[g.shot * tps + g.tile];
+  const int ie = a.inj_off[g.shot * tps + g.tile + 1];
+
+  float sbxx[4], sbzz[4], sbxz[4], txx[4], txz[4];
+  float qx[4] = {0, 0, 0, 0}, qz[4] = {0, 0, 0, 0}, dzqx[4] = {0, 0, 0, 0}, dzqz[4] = {0, 0, 0, 0};
+  float tzxz_cur = 0.0f, tzzz_cur = 0.0f;   // z-masked t_xz, t_zz of row m
+  auto tvals = [&](float lam, float mu, float sxx, float szz, float sxz, float& oxx, float& ozz, float& oxz) {
+    const float l2m = lam + 2.0f * mu;
+    oxx = dt * (l2m * sxx + lam * szz);
+    ozz = dt * (lam * sxx + l2m * szz);
+    oxz = dt * mu * sxz;
+  };
+  for (int k = 0; k < 3; ++k) {
+    const int r = i0 - 3 + k;
+    const uint32_t ro = g.row(r);
+    sbxx[k] = bload(rin, jc4, o_sxx + ro);
+    sbzz[k] = bload(rin, jc4, o_szz + ro);
+    sbxz[k] = bload(rin, jc4, o_sxz + ro);
+    float a_xx, a_zz, a_xz;
+    tvals(bload(rmod, jc4, m_lam + ro), bload(rmod, jc4, m_mu + ro), sbxx[k], sbzz[k], sbxz[k], a_xx, a_zz, a_xz);
+    txx[k] = xt(r) ? a_xx : 0.0f;
+    txz[k] = xt(r) ? a_xz : 0.0f;
+    if (k == 2) { tzxz_cur = zt ? a_xz : 0.0f; tzzz_cur = zt ? a_zz : 0.0f; }
+  }
+  sbxx[3] = sbzz[3] = sbxz[3] = txx[3] = txz[3] = 0.0f;
+  float pf_sxx = bload(rin, jc4, o_sxx + g.row(i0)), pf_szz = bload(rin, jc4, o_szz + g.row(i0)),
+        pf_sxz = bload(rin, jc4, o_sxz + g.row(i0));
+  float pf_lam = bload(rmod, jc4, m_lam + g.row(i0)), pf_mu = bload(rmod, jc4, m_mu + g.row(i0));
+  float pf_vbx = bload(rin, jc4, o_vx + g.row(i0 - 1)), pf_vbz = bload(rin, jc4, o_vz + g.row(i0 - 1));
+  float pf_rho = bload(rmod, jc4, m_rho + g.row(i0 - 1));
+  float pf_dsx = bload(raux, jc4, A_DSX * npt4 + g.row(i0 - 1)), pf_dsz = bload(raux, jc4, A_DSZ * npt4 + g.row(i0 - 1));
+  float pf_exx = bload(raux, jc4, A_EXX * npt4 + g.row(i0 - 3)), pf_ezz = bload(raux, jc4, A_EZZ * npt4 + g.row(i0 - 3)),
+        pf_exz = bload(raux, jc4, A_EXZ * npt4 + g.row(i0 - 3));
+
+  auto body = [&](auto phase, int m) __attribute__((always_inline)) {
+    constexpr int R = decltype(phase)::value;
+    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
+    sbxx[s3] = pf_sxx; sbzz[s3] = pf_szz; sbxz[s3] = pf_sxz;     // row m+1
+    float n_xx, n_zz, n_xz;
+    tvals(pf_lam, pf_mu, pf_sxx, pf_szz, pf_sxz, n_xx, n_zz, n_xz);
+    txx[s3] = xt(m + 1) ? n_xx : 0.0f;
+    txz[s3] = xt(m + 1) ? n_xz : 0.0f;
+    const float tzxz_next = zt ? n_xz : 0.0f, tzzz_next = zt ? n_zz : 0.0f;
+    const float vbx_o = pf_vbx, vbz_o = pf_vbz, rho = pf_rho, dsx = pf_dsx, dsz = pf_dsz;
+    const float exx = pf_exx, ezz = pf_ezz, exz = pf_exz;
+    {
+      const uint32_t r2 = g.row(m + 2), r1 = g.row(m + 1), rm1 = g.row(m - 1);
+      pf_sxx = bload(rin, jc4, o_sxx + r2);
+      pf_szz = bload(rin, jc4, o_szz + r2);
+      pf_sxz = bload(rin, jc4, o_sxz + r2);
+      pf_lam = bload(rmod, jc4, m_lam + r2);
+      pf_mu = bload(rmod, jc4, m_mu + r2);
+      pf_vbx = bload(rin, jc4, o_vx + r1);
+      pf_vbz = bload(rin, jc4, o_vz + r1);
+      pf_rho = bload(rmod, jc4, m_rho + r1);
+      pf_dsx = bload(raux, jc4, A_DSX * npt4 + r1);
+      pf_dsz = bload(raux, jc4, A_DSZ * npt4 + r1);
+      pf_exx = bload(raux, jc4, A_EXX * npt4 + rm1);
+      pf_ezz = bload(raux, jc4, A_EZZ * npt4 + rm1);
+      pf_exz = bload(raux, jc4, A_EXZ * npt4 + rm1);
+    }
+    const float bxm = a.bx[min(max(m, 0), nx - 1)];
+
+    // adjoint "velocity" at row m
+    lds_t[par][0][t] = tzxz_cur;
+    lds_t[par][1][t] = tzzz_cur;
+    __syncthreads();
+    float vbx = vbx_o, vbz = vbz_o;
+    while (ip < ie) {
+      const int4 e = a.inj[ip];
+      if (e.x > m) break;
+      if (e.x == m && e.y == j) {
+        vbx = vbx + a.val_x[int64_t(e.z) * a.nt_val + it];
+        vbz = vbz + a.val_z[int64_t(e.z) * a.nt_val + it];
+      }
+      ++ip;
+    }
+    vbx = vbx + (ndm(txx[s0], txx[s1], txx[s2], txx[s3], h8x, r8x) +
+                 ndm(lds_t[par][0][tm2], lds_t[par][0][tm1], tzxz_cur, lds_t[par][0][tp1], h8z, r8z));
+    vbz = vbz + (ndm(lds_t[par][1][tm2], lds_t[par][1][tm1], tzzz_cur, lds_t[par][1][tp1], h8z, r8z) +
+                 ndm(txz[s0], txz[s1], txz[s2], txz[s3], h8x, r8x));
+    const float b = (bxm * bzb) * bzt;
+    const float gx = b * vbx, gz = b * vbz;
+    if (g.jst && m >= i0 && m < i1) {
+      const uint32_t ro = uint32_t(m) * g.rowb;
+      bstore(gx, rout, jc4, o_vx + ro);
+      bstore(gz, rout, jc4, o_vz + ro);
+      const uint32_t ko = K_RHO * npt4 + ro;
+      bstore(bload(rk, jc4, ko) + (-dt * (gx * dsx + gz * dsz) / (rho * rho)), rk, jc4, ko);
+    }
+    const float qxv = dt * gx / rho, qzv = dt * gz / rho;
+    qx[s3] = xq(m) ? qxv : 0.0f;
+    qz[s3] = xq(m) ? qzv : 0.0f;
+    const float qxz = zq ? qxv : 0.0f, qzz = zq ? qzv : 0.0f;
+    lds_q[par][0][t] = qxz;
+    lds_q[par][1][t] = qzz;
+    __syncthreads();
+    dzqx[s2] = ndp(lds_q[par][0][tm1], qxz, lds_q[par][0][tp1], lds_q[par][0][tp2], h8z, r8z);
+    dzqz[s2] = ndp(lds_q[par][1][tm1], qzz, lds_q[par][1][tp1], lds_q[par][1][tp2], h8z, r8z);
+    tzxz_cur = tzxz_next;
+    tzzz_cur = tzzz_next;
+
+    // adjoint "stress" at row i = m-2
+    const int i = m - 2;
+    if (i >= i0 && g.jst) {
+      const uint32_t ro = uint32_t(i) * g.rowb;
+      const float nxx = sbxx[s0] + ndp(qx[s0], qx[s1], qx[s2], qx[s3], h8x, r8x);
+      const float nzz = sbzz[s0] + dzqz[s0];
+      const float nxz = sbxz[s0] + (dzqx[s0] + ndp(qz[s0], qz[s1], qz[s2], qz[s3], h8x, r8x));
+      bstore(nxx, rout, jc4, o_sxx + ro);
+      bstore(nzz, rout, jc4, o_szz + ro);
+      bstore(nxz, rout, jc4, o_sxz + ro);
+      const uint32_t kl = K_LAM * npt4 + ro, km = K_MU * npt4 + ro;
+      bstore(bload(rk, jc4, kl) + dt * (sbxx[s0] + sbzz[s0]) * (exx + ezz), rk, jc4, kl);
+      bstore(bload(rk, jc4, km) + dt * (2.0f * sbxx[s0] * exx + 2.0f * sbzz[s0] * ezz + sbxz[s0] * exz), rk, jc4, km);
+    }
+  };
+
+  const int mlast = i1 + 1;
+  for (int m = i0 - 1;;) {
+    body(Phase<0>{}, m);
+    if (++m > mlast) break;
+    body(Phase<1>{}, m);
+    if (++m > mlast) break;
+    body(Phase<2>{}, m);
+    if (++m > mlast) break;
+    body(Phase<3>{}, m);
+    if (++m > mlast) break;
+  }
+  if (a.advance) advance_clock(a.clock, it, -1);
+}
+
+// adjoint sources: one thread per (shot, component, receiver) scans time
+// backwards, A^n = sum_{k>=n} 2 w_k^2 (d^k - obs^k) (double accumulation),
+// writes dt * A^n into adj[comp][shot*nrec + r][n]; per-shot misfit partials
+// are reduced in a fixed order by a second kernel.
+__global__ void k_adjsrc(const float* __restrict__ obs_syn, const float* __restrict__ obs_ref,
+                         const float* __restrict__ taper, float* __restrict__ adj, double* __restrict__ part,
+                         int nshots, int ncomp, const int* __restrict__ comps, int nrec, int nt, float dt) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = nshots * ncomp * nrec;
+  if (g >= total) return;
+  const int r = g % nrec, c = (g / nrec) % ncomp, s = g / (nrec * ncomp);
+  const float* syn = obs_syn + ((int64_t(s) * 5 + comps[c]) * nrec + r) * nt;
+  const float* ref = obs_ref + ((int64_t(s) * ncomp + c) * nrec + r) * nt;
+  float* out = adj + (int64_t(c) * nshots * nrec + int64_t(s) * nrec + r) * nt;
+  double acc = 0.0, mis = 0.0;
+  for (int n = nt - 1; n >= 0; --n) {
+    const double w = taper[n];
+    const double d = double(syn[n]) - double(ref[n]);
+    acc += 2.0 * w * w * d;
+    mis += (w * d) * (w * d);
+    out[n] = float(double(dt) * acc);
+  }
+  part[g] = mis;
+}
+__global__ void k_misfit_shots(const double* __restrict__ part, double* __restrict__ mis, int nshots, int per) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nshots) return;
+  double acc = 0.0;
+  for (int q = 0; q < per; ++q) acc += part[int64_t(s) * per + q];
+  mis[s] = acc;
+}
+__global__ void k_reduce_shots(const float* __restrict__ kshot, float* __restrict__ kout, int nshots, int64_t n,
+                               int64_t shot_stride) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  float acc = kout[k];
+  for (int s = 0; s < nshots; ++s) acc = acc + kshot[int64_t(s) * shot_stride + k];
+  kout[k] = acc;
 }
 
 // ------------------------------------------------ plain SH imaging condition
 // div_uy, div_fw, interaction_muy (fd2d.wgsl:501-509, 571-588)
 __global__ void k_imaging_sh(const float* __restrict__ va, const float* __restrict__ vf, float* __restrict__ kmu,
-                             int nx, int nz, float h8x, float h8z, float ndt) {
+                             int nx, int nz, float h8x, float h8z, float r8x, float r8z, float ndt) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= nz) return;
@@ -447,12 +883,12 @@ __global__ void k_imaging_sh(const float* __restrict__ va, const float* __restri
   const bool xp = i >= 1 && i < nx - 2, zp = j >= 1 && j < nz - 2;
   float ax = 0.0f, az = 0.0f, fx = 0.0f, fz = 0.0f;
   if (xp) {
-    ax = dplus(va[k - nz], va[k], va[k + nz], va[k + 2 * nz], h8x);
-    fx = dplus(vf[k - nz], vf[k], vf[k + nz], vf[k + 2 * nz], h8x);
+    ax = dplus(va[k - nz], va[k], va[k + nz], va[k + 2 * nz], h8x, r8x);
+    fx = dplus(vf[k - nz], vf[k], vf[k + nz], vf[k + 2 * nz], h8x, r8x);
   }
   if (zp) {
-    az = dplus(va[k - 1], va[k], va[k + 1], va[k + 2], h8z);
-    fz = dplus(vf[k - 1], vf[k], vf[k + 1], vf[k + 2], h8z);
+    az = dplus(va[k - 1], va[k], va[k + 1], va[k + 2], h8z, r8z);
+    fz = dplus(vf[k - 1], vf[k], vf[k + 1], vf[k + 2], h8z, r8z);
   }
   kmu[k] = kmu[k] + (ax * fx + az * fz) * ndt;
 }
@@ -548,15 +984,21 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
 }
 
 template <int NT>
-hipError_t launch_sh_nt(const StepArgs& a, const Tiling& t, hipStream_t s) {
+hipError_t launch_sh_nt(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
   const int nblk = a.nshots * t.nstrips * t.nsegs;
-  hipLaunchKernelGGL(k_step_sh<NT>, dim3(nblk), dim3(NT), 0, s, a);
+  if (aux) hipLaunchKernelGGL((k_step_sh<NT, true>), dim3(nblk), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL((k_step_sh<NT, false>), dim3(nblk), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 template <int NT, bool SPIN>
-hipError_t launch_psv_nt(const StepArgs& a, const Tiling& t, hipStream_t s) {
+hipError_t launch_psv_nt(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
   const int nblk = a.nshots * t.nstrips * t.nsegs;
-  hipLaunchKernelGGL((k_step_psv<NT, SPIN>), dim3(nblk), dim3(NT), 0, s, a);
+  if constexpr (SPIN) {
+    hipLaunchKernelGGL((k_step_psv<NT, true, false>), dim3(nblk), dim3(NT), 0, s, a);
+  } else {
+    if (aux) hipLaunchKernelGGL((k_step_psv<NT, false, true>), dim3(nblk), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_step_psv<NT, false, false>), dim3(nblk), dim3(NT), 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -564,27 +1006,70 @@ hipError_t launch_psv_nt(const StepArgs& a, const Tiling& t, hipStream_t s) {
 
 bool tiling_supported(int nt) { return nt == 64 || nt == 128 || nt == 256; }
 
-hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s) {
+hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
   switch (t.nt_threads) {
-    case 64: return launch_sh_nt<64>(a, t, s);
-    case 128: return launch_sh_nt<128>(a, t, s);
-    case 256: return launch_sh_nt<256>(a, t, s);
+    case 64: return launch_sh_nt<64>(a, t, s, aux);
+    case 128: return launch_sh_nt<128>(a, t, s, aux);
+    case 256: return launch_sh_nt<256>(a, t, s, aux);
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStream_t s) {
+hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStream_t s, bool aux) {
+  if (spin && aux) return hipErrorInvalidValue;
   switch (t.nt_threads) {
-    case 64: return spin ? launch_psv_nt<64, true>(a, t, s) : launch_psv_nt<64, false>(a, t, s);
-    case 128: return spin ? launch_psv_nt<128, true>(a, t, s) : launch_psv_nt<128, false>(a, t, s);
-    case 256: return spin ? launch_psv_nt<256, true>(a, t, s) : launch_psv_nt<256, false>(a, t, s);
+    case 64: return spin ? launch_psv_nt<64, true>(a, t, s, aux) : launch_psv_nt<64, false>(a, t, s, aux);
+    case 128: return spin ? launch_psv_nt<128, true>(a, t, s, aux) : launch_psv_nt<128, false>(a, t, s, aux);
+    case 256: return spin ? launch_psv_nt<256, true>(a, t, s, aux) : launch_psv_nt<256, false>(a, t, s, aux);
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_adj_sh(const StepArgs& a, const Tiling& t, hipStream_t s) {
+  const int nblk = a.nshots * t.nstrips * t.nsegs;
+  switch (t.nt_threads) {
+    case 64: hipLaunchKernelGGL(k_adj_sh<64>, dim3(nblk), dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL(k_adj_sh<128>, dim3(nblk), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL(k_adj_sh<256>, dim3(nblk), dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_adj_psv(const StepArgs& a, const Tiling& t, hipStream_t s) {
+  const int nblk = a.nshots * t.nstrips * t.nsegs;
+  switch (t.nt_threads) {
+    case 64: hipLaunchKernelGGL(k_adj_psv<64>, dim3(nblk), dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL(k_adj_psv<128>, dim3(nblk), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL(k_adj_psv<256>, dim3(nblk), dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_adjsrc(const float* obs_syn, const float* obs_ref, const float* taper, float* adj, double* misfit,
+                         int nshots, int ncomp, const int* comps, int nrec, int nt, float dt, hipStream_t s) {
+  const int total = nshots * ncomp * nrec;
+  if (total > 0) {
+    // partials live after the per-shot misfits in the same buffer
+    double* part = misfit + nshots;
+    hipLaunchKernelGGL(k_adjsrc, dim3((total + 127) / 128), dim3(128), 0, s, obs_syn, obs_ref, taper, adj, part,
+                       nshots, ncomp, comps, nrec, nt, dt);
+    hipLaunchKernelGGL(k_misfit_shots, dim3((nshots + 63) / 64), dim3(64), 0, s, part, misfit, nshots, ncomp * nrec);
+  }
+  return hipGetLastError();
+}
+hipError_t launch_reduce_shots(const float* kshot, float* kout, int nshots, int64_t n, int64_t shot_stride,
+                               hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_reduce_shots, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, kshot, kout, nshots, n,
+                     shot_stride);
+  return hipGetLastError();
 }
 
 hipError_t launch_imaging_sh(const float* va, const float* vf, float* kmu, int nx, int nz, float h8x, float h8z,
                              float ndt, hipStream_t s) {
-  hipLaunchKernelGGL(k_imaging_sh, dim3((nz + 255) / 256, nx), dim3(256), 0, s, va, vf, kmu, nx, nz, h8x, h8z, ndt);
+  const float r8x = 1.0f / h8x, r8z = 1.0f / h8z;
+  hipLaunchKernelGGL(k_imaging_sh, dim3((nz + 255) / 256, nx), dim3(256), 0, s, va, vf, kmu, nx, nz, h8x, h8z, r8x,
+                     r8z, ndt);
   return hipGetLastError();
 }
 hipError_t launch_gsum(const float* sumx, const float* sumz, float* gsum, int nx, int nz, hipStream_t s) {

@@ -1,0 +1,111 @@
+"""Differentiable gradient (K_rho, K_mu, K_lambda; no counterpart in the
+reference, parity with it is unpinned).
+
+Pinned instead by:
+* the oracle's FP64 exact discrete adjoint against FP64 central finite
+  differences of the same scheme (relative step 1e-4) on tiny grids - this
+  pins the adjoint derivation (a dropped term, a wrong sign or a transposed
+  stencil fails it);
+* the GPU's FP32 checkpointed adjoint against the oracle's FP64 adjoint, and
+  its invariance to the checkpoint interval and to shot batching.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def problem(sh, nx, nz, nt, seed=0, nrec=5):
+    rng = np.random.default_rng(seed)
+    npt = nx * nz
+    model = {"rho": rng.uniform(1.0, 2.0, npt).astype(np.float32),
+             "vp": rng.uniform(2.0, 2.6, npt).astype(np.float32),
+             "vs": rng.uniform(0.8, 1.2, npt).astype(np.float32)}
+    src = [(nx // 3) * nz + nz // 2, (2 * nx // 3) * nz + nz // 3, 4 * nz + nz - 3]
+    rec = [3 * nz + 3, (nx - 3) * nz + nz - 4, (nx // 2) * nz + 2, 3 * nz + 3, (nx // 2) * nz + nz // 2][:nrec]
+    p = O.Problem(nx=nx, nz=nz, nt=nt, dx=float(np.float32(1.0)), dz=float(np.float32(1.25)),
+                  dt=float(np.float32(0.1)), sh=sh, psv=not sh, abs_left=True, abs_right=True, abs_bottom=True,
+                  abs_top=True, abs_width=5, abs_alpha=float(np.float32(0.2)), model=model, src_idx=src,
+                  rec_idx=rec)
+    stf = rng.standard_normal((3, len(src), nt)).astype(np.float32)
+    ncomp = 1 if sh else 2
+    return p, stf, ncomp
+
+
+@pytest.mark.parametrize("sh", [True, False], ids=["sh", "psv"])
+def test_oracle_adjoint_matches_fp64_finite_differences(sh):
+    p, stf, nc = problem(sh, 18, 15, 60)
+    o = O.Oracle(p, 8, stf=stf.astype(np.float64))
+    runs = [0, -1]
+    obs = np.random.default_rng(1).standard_normal((len(runs), nc, len(p.rec_idx), p.nt)) * 0.01
+    J, kr, km, kl = o.gradient(runs, obs)
+    assert J == pytest.approx(o.objective(runs, obs), rel=1e-14)
+    rng = np.random.default_rng(5)
+    params = [("RHO", kr), ("MU", km)] + ([] if sh else [("LAM", kl)])
+    for name, K in params:
+        base = o.field(O.F[name])
+        # points where the kernel is not negligible (inside the domain of dependence)
+        cand = np.flatnonzero(np.abs(K) > 1e-3 * np.abs(K).max())
+        fd, ad = [], []
+        for k in rng.choice(cand, 6, replace=False):
+            h = 1e-4 * abs(base[k])
+            f = base.copy()
+            f[k] += h
+            o.set_field(O.F[name], f)
+            jp = o.objective(runs, obs)
+            f[k] -= 2 * h
+            o.set_field(O.F[name], f)
+            jm = o.objective(runs, obs)
+            o.set_field(O.F[name], base)
+            fd.append((jp - jm) / (2 * h))
+            ad.append(K[k])
+        fd, ad = np.array(fd), np.array(ad)
+        assert np.abs(fd - ad).max() <= 1e-6 * np.abs(ad).max(), (name, fd, ad)
+    if sh:
+        assert np.all(kl == 0)
+
+
+def gpu_problem(seiswg, p, stf):
+    c = seiswg.Context.from_problem(p)
+    c.set_sources(p.src_idx, stf)
+    c.set_receivers(p.rec_idx)
+    return c
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sh", [True, False], ids=["sh", "psv"])
+@pytest.mark.parametrize("threads", [64, 256])
+def test_gpu_gradient_matches_oracle_adjoint(seiswg, sh, threads):
+    p, stf, nc = problem(sh, 46, 40, 120, seed=3)
+    runs = [0, 1, -1]
+    rng = np.random.default_rng(2)
+    obs = (rng.standard_normal((len(runs), nc, len(p.rec_idx), p.nt)) * 1e-3).astype(np.float32)
+    print(f"case sh={sh} threads={threads}", flush=True)
+    c = gpu_problem(seiswg, p, stf)
+    c.set_tiling(threads, 9)
+    mis, kr, km, kl = c.gradient(runs, obs, ckpt_every=16)
+    o = O.Oracle(p, 8, stf=stf.astype(np.float64))
+    J, okr, okm, okl = o.gradient(runs, obs.astype(np.float64))
+    per_run = [o.objective([r], obs[q:q + 1].astype(np.float64)) for q, r in enumerate(runs)]
+    np.testing.assert_allclose(mis, per_run, rtol=1e-5)
+    for name, g, w in (("K_rho", kr, okr), ("K_mu", km, okm)) + ((("K_lam", kl, okl),) if not sh else ()):
+        err = O.rel_l2(g, w)
+        print(f"{name}: rel-L2 vs FP64 oracle adjoint {err:.2e}")
+        assert err <= 1e-4, name
+    if sh:
+        assert not np.any(kl)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sh", [True, False], ids=["sh", "psv"])
+def test_gpu_gradient_independent_of_checkpoint_interval_and_batching(seiswg, sh):
+    p, stf, nc = problem(sh, 50, 44, 90, seed=4)
+    runs = [0, 1, 2, -1]
+    obs = (np.random.default_rng(7).standard_normal((len(runs), nc, len(p.rec_idx), p.nt)) * 1e-3).astype(np.float32)
+    c = gpu_problem(seiswg, p, stf)
+    ref = c.gradient(runs, obs, ckpt_every=90)
+    for ck, spl in ((7, 0), (25, 1), (11, 3)):
+        got = c.gradient(runs, obs, ckpt_every=ck, shots_per_launch=spl)
+        assert np.array_equal(got[0], ref[0])
+        for a, b in zip(got[1:], ref[1:]):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (ck, spl)

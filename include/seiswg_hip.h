@@ -147,14 +147,40 @@ SW_API int sw_adjoint_plain(sw_ctx* ctx, int32_t nruns, const int32_t* isrc, con
 SW_API int sw_gradient(sw_ctx* ctx, int32_t nshots, const int32_t* isrc, const float* obs_ref, int32_t ckpt_every,
                        int32_t shots_per_launch, double* misfit);
 
+/* Several independent problems (each its own context: grid, model, physics,
+ * sources, receivers, nt, dt, tile shape) advanced together: every time step
+ * is one launch per (physics, tile width) group covering all problems' shots,
+ * problems with fewer steps idling once done.  No counterpart in the
+ * reference (one problem per process).  All contexts must share one device;
+ * the multi object borrows them until destroyed. */
+typedef struct sw_multi sw_multi;
+SW_API int sw_multi_create(sw_ctx* const* problems, int32_t n, sw_multi** out);
+SW_API void sw_multi_destroy(sw_multi* m);
+/* Forward runs: problem q fires isrc[q][s] for its shot s (as sw_forward_batch);
+ * traces are read per problem with sw_read_obs. */
+SW_API int sw_multi_forward(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc);
+/* Gradients (as sw_gradient, all shots of a problem in one batch):
+ * obs_ref[q] is [nshots[q]][ncomp][nrec][nt] of problem q; misfit[q] may be
+ * NULL.  A common checkpoint interval; segments of all problems are aligned. */
+SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc,
+                             const float* const* obs_ref, int32_t ckpt_every, double* const* misfit);
+/* Device time (HIP events) of the last multi call and merged launches issued. */
+SW_API int sw_multi_timing(const sw_multi* m, double* ms, int64_t* launches);
+
 /* Gaussian smoothing of a kernel field in place (smooth_gradient,
  * solver.rs:1249-1274; init_gsum, gaussian_x, gaussian_z). */
 SW_API int sw_smooth(sw_ctx* ctx, int32_t field, float sigma);
 
 /* Tile shape of the fused step kernels: threads per workgroup (64, 128 or
- * 256; a workgroup owns threads-6 z-columns) and rows marched per workgroup. */
+ * 256; a workgroup owns threads-6 z-columns) and rows marched per workgroup.
+ * Without a call, each batch gets a heuristic shape sized to fill the GPU. */
 SW_API int sw_set_tiling(sw_ctx* ctx, int32_t threads, int32_t seglen);
 SW_API int sw_get_tiling(const sw_ctx* ctx, int32_t* threads, int32_t* seglen);
+/* Autotuner (no counterpart in the reference, which dispatches fixed 64-wide
+ * 1-D workgroups): times `steps` forward steps of `nshots` shots for every
+ * supported tile shape, keeps the fastest (as sw_set_tiling does) and
+ * returns it with its ms per step.  Tile shapes never change results. */
+SW_API int sw_autotune(sw_ctx* ctx, int32_t nshots, int32_t steps, int32_t* threads, int32_t* seglen, double* ms);
 
 /* Timing of the last run's time loop, from HIP events on the context stream. */
 typedef struct sw_timing {

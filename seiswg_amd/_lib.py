@@ -48,7 +48,8 @@ SNAP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.PO
 EXPORTS = ["sw_last_error", "sw_create", "sw_destroy", "sw_set_model", "sw_set_sources", "sw_set_receivers",
            "sw_forward", "sw_forward_batch", "sw_read_obs", "sw_read_field", "sw_write_field", "sw_prepare",
            "sw_advance", "sw_adjoint_plain", "sw_smooth", "sw_set_tiling", "sw_get_tiling", "sw_get_timing",
-           "sw_bench_copy", "sw_gradient",
+           "sw_bench_copy", "sw_gradient", "sw_autotune", "sw_multi_create", "sw_multi_destroy",
+           "sw_multi_forward", "sw_multi_gradient", "sw_multi_timing",
            "seiswg_run", "seiswg_ricker_table", "seiswg_infer_grid", "seiswg_grid_index"]
 
 _lib = None
@@ -81,6 +82,13 @@ def load():
     L.sw_set_tiling.argtypes = [vp, C.c_int32, C.c_int32]
     L.sw_get_tiling.argtypes = [vp, ip, ip]
     L.sw_get_timing.argtypes = [vp, C.POINTER(sw_timing)]
+    L.sw_multi_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
+    L.sw_multi_destroy.argtypes = [vp]
+    L.sw_multi_destroy.restype = None
+    L.sw_multi_forward.argtypes = [vp, ip, C.POINTER(ip)]
+    L.sw_multi_gradient.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(fp), C.c_int32, C.POINTER(dp)]
+    L.sw_multi_timing.argtypes = [vp, dp, C.POINTER(C.c_int64)]
+    L.sw_autotune.argtypes = [vp, C.c_int32, C.c_int32, ip, ip, dp]
     L.sw_gradient.argtypes = [vp, C.c_int32, ip, fp, C.c_int32, C.c_int32, dp]
     L.sw_bench_copy.argtypes = [C.c_int32, C.c_int64, C.c_int32, dp]
     L.seiswg_run.argtypes = [C.c_char_p]
@@ -89,7 +97,7 @@ def load():
     L.seiswg_grid_index.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32]
     L.seiswg_grid_index.restype = C.c_uint32
     for n in EXPORTS:
-        if n not in ("sw_destroy", "sw_last_error", "seiswg_grid_index"):
+        if n not in ("sw_destroy", "sw_last_error", "seiswg_grid_index", "sw_multi_destroy"):
             getattr(L, n).restype = C.c_int
     _lib = L
     return L
@@ -209,6 +217,12 @@ class Context:
     def set_tiling(self, threads: int, seglen: int):
         _check(self.L.sw_set_tiling(self.h, threads, seglen))
 
+    def autotune(self, nshots: int, steps: int = 10):
+        """Fastest tile shape for nshots shots: (threads, seglen, ms per step)."""
+        a, b, ms = C.c_int32(), C.c_int32(), C.c_double()
+        _check(self.L.sw_autotune(self.h, nshots, steps, C.byref(a), C.byref(b), C.byref(ms)))
+        return a.value, b.value, ms.value
+
     def tiling(self):
         a, b = C.c_int32(), C.c_int32()
         _check(self.L.sw_get_tiling(self.h, C.byref(a), C.byref(b)))
@@ -277,3 +291,53 @@ class Context:
         t = sw_timing()
         _check(self.L.sw_get_timing(self.h, C.byref(t)))
         return t
+
+
+class Multi:
+    """Several Contexts advanced together (sw_multi)."""
+
+    def __init__(self, contexts):
+        L = load()
+        self.ctxs = list(contexts)
+        arr = (C.c_void_p * len(self.ctxs))(*[c.h.value for c in self.ctxs])
+        h = C.c_void_p()
+        _check(L.sw_multi_create(arr, len(self.ctxs), C.byref(h)))
+        self.h, self.L = h, L
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sw_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _isrc(self, isrc_lists):
+        arrs = [np.ascontiguousarray(x, np.int32) for x in isrc_lists]
+        n = np.ascontiguousarray([len(a) for a in arrs], np.int32)
+        ptrs = (C.POINTER(C.c_int32) * len(arrs))(*[_iptr(a) for a in arrs])
+        return arrs, n, ptrs
+
+    def forward(self, isrc_lists):
+        arrs, n, ptrs = self._isrc(isrc_lists)
+        _check(self.L.sw_multi_forward(self.h, _iptr(n), ptrs))
+        for c, a in zip(self.ctxs, arrs):
+            c.nshots = len(a)
+
+    def gradient(self, isrc_lists, obs_refs, ckpt_every: int = 50):
+        """Returns per problem (misfit per shot, K_rho, K_mu, K_lambda)."""
+        arrs, n, ptrs = self._isrc(isrc_lists)
+        refs = [_f32(r) for r in obs_refs]
+        rp = (C.POINTER(C.c_float) * len(refs))(*[_fptr(r) for r in refs])
+        mis = [np.zeros(len(a), np.float64) for a in arrs]
+        mp = (C.POINTER(C.c_double) * len(mis))(*[m.ctypes.data_as(C.POINTER(C.c_double)) for m in mis])
+        _check(self.L.sw_multi_gradient(self.h, _iptr(n), ptrs, rp, ckpt_every, mp))
+        return [(m, c.field(F_K_RHO), c.field(F_K_MU), c.field(F_K_LAM)) for m, c in zip(mis, self.ctxs)]
+
+    def timing(self):
+        ms, n = C.c_double(), C.c_int64()
+        _check(self.L.sw_multi_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -66,6 +66,13 @@ namespace sw {
 void set_last_error(const std::string& msg) { g_err = msg; }
 }
 
+// bookkeeping of a running gradient (one batch of shots)
+struct GradPlan {
+  int K = 1, nck = 0, nb = 0, ncomp = 1, naux = 0;
+  size_t state_floats = 0;
+  std::vector<double> mis;
+};
+
 struct sw_ctx {
   sw_grid g{};
   int device = 0;
@@ -85,8 +92,10 @@ struct sw_ctx {
   std::vector<int32_t> src_k, rec_k;
   DBuf<float> stf;   // [3][nsrc][nt]
 
-  // tiling
+  // tiling: chosen per batch by a heuristic unless fixed by sw_set_tiling or
+  // sw_autotune
   Tiling tile{256, 64, 0, 0};
+  bool tile_fixed = false;
 
   // batch
   int nshots = 0;
@@ -107,7 +116,15 @@ struct sw_ctx {
   int nt_val = 0;
   bool record = true;
   bool sh_only = false;       // the adjoint workflow dispatches the SH kernels only
-  int aux_base = 0;           // step index of aux slot 0 (gradient recompute)
+  int aux_period = 1;         // checkpoint interval of the running gradient
+
+  // captured blocks of time steps (hipGraph), replayed while the launch
+  // arguments stay the same; `gen` changes whenever they may change
+  uint64_t gen = 1;
+  struct GraphEntry { uint64_t key; hipGraphExec_t exec; };
+  std::vector<GraphEntry> graphs;
+  uint64_t graph_gen = 0;
+  int graph_steps = 16;       // steps per captured block (even)
 
   // plain adjoint
   DBuf<float> kmu, k_lam, k_rho, gsum, gtmp, wtab, sumx, sumz, fwd_store, adj_src, ref, taper;
@@ -117,13 +134,16 @@ struct sw_ctx {
   // adjoint clock, receiver-injection lists, checkpoints, per-step aux
   // fields of the recomputed segment and per-shot kernels
   DBuf<float> adj[2];
-  int acur = 0;
   DBuf<int> aclock;
   DBuf<int> ainj_off;
   DBuf<int4> ainj;
   DBuf<float> ckpt, aux, kshot, adj_val, gref, gtaper;
   DBuf<double> gmis;
   DBuf<int> gcomps;
+  GradPlan gp;
+  std::vector<float> h_taper;
+  int h_comps[2] = {0, 0};
+  int h_clk[4] = {0, 0, 0, 0};
 
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -165,6 +185,22 @@ void build_taper(sw_ctx* c) {
     if (c->g.abs_bottom && j > (nz - w)) { float aw = alpha * float(w + j - nz); c->h_bzb[j] = std::exp(-aw * aw); }
     if (c->g.abs_top && (j + 1u) < w) { float aw = alpha * float(w - j - 1u); c->h_bzt[j] = std::exp(-aw * aw); }
   }
+}
+
+// Default tile shape: the widest strip and longest segment that still give
+// the GPU's 256 CUs several workgroups each (>= 1024 workgroups).
+void heuristic_tiling(sw_ctx* c, int nshots) {
+  Tiling& t = c->tile;
+  auto count = [&](int nt, int seg) {
+    const int64_t strips = (c->g.nz + (nt - 6) - 1) / (nt - 6), segs = (c->g.nx + seg - 1) / seg;
+    return int64_t(nshots) * strips * segs;
+  };
+  int nt = c->g.nz >= 400 ? 256 : c->g.nz >= 100 ? 128 : 64;
+  int seg = 128;
+  while (count(nt, seg) < 1024 && seg > 8) seg /= 2;
+  while (count(nt, seg) < 1024 && nt > 64) nt /= 2;
+  t.nt_threads = nt;
+  t.seglen = seg;
 }
 
 int check_tiling(sw_ctx* c) {
@@ -282,7 +318,9 @@ std::vector<Entry> source_entries(const sw_ctx* c, int isrc) {
 // lists (solver.rs:978-981: clear dynamic fields and traces).
 int prepare_batch(sw_ctx* c, int nshots, const std::vector<std::vector<Entry>>& inj, bool record) {
   if (!c->model_set) return fail(-2, "model not set");
+  ++c->gen;
   if (nshots < 1) return fail(-2, "nshots must be >= 1");
+  if (!c->tile_fixed) heuristic_tiling(c, nshots);
   int rc = check_tiling(c);
   if (rc) return rc;
   c->nshots = nshots;
@@ -310,7 +348,12 @@ int prepare_batch(sw_ctx* c, int nshots, const std::vector<std::vector<Entry>>& 
 
 enum StepMode { M_FWD = 0, M_FWD_AUX, M_ADJ };
 
-int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
+bool step_sh_on(const sw_ctx* c) { return uses_sh(c); }
+bool step_psv_on(const sw_ctx* c) { return uses_psv(c) && !c->sh_only; }
+
+// Launch arguments of one fused step of this context (the same for every
+// step: buffers are selected by the parity of the device clock).
+StepArgs make_args(sw_ctx* c, StepMode mode) {
   StepArgs a{};
   for (int q = 0; q < N_SLOTS; ++q) a.slot_off[q] = c->slot_index[q] < 0 ? -1 : int(c->slot_index[q] * c->npt);
   a.shot_floats = int(c->nslots * c->npt);
@@ -328,9 +371,14 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   a.record = (c->record && mode == M_FWD) ? 1 : 0;
   a.clock = c->clock.p;
   a.aux = c->aux.p;
-  a.aux_base = c->aux_base;
+  a.aux_period = c->aux_period;
   a.kgrad = c->kshot.p;
-  const bool sh = uses_sh(c), psv = uses_psv(c) && !c->sh_only, spin = uses_spin(c);
+  a.st[0] = c->state[0].p;
+  a.st[1] = c->state[1].p;
+  a.it_lo = -0x7fffffff;
+  a.it_hi = 0x7fffffff;
+  a.nblocks = c->nshots * c->tile.nstrips * c->tile.nsegs;
+  a.advance = 1;
   if (mode == M_ADJ) {
     a.inj_off = c->ainj_off.p; a.inj = c->ainj.p;
     a.val_x = c->adj_val.p;
@@ -338,38 +386,91 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     a.val_z = c->adj_val.p + size_t(c->nshots) * c->nrec * c->g.nt;
     a.nt_val = c->g.nt;
     a.clock = c->aclock.p;
-    a.advance = 1;
-    for (int s = 0; s < nsteps; ++s) {
-      a.state_in = c->adj[c->acur].p;
-      a.state_out = c->adj[c->acur ^ 1].p;
+    a.st[0] = c->adj[0].p;
+    a.st[1] = c->adj[1].p;
+  }
+  return a;
+}
+
+int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
+  StepArgs a = make_args(c, mode);
+  const bool sh = step_sh_on(c), psv = step_psv_on(c), spin = uses_spin(c);
+  const bool aux = mode == M_FWD_AUX;
+  StepArgs a_sh = a;
+  a_sh.advance = psv ? 0 : 1;   // with both physics the P-SV kernel advances the clock
+  // one time step: the fused kernel(s) of the enabled physics
+  auto enqueue = [&]() -> int {
+    if (mode == M_ADJ) {
       HIPCHK(sh ? launch_adj_sh(a, c->tile, c->stream) : launch_adj_psv(a, c->tile, c->stream));
       ++c->launches;
-      c->acur ^= 1;
+      return 0;
     }
-    return 0;
-  }
-  const bool aux = mode == M_FWD_AUX;
-  for (int s = 0; s < nsteps; ++s) {
-    a.state_in = c->state[c->cur].p;
-    a.state_out = c->state[c->cur ^ 1].p;
     if (sh) {
-      a.advance = psv ? 0 : 1;
-      HIPCHK(launch_step_sh(a, c->tile, c->stream, aux));
+      HIPCHK(launch_step_sh(a_sh, c->tile, c->stream, aux));
       ++c->launches;
     }
     if (psv) {
-      a.advance = 1;
       HIPCHK(launch_step_psv(a, c->tile, spin, c->stream, aux));
       ++c->launches;
     }
-    if (!sh && !psv) {
-      // nothing to integrate; keep the clock in step with the host
+    return 0;
+  };
+  auto count_step = [&](int n) {
+    if (mode != M_ADJ) c->cur = (c->cur + n) & 1;
+    if (mode == M_FWD) c->steps_done += n;
+  };
+  if (!sh && !psv) {
+    // nothing to integrate; keep the clock in step with the host
+    for (int s = 0; s < nsteps; ++s) {
       const int v[2] = {int(c->steps_done + 1), 0};
       HIPCHK(hipMemcpyAsync(c->clock.p, v, sizeof v, hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
+      count_step(1);
     }
-    c->cur ^= 1;
-    ++c->steps_done;
+    return 0;
+  }
+  int done = 0;
+  const int G = c->graph_steps;
+  if (G >= 2 && nsteps >= 2 * G && !getenv("SEISWG_NO_GRAPHS")) {
+    if (c->graph_gen != c->gen) {
+      for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.exec);
+      c->graphs.clear();
+      c->graph_gen = c->gen;
+    }
+    const uint64_t key = uint64_t(mode) | (uint64_t(a.record) << 4) | (uint64_t(c->sh_only) << 5) |
+                         (uint64_t(G) << 8);
+    hipGraphExec_t exec = nullptr;
+    for (auto& e : c->graphs)
+      if (e.key == key) exec = e.exec;
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      const int64_t l0 = c->launches;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      for (int q = 0; q < G && !rc; ++q) rc = enqueue();
+      hipError_t e_end = hipStreamEndCapture(c->stream, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      HIPCHK(e_end);
+      hipError_t e_inst = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHK(e_inst);
+      c->graphs.push_back({key, exec});
+      c->launches = l0;   // captured, not launched
+    }
+    const int64_t per = (mode != M_ADJ && sh && psv) ? 2 : 1;
+    for (; done + G <= nsteps; done += G) {
+      HIPCHK(hipGraphLaunch(exec, c->stream));
+      c->launches += per * G;
+      count_step(G);
+    }
+  }
+  for (; done < nsteps; ++done) {
+    int rc = enqueue();
+    if (rc) return rc;
+    count_step(1);
   }
   return 0;
 }
@@ -497,6 +598,7 @@ SW_API void sw_destroy(sw_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.exec);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -536,6 +638,7 @@ SW_API int sw_set_model(sw_ctx* c, int32_t kind, const float* const* f) {
   HIPCHK(hipMemcpyAsync(c->bzt.p, c->h_bzt.data(), c->h_bzt.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->model_set = true;
+  ++c->gen;
   return 0;
 }
 
@@ -544,6 +647,7 @@ SW_API int sw_set_sources(sw_ctx* c, int32_t nsrc, const int32_t* idx, const flo
   HIPCHK(hipSetDevice(c->device));
   c->nsrc = nsrc;
   c->src_k.assign(idx, idx + nsrc);
+  ++c->gen;
   const size_t n = size_t(3) * nsrc * size_t(c->g.nt);
   HIPCHK(c->stf.ensure(std::max<size_t>(n, 1)));
   if (n) HIPCHK(hipMemcpy(c->stf.p, stf, n * 4, hipMemcpyHostToDevice));
@@ -668,7 +772,49 @@ SW_API int sw_set_tiling(sw_ctx* c, int32_t threads, int32_t seglen) {
   if (seglen < 1) return fail(-2, "seglen must be >= 1");
   c->tile.nt_threads = threads;
   c->tile.seglen = seglen;
+  c->tile_fixed = true;
+  ++c->gen;
   c->nshots = 0;   // lists must be rebuilt
+  return 0;
+}
+
+SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* threads, int32_t* seglen, double* ms) {
+  if (!c || nshots < 1 || steps < 1) return fail(-1, "bad arguments");
+  if (!c->model_set) return fail(-2, "model not set");
+  if (c->g.nt < steps + 2) return fail(-2, "nt=%d too short to time %d steps", c->g.nt, steps);
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<int32_t> isrc(nshots);
+  for (int s = 0; s < nshots; ++s) isrc[s] = c->nsrc > 0 ? s % c->nsrc : -1;
+  double best = 1e300;
+  Tiling bt = c->tile;
+  for (int nt : {64, 128, 256}) {
+    for (int seg : {8, 16, 32, 64, 128, 256}) {
+      if (seg > 2 * c->g.nx) continue;
+      c->tile.nt_threads = nt;
+      c->tile.seglen = seg;
+      c->tile_fixed = true;
+      int rc = sw_prepare(c, nshots, isrc.data());
+      if (rc) return rc;
+      if ((rc = launch_steps(c, 2))) return rc;       // warm-up
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
+      if ((rc = launch_steps(c, steps))) return rc;
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
+      HIPCHK(hipEventSynchronize(c->ev1));
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+      if (t < best) {
+        best = t;
+        bt = c->tile;
+      }
+    }
+  }
+  c->tile = bt;
+  c->tile_fixed = true;
+  c->nshots = 0;
+  ++c->gen;
+  if (threads) *threads = bt.nt_threads;
+  if (seglen) *seglen = bt.seglen;
+  if (ms) *ms = best / steps;
   return 0;
 }
 
@@ -752,116 +898,404 @@ SW_API int sw_smooth(sw_ctx* c, int32_t field, float sigma) {
 }
 
 
-SW_API int sw_gradient(sw_ctx* c, int32_t nshots, const int32_t* isrc, const float* obs_ref, int32_t ckpt_every,
-                       int32_t shots_per_launch, double* misfit) {
-  if (!c || nshots < 1 || !isrc || !obs_ref) return fail(-1, "bad arguments");
+}  // extern "C"
+
+namespace {
+
+// ---- phases of the checkpointed adjoint gradient, shared by the one-problem
+// (sw_gradient) and the many-problem (sw_multi_gradient) drivers
+
+int grad_begin(sw_ctx* c, int ckpt_every) {
   if (uses_spin(c) || (uses_sh(c) == uses_psv(c)))
     return fail(-2, "the gradient needs exactly one of sh / psv, without spin");
   if (ckpt_every < 1) return fail(-2, "ckpt_every must be >= 1");
   if (!c->model_set) return fail(-2, "model not set");
-  HIPCHK(hipSetDevice(c->device));
   const bool sh = uses_sh(c);
-  const int nt = c->g.nt, nrec = c->nrec, ncomp = sh ? 1 : 2;
-  const int naux = sh ? NAUX_SH : NAUX_PSV;
+  const int nt = c->g.nt;
+  GradPlan& P = c->gp;
+  P.K = std::min(ckpt_every, std::max(nt, 1));
+  P.nck = (nt + P.K - 1) / P.K;
+  P.ncomp = sh ? 1 : 2;
+  P.naux = sh ? NAUX_SH : NAUX_PSV;
   const size_t npt = size_t(c->npt);
-  const int K = std::min(ckpt_every, std::max(nt, 1));
-  const int nck = (nt + K - 1) / K;
-  const int S = (shots_per_launch <= 0 || shots_per_launch > nshots) ? nshots : shots_per_launch;
-  for (int s = 0; s < nshots; ++s)
-    if (isrc[s] >= c->nsrc) return fail(-2, "source %d out of range (nsrc=%d)", isrc[s], c->nsrc);
-  // kernels summed over all shots
   for (DBuf<float>* k : {&c->k_rho, &c->kmu, &c->k_lam}) {
     HIPCHK(k->ensure(npt));
     HIPCHK(launch_fill(k->p, 0.0f, int64_t(npt), c->stream));
   }
   // misfit time taper (solver.rs:441-455), f32 on the host
-  std::vector<float> taper(std::max(nt, 1));
-  {
-    const float PI_F = 3.14159265358979323846f, dt = c->g.dt;
-    const float t_end = float(nt - 1) * dt, tw = t_end / 10.0f, t_min = tw, t_max = t_end - tw;
-    for (int n = 0; n < nt; ++n) {
-      const float t = float(n) * dt;
-      taper[n] = t <= t_min ? 0.5f + 0.5f * std::cos(PI_F * (t_min - t) / tw)
-                 : t >= t_max ? 0.5f + 0.5f * std::cos(PI_F * (t_max - t) / tw) : 1.0f;
+  c->h_taper.assign(std::max(nt, 1), 1.0f);
+  const float PI_F = 3.14159265358979323846f, dt = c->g.dt;
+  const float t_end = float(nt - 1) * dt, tw = t_end / 10.0f, t_min = tw, t_max = t_end - tw;
+  for (int n = 0; n < nt; ++n) {
+    const float t = float(n) * dt;
+    c->h_taper[n] = t <= t_min ? 0.5f + 0.5f * std::cos(PI_F * (t_min - t) / tw)
+                    : t >= t_max ? 0.5f + 0.5f * std::cos(PI_F * (t_max - t) / tw) : 1.0f;
+  }
+  HIPCHK(c->gtaper.ensure(c->h_taper.size()));
+  HIPCHK(hipMemcpyAsync(c->gtaper.p, c->h_taper.data(), c->h_taper.size() * 4, hipMemcpyHostToDevice, c->stream));
+  c->h_comps[0] = sh ? SW_OBS_Y : SW_OBS_X;
+  c->h_comps[1] = SW_OBS_Z;
+  HIPCHK(c->gcomps.ensure(2));
+  HIPCHK(hipMemcpyAsync(c->gcomps.p, c->h_comps, sizeof c->h_comps, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// a batch of nb shots: forward buffers, source and receiver injection lists,
+// checkpoint storage
+int grad_batch_setup(sw_ctx* c, int nb, const int32_t* isrc) {
+  GradPlan& P = c->gp;
+  const int nrec = c->nrec;
+  std::vector<std::vector<Entry>> inj(nb), ainj(nb);
+  for (int s = 0; s < nb; ++s) {
+    if (isrc[s] >= c->nsrc) return fail(-2, "source %d out of range (nsrc=%d)", isrc[s], c->nsrc);
+    inj[s] = source_entries(c, isrc[s]);
+    for (int r = 0; r < nrec; ++r) {
+      const uint32_t k = uint32_t(c->rec_k[r]);
+      if (int64_t(k) >= c->npt) continue;
+      ainj[s].push_back({int(k / uint32_t(c->g.nz)), int(k % uint32_t(c->g.nz)), s * nrec + r});
     }
   }
-  HIPCHK(c->gtaper.ensure(taper.size()));
-  HIPCHK(hipMemcpyAsync(c->gtaper.p, taper.data(), taper.size() * 4, hipMemcpyHostToDevice, c->stream));
-  const int comps_h[2] = {sh ? SW_OBS_Y : SW_OBS_X, SW_OBS_Z};
-  HIPCHK(c->gcomps.ensure(2));
-  HIPCHK(hipMemcpyAsync(c->gcomps.p, comps_h, sizeof comps_h, hipMemcpyHostToDevice, c->stream));
-  const size_t per_shot_obs = size_t(ncomp) * nrec * nt;
-  const size_t sstride = size_t(c->nsrc) * nt;
-  std::vector<double> mis_h(S);
+  const size_t sstride = size_t(c->nsrc) * c->g.nt;
+  c->val_x = c->stf.p; c->val_y = c->stf.p + sstride; c->val_z = c->stf.p + 2 * sstride; c->nt_val = c->g.nt;
+  int rc = prepare_batch(c, nb, inj, true);
+  if (rc) return rc;
+  if ((rc = upload_inj(c, ainj, c->ainj_off, c->ainj))) return rc;
+  P.nb = nb;
+  P.state_floats = size_t(nb) * c->nslots * size_t(c->npt);
+  HIPCHK(c->ckpt.ensure(std::max<size_t>(size_t(P.nck) * P.state_floats, 1)));
+  return 0;
+}
+
+int grad_save_ckpt(sw_ctx* c, int q) {
+  if (q > 0)
+    HIPCHK(hipMemcpyAsync(c->ckpt.p + size_t(q) * c->gp.state_floats, c->state[c->cur].p, c->gp.state_floats * 4,
+                          hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+// adjoint sources, per-shot misfits (host copy in c->gp.mis after a sync) and
+// the backward-pass buffers
+int grad_adjsrc_and_bwd_setup(sw_ctx* c, const float* obs_ref_batch) {
+  GradPlan& P = c->gp;
+  const int nb = P.nb, nrec = c->nrec, nt = c->g.nt;
+  const size_t npt = size_t(c->npt);
+  const size_t per_shot_obs = size_t(P.ncomp) * nrec * nt;
+  HIPCHK(c->gref.ensure(std::max<size_t>(size_t(nb) * per_shot_obs, 1)));
+  HIPCHK(hipMemcpyAsync(c->gref.p, obs_ref_batch, size_t(nb) * per_shot_obs * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c->adj_val.ensure(std::max<size_t>(size_t(P.ncomp) * nb * nrec * nt, 1)));
+  HIPCHK(c->gmis.ensure(size_t(nb) + size_t(nb) * P.ncomp * std::max(nrec, 1)));
+  HIPCHK(launch_adjsrc(c->obs.p, c->gref.p, c->gtaper.p, c->adj_val.p, c->gmis.p, nb, P.ncomp, c->gcomps.p, nrec, nt,
+                       c->g.dt, c->stream));
+  P.mis.assign(nb, 0.0);
+  HIPCHK(hipMemcpyAsync(P.mis.data(), c->gmis.p, size_t(nb) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c->aux.ensure(std::max<size_t>(size_t(P.K) * nb * P.naux * npt, 1)));
+  HIPCHK(c->kshot.ensure(size_t(nb) * NK * npt));
+  HIPCHK(launch_fill(c->kshot.p, 0.0f, int64_t(nb) * NK * int64_t(npt), c->stream));
+  for (int p = 0; p < 2; ++p) {
+    HIPCHK(c->adj[p].ensure(std::max<size_t>(P.state_floats, 1)));
+    HIPCHK(launch_fill(c->adj[p].p, 0.0f, int64_t(P.state_floats), c->stream));
+  }
+  HIPCHK(c->aclock.ensure(2));
+  c->aux_period = P.K;
+  ++c->gen;
+  HIPCHK(hipStreamSynchronize(c->stream));   // obs_ref_batch and P.mis are host memory
+  return 0;
+}
+
+// segment q of the backward pass: restore its checkpoint and set the clocks;
+// returns the segment's step range [n0, n1) through the arguments
+int grad_segment_setup(sw_ctx* c, int q, int& n0, int& n1) {
+  GradPlan& P = c->gp;
+  n0 = q * P.K;
+  n1 = std::min(c->g.nt, n0 + P.K);
+  c->cur = n0 & 1;
+  if (q == 0) HIPCHK(launch_fill(c->state[c->cur].p, 0.0f, int64_t(P.state_floats), c->stream));
+  else
+    HIPCHK(hipMemcpyAsync(c->state[c->cur].p, c->ckpt.p + size_t(q) * P.state_floats, P.state_floats * 4,
+                          hipMemcpyDeviceToDevice, c->stream));
+  c->h_clk[0] = n0; c->h_clk[1] = 0; c->h_clk[2] = n1 - 1; c->h_clk[3] = 0;
+  HIPCHK(hipMemcpyAsync(c->clock.p, c->h_clk, 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->aclock.p, c->h_clk + 2, 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+// sum the batch's per-shot kernels into the context's K fields (fixed order)
+int grad_reduce(sw_ctx* c) {
+  const size_t npt = size_t(c->npt);
+  const int nb = c->gp.nb;
+  HIPCHK(launch_reduce_shots(c->kshot.p + size_t(K_RHO) * npt, c->k_rho.p, nb, int64_t(npt), NK * int64_t(npt),
+                             c->stream));
+  HIPCHK(launch_reduce_shots(c->kshot.p + size_t(K_MU) * npt, c->kmu.p, nb, int64_t(npt), NK * int64_t(npt),
+                             c->stream));
+  HIPCHK(launch_reduce_shots(c->kshot.p + size_t(K_LAM) * npt, c->k_lam.p, nb, int64_t(npt), NK * int64_t(npt),
+                             c->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+SW_API int sw_gradient(sw_ctx* c, int32_t nshots, const int32_t* isrc, const float* obs_ref, int32_t ckpt_every,
+                       int32_t shots_per_launch, double* misfit) {
+  if (!c || nshots < 1 || !isrc || !obs_ref) return fail(-1, "bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  int rc = grad_begin(c, ckpt_every);
+  if (rc) return rc;
+  const GradPlan& P = c->gp;
+  const int nt = c->g.nt;
+  const int S = (shots_per_launch <= 0 || shots_per_launch > nshots) ? nshots : shots_per_launch;
+  const size_t per_shot_obs = size_t(P.ncomp) * c->nrec * nt;
   for (int b0 = 0; b0 < nshots; b0 += S) {
     const int nb = std::min(S, nshots - b0);
-    // ---- forward with checkpoints (traces recorded)
-    std::vector<std::vector<Entry>> inj(nb), ainj(nb);
-    for (int s = 0; s < nb; ++s) {
-      inj[s] = source_entries(c, isrc[b0 + s]);
-      for (int r = 0; r < nrec; ++r) {
-        const uint32_t k = uint32_t(c->rec_k[r]);
-        if (int64_t(k) >= c->npt) continue;
-        ainj[s].push_back({int(k / uint32_t(c->g.nz)), int(k % uint32_t(c->g.nz)), s * nrec + r});
-      }
+    if ((rc = grad_batch_setup(c, nb, isrc + b0))) return rc;
+    // forward with checkpoints (traces recorded)
+    for (int q = 0; q < P.nck; ++q) {
+      if ((rc = grad_save_ckpt(c, q))) return rc;
+      if ((rc = launch_steps(c, std::min(P.K, nt - q * P.K), M_FWD))) return rc;
     }
-    c->val_x = c->stf.p; c->val_y = c->stf.p + sstride; c->val_z = c->stf.p + 2 * sstride; c->nt_val = nt;
-    int rc = prepare_batch(c, nb, inj, true);
-    if (rc) return rc;
-    if ((rc = upload_inj(c, ainj, c->ainj_off, c->ainj))) return rc;
-    const size_t state_floats = size_t(nb) * c->nslots * npt;
-    HIPCHK(c->ckpt.ensure(std::max<size_t>(size_t(nck) * state_floats, 1)));
-    for (int q = 0; q < nck; ++q) {
-      if (q > 0)
-        HIPCHK(hipMemcpyAsync(c->ckpt.p + size_t(q) * state_floats, c->state[c->cur].p, state_floats * 4,
-                              hipMemcpyDeviceToDevice, c->stream));
-      if ((rc = launch_steps(c, std::min(K, nt - q * K), M_FWD))) return rc;
-    }
-    // ---- adjoint sources and misfit
-    HIPCHK(c->gref.ensure(std::max<size_t>(size_t(nb) * per_shot_obs, 1)));
-    HIPCHK(hipMemcpyAsync(c->gref.p, obs_ref + size_t(b0) * per_shot_obs, size_t(nb) * per_shot_obs * 4,
-                          hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c->adj_val.ensure(std::max<size_t>(size_t(ncomp) * nb * nrec * nt, 1)));
-    HIPCHK(c->gmis.ensure(size_t(nb) + size_t(nb) * ncomp * std::max(nrec, 1)));
-    HIPCHK(launch_adjsrc(c->obs.p, c->gref.p, c->gtaper.p, c->adj_val.p, c->gmis.p, nb, ncomp, c->gcomps.p, nrec, nt,
-                         c->g.dt, c->stream));
-    HIPCHK(hipMemcpyAsync(mis_h.data(), c->gmis.p, size_t(nb) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    // ---- backward: per segment, recompute the forward steps (aux kept), then the adjoint steps
-    HIPCHK(c->aux.ensure(std::max<size_t>(size_t(K) * nb * naux * npt, 1)));
-    HIPCHK(c->kshot.ensure(size_t(nb) * NK * npt));
-    HIPCHK(launch_fill(c->kshot.p, 0.0f, int64_t(nb) * NK * int64_t(npt), c->stream));
-    for (int p = 0; p < 2; ++p) {
-      HIPCHK(c->adj[p].ensure(std::max<size_t>(state_floats, 1)));
-      HIPCHK(launch_fill(c->adj[p].p, 0.0f, int64_t(state_floats), c->stream));
-    }
-    c->acur = 0;
-    HIPCHK(c->aclock.ensure(2));
-    for (int q = nck - 1; q >= 0; --q) {
-      const int n0 = q * K, n1 = std::min(nt, n0 + K);
-      c->cur = 0;
-      if (q == 0) HIPCHK(launch_fill(c->state[0].p, 0.0f, int64_t(state_floats), c->stream));
-      else
-        HIPCHK(hipMemcpyAsync(c->state[0].p, c->ckpt.p + size_t(q) * state_floats, state_floats * 4,
-                              hipMemcpyDeviceToDevice, c->stream));
-      const int fclk[2] = {n0, 0}, aclk[2] = {n1 - 1, 0};
-      HIPCHK(hipMemcpyAsync(c->clock.p, fclk, sizeof fclk, hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->aclock.p, aclk, sizeof aclk, hipMemcpyHostToDevice, c->stream));
-      c->aux_base = n0;
+    if ((rc = grad_adjsrc_and_bwd_setup(c, obs_ref + size_t(b0) * per_shot_obs))) return rc;
+    // backward: per segment, recompute the forward steps keeping aux, then the adjoint steps
+    for (int q = P.nck - 1; q >= 0; --q) {
+      int n0, n1;
+      if ((rc = grad_segment_setup(c, q, n0, n1))) return rc;
       if ((rc = launch_steps(c, n1 - n0, M_FWD_AUX))) return rc;
       if ((rc = launch_steps(c, n1 - n0, M_ADJ))) return rc;
-      HIPCHK(hipStreamSynchronize(c->stream));   // the clock values above live on the host stack
+      HIPCHK(hipStreamSynchronize(c->stream));   // h_clk is reused by the next segment
     }
-    // ---- sum the shots' kernels into the context's K fields (fixed order)
-    HIPCHK(launch_reduce_shots(c->kshot.p + size_t(K_RHO) * npt, c->k_rho.p, nb, int64_t(npt), NK * int64_t(npt),
-                               c->stream));
-    HIPCHK(launch_reduce_shots(c->kshot.p + size_t(K_MU) * npt, c->kmu.p, nb, int64_t(npt), NK * int64_t(npt),
-                               c->stream));
-    HIPCHK(launch_reduce_shots(c->kshot.p + size_t(K_LAM) * npt, c->k_lam.p, nb, int64_t(npt), NK * int64_t(npt),
-                               c->stream));
+    if ((rc = grad_reduce(c))) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     if (misfit)
-      for (int s = 0; s < nb; ++s) misfit[b0 + s] = mis_h[s];
+      for (int s = 0; s < nb; ++s) misfit[b0 + s] = P.mis[s];
   }
+  return 0;
+}
+
+}  // extern "C"
+
+struct sw_multi {
+  std::vector<sw_ctx*> p;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DBuf<StepArgs> args;
+  DBuf<int2> work;
+  double loop_ms = 0;
+  int64_t launches = 0;
+};
+
+namespace {
+
+// Swaps every problem's stream for the shared one while a multi call runs, so
+// all per-problem copies and the merged launches are ordered on one stream.
+struct SharedStream {
+  sw_multi* m;
+  std::vector<hipStream_t> saved;
+  explicit SharedStream(sw_multi* x) : m(x) {
+    for (sw_ctx* c : m->p) { saved.push_back(c->stream); c->stream = m->stream; }
+  }
+  ~SharedStream() {
+    for (size_t q = 0; q < m->p.size(); ++q) m->p[q]->stream = saved[q];
+  }
+};
+
+// nlaunch merged launches of `mode`; problem q takes part while its clock is
+// in [win[q].x, win[q].y).  One launch per (kernel kind, tile width) group;
+// SH groups go before P-SV groups, as the reference dispatches them.
+int multi_steps(sw_multi* m, StepMode mode, int nlaunch, const std::vector<int2>& win) {
+  struct Ent { int kind, nt; StepArgs a; };
+  std::vector<Ent> ents;
+  const bool aux = mode == M_FWD_AUX;
+  for (size_t q = 0; q < m->p.size(); ++q) {
+    sw_ctx* c = m->p[q];
+    if (win[q].x >= win[q].y) continue;
+    StepArgs a = make_args(c, mode);
+    a.it_lo = win[q].x;
+    a.it_hi = win[q].y;
+    const bool sh = step_sh_on(c), psv = step_psv_on(c), spin = uses_spin(c);
+    const int nt = c->tile.nt_threads;
+    if (mode == M_ADJ) {
+      ents.push_back({sh ? MK_ADJ_SH : MK_ADJ_PSV, nt, a});
+      continue;
+    }
+    if (sh) {
+      StepArgs b = a;
+      b.advance = psv ? 0 : 1;
+      ents.push_back({aux ? MK_SH_AUX : MK_SH, nt, b});
+    }
+    if (psv) ents.push_back({aux ? MK_PSV_AUX : (spin ? MK_SPIN : MK_PSV), nt, a});
+  }
+  if (ents.empty() || nlaunch <= 0) return 0;
+  auto rank = [](int kind) { return (kind == MK_SH || kind == MK_SH_AUX || kind == MK_ADJ_SH) ? 0 : 1; };
+  std::stable_sort(ents.begin(), ents.end(), [&](const Ent& x, const Ent& y) {
+    return rank(x.kind) != rank(y.kind) ? rank(x.kind) < rank(y.kind)
+                                         : (x.kind != y.kind ? x.kind < y.kind : x.nt < y.nt);
+  });
+  std::vector<StepArgs> hargs;
+  std::vector<int2> hwork;
+  struct Grp { int kind, nt, off, nb; };
+  std::vector<Grp> grps;
+  for (size_t e = 0; e < ents.size(); ++e) {
+    if (grps.empty() || grps.back().kind != ents[e].kind || grps.back().nt != ents[e].nt)
+      grps.push_back({ents[e].kind, ents[e].nt, int(hwork.size()), 0});
+    hargs.push_back(ents[e].a);
+    for (int b = 0; b < ents[e].a.nblocks; ++b) hwork.push_back(make_int2(int(e), b));
+    grps.back().nb += ents[e].a.nblocks;
+  }
+  HIPCHK(m->args.ensure(hargs.size()));
+  HIPCHK(m->work.ensure(hwork.size()));
+  HIPCHK(hipMemcpyAsync(m->args.p, hargs.data(), hargs.size() * sizeof(StepArgs), hipMemcpyHostToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(m->work.p, hwork.data(), hwork.size() * sizeof(int2), hipMemcpyHostToDevice, m->stream));
+  for (int l = 0; l < nlaunch; ++l)
+    for (const Grp& g : grps) {
+      HIPCHK(launch_multi(g.kind, g.nt, m->args.p, m->work.p + g.off, g.nb, m->stream));
+      ++m->launches;
+    }
+  HIPCHK(hipStreamSynchronize(m->stream));   // hargs / hwork are host memory
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+SW_API int sw_multi_create(sw_ctx* const* problems, int32_t n, sw_multi** out) {
+  if (!problems || n < 1 || !out) return fail(-1, "bad arguments");
+  for (int q = 0; q < n; ++q) {
+    if (!problems[q]) return fail(-1, "problem %d is null", q);
+    if (problems[q]->device != problems[0]->device) return fail(-2, "problems must share one HIP device");
+    if (!problems[q]->model_set) return fail(-2, "problem %d has no model", q);
+    for (int r = 0; r < q; ++r)
+      if (problems[r] == problems[q]) return fail(-2, "problem %d appears twice", q);
+  }
+  HIPCHK(hipSetDevice(problems[0]->device));
+  sw_multi* m = new sw_multi;
+  m->p.assign(problems, problems + n);
+  m->device = problems[0]->device;
+  hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete m;
+    return fail(-3, "stream creation failed: %s", hipGetErrorString(e));
+  }
+  *out = m;
+  return 0;
+}
+
+SW_API void sw_multi_destroy(sw_multi* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  if (m->stream) {
+    (void)hipStreamSynchronize(m->stream);
+    (void)hipStreamDestroy(m->stream);
+  }
+  delete m;
+}
+
+SW_API int sw_multi_forward(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc) {
+  if (!m || !nshots || !isrc) return fail(-1, "bad arguments");
+  HIPCHK(hipSetDevice(m->device));
+  SharedStream guard(m);
+  const int n = int(m->p.size());
+  int maxnt = 0;
+  std::vector<int2> win(n);
+  for (int q = 0; q < n; ++q) {
+    int rc = sw_prepare(m->p[q], nshots[q], isrc[q]);
+    if (rc) return rc;
+    win[q] = make_int2(0, m->p[q]->g.nt);
+    maxnt = std::max(maxnt, m->p[q]->g.nt);
+  }
+  hipEvent_t e0 = m->p[0]->ev0, e1 = m->p[0]->ev1;
+  HIPCHK(hipEventRecord(e0, m->stream));
+  int rc = multi_steps(m, M_FWD, maxnt, win);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(e1, m->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  m->loop_ms = ms;
+  for (sw_ctx* c : m->p) {
+    c->steps_done = c->g.nt;
+    c->cur = c->g.nt & 1;
+  }
+  return 0;
+}
+
+SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc,
+                             const float* const* obs_ref, int32_t ckpt_every, double* const* misfit) {
+  if (!m || !nshots || !isrc || !obs_ref) return fail(-1, "bad arguments");
+  HIPCHK(hipSetDevice(m->device));
+  SharedStream guard(m);
+  const int n = int(m->p.size());
+  int rc = 0, max_nck = 0;
+  hipEvent_t e0 = m->p[0]->ev0, e1 = m->p[0]->ev1;
+  HIPCHK(hipEventRecord(e0, m->stream));
+  for (int q = 0; q < n; ++q) {
+    sw_ctx* c = m->p[q];
+    if ((rc = grad_begin(c, ckpt_every))) return rc;
+    if ((rc = grad_batch_setup(c, nshots[q], isrc[q]))) return rc;
+    max_nck = std::max(max_nck, c->gp.nck);
+  }
+  const int K = m->p[0]->gp.K;
+  for (sw_ctx* c : m->p)
+    if (c->gp.K != K) return fail(-2, "ckpt_every exceeds nt of some problem; use ckpt_every <= min nt");
+  // forward with checkpoints, all problems in lockstep
+  std::vector<int2> win(n);
+  for (int s = 0; s < max_nck; ++s) {
+    int nl = 0;
+    for (int q = 0; q < n; ++q) {
+      sw_ctx* c = m->p[q];
+      win[q] = make_int2(0, c->g.nt);
+      if (s < c->gp.nck) {
+        if ((rc = grad_save_ckpt(c, s))) return rc;
+        nl = std::max(nl, std::min(K, c->g.nt - s * K));
+      }
+    }
+    if ((rc = multi_steps(m, M_FWD, nl, win))) return rc;
+    for (sw_ctx* c : m->p) {
+      const int done = std::min(c->g.nt, s * K + nl);
+      c->steps_done = done;
+      c->cur = done & 1;
+    }
+  }
+  for (int q = 0; q < n; ++q) {
+    const size_t per = size_t(m->p[q]->gp.ncomp) * m->p[q]->nrec * m->p[q]->g.nt;
+    (void)per;
+    if ((rc = grad_adjsrc_and_bwd_setup(m->p[q], obs_ref[q]))) return rc;
+  }
+  // backward, segment by segment from the last
+  for (int s = max_nck - 1; s >= 0; --s) {
+    int nl = 0;
+    for (int q = 0; q < n; ++q) {
+      sw_ctx* c = m->p[q];
+      if (s < c->gp.nck) {
+        int n0, n1;
+        if ((rc = grad_segment_setup(c, s, n0, n1))) return rc;
+        win[q] = make_int2(n0, n1);
+        nl = std::max(nl, n1 - n0);
+      } else {
+        win[q] = make_int2(0, 0);
+      }
+    }
+    if ((rc = multi_steps(m, M_FWD_AUX, nl, win))) return rc;
+    if ((rc = multi_steps(m, M_ADJ, nl, win))) return rc;
+  }
+  for (sw_ctx* c : m->p)
+    if ((rc = grad_reduce(c))) return rc;
+  HIPCHK(hipEventRecord(e1, m->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  m->loop_ms = ms;
+  for (int q = 0; q < n; ++q)
+    if (misfit && misfit[q])
+      for (int s = 0; s < nshots[q]; ++s) misfit[q][s] = m->p[q]->gp.mis[s];
+  return 0;
+}
+
+SW_API int sw_multi_timing(const sw_multi* m, double* ms, int64_t* launches) {
+  if (!m) return fail(-1, "null argument");
+  if (ms) *ms = m->loop_ms;
+  if (launches) *launches = m->launches;
   return 0;
 }
 

@@ -34,9 +34,10 @@ struct Tiling {
 enum ModelField : int { MF_RHO = 0, MF_LAM, MF_MU, MF_NU, MF_J, MF_LAM_C, MF_MU_C, MF_NU_C, MF_N };
 
 struct StepArgs {
-  // state buffers laid out [shot][slot][npt]
-  const float* state_in;
-  float* state_out;
+  // ping-pong state buffers, each laid out [shot][slot][npt].  A forward step
+  // `it` reads st[it & 1] and writes st[(it+1) & 1]; an adjoint step `it`
+  // reads st[(it+1) & 1] and writes st[it & 1].
+  float* st[2];
   int slot_off[N_SLOTS];      // float offset of each slot inside one shot (-1: not stored)
   int shot_floats;            // floats per shot
   const float* model;         // [MF_N][npt], shared by all shots
@@ -58,11 +59,13 @@ struct StepArgs {
   int record;                 // 1: accumulate u and write traces
   int* clock;                 // [0] = current step it, [1] = finished-block counter
   int advance;                // forward: last block sets clock[0] = it+1; adjoint: it-1
+  int it_lo, it_hi;           // the launch is a no-op unless it_lo <= it < it_hi
+  int nblocks;                // workgroups of this problem in the launch
   // gradient support: per-step auxiliary fields (forward recompute writes,
-  // adjoint reads) laid out [step - aux_base][shot][naux][npt], and the
+  // adjoint reads) laid out [step % aux_period][shot][naux][npt], and the
   // per-shot kernels [shot][3][npt] (rho, mu, lambda)
   float* aux;
-  int aux_base;
+  int aux_period;
   float* kgrad;
 };
 
@@ -76,6 +79,13 @@ hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, boo
 hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStream_t s, bool aux = false);
 hipError_t launch_adj_sh(const StepArgs& a, const Tiling& t, hipStream_t s);
 hipError_t launch_adj_psv(const StepArgs& a, const Tiling& t, hipStream_t s);
+
+// Several independent problems advanced by one launch: args[p] in device
+// memory, work[b] = (problem, workgroup of that problem) for every workgroup
+// b of the launch; all problems share the tile width nt_threads.
+enum MultiKind : int { MK_SH = 0, MK_PSV, MK_SPIN, MK_SH_AUX, MK_PSV_AUX, MK_ADJ_SH, MK_ADJ_PSV };
+hipError_t launch_multi(int kind, int nt_threads, const StepArgs* args, const int2* work, int nblocks,
+                        hipStream_t s);
 // adjoint sources dt * sum_{k>=n} 2 w_k^2 (d^k - obs^k) and per-shot misfit
 hipError_t launch_adjsrc(const float* obs_syn, const float* obs_ref, const float* taper, float* adj, double* misfit,
                          int nshots, int ncomp, const int* comps, int nrec, int nt, float dt, hipStream_t s);

@@ -64,12 +64,12 @@ __device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, uint32
 }
 
 // Advance the step clock once every workgroup of this launch has read it.
-__device__ __forceinline__ void advance_clock(int* clock, int it, int dir = 1) {
+__device__ __forceinline__ void advance_clock(int* clock, int it, int dir, int nblk) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
     const int done = atomicAdd(&clock[1], 1);
-    if (done == int(gridDim.x) - 1) {
+    if (done == nblk - 1) {
       atomicExch(&clock[1], 0);
       atomicExch(&clock[0], it + dir);
     }
@@ -92,10 +92,9 @@ struct TileGeom {
 };
 
 template <int NT>
-__device__ __forceinline__ TileGeom tile_geom(const StepArgs& a) {
+__device__ __forceinline__ TileGeom tile_geom(const StepArgs& a, int logical) {
   constexpr int W = NT - 6;
   TileGeom g;
-  const int logical = xcd_remap(blockIdx.x, gridDim.x);
   const int tps = a.nstrips * a.nsegs;
   g.shot = logical / tps;
   g.tile = logical - g.shot * tps;
@@ -117,16 +116,17 @@ __device__ __forceinline__ TileGeom tile_geom(const StepArgs& a) {
 // ---------------------------------------------------------------- SH step
 // div_sy, stf_dsy, add_vy, div_vy, add_sy_sh, save_obs_y_sh (fd2d.wgsl:242-301)
 template <int NT, bool AUX>
-__global__ __launch_bounds__(NT) void k_step_sh(const StepArgs a) {
+__device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   __shared__ float lds_s[2][NT];
   __shared__ float lds_v[2][NT];
-  const TileGeom g = tile_geom<NT>(a);
+  const TileGeom g = tile_geom<NT>(a, logical);
   const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (it < a.it_lo || it >= a.it_hi) return;
   const int64_t so = int64_t(g.shot) * a.shot_floats;
   const uint32_t sbytes = 4u * uint32_t(a.shot_floats);
-  const auto rin = make_rsrc(a.state_in + so, sbytes);
-  const auto rout = make_rsrc(a.state_out + so, sbytes);
+  const auto rin = make_rsrc(a.st[it & 1] + so, sbytes);
+  const auto rout = make_rsrc(a.st[(it + 1) & 1] + so, sbytes);
   const uint32_t npt4 = 4u * uint32_t(a.npt);
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
   const uint32_t o_vy = 4u * a.slot_off[S_VY], o_sxy = 4u * a.slot_off[S_SXY], o_szy = 4u * a.slot_off[S_SZY];
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(NT) void k_step_sh(const StepArgs a) {
   float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
   float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const auto raux =
-      make_rsrc(AUX ? a.aux + (int64_t(it - a.aux_base) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt) : a.model,
+      make_rsrc(AUX ? a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt) : a.model,
                 AUX ? uint32_t(NAUX_SH) * npt4 : 0u);
 
   // rings: stress row r in slot (r - (i0-3)) & 3; new-velocity row r in slot
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(NT) void k_step_sh(const StepArgs a) {
     body(Phase<3>{}, m);
     if (++m > mlast) break;
   }
-  if (a.advance) advance_clock(a.clock, it);
+  if (a.advance) advance_clock(a.clock, it, 1, a.nblocks);
 }
 
 // ------------------------------------------------------- P-SV (+spin) step
@@ -247,18 +247,19 @@ __global__ __launch_bounds__(NT) void k_step_sh(const StepArgs a) {
 // add_vy_c, div_vy_c, add_sy_c, save_obs_x/z, compute_du_grad, save_obs_ry
 // (fd2d.wgsl:308-492)
 template <int NT, bool SPIN, bool AUX>
-__global__ __launch_bounds__(NT) void k_step_psv(const StepArgs a) {
+__device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   constexpr int NSL = SPIN ? 4 : 2;   // sxz, szz, syz_c, syy_c
   constexpr int NVL = SPIN ? 3 : 2;   // vx, vz, vy_c
   __shared__ float lds_s[2][NSL][NT];
   __shared__ float lds_v[2][NVL][NT];
-  const TileGeom g = tile_geom<NT>(a);
+  const TileGeom g = tile_geom<NT>(a, logical);
   const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (it < a.it_lo || it >= a.it_hi) return;
   const int64_t so = int64_t(g.shot) * a.shot_floats;
   const uint32_t sbytes = 4u * uint32_t(a.shot_floats);
-  const auto rin = make_rsrc(a.state_in + so, sbytes);
-  const auto rout = make_rsrc(a.state_out + so, sbytes);
+  const auto rin = make_rsrc(a.st[it & 1] + so, sbytes);
+  const auto rout = make_rsrc(a.st[(it + 1) & 1] + so, sbytes);
   const uint32_t npt4 = 4u * uint32_t(a.npt);
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
   const uint32_t o_vx = 4u * a.slot_off[S_VX], o_vz = 4u * a.slot_off[S_VZ];
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(NT) void k_step_psv(const StepArgs a) {
   float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const int nobs = a.nrec * a.nt;
   const auto raux =
-      make_rsrc(AUX ? a.aux + (int64_t(it - a.aux_base) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt) : a.model,
+      make_rsrc(AUX ? a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt) : a.model,
                 AUX ? uint32_t(NAUX_PSV) * npt4 : 0u);
 
   // rings as in the SH kernel
@@ -516,7 +517,7 @@ __global__ __launch_bounds__(NT) void k_step_psv(const StepArgs a) {
     body(Phase<3>{}, m);
     if (++m > mlast) break;
   }
-  if (a.advance) advance_clock(a.clock, it);
+  if (a.advance) advance_clock(a.clock, it, 1, a.nblocks);
 }
 
 // ------------------------------------------------------ adjoint time steps
@@ -542,20 +543,21 @@ __device__ __forceinline__ float ndp(float fm1, float f0, float fp1, float fp2, 
 }
 
 template <int NT>
-__global__ __launch_bounds__(NT) void k_adj_sh(const StepArgs a) {
+__device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   __shared__ float lds_t[2][NT];
   __shared__ float lds_q[2][NT];
-  const TileGeom g = tile_geom<NT>(a);
+  const TileGeom g = tile_geom<NT>(a, logical);
   const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (it < a.it_lo || it >= a.it_hi) return;
   const int64_t so = int64_t(g.shot) * a.shot_floats;
   const uint32_t sbytes = 4u * uint32_t(a.shot_floats);
-  const auto rin = make_rsrc(a.state_in + so, sbytes);
-  const auto rout = make_rsrc(a.state_out + so, sbytes);
+  const auto rin = make_rsrc(a.st[(it + 1) & 1] + so, sbytes);
+  const auto rout = make_rsrc(a.st[it & 1] + so, sbytes);
   const uint32_t npt4 = 4u * uint32_t(a.npt);
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
   const auto raux =
-      make_rsrc(a.aux + (int64_t(it - a.aux_base) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt), NAUX_SH * npt4);
+      make_rsrc(a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt), NAUX_SH * npt4);
   const auto rk = make_rsrc(a.kgrad + int64_t(g.shot) * NK * int64_t(a.npt), NK * npt4);
   const uint32_t o_vy = 4u * a.slot_off[S_VY], o_sxy = 4u * a.slot_off[S_SXY], o_szy = 4u * a.slot_off[S_SZY];
   const uint32_t m_rho = MF_RHO * npt4, m_mu = MF_MU * npt4;
@@ -669,23 +671,24 @@ __global__ __launch_bounds__(NT) void k_adj_sh(const StepArgs a) {
     body(Phase<3>{}, m);
     if (++m > mlast) break;
   }
-  if (a.advance) advance_clock(a.clock, it, -1);
+  if (a.advance) advance_clock(a.clock, it, -1, a.nblocks);
 }
 
 template <int NT>
-__global__ __launch_bounds__(NT) void k_adj_psv(const StepArgs a) {
+__device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   __shared__ float lds_t[2][2][NT];   // z-masked t_xz, t_zz of row m
   __shared__ float lds_q[2][2][NT];   // z-masked qx, qz of row m
-  const TileGeom g = tile_geom<NT>(a);
+  const TileGeom g = tile_geom<NT>(a, logical);
   const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (it < a.it_lo || it >= a.it_hi) return;
   const int64_t so = int64_t(g.shot) * a.shot_floats;
   const uint32_t sbytes = 4u * uint32_t(a.shot_floats);
-  const auto rin = make_rsrc(a.state_in + so, sbytes);
-  const auto rout = make_rsrc(a.state_out + so, sbytes);
+  const auto rin = make_rsrc(a.st[(it + 1) & 1] + so, sbytes);
+  const auto rout = make_rsrc(a.st[it & 1] + so, sbytes);
   const uint32_t npt4 = 4u * uint32_t(a.npt);
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
-  const auto raux = make_rsrc(a.aux + (int64_t(it - a.aux_base) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt),
+  const auto raux = make_rsrc(a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt),
                               NAUX_PSV * npt4);
   const auto rk = make_rsrc(a.kgrad + int64_t(g.shot) * NK * int64_t(a.npt), NK * npt4);
   const uint32_t o_vx = 4u * a.slot_off[S_VX], o_vz = 4u * a.slot_off[S_VZ];
@@ -829,7 +832,53 @@ __global__ __launch_bounds__(NT) void k_adj_psv(const StepArgs a) {
     body(Phase<3>{}, m);
     if (++m > mlast) break;
   }
-  if (a.advance) advance_clock(a.clock, it, -1);
+  if (a.advance) advance_clock(a.clock, it, -1, a.nblocks);
+}
+
+// --------------------------------------------------------- entry points
+template <int NT, bool AUX>
+__global__ __launch_bounds__(NT) void k_step_sh(const StepArgs a) {
+  step_sh<NT, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+template <int NT, bool SPIN, bool AUX>
+__global__ __launch_bounds__(NT) void k_step_psv(const StepArgs a) {
+  step_psv<NT, SPIN, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+template <int NT>
+__global__ __launch_bounds__(NT) void k_adj_sh(const StepArgs a) {
+  adj_sh<NT>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+template <int NT>
+__global__ __launch_bounds__(NT) void k_adj_psv(const StepArgs a) {
+  adj_psv<NT>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+// multi-problem entry point: one launch advances several problems
+template <int NT, int KIND>
+__global__ __launch_bounds__(NT) void k_multi(const StepArgs* __restrict__ args, const int2* __restrict__ work) {
+  const int2 w = work[xcd_remap(blockIdx.x, gridDim.x)];
+  const StepArgs& a = args[w.x];
+  if constexpr (KIND == MK_SH) step_sh<NT, false>(a, w.y);
+  else if constexpr (KIND == MK_PSV) step_psv<NT, false, false>(a, w.y);
+  else if constexpr (KIND == MK_SPIN) step_psv<NT, true, false>(a, w.y);
+  else if constexpr (KIND == MK_SH_AUX) step_sh<NT, true>(a, w.y);
+  else if constexpr (KIND == MK_PSV_AUX) step_psv<NT, false, true>(a, w.y);
+  else if constexpr (KIND == MK_ADJ_SH) adj_sh<NT>(a, w.y);
+  else adj_psv<NT>(a, w.y);
+}
+
+template <int NT>
+hipError_t launch_multi_nt(int kind, const StepArgs* args, const int2* work, int nb, hipStream_t s) {
+  switch (kind) {
+    case MK_SH: hipLaunchKernelGGL((k_multi<NT, MK_SH>), dim3(nb), dim3(NT), 0, s, args, work); break;
+    case MK_PSV: hipLaunchKernelGGL((k_multi<NT, MK_PSV>), dim3(nb), dim3(NT), 0, s, args, work); break;
+    case MK_SPIN: hipLaunchKernelGGL((k_multi<NT, MK_SPIN>), dim3(nb), dim3(NT), 0, s, args, work); break;
+    case MK_SH_AUX: hipLaunchKernelGGL((k_multi<NT, MK_SH_AUX>), dim3(nb), dim3(NT), 0, s, args, work); break;
+    case MK_PSV_AUX: hipLaunchKernelGGL((k_multi<NT, MK_PSV_AUX>), dim3(nb), dim3(NT), 0, s, args, work); break;
+    case MK_ADJ_SH: hipLaunchKernelGGL((k_multi<NT, MK_ADJ_SH>), dim3(nb), dim3(NT), 0, s, args, work); break;
+    case MK_ADJ_PSV: hipLaunchKernelGGL((k_multi<NT, MK_ADJ_PSV>), dim3(nb), dim3(NT), 0, s, args, work); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 // adjoint sources: one thread per (shot, component, receiver) scans time
@@ -1021,6 +1070,16 @@ hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStr
     case 64: return spin ? launch_psv_nt<64, true>(a, t, s, aux) : launch_psv_nt<64, false>(a, t, s, aux);
     case 128: return spin ? launch_psv_nt<128, true>(a, t, s, aux) : launch_psv_nt<128, false>(a, t, s, aux);
     case 256: return spin ? launch_psv_nt<256, true>(a, t, s, aux) : launch_psv_nt<256, false>(a, t, s, aux);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_multi(int kind, int nt_threads, const StepArgs* args, const int2* work, int nb, hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  switch (nt_threads) {
+    case 64: return launch_multi_nt<64>(kind, args, work, nb, s);
+    case 128: return launch_multi_nt<128>(kind, args, work, nb, s);
+    case 256: return launch_multi_nt<256>(kind, args, work, nb, s);
   }
   return hipErrorInvalidValue;
 }

@@ -109,3 +109,49 @@ def test_gpu_gradient_independent_of_checkpoint_interval_and_batching(seiswg, sh
         assert np.array_equal(got[0], ref[0])
         for a, b in zip(got[1:], ref[1:]):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (ck, spl)
+
+
+def multi_problems(seiswg):
+    """Uneven problems: SH and P-SV, different grids, nt, dt and tile shapes."""
+    specs = [(True, 40, 36, 70, 0.1, (64, 7)), (False, 52, 30, 90, 0.08, (128, 16)),
+             (False, 34, 70, 55, 0.1, (64, 32)), (True, 60, 44, 90, 0.05, (128, 9))]
+    out = []
+    for q, (sh, nx, nz, nt, dt, tl) in enumerate(specs):
+        p, stf, nc = problem(sh, nx, nz, nt, seed=10 + q)
+        p.dt = float(np.float32(dt))
+        c = gpu_problem(seiswg, p, stf)
+        c.set_tiling(*tl)
+        out.append((p, stf, nc, c))
+    return out
+
+
+@pytest.mark.gpu
+def test_multi_problem_forward_equals_single_runs(seiswg):
+    probs = multi_problems(seiswg)
+    runs = [[0, -1], [1], [0, 1, 2], [-1, 2]]
+    single = []
+    for (p, stf, nc, c), r in zip(probs, runs):
+        c.forward_batch(r)
+        comp = 1 if p.sh else 0
+        single.append([c.obs(s, comp).copy() for s in range(len(r))])
+    m = seiswg.Multi([c for *_, c in probs])
+    m.forward(runs)
+    for (p, stf, nc, c), r, ref in zip(probs, runs, single):
+        comp = 1 if p.sh else 0
+        for s in range(len(r)):
+            assert np.array_equal(c.obs(s, comp), ref[s])
+
+
+@pytest.mark.gpu
+def test_multi_problem_gradient_equals_single_gradients(seiswg):
+    probs = multi_problems(seiswg)
+    runs = [[0, -1], [1, 2], [0], [2, 0, 1]]
+    refs = [(np.random.default_rng(q).standard_normal((len(r), nc, len(p.rec_idx), p.nt)) * 1e-3).astype(np.float32)
+            for q, ((p, stf, nc, c), r) in enumerate(zip(probs, runs))]
+    single = [c.gradient(r, o, ckpt_every=20) for (p, stf, nc, c), r, o in zip(probs, runs, refs)]
+    m = seiswg.Multi([c for *_, c in probs])
+    got = m.gradient(runs, refs, ckpt_every=20)
+    for a, b in zip(got, single):
+        assert np.array_equal(a[0], b[0])
+        for x, y in zip(a[1:], b[1:]):
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32))

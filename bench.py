@@ -71,6 +71,48 @@ def cpu_baseline(synth, model, geom, src_idx, rec_idx, budget_s=12.0):
                       f"{n} time steps in {el:.1f} s"}
 
 
+def gradient_bench(seiswg, synth, nt_override=0):
+    """Checkpointed P-SV adjoint gradient (1000x600, 16 shots, 200 receivers,
+    checkpoint every 50 steps): shots/s with all shots batched per launch vs
+    the plain one-shot-per-launch path on the same GPU, same kernels, same
+    results (bit-identical kernels are checked)."""
+    c = synth.C3
+    init, true, geom = synth.c3_problem()
+    nt = nt_override or c["nt"]
+
+    def ctx_for(model):
+        x = seiswg.Context(c["nx"], c["nz"], nt, c["h"], c["h"], c["dt"], sh=False, psv=True, abs_left=True,
+                           abs_right=True, abs_bottom=True, abs_top=False, abs_width=c["abs_width"], abs_alpha=0.015)
+        x.set_model(seiswg.abi.MODEL_VPVS, [model["rho"], model["vp"], model["vs"]])
+        rows = geom.src_rows()
+        x.set_sources([seiswg.grid_index(r[0], r[1], c["h"], c["h"], c["nz"]) for r in rows],
+                      seiswg.ricker_table([[r[3], r[4], r[5], r[6]] for r in rows], nt, c["dt"]))
+        x.set_receivers([seiswg.grid_index(a, b, c["h"], c["h"], c["nz"]) for a, b in zip(geom.rec_x, geom.rec_z)])
+        return x
+
+    import numpy as np
+    shots = list(range(c["nshots"]))
+    ct = ctx_for(true)
+    ct.forward_batch(shots)
+    obs = np.stack([np.stack([ct.obs(s, seiswg.abi.OBS_X), ct.obs(s, seiswg.abi.OBS_Z)]) for s in shots])
+    ct.close()
+    ci = ctx_for(init)
+    ci.gradient(shots[:1], obs[:1], ckpt_every=c["ckpt"])          # warm-up (allocations, code load)
+    t0 = time.perf_counter()
+    mis_b, kr_b, km_b, kl_b = ci.gradient(shots, obs, ckpt_every=c["ckpt"])
+    t_batched = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mis_p, kr_p, km_p, kl_p = ci.gradient(shots, obs, ckpt_every=c["ckpt"], shots_per_launch=1)
+    t_plain = time.perf_counter() - t0
+    same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+               for a, b in ((kr_b, kr_p), (km_b, km_p), (kl_b, kl_p)))
+    return {"workload": f"checkpointed P-SV adjoint gradient, 1000x600, 16 shots, 200 receivers, nt={nt}, "
+                        f"checkpoint every {c['ckpt']}", "shots_per_s_batched": len(shots) / t_batched,
+            "shots_per_s_plain": len(shots) / t_plain, "speedup": t_plain / t_batched,
+            "s_batched": t_batched, "s_plain": t_plain, "kernels_bit_identical": bool(same),
+            "misfit_total": float(mis_b.sum())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +121,10 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="tile width (64/128/256); 0 = library default")
     ap.add_argument("--seglen", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-gradient", action="store_true", help="skip the batched-vs-plain gradient measurement")
+    ap.add_argument("--gradient-nt", type=int, default=0, help="time steps of the gradient workload (0: 3000)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain 1 GiB copy kernel (calibrates the HBM counters of a profiler run)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the time loop shards by shots with "
@@ -86,6 +132,8 @@ def main():
         return 2
     import seiswg_amd as seiswg
     from seiswg_amd import synth
+    if a.calibrate_copy:
+        seiswg.bench_copy(1 << 30, 5)
     nt = max(synth.C2["nt"], a.warmup + a.steps)
     ctx, model, geom, src_idx, rec_idx = build_context(seiswg, synth, nt)
     if a.threads or a.seglen:
@@ -119,6 +167,9 @@ def main():
                      "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic,
                      "kernel_avg_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_launch": algo_bytes},
     }
+    ctx.close()
+    if not a.no_gradient:
+        out["gradient"] = gradient_bench(seiswg, synth, a.gradient_nt)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(synth, model, geom, src_idx, rec_idx)
     print(json.dumps(out), flush=True)

@@ -112,3 +112,21 @@ def c2_problem():
     geom = line_geometry(c["nx"], c["h"], c["nshots"], c["nrec"], rec_first=2, rec_step=5, f0=c["f0"], t0=c["t0"],
                          ang=c["ang"])
     return model, geom
+
+
+C3 = dict(nx=1000, nz=600, h=10.0, nshots=16, nrec=200, nt=3000, dt=1e-3, f0=8.0, t0=0.15, ang=90.0,
+          abs_width=40, seed=2, anomaly_seed=3, ckpt=50)
+
+
+def c3_problem():
+    """Checkpointed P-SV gradient workload: 1000x600 initial model (the C2
+    recipe, seed 2), true model with three +/-8% vs Gaussian anomalies of
+    radius 150 m (seed 3); 16 shots at 50 m depth and 200 receivers at 10 m,
+    evenly spaced."""
+    c = C3
+    init = layered_random_model(c["nx"], c["nz"], c["h"], c["seed"])
+    true = gaussian_anomalies(init, c["nx"], c["nz"], c["h"], c["anomaly_seed"])
+    step = (c["nx"] - 4) // c["nrec"]
+    geom = line_geometry(c["nx"], c["h"], c["nshots"], c["nrec"], rec_first=2, rec_step=step, f0=c["f0"],
+                         t0=c["t0"], ang=c["ang"])
+    return init, true, geom

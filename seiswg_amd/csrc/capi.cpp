@@ -188,7 +188,8 @@ void build_taper(sw_ctx* c) {
 }
 
 // Default tile shape: the widest strip and longest segment that still give
-// the GPU's 256 CUs several workgroups each (>= 1024 workgroups).
+// the GPU's 256 CUs several workgroups each (>= 1024 workgroups).  Long
+// segments amortise each workgroup's start-up (list lookups, first rows).
 void heuristic_tiling(sw_ctx* c, int nshots) {
   Tiling& t = c->tile;
   auto count = [&](int nt, int seg) {
@@ -196,7 +197,7 @@ void heuristic_tiling(sw_ctx* c, int nshots) {
     return int64_t(nshots) * strips * segs;
   };
   int nt = c->g.nz >= 400 ? 256 : c->g.nz >= 100 ? 128 : 64;
-  int seg = 128;
+  int seg = 512;
   while (count(nt, seg) < 1024 && seg > 8) seg /= 2;
   while (count(nt, seg) < 1024 && nt > 64) nt /= 2;
   t.nt_threads = nt;
@@ -629,6 +630,10 @@ SW_API int sw_set_model(sw_ctx* c, int32_t kind, const float* const* f) {
   } else {
     return fail(-2, "unknown model kind %d", kind);
   }
+  // correctly rounded reciprocals for the exact per-cell divisions
+  HIPCHK(launch_recip(c->mf(MF_RHO), c->mf(MF_IRHO), int64_t(n), c->stream));
+  if (kind == SW_MODEL_LAME) HIPCHK(launch_recip(c->mf(MF_J), c->mf(MF_IJ), int64_t(n), c->stream));
+  else HIPCHK(launch_fill(c->mf(MF_IJ), 1.0f, int64_t(n), c->stream));
   build_taper(c);
   HIPCHK(c->bx.ensure(size_t(c->g.nx)));
   HIPCHK(c->bzb.ensure(size_t(c->g.nz)));
@@ -762,6 +767,8 @@ SW_API int sw_write_field(sw_ctx* c, int32_t shot, int32_t field, const float* i
   }
   if (!dst) return fail(-2, "field %d is not materialised for this configuration", field);
   HIPCHK(hipMemcpyAsync(dst, in, size_t(c->npt) * 4, hipMemcpyHostToDevice, c->stream));
+  if (field == SW_F_RHO) HIPCHK(launch_recip(c->mf(MF_RHO), c->mf(MF_IRHO), c->npt, c->stream));
+  if (field == SW_F_J) HIPCHK(launch_recip(c->mf(MF_J), c->mf(MF_IJ), c->npt, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -31,7 +31,8 @@ struct Tiling {
 };
 
 // Rows of the model buffer [MF_N][npt].
-enum ModelField : int { MF_RHO = 0, MF_LAM, MF_MU, MF_NU, MF_J, MF_LAM_C, MF_MU_C, MF_NU_C, MF_N };
+// MF_IRHO / MF_IJ hold the correctly rounded reciprocals 1/rho, 1/J.
+enum ModelField : int { MF_RHO = 0, MF_LAM, MF_MU, MF_NU, MF_J, MF_LAM_C, MF_MU_C, MF_NU_C, MF_IRHO, MF_IJ, MF_N };
 
 struct StepArgs {
   // ping-pong state buffers, each laid out [shot][slot][npt].  A forward step
@@ -104,6 +105,7 @@ hipError_t launch_misfit(const float* syn, const float* ref, const float* taper,
 hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t s);
 hipError_t launch_vps2lm(const float* rho, float* lam_vp, float* mu_vs, int64_t npt, hipStream_t s);
 hipError_t launch_fill(float* p, float v, int64_t n, hipStream_t s);
+hipError_t launch_recip(const float* x, float* out, int64_t n, hipStream_t s);
 hipError_t launch_copy(const float* src, float* dst, int64_t n, hipStream_t s);
 
 }  // namespace sw

@@ -36,6 +36,10 @@ __device__ __forceinline__ float div_const(float x, float y, float r) {
   return __builtin_fmaf(rem, r, q0);
 }
 
+// x / y for a per-cell divisor with its correctly rounded reciprocal r stored
+// beside it: the same Markstein correction, exact for normal results.
+__device__ __forceinline__ float div_cell(float x, float y, float r) { return div_const(x, y, r); }
+
 // fd2d.wgsl:151-164 (backward) and 166-180 (forward) differences, with the
 // reference's 1/8 weight on the 3-cell difference; h8 = 8*h, r8 = 1/h8.
 __device__ __forceinline__ float dminus(float fm2, float fm1, float f0, float fp1, float h8, float r8) {
@@ -130,17 +134,20 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   const uint32_t npt4 = 4u * uint32_t(a.npt);
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
   const uint32_t o_vy = 4u * a.slot_off[S_VY], o_sxy = 4u * a.slot_off[S_SXY], o_szy = 4u * a.slot_off[S_SZY];
-  const uint32_t m_rho = MF_RHO * npt4, m_mu = MF_MU * npt4;
+  const uint32_t m_rho = MF_RHO * npt4, m_mu = MF_MU * npt4, m_irho = MF_IRHO * npt4;
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
+  const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
   const float dt = a.dt, h8x = a.h8x, h8z = a.h8z, r8x = a.r8x, r8z = a.r8z;
   const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;
   int ip = a.inj_off[g.shot * tps + g.tile];
   const int ie = a.inj_off[g.shot * tps + g.tile + 1];
+  int inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;     // row of the next injection entry
   int rp = 0, re = 0;
   if (a.record) { rp = a.rec_off[g.tile]; re = a.rec_off[g.tile + 1]; }
+  int rec_row = rp < re ? a.rec[rp].x : 0x7fffffff;     // row of the next receiver entry
   float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
   float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const auto raux =
@@ -155,23 +162,35 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     szy[q] = bload(rin, jc4, o_szy + g.row(i0 - 3 + q));
   }
   sxy[3] = szy[3] = 0.0f;
-  float pf_sxy = bload(rin, jc4, o_sxy + g.row(i0)), pf_szy = bload(rin, jc4, o_szy + g.row(i0));
-  float pf_vy = bload(rin, jc4, o_vy + g.row(i0 - 1));
-  float pf_rho = bload(rmod, jc4, m_rho + g.row(i0 - 1));
-  float pf_mu = bload(rmod, jc4, m_mu + g.row(i0 - 3));
+  // 2-deep prefetch FIFO, slot (m - (i0-1)) & 1: stress row m+1, velocity and
+  // density row m, shear modulus row m-2 of iteration m
+  float pf_sxy[2], pf_szy[2], pf_vy[2], pf_rho[2], pf_irho[2], pf_mu[2];
+  for (int q = 0; q < 2; ++q) {
+    pf_sxy[q] = bload(rin, jc4, o_sxy + g.row(i0 + q));
+    pf_szy[q] = bload(rin, jc4, o_szy + g.row(i0 + q));
+    pf_vy[q] = bload(rin, jc4, o_vy + g.row(i0 - 1 + q));
+    pf_rho[q] = bload(rmod, jc4, m_rho + g.row(i0 - 1 + q));
+    pf_irho[q] = bload(rmod, jc4, m_irho + g.row(i0 - 1 + q));
+    pf_mu[q] = bload(rmod, jc4, m_mu + g.row(i0 - 3 + q));
+  }
 
   auto body = [&](auto phase, int m) __attribute__((always_inline)) {
     constexpr int R = decltype(phase)::value;
     constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
-    sxy[s3] = pf_sxy;          // row m+1
-    szy[s3] = pf_szy;
-    const float vy_o = pf_vy, rho = pf_rho, mu = pf_mu;
-    pf_sxy = bload(rin, jc4, o_sxy + g.row(m + 2));
-    pf_szy = bload(rin, jc4, o_szy + g.row(m + 2));
-    pf_vy = bload(rin, jc4, o_vy + g.row(m + 1));
-    pf_rho = bload(rmod, jc4, m_rho + g.row(m + 1));
-    pf_mu = bload(rmod, jc4, m_mu + g.row(m - 1));
-    const float bxm = a.bx[min(max(m, 0), nx - 1)];
+    constexpr int f = R & 1;
+    sxy[s3] = pf_sxy[f];       // row m+1
+    szy[s3] = pf_szy[f];
+    const float vy_o = pf_vy[f], rho = pf_rho[f], irho = pf_irho[f], mu = pf_mu[f];
+    {
+      const uint32_t r3 = g.row(m + 3), r2 = g.row(m + 2), r0 = g.row(m);
+      pf_sxy[f] = bload(rin, jc4, o_sxy + r3);
+      pf_szy[f] = bload(rin, jc4, o_szy + r3);
+      pf_vy[f] = bload(rin, jc4, o_vy + r2);
+      pf_rho[f] = bload(rmod, jc4, m_rho + r2);
+      pf_irho[f] = bload(rmod, jc4, m_irho + r2);
+      pf_mu[f] = bload(rmod, jc4, m_mu + r0);
+    }
+    const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // velocity half-step at row m
     lds_s[par][t] = szy[s2];
@@ -179,14 +198,16 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     const bool xm_ok = m >= 2 && m < nx - 2;
     float dsy = (xm_ok ? dminus(sxy[s0], sxy[s1], sxy[s2], sxy[s3], h8x, r8x) : 0.0f) +
                 (g.zm_ok ? dminus(lds_s[par][tm2], lds_s[par][tm1], szy[s2], lds_s[par][tp1], h8z, r8z) : 0.0f);
-    while (ip < ie) {
-      const int4 e = a.inj[ip];
-      if (e.x > m) break;
-      if (e.x == m && e.y == j) dsy = dsy + a.val_y[int64_t(e.z) * a.nt_val + it];
-      ++ip;
+    if (m == inj_row) {
+      for (; ip < ie; ++ip) {
+        const int4 e = a.inj[ip];
+        if (e.x != m) break;
+        if (e.y == j) dsy = dsy + a.val_y[int64_t(e.z) * a.nt_val + it];
+      }
+      inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;
     }
     const float b = (bxm * bzb) * bzt;
-    const float vyn = b * (vy_o + dt * dsy / rho);
+    const float vyn = b * (vy_o + div_cell(dt * dsy, rho, irho));
     if (g.jst && m >= i0 && m < i1) {
       bstore(vyn, rout, jc4, o_vy + uint32_t(m) * g.rowb);
       if constexpr (AUX) bstore(dsy, raux, jc4, A_DSY * npt4 + uint32_t(m) * g.rowb);
@@ -197,16 +218,18 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     dvydz[s2] = g.zp_ok ? dplus(lds_v[par][tm1], vyn, lds_v[par][tp1], lds_v[par][tp2], h8z, r8z) : 0.0f;
 
     // receivers on row m: uy += vy*dt, trace = uy
-    while (rp < re) {
-      const int4 e = a.rec[rp];
-      if (e.x > m) break;
-      if (e.x == m && e.y == j) {
-        float* u = uacc + e.z * N_UACC;
-        const float uy = u[U_UY] + vyn * dt;
-        u[U_UY] = uy;
-        obs[(1 * a.nrec + e.z) * a.nt + it] = uy;
+    if (m == rec_row) {
+      for (; rp < re; ++rp) {
+        const int4 e = a.rec[rp];
+        if (e.x != m) break;
+        if (e.y == j) {
+          float* u = uacc + e.z * N_UACC;
+          const float uy = u[U_UY] + vyn * dt;
+          u[U_UY] = uy;
+          obs[(1 * a.nrec + e.z) * a.nt + it] = uy;
+        }
       }
-      ++rp;
+      rec_row = rp < re ? a.rec[rp].x : 0x7fffffff;
     }
 
     // stress half-step at row i = m-2
@@ -271,16 +294,20 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   }
   const uint32_t m_rho = MF_RHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4;
   const uint32_t m_nu = MF_NU * npt4, m_j = MF_J * npt4, m_muc = MF_MU_C * npt4, m_nuc = MF_NU_C * npt4;
+  const uint32_t m_irho = MF_IRHO * npt4, m_ij = MF_IJ * npt4;
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
+  const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
   const float dt = a.dt, h8x = a.h8x, h8z = a.h8z, r8x = a.r8x, r8z = a.r8z;
   const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;
   int ip = a.inj_off[g.shot * tps + g.tile];
   const int ie = a.inj_off[g.shot * tps + g.tile + 1];
+  int inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;
   int rp = 0, re = 0;
   if (a.record) { rp = a.rec_off[g.tile]; re = a.rec_off[g.tile + 1]; }
+  int rec_row = rp < re ? a.rec[rp].x : 0x7fffffff;   // first row of the pending receivers
   float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
   float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const int nobs = a.nrec * a.nt;
@@ -305,52 +332,67 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   }
   sxx[3] = sxz[3] = szz[3] = syx[3] = syy[3] = syz[3] = 0.0f;
   if constexpr (!SPIN) syx[0] = syx[1] = syx[2] = syy[0] = syy[1] = syy[2] = syz[0] = syz[1] = syz[2] = 0.0f;
-  float pf_sxx = bload(rin, jc4, o_sxx + g.row(i0)), pf_sxz = bload(rin, jc4, o_sxz + g.row(i0)),
-        pf_szz = bload(rin, jc4, o_szz + g.row(i0));
-  float pf_syx = 0, pf_syy = 0, pf_syz = 0, pf_vyc = 0, pf_jj = 1.0f, pf_nu = 0, pf_muc = 0, pf_nuc = 0;
-  if constexpr (SPIN) {
-    pf_syx = bload(rin, jc4, o_syx + g.row(i0));
-    pf_syy = bload(rin, jc4, o_syy + g.row(i0));
-    pf_syz = bload(rin, jc4, o_syz + g.row(i0));
-    pf_vyc = bload(rin, jc4, o_vyc + g.row(i0 - 1));
-    pf_jj = bload(rmod, jc4, m_j + g.row(i0 - 1));
-    pf_nu = bload(rmod, jc4, m_nu + g.row(i0 - 3));
-    pf_muc = bload(rmod, jc4, m_muc + g.row(i0 - 3));
-    pf_nuc = bload(rmod, jc4, m_nuc + g.row(i0 - 3));
+  // 2-deep prefetch FIFO, slot (m - (i0-1)) & 1: stress rows m+1, velocity /
+  // density / J rows m, moduli rows m-2 of iteration m
+  float pf_sxx[2], pf_sxz[2], pf_szz[2], pf_vx[2], pf_vz[2], pf_rho[2], pf_irho[2], pf_lam[2], pf_mu[2];
+  float pf_syx[2] = {0, 0}, pf_syy[2] = {0, 0}, pf_syz[2] = {0, 0}, pf_vyc[2] = {0, 0}, pf_jj[2] = {1, 1},
+        pf_ij[2] = {1, 1}, pf_nu[2] = {0, 0}, pf_muc[2] = {0, 0}, pf_nuc[2] = {0, 0};
+  for (int q = 0; q < 2; ++q) {
+    const uint32_t r1 = g.row(i0 + q), r0 = g.row(i0 - 1 + q), rm2 = g.row(i0 - 3 + q);
+    pf_sxx[q] = bload(rin, jc4, o_sxx + r1);
+    pf_sxz[q] = bload(rin, jc4, o_sxz + r1);
+    pf_szz[q] = bload(rin, jc4, o_szz + r1);
+    pf_vx[q] = bload(rin, jc4, o_vx + r0);
+    pf_vz[q] = bload(rin, jc4, o_vz + r0);
+    pf_rho[q] = bload(rmod, jc4, m_rho + r0);
+    pf_irho[q] = bload(rmod, jc4, m_irho + r0);
+    pf_lam[q] = bload(rmod, jc4, m_lam + rm2);
+    pf_mu[q] = bload(rmod, jc4, m_mu + rm2);
+    if constexpr (SPIN) {
+      pf_syx[q] = bload(rin, jc4, o_syx + r1);
+      pf_syy[q] = bload(rin, jc4, o_syy + r1);
+      pf_syz[q] = bload(rin, jc4, o_syz + r1);
+      pf_vyc[q] = bload(rin, jc4, o_vyc + r0);
+      pf_jj[q] = bload(rmod, jc4, m_j + r0);
+      pf_ij[q] = bload(rmod, jc4, m_ij + r0);
+      pf_nu[q] = bload(rmod, jc4, m_nu + rm2);
+      pf_muc[q] = bload(rmod, jc4, m_muc + rm2);
+      pf_nuc[q] = bload(rmod, jc4, m_nuc + rm2);
+    }
   }
-  float pf_vx = bload(rin, jc4, o_vx + g.row(i0 - 1)), pf_vz = bload(rin, jc4, o_vz + g.row(i0 - 1));
-  float pf_rho = bload(rmod, jc4, m_rho + g.row(i0 - 1));
-  float pf_lam = bload(rmod, jc4, m_lam + g.row(i0 - 3)), pf_mu = bload(rmod, jc4, m_mu + g.row(i0 - 3));
 
   auto body = [&](auto phase, int m) __attribute__((always_inline)) {
     constexpr int R = decltype(phase)::value;
     constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
-    sxx[s3] = pf_sxx; sxz[s3] = pf_sxz; szz[s3] = pf_szz;     // row m+1
-    if constexpr (SPIN) { syx[s3] = pf_syx; syy[s3] = pf_syy; syz[s3] = pf_syz; }
-    const float vx_o = pf_vx, vz_o = pf_vz, rho = pf_rho, lam = pf_lam, mu = pf_mu;
-    const float vyc_o = pf_vyc, jj = pf_jj, nu = pf_nu, muc = pf_muc, nuc = pf_nuc;
+    constexpr int f = R & 1;
+    sxx[s3] = pf_sxx[f]; sxz[s3] = pf_sxz[f]; szz[s3] = pf_szz[f];     // row m+1
+    if constexpr (SPIN) { syx[s3] = pf_syx[f]; syy[s3] = pf_syy[f]; syz[s3] = pf_syz[f]; }
+    const float vx_o = pf_vx[f], vz_o = pf_vz[f], rho = pf_rho[f], irho = pf_irho[f], lam = pf_lam[f], mu = pf_mu[f];
+    const float vyc_o = pf_vyc[f], jj = pf_jj[f], ij = pf_ij[f], nu = pf_nu[f], muc = pf_muc[f], nuc = pf_nuc[f];
     {
-      const uint32_t r2 = g.row(m + 2), r1 = g.row(m + 1), rm1 = g.row(m - 1);
-      pf_sxx = bload(rin, jc4, o_sxx + r2);
-      pf_sxz = bload(rin, jc4, o_sxz + r2);
-      pf_szz = bload(rin, jc4, o_szz + r2);
-      pf_vx = bload(rin, jc4, o_vx + r1);
-      pf_vz = bload(rin, jc4, o_vz + r1);
-      pf_rho = bload(rmod, jc4, m_rho + r1);
-      pf_lam = bload(rmod, jc4, m_lam + rm1);
-      pf_mu = bload(rmod, jc4, m_mu + rm1);
+      const uint32_t r3 = g.row(m + 3), r2 = g.row(m + 2), r0 = g.row(m);
+      pf_sxx[f] = bload(rin, jc4, o_sxx + r3);
+      pf_sxz[f] = bload(rin, jc4, o_sxz + r3);
+      pf_szz[f] = bload(rin, jc4, o_szz + r3);
+      pf_vx[f] = bload(rin, jc4, o_vx + r2);
+      pf_vz[f] = bload(rin, jc4, o_vz + r2);
+      pf_rho[f] = bload(rmod, jc4, m_rho + r2);
+      pf_irho[f] = bload(rmod, jc4, m_irho + r2);
+      pf_lam[f] = bload(rmod, jc4, m_lam + r0);
+      pf_mu[f] = bload(rmod, jc4, m_mu + r0);
       if constexpr (SPIN) {
-        pf_syx = bload(rin, jc4, o_syx + r2);
-        pf_syy = bload(rin, jc4, o_syy + r2);
-        pf_syz = bload(rin, jc4, o_syz + r2);
-        pf_vyc = bload(rin, jc4, o_vyc + r1);
-        pf_jj = bload(rmod, jc4, m_j + r1);
-        pf_nu = bload(rmod, jc4, m_nu + rm1);
-        pf_muc = bload(rmod, jc4, m_muc + rm1);
-        pf_nuc = bload(rmod, jc4, m_nuc + rm1);
+        pf_syx[f] = bload(rin, jc4, o_syx + r3);
+        pf_syy[f] = bload(rin, jc4, o_syy + r3);
+        pf_syz[f] = bload(rin, jc4, o_syz + r3);
+        pf_vyc[f] = bload(rin, jc4, o_vyc + r2);
+        pf_jj[f] = bload(rmod, jc4, m_j + r2);
+        pf_ij[f] = bload(rmod, jc4, m_ij + r2);
+        pf_nu[f] = bload(rmod, jc4, m_nu + r0);
+        pf_muc[f] = bload(rmod, jc4, m_muc + r0);
+        pf_nuc[f] = bload(rmod, jc4, m_nuc + r0);
       }
     }
-    const float bxm = a.bx[min(max(m, 0), nx - 1)];
+    const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // ---- velocity half-step at row m
     lds_s[par][0][t] = sxz[s2];
@@ -375,20 +417,22 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
       dsz = dsz + (-(xm_ok ? dminus(syy[s0], syy[s1], syy[s2], syy[s3], h8x, r8x) : 0.0f));
       dsyc = dsyc + 2.0f * syy[s2];
     }
-    while (ip < ie) {
-      const int4 e = a.inj[ip];
-      if (e.x > m) break;
-      if (e.x == m && e.y == j) {
-        dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it];
-        dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it];
+    if (m == inj_row) {
+      for (; ip < ie; ++ip) {
+        const int4 e = a.inj[ip];
+        if (e.x != m) break;
+        if (e.y == j) {
+          dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it];
+          dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it];
+        }
       }
-      ++ip;
+      inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;
     }
     const float b = (bxm * bzb) * bzt;
-    const float vxn = b * (vx_o + dt * dsx / rho);
-    const float vzn = b * (vz_o + dt * dsz / rho);
+    const float vxn = b * (vx_o + div_cell(dt * dsx, rho, irho));
+    const float vzn = b * (vz_o + div_cell(dt * dsz, rho, irho));
     float vycn = 0.0f;
-    if constexpr (SPIN) vycn = b * (vyc_o + dt * dsyc / jj);
+    if constexpr (SPIN) vycn = b * (vyc_o + div_cell(dt * dsyc, jj, ij));
     if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
       bstore(vxn, rout, jc4, o_vx + ro);
@@ -413,12 +457,12 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
       dvycdz[s2] = zp_ok ? dplus(lds_v[par][2][tm1], vycn, lds_v[par][2][tp1], lds_v[par][2][tp2], h8z, r8z) : 0.0f;
 
     // ---- receivers: u = sum v*dt at the cells the traces need
-    if (a.record) {
-      if constexpr (!SPIN) {
-        while (rp < re) {
+    if constexpr (!SPIN) {
+      if (m == rec_row) {
+        for (; rp < re; ++rp) {
           const int4 e = a.rec[rp];
-          if (e.x > m) break;
-          if (e.x == m && e.y == j) {
+          if (e.x != m) break;
+          if (e.y == j) {
             float* u = uacc + e.z * N_UACC;
             const float ux = u[U_UX] + vxn * dt, uz = u[U_UZ] + vzn * dt;
             u[U_UX] = ux;
@@ -426,11 +470,12 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
             obs[0 * nobs + e.z * a.nt + it] = ux;
             obs[2 * nobs + e.z * a.nt + it] = uz;
           }
-          ++rp;
         }
-      } else {
-        // a receiver on row ir needs uz on rows ir-1..ir+2 and ux on columns j-1..j+2
-        while (rp < re && a.rec[rp].x + 2 < m) ++rp;
+        rec_row = rp < re ? a.rec[rp].x : 0x7fffffff;
+      }
+    } else {
+      // a receiver on row ir needs uz on rows ir-1..ir+2 and ux on columns j-1..j+2
+      if (m >= rec_row - 1) {
         for (int q = rp; q < re; ++q) {
           const int4 e = a.rec[q];
           if (e.x - 1 > m) break;
@@ -467,6 +512,8 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
             }
           }
         }
+        while (rp < re && a.rec[rp].x + 2 <= m) ++rp;
+        rec_row = rp < re ? a.rec[rp].x : 0x7fffffff;
       }
     }
 
@@ -1022,6 +1069,11 @@ __global__ void k_vps2lm(const float* __restrict__ rho, float* __restrict__ lam,
   lam[k] = (vp > vs) ? r * (vp * vp - 2.0f * vs * vs) : 0.0f;
 }
 
+__global__ void k_recip(const float* __restrict__ x, float* __restrict__ out, int64_t n) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = 1.0f / x[k];
+}
+
 __global__ void k_fill(float* __restrict__ p, float v, int64_t n) {
   const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k < n) p[k] = v;
@@ -1159,6 +1211,11 @@ hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStrea
 }
 hipError_t launch_vps2lm(const float* rho, float* lam, float* mu, int64_t npt, hipStream_t s) {
   hipLaunchKernelGGL(k_vps2lm, dim3(unsigned((npt + 255) / 256)), dim3(256), 0, s, rho, lam, mu, npt);
+  return hipGetLastError();
+}
+hipError_t launch_recip(const float* x, float* out, int64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_recip, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, x, out, n);
   return hipGetLastError();
 }
 hipError_t launch_fill(float* p, float v, int64_t n, hipStream_t s) {

@@ -80,6 +80,8 @@ struct sw_ctx {
   int64_t npt = 0;
   float h8x = 0, h8z = 0;
   bool model_set = false;
+  // every rho / J in [2^-20, 2^20]: divisions by them may use the reciprocal form
+  bool rho_fast = false, j_fast = false;
 
   // model fields [MF_N][npt]: rho, lambda, mu, nu, j, lambda_c, mu_c, nu_c
   DBuf<float> model;
@@ -190,6 +192,12 @@ void build_taper(sw_ctx* c) {
 // Default tile shape: the widest strip and longest segment that still give
 // the GPU's 256 CUs several workgroups each (>= 1024 workgroups).  Long
 // segments amortise each workgroup's start-up (list lookups, first rows).
+bool all_fast_divisors(const float* v, size_t n) {
+  for (size_t k = 0; k < n; ++k)
+    if (!fast_divisor(v[k])) return false;
+  return true;
+}
+
 void heuristic_tiling(sw_ctx* c, int nshots) {
   Tiling& t = c->tile;
   auto count = [&](int nt, int seg) {
@@ -365,6 +373,8 @@ StepArgs make_args(sw_ctx* c, StepMode mode) {
   a.nstrips = c->tile.nstrips; a.nsegs = c->tile.nsegs; a.seglen = c->tile.seglen;
   a.dt = c->g.dt; a.h8x = c->h8x; a.h8z = c->h8z;
   a.r8x = 1.0f / c->h8x; a.r8z = 1.0f / c->h8z;
+  a.fast_div = ((fast_divisor(c->h8x) && fast_divisor(c->h8z)) ? FD_CONST : 0) |
+               (c->rho_fast ? FD_RHO : 0) | (c->j_fast ? FD_J : 0);
   a.inj_off = c->inj_off.p; a.inj = c->inj.p;
   a.val_x = c->val_x; a.val_y = c->val_y; a.val_z = c->val_z; a.nt_val = c->nt_val;
   a.rec_off = c->rec_off.p; a.rec = c->rec.p;
@@ -619,6 +629,8 @@ SW_API int sw_set_model(sw_ctx* c, int32_t kind, const float* const* f) {
   int rc = 0;
   if (kind == SW_MODEL_VPVS) {
     if (!f[0] || !f[1] || !f[2]) return fail(-1, "rho, vp and vs are required");
+    c->rho_fast = all_fast_divisors(f[0], n);
+    c->j_fast = false;   // J = 0: the spin fields need the LAME model kind
     if ((rc = up(MF_RHO, f[0])) || (rc = up(MF_LAM, f[1])) || (rc = up(MF_MU, f[2]))) return rc;
     HIPCHK(launch_vps2lm(c->mf(MF_RHO), c->mf(MF_LAM), c->mf(MF_MU), c->npt, c->stream));
   } else if (kind == SW_MODEL_LAME) {
@@ -627,6 +639,8 @@ SW_API int sw_set_model(sw_ctx* c, int32_t kind, const float* const* f) {
       if (!f[q]) return fail(-1, "model field %d missing", q);
       if ((rc = up(order[q], f[q]))) return rc;
     }
+    c->rho_fast = all_fast_divisors(f[0], n);
+    c->j_fast = all_fast_divisors(f[4], n);
   } else {
     return fail(-2, "unknown model kind %d", kind);
   }
@@ -767,8 +781,14 @@ SW_API int sw_write_field(sw_ctx* c, int32_t shot, int32_t field, const float* i
   }
   if (!dst) return fail(-2, "field %d is not materialised for this configuration", field);
   HIPCHK(hipMemcpyAsync(dst, in, size_t(c->npt) * 4, hipMemcpyHostToDevice, c->stream));
-  if (field == SW_F_RHO) HIPCHK(launch_recip(c->mf(MF_RHO), c->mf(MF_IRHO), c->npt, c->stream));
-  if (field == SW_F_J) HIPCHK(launch_recip(c->mf(MF_J), c->mf(MF_IJ), c->npt, c->stream));
+  if (field == SW_F_RHO) {
+    HIPCHK(launch_recip(c->mf(MF_RHO), c->mf(MF_IRHO), c->npt, c->stream));
+    c->rho_fast = all_fast_divisors(in, size_t(c->npt));
+  }
+  if (field == SW_F_J) {
+    HIPCHK(launch_recip(c->mf(MF_J), c->mf(MF_IJ), c->npt, c->stream));
+    c->j_fast = all_fast_divisors(in, size_t(c->npt));
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

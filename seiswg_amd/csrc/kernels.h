@@ -34,6 +34,12 @@ struct Tiling {
 // MF_IRHO / MF_IJ hold the correctly rounded reciprocals 1/rho, 1/J.
 enum ModelField : int { MF_RHO = 0, MF_LAM, MF_MU, MF_NU, MF_J, MF_LAM_C, MF_MU_C, MF_NU_C, MF_IRHO, MF_IJ, MF_N };
 
+// Exact divisions by 8h (FD_CONST), rho (FD_RHO) and J (FD_J) may use the
+// reciprocal-and-correction form when every such divisor lies in
+// [2^-20, 2^20]; otherwise the kernels divide with the IEEE sequence.
+enum FastDiv : int { FD_CONST = 1, FD_RHO = 2, FD_J = 4 };
+inline bool fast_divisor(float y) { return y >= 0x1p-20f && y <= 0x1p20f; }
+
 struct StepArgs {
   // ping-pong state buffers, each laid out [shot][slot][npt].  A forward step
   // `it` reads st[it & 1] and writes st[(it+1) & 1]; an adjoint step `it`
@@ -48,6 +54,7 @@ struct StepArgs {
   int nstrips, nsegs, seglen;
   float dt, h8x, h8z;         // h8 = 8*h (fd2d.wgsl:156)
   float r8x, r8z;             // correctly rounded 1/h8
+  int fast_div;               // FastDiv bits
   // body-force injections: CSR over (shot, tile), entries (i, j, q) sorted by i
   const int* inj_off; const int4* inj;
   const float* val_x; const float* val_y; const float* val_z;   // [q][nt_val]
@@ -93,6 +100,8 @@ hipError_t launch_adjsrc(const float* obs_syn, const float* obs_ref, const float
 hipError_t launch_reduce_shots(const float* kshot, float* kout, int nshots, int64_t n, int64_t shot_stride,
                                hipStream_t s);
 bool tiling_supported(int nt_threads);
+// resident workgroups per CU of the step kernel of a MultiKind at a tile width
+int step_blocks_per_cu(int kind, int nt_threads);
 
 hipError_t launch_imaging_sh(const float* vadj, const float* vfw, float* kmu, int nx, int nz,
                              float h8x, float h8z, float ndt, hipStream_t s);

@@ -24,29 +24,38 @@
 namespace sw {
 namespace {
 
-// x / y for a divisor y that is the same for the whole launch, with r the
-// correctly rounded 1/y: q0 = x*r is within 1 ulp of x/y, the remainder
-// x - q0*y is exact in one FMA and q0 + rem*r (one FMA) is the correctly
-// rounded quotient (Markstein's correction), i.e. bit-identical to IEEE
-// division for normal results, at 3 VALU instead of the ~9 of the generic
-// division sequence.
-__device__ __forceinline__ float div_const(float x, float y, float r) {
-  const float q0 = x * r;
-  const float rem = __builtin_fmaf(-q0, y, x);
-  return __builtin_fmaf(rem, r, q0);
+// Exact division x / y with the correctly rounded reciprocal r = RN(1/y)
+// known in advance (Markstein's correction): q0 = x*r, e = q0*y - x (exact in
+// one FMA), q = q0 - e*r.  For y in [2^-20, 2^20] (checked on the host, which
+// clears `fast` otherwise) and x zero or 2^-100 <= |x| <= 2^100, q equals the
+// IEEE quotient bit for bit, signed zeros included (verified exhaustively on
+// whole binades at the range ends and in between).  Outside that range -
+// numerators so small that the quotient is subnormal, where the correction
+// can miss by one unit - the lanes concerned take the IEEE division.  3 VALU
+// plus a 3-VALU range test instead of the ~10 of the generic sequence.
+struct Div {
+  float y, r;
+  bool fast;
+};
+__device__ __forceinline__ bool markstein_range(float x) {
+  const float ax = __builtin_fabsf(x);
+  return __builtin_amdgcn_fmed3f(ax, 0x1p-100f, 0x1p100f) == ax || x == 0.0f;
 }
-
-// x / y for a per-cell divisor with its correctly rounded reciprocal r stored
-// beside it: the same Markstein correction, exact for normal results.
-__device__ __forceinline__ float div_cell(float x, float y, float r) { return div_const(x, y, r); }
+__device__ __forceinline__ float div_exact(float x, Div d) {
+  const float q0 = x * d.r;
+  const float e = __builtin_fmaf(q0, d.y, -x);
+  const float q = __builtin_fmaf(-e, d.r, q0);
+  if (__builtin_expect(d.fast && markstein_range(x), 1)) return q;
+  return x / d.y;
+}
 
 // fd2d.wgsl:151-164 (backward) and 166-180 (forward) differences, with the
-// reference's 1/8 weight on the 3-cell difference; h8 = 8*h, r8 = 1/h8.
-__device__ __forceinline__ float dminus(float fm2, float fm1, float f0, float fp1, float h8, float r8) {
-  return div_const(9.0f * (f0 - fm1) - (fp1 - fm2), h8, r8);
+// reference's 1/8 weight on the 3-cell difference; d.y = 8*h, d.r = 1/(8*h).
+__device__ __forceinline__ float dminus(float fm2, float fm1, float f0, float fp1, Div d) {
+  return div_exact(9.0f * (f0 - fm1) - (fp1 - fm2), d);
 }
-__device__ __forceinline__ float dplus(float fm1, float f0, float fp1, float fp2, float h8, float r8) {
-  return div_const(9.0f * (fp1 - f0) - (fp2 - fm1), h8, r8);
+__device__ __forceinline__ float dplus(float fm1, float f0, float fp1, float fp2, Div d) {
+  return div_exact(9.0f * (fp1 - f0) - (fp2 - fm1), d);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Map them so that
@@ -124,7 +133,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   __shared__ float lds_s[2][NT];
   __shared__ float lds_v[2][NT];
   const TileGeom g = tile_geom<NT>(a, logical);
-  const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
+  const int t = g.t, j = g.j, nx = a.nx, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (it < a.it_lo || it >= a.it_hi) return;
   const int64_t so = int64_t(g.shot) * a.shot_floats;
@@ -139,7 +148,9 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
-  const float dt = a.dt, h8x = a.h8x, h8z = a.h8z, r8x = a.r8x, r8z = a.r8z;
+  const float dt = a.dt;
+  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const bool fast_rho = (a.fast_div & FD_RHO) != 0;
   const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;
   int ip = a.inj_off[g.shot * tps + g.tile];
@@ -196,8 +207,8 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     lds_s[par][t] = szy[s2];
     __syncthreads();
     const bool xm_ok = m >= 2 && m < nx - 2;
-    float dsy = (xm_ok ? dminus(sxy[s0], sxy[s1], sxy[s2], sxy[s3], h8x, r8x) : 0.0f) +
-                (g.zm_ok ? dminus(lds_s[par][tm2], lds_s[par][tm1], szy[s2], lds_s[par][tp1], h8z, r8z) : 0.0f);
+    float dsy = (xm_ok ? dminus(sxy[s0], sxy[s1], sxy[s2], sxy[s3], Dx) : 0.0f) +
+                (g.zm_ok ? dminus(lds_s[par][tm2], lds_s[par][tm1], szy[s2], lds_s[par][tp1], Dz) : 0.0f);
     if (m == inj_row) {
       for (; ip < ie; ++ip) {
         const int4 e = a.inj[ip];
@@ -207,7 +218,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
       inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;
     }
     const float b = (bxm * bzb) * bzt;
-    const float vyn = b * (vy_o + div_cell(dt * dsy, rho, irho));
+    const float vyn = b * (vy_o + div_exact(dt * dsy, Div{rho, irho, fast_rho}));
     if (g.jst && m >= i0 && m < i1) {
       bstore(vyn, rout, jc4, o_vy + uint32_t(m) * g.rowb);
       if constexpr (AUX) bstore(dsy, raux, jc4, A_DSY * npt4 + uint32_t(m) * g.rowb);
@@ -215,7 +226,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     vy[s3] = vyn;              // new-velocity ring: rows m-3..m in slots s0..s3
     lds_v[par][t] = vyn;
     __syncthreads();
-    dvydz[s2] = g.zp_ok ? dplus(lds_v[par][tm1], vyn, lds_v[par][tp1], lds_v[par][tp2], h8z, r8z) : 0.0f;
+    dvydz[s2] = g.zp_ok ? dplus(lds_v[par][tm1], vyn, lds_v[par][tp1], lds_v[par][tp2], Dz) : 0.0f;
 
     // receivers on row m: uy += vy*dt, trace = uy
     if (m == rec_row) {
@@ -237,7 +248,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     if (i >= i0) {
       const bool xp_ok = i >= 1 && i < nx - 2;
       const float mu_dt = mu * dt;
-      const float dvydx = xp_ok ? dplus(vy[s0], vy[s1], vy[s2], vy[s3], h8x, r8x) : 0.0f;
+      const float dvydx = xp_ok ? dplus(vy[s0], vy[s1], vy[s2], vy[s3], Dx) : 0.0f;
       const float nsxy = sxy[s0] + mu_dt * dvydx;
       const float nszy = szy[s0] + mu_dt * dvydz[s0];
       if (g.jst) {
@@ -299,7 +310,9 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
-  const float dt = a.dt, h8x = a.h8x, h8z = a.h8z, r8x = a.r8x, r8z = a.r8z;
+  const float dt = a.dt;
+  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const bool fast_rho = (a.fast_div & FD_RHO) != 0, fast_j = (a.fast_div & FD_J) != 0;
   const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;
   int ip = a.inj_off[g.shot * tps + g.tile];
@@ -401,20 +414,20 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     __syncthreads();
     const bool xm_ok = m >= 2 && m < nx - 2;
     const bool zm_ok = g.zm_ok;
-    float dsx = (xm_ok ? dminus(sxx[s0], sxx[s1], sxx[s2], sxx[s3], h8x, r8x) : 0.0f) +
-                (zm_ok ? dminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[s2], lds_s[par][0][tp1], h8z, r8z)
+    float dsx = (xm_ok ? dminus(sxx[s0], sxx[s1], sxx[s2], sxx[s3], Dx) : 0.0f) +
+                (zm_ok ? dminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[s2], lds_s[par][0][tp1], Dz)
                        : 0.0f);
-    float dsz = (xm_ok ? dminus(sxz[s0], sxz[s1], sxz[s2], sxz[s3], h8x, r8x) : 0.0f) +
-                (zm_ok ? dminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[s2], lds_s[par][1][tp1], h8z, r8z)
+    float dsz = (xm_ok ? dminus(sxz[s0], sxz[s1], sxz[s2], sxz[s3], Dx) : 0.0f) +
+                (zm_ok ? dminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[s2], lds_s[par][1][tp1], Dz)
                        : 0.0f);
     float dsyc = 0.0f;
     if constexpr (SPIN) {
-      dsyc = (xm_ok ? dminus(syx[s0], syx[s1], syx[s2], syx[s3], h8x, r8x) : 0.0f) +
-             (zm_ok ? dminus(lds_s[par][2][tm2], lds_s[par][2][tm1], syz[s2], lds_s[par][2][tp1], h8z, r8z)
+      dsyc = (xm_ok ? dminus(syx[s0], syx[s1], syx[s2], syx[s3], Dx) : 0.0f) +
+             (zm_ok ? dminus(lds_s[par][2][tm2], lds_s[par][2][tm1], syz[s2], lds_s[par][2][tp1], Dz)
                     : 0.0f);
-      dsx = dsx + (zm_ok ? dminus(lds_s[par][3][tm2], lds_s[par][3][tm1], syy[s2], lds_s[par][3][tp1], h8z, r8z)
+      dsx = dsx + (zm_ok ? dminus(lds_s[par][3][tm2], lds_s[par][3][tm1], syy[s2], lds_s[par][3][tp1], Dz)
                          : 0.0f);
-      dsz = dsz + (-(xm_ok ? dminus(syy[s0], syy[s1], syy[s2], syy[s3], h8x, r8x) : 0.0f));
+      dsz = dsz + (-(xm_ok ? dminus(syy[s0], syy[s1], syy[s2], syy[s3], Dx) : 0.0f));
       dsyc = dsyc + 2.0f * syy[s2];
     }
     if (m == inj_row) {
@@ -429,10 +442,10 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
       inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;
     }
     const float b = (bxm * bzb) * bzt;
-    const float vxn = b * (vx_o + div_cell(dt * dsx, rho, irho));
-    const float vzn = b * (vz_o + div_cell(dt * dsz, rho, irho));
+    const float vxn = b * (vx_o + div_exact(dt * dsx, Div{rho, irho, fast_rho}));
+    const float vzn = b * (vz_o + div_exact(dt * dsz, Div{rho, irho, fast_rho}));
     float vycn = 0.0f;
-    if constexpr (SPIN) vycn = b * (vyc_o + div_cell(dt * dsyc, jj, ij));
+    if constexpr (SPIN) vycn = b * (vyc_o + div_exact(dt * dsyc, Div{jj, ij, fast_j}));
     if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
       bstore(vxn, rout, jc4, o_vx + ro);
@@ -451,10 +464,10 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     if constexpr (SPIN) lds_v[par][2][t] = vycn;
     __syncthreads();
     const bool zp_ok = g.zp_ok;
-    dvxdz[s2] = zp_ok ? dplus(lds_v[par][0][tm1], vxn, lds_v[par][0][tp1], lds_v[par][0][tp2], h8z, r8z) : 0.0f;
-    dvzdz[s2] = zp_ok ? dplus(lds_v[par][1][tm1], vzn, lds_v[par][1][tp1], lds_v[par][1][tp2], h8z, r8z) : 0.0f;
+    dvxdz[s2] = zp_ok ? dplus(lds_v[par][0][tm1], vxn, lds_v[par][0][tp1], lds_v[par][0][tp2], Dz) : 0.0f;
+    dvzdz[s2] = zp_ok ? dplus(lds_v[par][1][tm1], vzn, lds_v[par][1][tp1], lds_v[par][1][tp2], Dz) : 0.0f;
     if constexpr (SPIN)
-      dvycdz[s2] = zp_ok ? dplus(lds_v[par][2][tm1], vycn, lds_v[par][2][tp1], lds_v[par][2][tp2], h8z, r8z) : 0.0f;
+      dvycdz[s2] = zp_ok ? dplus(lds_v[par][2][tm1], vycn, lds_v[par][2][tp1], lds_v[par][2][tp2], Dz) : 0.0f;
 
     // ---- receivers: u = sum v*dt at the cells the traces need
     if constexpr (!SPIN) {
@@ -499,8 +512,8 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
               const float ux = u[U_UX], uz = u[U_UZ], uyc = u[U_UYC];
               const bool xp = e.x >= 1 && e.x < nx - 2;
               const bool zp = e.y >= 1 && e.y < nz - 2;
-              const float duzdx = xp ? dplus(u[U_UZ_M1], uz, u[U_UZ_P1], uz_p2, h8x, r8x) : 0.0f;
-              const float duxdz = zp ? dplus(u[U_UX_M1], ux, u[U_UX_P1], u[U_UX_P2], h8z, r8z) : 0.0f;
+              const float duzdx = xp ? dplus(u[U_UZ_M1], uz, u[U_UZ_P1], uz_p2, Dx) : 0.0f;
+              const float duxdz = zp ? dplus(u[U_UX_M1], ux, u[U_UX_P1], u[U_UX_P2], Dz) : 0.0f;
               const float curl = -(duzdx - duxdz);
               const float iso = uyc - 0.5f * curl;
               const int kt = e.z * a.nt + it;
@@ -521,8 +534,8 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     const int i = m - 2;
     if (i >= i0) {
       const bool xp_ok = i >= 1 && i < nx - 2;
-      const float dvxdx = xp_ok ? dplus(vx[s0], vx[s1], vx[s2], vx[s3], h8x, r8x) : 0.0f;
-      const float dvzdx = xp_ok ? dplus(vz[s0], vz[s1], vz[s2], vz[s3], h8x, r8x) : 0.0f;
+      const float dvxdx = xp_ok ? dplus(vx[s0], vx[s1], vx[s2], vx[s3], Dx) : 0.0f;
+      const float dvzdx = xp_ok ? dplus(vz[s0], vz[s1], vz[s2], vz[s3], Dx) : 0.0f;
       const float l2m = lam + 2.0f * mu;
       const float nsxx = sxx[s0] + dt * (l2m * dvxdx + lam * dvzdz[s0]);
       const float nszz = szz[s0] + dt * (l2m * dvzdz[s0] + lam * dvxdx);
@@ -539,7 +552,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
         }
       }
       if constexpr (SPIN) {
-        const float dvycdx = xp_ok ? dplus(vyc[s0], vyc[s1], vyc[s2], vyc[s3], h8x, r8x) : 0.0f;
+        const float dvycdx = xp_ok ? dplus(vyc[s0], vyc[s1], vyc[s2], vyc[s3], Dx) : 0.0f;
         const float nsyy = syy[s0] + 2.0f * dt * nu * (-vyc[s1] - 0.5f * (dvzdx - dvxdz[s0]));
         const float cc = dt * (muc + nuc);
         const float nsyx = syx[s0] + cc * dvycdx;
@@ -582,11 +595,11 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
 // outside the rows/columns on which the forward operator is active (D+: 1 <=
 // i < n-2, D-: 2 <= i < n-2).  ds and the strains come from the aux fields
 // of the recomputed forward step.  Same marching/tiling as the forward step.
-__device__ __forceinline__ float ndm(float fm2, float fm1, float f0, float fp1, float h8, float r8) {
-  return -div_const(9.0f * (f0 - fm1) - (fp1 - fm2), h8, r8);
+__device__ __forceinline__ float ndm(float fm2, float fm1, float f0, float fp1, Div d) {
+  return -div_exact(9.0f * (f0 - fm1) - (fp1 - fm2), d);
 }
-__device__ __forceinline__ float ndp(float fm1, float f0, float fp1, float fp2, float h8, float r8) {
-  return -div_const(9.0f * (fp1 - f0) - (fp2 - fm1), h8, r8);
+__device__ __forceinline__ float ndp(float fm1, float f0, float fp1, float fp2, Div d) {
+  return -div_exact(9.0f * (fp1 - f0) - (fp2 - fm1), d);
 }
 
 template <int NT>
@@ -611,7 +624,8 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
-  const float dt = a.dt, h8x = a.h8x, h8z = a.h8z, r8x = a.r8x, r8z = a.r8z;
+  const float dt = a.dt;
+  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
   const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
   const bool zt = j >= 1 && j < nz - 2, zq = j >= 2 && j < nz - 2;
   auto xt = [&](int r) { return r >= 1 && r < nx - 2; };
@@ -675,8 +689,8 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
       if (e.x == m && e.y == j) vb = vb + a.val_y[int64_t(e.z) * a.nt_val + it];
       ++ip;
     }
-    vb = vb + (ndm(tx[s0], tx[s1], tx[s2], tx[s3], h8x, r8x) +
-               ndm(lds_t[par][tm2], lds_t[par][tm1], tz_cur, lds_t[par][tp1], h8z, r8z));
+    vb = vb + (ndm(tx[s0], tx[s1], tx[s2], tx[s3], Dx) +
+               ndm(lds_t[par][tm2], lds_t[par][tm1], tz_cur, lds_t[par][tp1], Dz));
     const float b = (bxm * bzb) * bzt;
     const float gv = b * vb;
     const bool own_m = g.jst && m >= i0 && m < i1;
@@ -691,14 +705,14 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     const float qz = zq ? qv : 0.0f;
     lds_q[par][t] = qz;
     __syncthreads();
-    dzq[s2] = ndp(lds_q[par][tm1], qz, lds_q[par][tp1], lds_q[par][tp2], h8z, r8z);
+    dzq[s2] = ndp(lds_q[par][tm1], qz, lds_q[par][tp1], lds_q[par][tp2], Dz);
     tz_cur = tz_next;
 
     // adjoint "stress" at row i = m-2
     const int i = m - 2;
     if (i >= i0 && g.jst) {
       const uint32_t ro = uint32_t(i) * g.rowb;
-      const float nsx = sbx[s0] + ndp(q[s0], q[s1], q[s2], q[s3], h8x, r8x);
+      const float nsx = sbx[s0] + ndp(q[s0], q[s1], q[s2], q[s3], Dx);
       const float nsz = sbz[s0] + dzq[s0];
       bstore(nsx, rout, jc4, o_sxy + ro);
       bstore(nsz, rout, jc4, o_szy + ro);
@@ -744,7 +758,8 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
-  const float dt = a.dt, h8x = a.h8x, h8z = a.h8z, r8x = a.r8x, r8z = a.r8z;
+  const float dt = a.dt;
+  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
   const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
   const bool zt = j >= 1 && j < nz - 2, zq = j >= 2 && j < nz - 2;
   auto xt = [&](int r) { return r >= 1 && r < nx - 2; };
@@ -827,10 +842,10 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
       }
       ++ip;
     }
-    vbx = vbx + (ndm(txx[s0], txx[s1], txx[s2], txx[s3], h8x, r8x) +
-                 ndm(lds_t[par][0][tm2], lds_t[par][0][tm1], tzxz_cur, lds_t[par][0][tp1], h8z, r8z));
-    vbz = vbz + (ndm(lds_t[par][1][tm2], lds_t[par][1][tm1], tzzz_cur, lds_t[par][1][tp1], h8z, r8z) +
-                 ndm(txz[s0], txz[s1], txz[s2], txz[s3], h8x, r8x));
+    vbx = vbx + (ndm(txx[s0], txx[s1], txx[s2], txx[s3], Dx) +
+                 ndm(lds_t[par][0][tm2], lds_t[par][0][tm1], tzxz_cur, lds_t[par][0][tp1], Dz));
+    vbz = vbz + (ndm(lds_t[par][1][tm2], lds_t[par][1][tm1], tzzz_cur, lds_t[par][1][tp1], Dz) +
+                 ndm(txz[s0], txz[s1], txz[s2], txz[s3], Dx));
     const float b = (bxm * bzb) * bzt;
     const float gx = b * vbx, gz = b * vbz;
     if (g.jst && m >= i0 && m < i1) {
@@ -847,8 +862,8 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     lds_q[par][0][t] = qxz;
     lds_q[par][1][t] = qzz;
     __syncthreads();
-    dzqx[s2] = ndp(lds_q[par][0][tm1], qxz, lds_q[par][0][tp1], lds_q[par][0][tp2], h8z, r8z);
-    dzqz[s2] = ndp(lds_q[par][1][tm1], qzz, lds_q[par][1][tp1], lds_q[par][1][tp2], h8z, r8z);
+    dzqx[s2] = ndp(lds_q[par][0][tm1], qxz, lds_q[par][0][tp1], lds_q[par][0][tp2], Dz);
+    dzqz[s2] = ndp(lds_q[par][1][tm1], qzz, lds_q[par][1][tp1], lds_q[par][1][tp2], Dz);
     tzxz_cur = tzxz_next;
     tzzz_cur = tzzz_next;
 
@@ -856,9 +871,9 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     const int i = m - 2;
     if (i >= i0 && g.jst) {
       const uint32_t ro = uint32_t(i) * g.rowb;
-      const float nxx = sbxx[s0] + ndp(qx[s0], qx[s1], qx[s2], qx[s3], h8x, r8x);
+      const float nxx = sbxx[s0] + ndp(qx[s0], qx[s1], qx[s2], qx[s3], Dx);
       const float nzz = sbzz[s0] + dzqz[s0];
-      const float nxz = sbxz[s0] + (dzqx[s0] + ndp(qz[s0], qz[s1], qz[s2], qz[s3], h8x, r8x));
+      const float nxz = sbxz[s0] + (dzqx[s0] + ndp(qz[s0], qz[s1], qz[s2], qz[s3], Dx));
       bstore(nxx, rout, jc4, o_sxx + ro);
       bstore(nzz, rout, jc4, o_szz + ro);
       bstore(nxz, rout, jc4, o_sxz + ro);
@@ -971,7 +986,9 @@ __global__ void k_reduce_shots(const float* __restrict__ kshot, float* __restric
 // ------------------------------------------------ plain SH imaging condition
 // div_uy, div_fw, interaction_muy (fd2d.wgsl:501-509, 571-588)
 __global__ void k_imaging_sh(const float* __restrict__ va, const float* __restrict__ vf, float* __restrict__ kmu,
-                             int nx, int nz, float h8x, float h8z, float r8x, float r8z, float ndt) {
+                             int nx, int nz, float h8x, float h8z, float r8x, float r8z, float ndt,
+                             int fast_div) {
+  const Div Dx{h8x, r8x, (fast_div & FD_CONST) != 0}, Dz{h8z, r8z, (fast_div & FD_CONST) != 0};
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= nz) return;
@@ -979,12 +996,12 @@ __global__ void k_imaging_sh(const float* __restrict__ va, const float* __restri
   const bool xp = i >= 1 && i < nx - 2, zp = j >= 1 && j < nz - 2;
   float ax = 0.0f, az = 0.0f, fx = 0.0f, fz = 0.0f;
   if (xp) {
-    ax = dplus(va[k - nz], va[k], va[k + nz], va[k + 2 * nz], h8x, r8x);
-    fx = dplus(vf[k - nz], vf[k], vf[k + nz], vf[k + 2 * nz], h8x, r8x);
+    ax = dplus(va[k - nz], va[k], va[k + nz], va[k + 2 * nz], Dx);
+    fx = dplus(vf[k - nz], vf[k], vf[k + nz], vf[k + 2 * nz], Dx);
   }
   if (zp) {
-    az = dplus(va[k - 1], va[k], va[k + 1], va[k + 2], h8z, r8z);
-    fz = dplus(vf[k - 1], vf[k], vf[k + 1], vf[k + 2], h8z, r8z);
+    az = dplus(va[k - 1], va[k], va[k + 1], va[k + 2], Dz);
+    fz = dplus(vf[k - 1], vf[k], vf[k + 1], vf[k + 2], Dz);
   }
   kmu[k] = kmu[k] + (ax * fx + az * fz) * ndt;
 }
@@ -1107,6 +1124,36 @@ hipError_t launch_psv_nt(const StepArgs& a, const Tiling& t, hipStream_t s, bool
 
 bool tiling_supported(int nt) { return nt == 64 || nt == 128 || nt == 256; }
 
+namespace {
+template <typename K>
+int occupancy(K k, int nt) {
+  int n = 0;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, nt, 0) == hipSuccess ? n : 0;
+}
+template <int NT>
+int blocks_per_cu_nt(int kind) {
+  switch (kind) {
+    case MK_SH: return occupancy(k_step_sh<NT, false>, NT);
+    case MK_PSV: return occupancy(k_step_psv<NT, false, false>, NT);
+    case MK_SPIN: return occupancy(k_step_psv<NT, true, false>, NT);
+    case MK_SH_AUX: return occupancy(k_step_sh<NT, true>, NT);
+    case MK_PSV_AUX: return occupancy(k_step_psv<NT, false, true>, NT);
+    case MK_ADJ_SH: return occupancy(k_adj_sh<NT>, NT);
+    case MK_ADJ_PSV: return occupancy(k_adj_psv<NT>, NT);
+  }
+  return 0;
+}
+}  // namespace
+
+int step_blocks_per_cu(int kind, int nt_threads) {
+  switch (nt_threads) {
+    case 64: return blocks_per_cu_nt<64>(kind);
+    case 128: return blocks_per_cu_nt<128>(kind);
+    case 256: return blocks_per_cu_nt<256>(kind);
+  }
+  return 0;
+}
+
 hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
   switch (t.nt_threads) {
     case 64: return launch_sh_nt<64>(a, t, s, aux);
@@ -1179,8 +1226,9 @@ hipError_t launch_reduce_shots(const float* kshot, float* kout, int nshots, int6
 hipError_t launch_imaging_sh(const float* va, const float* vf, float* kmu, int nx, int nz, float h8x, float h8z,
                              float ndt, hipStream_t s) {
   const float r8x = 1.0f / h8x, r8z = 1.0f / h8z;
+  const int fd = (fast_divisor(h8x) && fast_divisor(h8z)) ? FD_CONST : 0;
   hipLaunchKernelGGL(k_imaging_sh, dim3((nz + 255) / 256, nx), dim3(256), 0, s, va, vf, kmu, nx, nz, h8x, h8z, r8x,
-                     r8z, ndt);
+                     r8z, ndt, fd);
   return hipGetLastError();
 }
 hipError_t launch_gsum(const float* sumx, const float* sumz, float* gsum, int nx, int nz, hipStream_t s) {

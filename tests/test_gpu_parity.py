@@ -93,6 +93,39 @@ def test_steps_from_random_state_match_oracle(seiswg, phys, tiling):
     for comp in comps:
         exact.append(compare(c.obs(0, comp)[:, :nsteps], obs[comp][:, :nsteps], f"obs{comp}"))
     print(f"bit-identical: {sum(exact)}/{len(exact)}")
+    assert all(exact)
+
+
+@pytest.mark.parametrize("phys", ["sh", "psv", "spin"])
+def test_steps_through_subnormal_range_match_oracle_bitwise(seiswg, phys):
+    """State fields spanning zero, the subnormal range and normal magnitudes:
+    divisions whose quotient is subnormal take the IEEE path in the kernels,
+    the others the reciprocal-and-correction path; both must equal the
+    oracle's IEEE divisions bit for bit."""
+    sh, psv, spin = phys == "sh", phys in ("psv", "spin"), phys == "spin"
+    nx, nz, nsteps = 40, 75, 3
+    p, stf = random_problem(nx, nz, 6, sh, psv, spin, seed=21)
+    stf = (stf * 1e-40).astype(np.float32)
+    print(f"case phys={phys}", flush=True)
+    c = gpu_context(p, stf, seiswg)
+    c.set_tiling(64, 9)
+    o = O.Oracle(p, 4, stf=stf.astype(np.float64))
+    o.clear()
+    rng = np.random.default_rng(8)
+    fields = (STATE["sh"] if sh else []) + (STATE["psv"] if psv else []) + (STATE["spin"] if spin else [])
+    c.prepare([-1])
+    for name in fields:
+        mag = 10.0 ** rng.uniform(-45, 2, nx * nz)
+        v = (rng.standard_normal(nx * nz) * mag * (rng.uniform(size=nx * nz) > 0.3)).astype(np.float32)
+        c.write_field(getattr(seiswg.abi, "F_" + name), v)
+        o.set_field(O.F[name], v)
+    c.advance(nsteps)
+    for it in range(nsteps):
+        o.step(it, -1)
+    for name in fields:
+        got = c.field(getattr(seiswg.abi, "F_" + name))
+        want = np.asarray(o.field(O.F[name]), np.float64).astype(np.float32)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), name
 
 
 @pytest.mark.parametrize("threads", [64, 256])

@@ -120,6 +120,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--threads", type=int, default=0, help="tile width (64/128/256); 0 = library default")
     ap.add_argument("--seglen", type=int, default=0)
+    ap.add_argument("--group", type=int, default=0, help="shots per workgroup (1, 2, 4); 0 = library default")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gradient", action="store_true", help="skip the batched-vs-plain gradient measurement")
     ap.add_argument("--gradient-nt", type=int, default=0, help="time steps of the gradient workload (0: 3000)")
@@ -136,11 +137,12 @@ def main():
         seiswg.bench_copy(1 << 30, 5)
     nt = max(synth.C2["nt"], a.warmup + a.steps)
     ctx, model, geom, src_idx, rec_idx = build_context(seiswg, synth, nt)
-    if a.threads or a.seglen:
-        t, s = ctx.tiling()
-        ctx.set_tiling(a.threads or t, a.seglen or s)
     nshots = synth.C2["nshots"]
     ctx.prepare(list(range(nshots)))
+    if a.threads or a.seglen or a.group:
+        t, s, g = ctx.tiling()
+        ctx.set_tiling(a.threads or t, a.seglen or s, a.group or g)
+        ctx.prepare(list(range(nshots)))
     if a.warmup:
         ctx.advance(a.warmup)
     t0 = time.perf_counter()

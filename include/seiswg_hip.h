@@ -171,16 +171,20 @@ SW_API int sw_multi_timing(const sw_multi* m, double* ms, int64_t* launches);
  * solver.rs:1249-1274; init_gsum, gaussian_x, gaussian_z). */
 SW_API int sw_smooth(sw_ctx* ctx, int32_t field, float sigma);
 
-/* Tile shape of the fused step kernels: threads per workgroup (64, 128 or
- * 256; a workgroup owns threads-6 z-columns) and rows marched per workgroup.
- * Without a call, each batch gets a heuristic shape sized to fill the GPU. */
-SW_API int sw_set_tiling(sw_ctx* ctx, int32_t threads, int32_t seglen);
-SW_API int sw_get_tiling(const sw_ctx* ctx, int32_t* threads, int32_t* seglen);
-/* Autotuner (no counterpart in the reference, which dispatches fixed 64-wide
- * 1-D workgroups): times `steps` forward steps of `nshots` shots for every
- * supported tile shape, keeps the fastest (as sw_set_tiling does) and
- * returns it with its ms per step.  Tile shapes never change results. */
-SW_API int sw_autotune(sw_ctx* ctx, int32_t nshots, int32_t steps, int32_t* threads, int32_t* seglen, double* ms);
+/* Tile shape of the fused step kernels (no counterpart in the reference,
+ * which dispatches fixed 64-wide 1-D workgroups): threads per shot (64, 128
+ * or 256; they own threads-6 z-columns), rows marched per workgroup, and
+ * shots per workgroup of the forward steps (0: automatic; 1; 2 or 4 with
+ * threads >= 128; reduced to a divisor of each batch's shot count).  Without
+ * a call, each batch gets a heuristic shape sized to fill the GPU.
+ * sw_get_tiling reports the shape of the last prepared batch. */
+SW_API int sw_set_tiling(sw_ctx* ctx, int32_t threads, int32_t seglen, int32_t shots_per_block);
+SW_API int sw_get_tiling(const sw_ctx* ctx, int32_t* threads, int32_t* seglen, int32_t* shots_per_block);
+/* Autotuner: times `steps` forward steps of `nshots` shots for a set of
+ * tile shapes, keeps the fastest (as sw_set_tiling does) and returns it with
+ * its ms per step.  Tile shapes never change results. */
+SW_API int sw_autotune(sw_ctx* ctx, int32_t nshots, int32_t steps, int32_t* threads, int32_t* seglen,
+                       int32_t* shots_per_block, double* ms);
 
 /* Timing of the last run's time loop, from HIP events on the context stream. */
 typedef struct sw_timing {

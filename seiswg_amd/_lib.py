@@ -79,8 +79,8 @@ def load():
     L.sw_advance.argtypes = [vp, C.c_int32]
     L.sw_adjoint_plain.argtypes = [vp, C.c_int32, ip, fp, C.c_int32, C.c_float, dp]
     L.sw_smooth.argtypes = [vp, C.c_int32, C.c_float]
-    L.sw_set_tiling.argtypes = [vp, C.c_int32, C.c_int32]
-    L.sw_get_tiling.argtypes = [vp, ip, ip]
+    L.sw_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+    L.sw_get_tiling.argtypes = [vp, ip, ip, ip]
     L.sw_get_timing.argtypes = [vp, C.POINTER(sw_timing)]
     L.sw_multi_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
     L.sw_multi_destroy.argtypes = [vp]
@@ -88,7 +88,7 @@ def load():
     L.sw_multi_forward.argtypes = [vp, ip, C.POINTER(ip)]
     L.sw_multi_gradient.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(fp), C.c_int32, C.POINTER(dp)]
     L.sw_multi_timing.argtypes = [vp, dp, C.POINTER(C.c_int64)]
-    L.sw_autotune.argtypes = [vp, C.c_int32, C.c_int32, ip, ip, dp]
+    L.sw_autotune.argtypes = [vp, C.c_int32, C.c_int32, ip, ip, ip, dp]
     L.sw_gradient.argtypes = [vp, C.c_int32, ip, fp, C.c_int32, C.c_int32, dp]
     L.sw_bench_copy.argtypes = [C.c_int32, C.c_int64, C.c_int32, dp]
     L.seiswg_run.argtypes = [C.c_char_p]
@@ -214,19 +214,20 @@ class Context:
         self.nrec = len(idx)
         _check(self.L.sw_set_receivers(self.h, len(idx), _iptr(idx)))
 
-    def set_tiling(self, threads: int, seglen: int):
-        _check(self.L.sw_set_tiling(self.h, threads, seglen))
+    def set_tiling(self, threads: int, seglen: int, shots_per_block: int = 0):
+        _check(self.L.sw_set_tiling(self.h, threads, seglen, shots_per_block))
 
     def autotune(self, nshots: int, steps: int = 10):
-        """Fastest tile shape for nshots shots: (threads, seglen, ms per step)."""
-        a, b, ms = C.c_int32(), C.c_int32(), C.c_double()
-        _check(self.L.sw_autotune(self.h, nshots, steps, C.byref(a), C.byref(b), C.byref(ms)))
-        return a.value, b.value, ms.value
+        """Fastest tile shape for nshots shots: (threads, seglen, shots per block, ms per step)."""
+        a, b, g, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        _check(self.L.sw_autotune(self.h, nshots, steps, C.byref(a), C.byref(b), C.byref(g), C.byref(ms)))
+        return a.value, b.value, g.value, ms.value
 
     def tiling(self):
-        a, b = C.c_int32(), C.c_int32()
-        _check(self.L.sw_get_tiling(self.h, C.byref(a), C.byref(b)))
-        return a.value, b.value
+        """(threads, seglen, shots per block) of the last prepared batch."""
+        a, b, g = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(self.L.sw_get_tiling(self.h, C.byref(a), C.byref(b), C.byref(g)))
+        return a.value, b.value, g.value
 
     def forward(self, isrc: int = -1, snap: int = 0, snapshots: dict | None = None):
         return self.forward_batch([isrc], snap, snapshots)

@@ -98,6 +98,8 @@ struct sw_ctx {
   // sw_autotune
   Tiling tile{256, 64, 0, 0};
   bool tile_fixed = false;
+  int group_req = 0;          // shots per forward workgroup requested (0: automatic)
+  int ncu = 0;                // compute units of the device
 
   // batch
   int nshots = 0;
@@ -198,27 +200,54 @@ bool all_fast_divisors(const float* v, size_t n) {
   return true;
 }
 
-void heuristic_tiling(sw_ctx* c, int nshots) {
-  Tiling& t = c->tile;
-  auto count = [&](int nt, int seg) {
-    const int64_t strips = (c->g.nz + (nt - 6) - 1) / (nt - 6), segs = (c->g.nx + seg - 1) / seg;
-    return int64_t(nshots) * strips * segs;
-  };
-  int nt = c->g.nz >= 400 ? 256 : c->g.nz >= 100 ? 128 : 64;
-  int seg = 512;
-  while (count(nt, seg) < 1024 && seg > 8) seg /= 2;
-  while (count(nt, seg) < 1024 && nt > 64) nt /= 2;
-  t.nt_threads = nt;
-  t.seglen = seg;
+// Shots per forward workgroup: the requested count (0: 2 where the tile is
+// wide enough), reduced to one that the kernels support and that divides
+// the number of shots.
+int effective_group(int nt, int requested, int nshots, bool spin) {
+  int g = requested > 0 ? requested : (nt >= 128 ? 2 : 1);
+  while (g > 1 && (!group_supported(nt, g) || nshots % g != 0 || (spin && nt * g > 512))) g /= 2;
+  return g;
 }
 
-int check_tiling(sw_ctx* c) {
+// Tile width from the column count.  Shots per workgroup: the largest of
+// 4, 2, 1 that divides the batch and whose resident workgroups still give a
+// CU at least 16 waves (enough to keep its memory pipeline busy); more shots
+// per workgroup read each model row fewer times.  Segments: as many as fill
+// every CU's resident slots once (whole workgroups per CU, no second round),
+// but at least 8 rows each.
+void heuristic_tiling(sw_ctx* c, int nshots) {
+  Tiling& t = c->tile;
+  const int nt = c->g.nz >= 400 ? 256 : c->g.nz >= 100 ? 128 : 64;
+  const bool spin = uses_spin(c);
+  const int kind = uses_psv(c) ? (spin ? MK_SPIN : MK_PSV) : MK_SH;
+  const int64_t ncu = c->ncu > 0 ? c->ncu : 256, nstrips = (c->g.nz + (nt - 6) - 1) / (nt - 6);
+  int group = 1;
+  int64_t resident = std::max(step_blocks_per_cu(kind, nt, 1), 1);
+  for (int g : {4, 2, 1}) {
+    if (g != effective_group(nt, c->group_req > 0 ? c->group_req : g, nshots, spin)) continue;
+    const int64_t r = step_blocks_per_cu(kind, nt, g);
+    if (g > 1 && r * g * nt / 64 < 16) continue;
+    group = g;
+    resident = std::max<int64_t>(r, 1);
+    break;
+  }
+  const int64_t base = int64_t(nshots / group) * nstrips;
+  const int64_t nsegs = std::max<int64_t>(1, (ncu * resident) / base);
+  int seg = int(std::max<int64_t>((c->g.nx + nsegs - 1) / nsegs, 8));
+  seg = std::min(seg, c->g.nx);
+  t.nt_threads = nt;
+  t.seglen = seg;
+  t.group = group;
+}
+
+int check_tiling(sw_ctx* c, int nshots) {
   Tiling& t = c->tile;
   if (!tiling_supported(t.nt_threads)) return fail(-2, "unsupported tile width %d", t.nt_threads);
   if (t.seglen < 1) return fail(-2, "seglen must be >= 1");
   const int W = t.nt_threads - 6;
   t.nstrips = (c->g.nz + W - 1) / W;
   t.nsegs = (c->g.nx + t.seglen - 1) / t.seglen;
+  t.group = effective_group(t.nt_threads, c->tile_fixed ? c->group_req : t.group, nshots, uses_spin(c));
   return 0;
 }
 
@@ -330,7 +359,7 @@ int prepare_batch(sw_ctx* c, int nshots, const std::vector<std::vector<Entry>>& 
   ++c->gen;
   if (nshots < 1) return fail(-2, "nshots must be >= 1");
   if (!c->tile_fixed) heuristic_tiling(c, nshots);
-  int rc = check_tiling(c);
+  int rc = check_tiling(c, nshots);
   if (rc) return rc;
   c->nshots = nshots;
   const size_t slot_elems = size_t(nshots) * size_t(c->npt);
@@ -388,7 +417,8 @@ StepArgs make_args(sw_ctx* c, StepMode mode) {
   a.st[1] = c->state[1].p;
   a.it_lo = -0x7fffffff;
   a.it_hi = 0x7fffffff;
-  a.nblocks = c->nshots * c->tile.nstrips * c->tile.nsegs;
+  // forward steps run tile.group shots per workgroup, adjoint steps one
+  a.nblocks = (mode == M_ADJ ? c->nshots : c->nshots / c->tile.group) * c->tile.nstrips * c->tile.nsegs;
   a.advance = 1;
   if (mode == M_ADJ) {
     a.inj_off = c->ainj_off.p; a.inj = c->ainj.p;
@@ -593,6 +623,7 @@ SW_API int sw_create(const sw_grid* g, int32_t device, sw_ctx** out) {
   c->npt = int64_t(g->nx) * g->nz;
   c->h8x = 8.0f * g->dx;
   c->h8z = 8.0f * g->dz;
+  if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) c->ncu = 0;
   compute_slots(c);
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
@@ -793,19 +824,23 @@ SW_API int sw_write_field(sw_ctx* c, int32_t shot, int32_t field, const float* i
   return 0;
 }
 
-SW_API int sw_set_tiling(sw_ctx* c, int32_t threads, int32_t seglen) {
+SW_API int sw_set_tiling(sw_ctx* c, int32_t threads, int32_t seglen, int32_t shots_per_block) {
   if (!c) return fail(-1, "null ctx");
   if (!tiling_supported(threads)) return fail(-2, "threads must be 64, 128 or 256");
   if (seglen < 1) return fail(-2, "seglen must be >= 1");
+  if (shots_per_block != 0 && !group_supported(threads, shots_per_block))
+    return fail(-2, "shots_per_block must be 0 (automatic), 1, or 2 or 4 with threads >= 128");
   c->tile.nt_threads = threads;
   c->tile.seglen = seglen;
+  c->group_req = shots_per_block;
   c->tile_fixed = true;
   ++c->gen;
   c->nshots = 0;   // lists must be rebuilt
   return 0;
 }
 
-SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* threads, int32_t* seglen, double* ms) {
+SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* threads, int32_t* seglen,
+                       int32_t* shots_per_block, double* ms) {
   if (!c || nshots < 1 || steps < 1) return fail(-1, "bad arguments");
   if (!c->model_set) return fail(-2, "model not set");
   if (c->g.nt < steps + 2) return fail(-2, "nt=%d too short to time %d steps", c->g.nt, steps);
@@ -814,41 +849,50 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
   for (int s = 0; s < nshots; ++s) isrc[s] = c->nsrc > 0 ? s % c->nsrc : -1;
   double best = 1e300;
   Tiling bt = c->tile;
+  int bg = c->group_req;
   for (int nt : {64, 128, 256}) {
-    for (int seg : {8, 16, 32, 64, 128, 256}) {
-      if (seg > 2 * c->g.nx) continue;
-      c->tile.nt_threads = nt;
-      c->tile.seglen = seg;
-      c->tile_fixed = true;
-      int rc = sw_prepare(c, nshots, isrc.data());
-      if (rc) return rc;
-      if ((rc = launch_steps(c, 2))) return rc;       // warm-up
-      HIPCHK(hipEventRecord(c->ev0, c->stream));
-      if ((rc = launch_steps(c, steps))) return rc;
-      HIPCHK(hipEventRecord(c->ev1, c->stream));
-      HIPCHK(hipEventSynchronize(c->ev1));
-      float t = 0;
-      HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
-      if (t < best) {
-        best = t;
-        bt = c->tile;
+    for (int grp : {1, 2, 4}) {
+      if (!group_supported(nt, grp) || nshots % grp != 0) continue;
+      for (int seg : {32, 64, 128, 256, 400, 512}) {
+        if (seg > 2 * c->g.nx) continue;
+        c->tile.nt_threads = nt;
+        c->tile.seglen = seg;
+        c->group_req = grp;
+        c->tile_fixed = true;
+        int rc = sw_prepare(c, nshots, isrc.data());
+        if (rc) return rc;
+        if ((rc = launch_steps(c, 2))) return rc;       // warm-up
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        if ((rc = launch_steps(c, steps))) return rc;
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipEventSynchronize(c->ev1));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+        if (t < best) {
+          best = t;
+          bt = c->tile;
+          bg = grp;
+        }
       }
     }
   }
   c->tile = bt;
+  c->group_req = bg;
   c->tile_fixed = true;
   c->nshots = 0;
   ++c->gen;
   if (threads) *threads = bt.nt_threads;
   if (seglen) *seglen = bt.seglen;
+  if (shots_per_block) *shots_per_block = bt.group;
   if (ms) *ms = best / steps;
   return 0;
 }
 
-SW_API int sw_get_tiling(const sw_ctx* c, int32_t* threads, int32_t* seglen) {
+SW_API int sw_get_tiling(const sw_ctx* c, int32_t* threads, int32_t* seglen, int32_t* shots_per_block) {
   if (!c || !threads || !seglen) return fail(-1, "null argument");
   *threads = c->tile.nt_threads;
   *seglen = c->tile.seglen;
+  if (shots_per_block) *shots_per_block = c->tile.group;
   return 0;
 }
 
@@ -1137,6 +1181,7 @@ int multi_steps(sw_multi* m, StepMode mode, int nlaunch, const std::vector<int2>
     sw_ctx* c = m->p[q];
     if (win[q].x >= win[q].y) continue;
     StepArgs a = make_args(c, mode);
+    a.nblocks = c->nshots * c->tile.nstrips * c->tile.nsegs;   // merged launches run one shot per workgroup
     a.it_lo = win[q].x;
     a.it_hi = win[q].y;
     const bool sh = step_sh_on(c), psv = step_psv_on(c), spin = uses_spin(c);

@@ -25,9 +25,10 @@ enum UaccSlot : int {
 };
 
 struct Tiling {
-  int nt_threads;   // threads per workgroup = columns per strip incl. 6 halo
+  int nt_threads;   // threads per shot = columns per strip incl. 6 halo
   int seglen;       // rows (x) marched by one workgroup
   int nstrips, nsegs;
+  int group = 1;    // shots per workgroup of the forward steps (1, 2 or 4)
 };
 
 // Rows of the model buffer [MF_N][npt].
@@ -100,8 +101,10 @@ hipError_t launch_adjsrc(const float* obs_syn, const float* obs_ref, const float
 hipError_t launch_reduce_shots(const float* kshot, float* kout, int nshots, int64_t n, int64_t shot_stride,
                                hipStream_t s);
 bool tiling_supported(int nt_threads);
+bool group_supported(int nt_threads, int group);
 // resident workgroups per CU of the step kernel of a MultiKind at a tile width
-int step_blocks_per_cu(int kind, int nt_threads);
+// and shots per workgroup (adjoint kinds: group 1 only)
+int step_blocks_per_cu(int kind, int nt_threads, int group);
 
 hipError_t launch_imaging_sh(const float* vadj, const float* vfw, float* kmu, int nx, int nz,
                              float h8x, float h8z, float ndt, hipStream_t s);

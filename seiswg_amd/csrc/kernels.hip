@@ -97,22 +97,30 @@ using Phase = std::integral_constant<int, R>;
 // for rows/columns outside the grid only feed halo lanes whose results are
 // discarded or zeroed by the edge rules of the FD operators.
 struct TileGeom {
-  int shot, tile, t, j, i0, i1;
+  int shot, tile, t, lane, base, j, i0, i1;
   bool jin, jst, zm_ok, zp_ok;
   uint32_t jc4, rowb;
   int nx;
   __device__ __forceinline__ uint32_t row(int m) const { return uint32_t(min(max(m, 0), nx - 1)) * rowb; }
 };
 
-template <int NT>
+// A workgroup of S*NT threads runs the same tile of S consecutive shots, one
+// NT-thread group per shot (S divides the shot count): the groups share the
+// barriers, so their model-field loads of a row come close together in time
+// and all but the first are served by the caches.
+template <int NT, int S = 1>
 __device__ __forceinline__ TileGeom tile_geom(const StepArgs& a, int logical) {
   constexpr int W = NT - 6;
   TileGeom g;
   const int tps = a.nstrips * a.nsegs;
-  g.shot = logical / tps;
-  g.tile = logical - g.shot * tps;
+  const int grp = logical / tps;
+  g.tile = logical - grp * tps;
+  const int sub = S == 1 ? 0 : __builtin_amdgcn_readfirstlane(int(threadIdx.x) / NT);   // NT is a multiple of 64
+  g.shot = grp * S + sub;
+  g.base = sub * NT;
+  g.lane = int(threadIdx.x);
   const int strip = g.tile % a.nstrips, seg = g.tile / a.nstrips;
-  g.t = threadIdx.x;
+  g.t = g.lane - g.base;
   g.nx = a.nx;
   g.j = strip * W - 3 + g.t;
   g.i0 = seg * a.seglen;
@@ -128,12 +136,12 @@ __device__ __forceinline__ TileGeom tile_geom(const StepArgs& a, int logical) {
 
 // ---------------------------------------------------------------- SH step
 // div_sy, stf_dsy, add_vy, div_vy, add_sy_sh, save_obs_y_sh (fd2d.wgsl:242-301)
-template <int NT, bool AUX>
+template <int NT, int S, bool AUX>
 __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
-  __shared__ float lds_s[2][NT];
-  __shared__ float lds_v[2][NT];
-  const TileGeom g = tile_geom<NT>(a, logical);
-  const int t = g.t, j = g.j, nx = a.nx, i0 = g.i0, i1 = g.i1;
+  __shared__ float lds_s[2][S * NT];
+  __shared__ float lds_v[2][S * NT];
+  const TileGeom g = tile_geom<NT, S>(a, logical);
+  const int t = g.t, lane = g.lane, j = g.j, nx = a.nx, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (it < a.it_lo || it >= a.it_hi) return;
   const int64_t so = int64_t(g.shot) * a.shot_floats;
@@ -151,7 +159,8 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
   const bool fast_rho = (a.fast_div & FD_RHO) != 0;
-  const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
+  const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
+            tp2 = g.base + min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;
   int ip = a.inj_off[g.shot * tps + g.tile];
   const int ie = a.inj_off[g.shot * tps + g.tile + 1];
@@ -204,7 +213,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // velocity half-step at row m
-    lds_s[par][t] = szy[s2];
+    lds_s[par][lane] = szy[s2];
     __syncthreads();
     const bool xm_ok = m >= 2 && m < nx - 2;
     float dsy = (xm_ok ? dminus(sxy[s0], sxy[s1], sxy[s2], sxy[s3], Dx) : 0.0f) +
@@ -224,7 +233,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
       if constexpr (AUX) bstore(dsy, raux, jc4, A_DSY * npt4 + uint32_t(m) * g.rowb);
     }
     vy[s3] = vyn;              // new-velocity ring: rows m-3..m in slots s0..s3
-    lds_v[par][t] = vyn;
+    lds_v[par][lane] = vyn;
     __syncthreads();
     dvydz[s2] = g.zp_ok ? dplus(lds_v[par][tm1], vyn, lds_v[par][tp1], lds_v[par][tp2], Dz) : 0.0f;
 
@@ -280,14 +289,14 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
 // div_sxz, div_sy_c, div_sxyz_c, stf_dsxz, add_vxz, div_vxz, add_sxz,
 // add_vy_c, div_vy_c, add_sy_c, save_obs_x/z, compute_du_grad, save_obs_ry
 // (fd2d.wgsl:308-492)
-template <int NT, bool SPIN, bool AUX>
+template <int NT, int S, bool SPIN, bool AUX>
 __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   constexpr int NSL = SPIN ? 4 : 2;   // sxz, szz, syz_c, syy_c
   constexpr int NVL = SPIN ? 3 : 2;   // vx, vz, vy_c
-  __shared__ float lds_s[2][NSL][NT];
-  __shared__ float lds_v[2][NVL][NT];
-  const TileGeom g = tile_geom<NT>(a, logical);
-  const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
+  __shared__ float lds_s[2][NSL][S * NT];
+  __shared__ float lds_v[2][NVL][S * NT];
+  const TileGeom g = tile_geom<NT, S>(a, logical);
+  const int t = g.t, lane = g.lane, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (it < a.it_lo || it >= a.it_hi) return;
   const int64_t so = int64_t(g.shot) * a.shot_floats;
@@ -313,7 +322,8 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
   const bool fast_rho = (a.fast_div & FD_RHO) != 0, fast_j = (a.fast_div & FD_J) != 0;
-  const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
+  const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
+            tp2 = g.base + min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;
   int ip = a.inj_off[g.shot * tps + g.tile];
   const int ie = a.inj_off[g.shot * tps + g.tile + 1];
@@ -408,9 +418,9 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // ---- velocity half-step at row m
-    lds_s[par][0][t] = sxz[s2];
-    lds_s[par][1][t] = szz[s2];
-    if constexpr (SPIN) { lds_s[par][2][t] = syz[s2]; lds_s[par][3][t] = syy[s2]; }
+    lds_s[par][0][lane] = sxz[s2];
+    lds_s[par][1][lane] = szz[s2];
+    if constexpr (SPIN) { lds_s[par][2][lane] = syz[s2]; lds_s[par][3][lane] = syy[s2]; }
     __syncthreads();
     const bool xm_ok = m >= 2 && m < nx - 2;
     const bool zm_ok = g.zm_ok;
@@ -459,9 +469,9 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     vx[s3] = vxn;
     vz[s3] = vzn;
     if constexpr (SPIN) vyc[s3] = vycn;
-    lds_v[par][0][t] = vxn;
-    lds_v[par][1][t] = vzn;
-    if constexpr (SPIN) lds_v[par][2][t] = vycn;
+    lds_v[par][0][lane] = vxn;
+    lds_v[par][1][lane] = vzn;
+    if constexpr (SPIN) lds_v[par][2][lane] = vycn;
     __syncthreads();
     const bool zp_ok = g.zp_ok;
     dvxdz[s2] = zp_ok ? dplus(lds_v[par][0][tm1], vxn, lds_v[par][0][tp1], lds_v[par][0][tp2], Dz) : 0.0f;
@@ -898,13 +908,18 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
 }
 
 // --------------------------------------------------------- entry points
-template <int NT, bool AUX>
-__global__ __launch_bounds__(NT) void k_step_sh(const StepArgs a) {
-  step_sh<NT, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
+template <int NT, int S, bool AUX>
+__global__ __launch_bounds__(NT * S) void k_step_sh(const StepArgs a) {
+  step_sh<NT, S, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
-template <int NT, bool SPIN, bool AUX>
-__global__ __launch_bounds__(NT) void k_step_psv(const StepArgs a) {
-  step_psv<NT, SPIN, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
+// P-SV without spin fits 80 VGPRs: ask for 6 waves per SIMD at every group size
+template <int NT, int S, bool AUX>
+__global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(6))) void k_step_psv(const StepArgs a) {
+  step_psv<NT, S, false, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+template <int NT, int S>
+__global__ __launch_bounds__(NT * S) void k_step_spin(const StepArgs a) {
+  step_psv<NT, S, true, false>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 template <int NT>
 __global__ __launch_bounds__(NT) void k_adj_sh(const StepArgs a) {
@@ -919,11 +934,11 @@ template <int NT, int KIND>
 __global__ __launch_bounds__(NT) void k_multi(const StepArgs* __restrict__ args, const int2* __restrict__ work) {
   const int2 w = work[xcd_remap(blockIdx.x, gridDim.x)];
   const StepArgs& a = args[w.x];
-  if constexpr (KIND == MK_SH) step_sh<NT, false>(a, w.y);
-  else if constexpr (KIND == MK_PSV) step_psv<NT, false, false>(a, w.y);
-  else if constexpr (KIND == MK_SPIN) step_psv<NT, true, false>(a, w.y);
-  else if constexpr (KIND == MK_SH_AUX) step_sh<NT, true>(a, w.y);
-  else if constexpr (KIND == MK_PSV_AUX) step_psv<NT, false, true>(a, w.y);
+  if constexpr (KIND == MK_SH) step_sh<NT, 1, false>(a, w.y);
+  else if constexpr (KIND == MK_PSV) step_psv<NT, 1, false, false>(a, w.y);
+  else if constexpr (KIND == MK_SPIN) step_psv<NT, 1, true, false>(a, w.y);
+  else if constexpr (KIND == MK_SH_AUX) step_sh<NT, 1, true>(a, w.y);
+  else if constexpr (KIND == MK_PSV_AUX) step_psv<NT, 1, false, true>(a, w.y);
   else if constexpr (KIND == MK_ADJ_SH) adj_sh<NT>(a, w.y);
   else adj_psv<NT>(a, w.y);
 }
@@ -1101,28 +1116,44 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
   for (int64_t k = int64_t(blockIdx.x) * 256 + threadIdx.x; k < n4; k += int64_t(gridDim.x) * 256) dst[k] = src[k];
 }
 
-template <int NT>
-hipError_t launch_sh_nt(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
-  const int nblk = a.nshots * t.nstrips * t.nsegs;
-  if (aux) hipLaunchKernelGGL((k_step_sh<NT, true>), dim3(nblk), dim3(NT), 0, s, a);
-  else hipLaunchKernelGGL((k_step_sh<NT, false>), dim3(nblk), dim3(NT), 0, s, a);
-  return hipGetLastError();
-}
-template <int NT, bool SPIN>
-hipError_t launch_psv_nt(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
-  const int nblk = a.nshots * t.nstrips * t.nsegs;
-  if constexpr (SPIN) {
-    hipLaunchKernelGGL((k_step_psv<NT, true, false>), dim3(nblk), dim3(NT), 0, s, a);
-  } else {
-    if (aux) hipLaunchKernelGGL((k_step_psv<NT, false, true>), dim3(nblk), dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL((k_step_psv<NT, false, false>), dim3(nblk), dim3(NT), 0, s, a);
+// forward step launch for a tile width NT and S shots per workgroup; kind is
+// MK_SH, MK_SH_AUX, MK_PSV, MK_PSV_AUX or MK_SPIN
+template <int NT, int S>
+hipError_t launch_fwd_ns(int kind, const StepArgs& a, const Tiling& t, hipStream_t s) {
+  const int nblk = a.nshots / S * t.nstrips * t.nsegs;
+  const dim3 grid(nblk), block(NT * S);
+  switch (kind) {
+    case MK_SH: hipLaunchKernelGGL((k_step_sh<NT, S, false>), grid, block, 0, s, a); break;
+    case MK_SH_AUX: hipLaunchKernelGGL((k_step_sh<NT, S, true>), grid, block, 0, s, a); break;
+    case MK_PSV: hipLaunchKernelGGL((k_step_psv<NT, S, false>), grid, block, 0, s, a); break;
+    case MK_PSV_AUX: hipLaunchKernelGGL((k_step_psv<NT, S, true>), grid, block, 0, s, a); break;
+    case MK_SPIN:
+      // the spin kernel needs more than 64 VGPRs: at most 512 threads per workgroup
+      if constexpr (NT * S <= 512) hipLaunchKernelGGL((k_step_spin<NT, S>), grid, block, 0, s, a);
+      else return hipErrorInvalidValue;
+      break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+hipError_t launch_fwd(int kind, const StepArgs& a, const Tiling& t, hipStream_t s) {
+  if (t.group < 1 || a.nshots % t.group != 0) return hipErrorInvalidValue;
+  switch (t.nt_threads * 8 + t.group) {
+    case 64 * 8 + 1: return launch_fwd_ns<64, 1>(kind, a, t, s);
+    case 128 * 8 + 1: return launch_fwd_ns<128, 1>(kind, a, t, s);
+    case 128 * 8 + 2: return launch_fwd_ns<128, 2>(kind, a, t, s);
+    case 128 * 8 + 4: return launch_fwd_ns<128, 4>(kind, a, t, s);
+    case 256 * 8 + 1: return launch_fwd_ns<256, 1>(kind, a, t, s);
+    case 256 * 8 + 2: return launch_fwd_ns<256, 2>(kind, a, t, s);
+    case 256 * 8 + 4: return launch_fwd_ns<256, 4>(kind, a, t, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace
 
 bool tiling_supported(int nt) { return nt == 64 || nt == 128 || nt == 256; }
+bool group_supported(int nt, int group) { return group == 1 || (nt >= 128 && (group == 2 || group == 4)); }
 
 namespace {
 template <typename K>
@@ -1130,47 +1161,45 @@ int occupancy(K k, int nt) {
   int n = 0;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, nt, 0) == hipSuccess ? n : 0;
 }
-template <int NT>
-int blocks_per_cu_nt(int kind) {
+template <int NT, int S>
+int blocks_per_cu_ns(int kind) {
   switch (kind) {
-    case MK_SH: return occupancy(k_step_sh<NT, false>, NT);
-    case MK_PSV: return occupancy(k_step_psv<NT, false, false>, NT);
-    case MK_SPIN: return occupancy(k_step_psv<NT, true, false>, NT);
-    case MK_SH_AUX: return occupancy(k_step_sh<NT, true>, NT);
-    case MK_PSV_AUX: return occupancy(k_step_psv<NT, false, true>, NT);
-    case MK_ADJ_SH: return occupancy(k_adj_sh<NT>, NT);
-    case MK_ADJ_PSV: return occupancy(k_adj_psv<NT>, NT);
+    case MK_SH: return occupancy(k_step_sh<NT, S, false>, NT * S);
+    case MK_PSV: return occupancy(k_step_psv<NT, S, false>, NT * S);
+    case MK_SPIN:
+      if constexpr (NT * S <= 512) return occupancy(k_step_spin<NT, S>, NT * S);
+      return 0;
+    case MK_SH_AUX: return occupancy(k_step_sh<NT, S, true>, NT * S);
+    case MK_PSV_AUX: return occupancy(k_step_psv<NT, S, true>, NT * S);
+  }
+  if constexpr (S == 1) {
+    if (kind == MK_ADJ_SH) return occupancy(k_adj_sh<NT>, NT);
+    if (kind == MK_ADJ_PSV) return occupancy(k_adj_psv<NT>, NT);
   }
   return 0;
 }
 }  // namespace
 
-int step_blocks_per_cu(int kind, int nt_threads) {
-  switch (nt_threads) {
-    case 64: return blocks_per_cu_nt<64>(kind);
-    case 128: return blocks_per_cu_nt<128>(kind);
-    case 256: return blocks_per_cu_nt<256>(kind);
+int step_blocks_per_cu(int kind, int nt_threads, int group) {
+  switch (nt_threads * 8 + group) {
+    case 64 * 8 + 1: return blocks_per_cu_ns<64, 1>(kind);
+    case 128 * 8 + 1: return blocks_per_cu_ns<128, 1>(kind);
+    case 128 * 8 + 2: return blocks_per_cu_ns<128, 2>(kind);
+    case 128 * 8 + 4: return blocks_per_cu_ns<128, 4>(kind);
+    case 256 * 8 + 1: return blocks_per_cu_ns<256, 1>(kind);
+    case 256 * 8 + 2: return blocks_per_cu_ns<256, 2>(kind);
+    case 256 * 8 + 4: return blocks_per_cu_ns<256, 4>(kind);
   }
   return 0;
 }
 
 hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
-  switch (t.nt_threads) {
-    case 64: return launch_sh_nt<64>(a, t, s, aux);
-    case 128: return launch_sh_nt<128>(a, t, s, aux);
-    case 256: return launch_sh_nt<256>(a, t, s, aux);
-  }
-  return hipErrorInvalidValue;
+  return launch_fwd(aux ? MK_SH_AUX : MK_SH, a, t, s);
 }
 
 hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStream_t s, bool aux) {
   if (spin && aux) return hipErrorInvalidValue;
-  switch (t.nt_threads) {
-    case 64: return spin ? launch_psv_nt<64, true>(a, t, s, aux) : launch_psv_nt<64, false>(a, t, s, aux);
-    case 128: return spin ? launch_psv_nt<128, true>(a, t, s, aux) : launch_psv_nt<128, false>(a, t, s, aux);
-    case 256: return spin ? launch_psv_nt<256, true>(a, t, s, aux) : launch_psv_nt<256, false>(a, t, s, aux);
-  }
-  return hipErrorInvalidValue;
+  return launch_fwd(spin ? MK_SPIN : aux ? MK_PSV_AUX : MK_PSV, a, t, s);
 }
 
 hipError_t launch_multi(int kind, int nt_threads, const StepArgs* args, const int2* work, int nb, hipStream_t s) {

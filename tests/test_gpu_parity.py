@@ -128,17 +128,20 @@ def test_steps_through_subnormal_range_match_oracle_bitwise(seiswg, phys):
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), name
 
 
-@pytest.mark.parametrize("threads", [64, 256])
-def test_single_source_runs_and_batched_shots_agree(seiswg, threads):
+@pytest.mark.parametrize("threads,group", [(64, 1), (128, 2), (256, 4), (256, 2)])
+def test_single_source_runs_and_batched_shots_agree(seiswg, threads, group):
+    """Shots batched in one launch, several per workgroup, equal single runs."""
     nx, nz, nt = 60, 70, 40
     p, stf = random_problem(nx, nz, nt, False, True, False, seed=3)
+    print(f"case threads={threads} group={group}", flush=True)
     c = gpu_context(p, stf, seiswg)
-    c.set_tiling(threads, 16)
+    c.set_tiling(threads, 16, group)
     singles = []
     for s in range(4):
         c.forward(s)
         singles.append((c.obs(0, O.OBS_X).copy(), c.field(seiswg.abi.F_VZ).copy()))
     c.forward_batch([0, 1, 2, 3])
+    assert c.tiling() == (threads, 16, group)
     for s in range(4):
         assert np.array_equal(c.obs(s, O.OBS_X), singles[s][0])
         assert np.array_equal(c.field(seiswg.abi.F_VZ, shot=s), singles[s][1])
@@ -146,6 +149,36 @@ def test_single_source_runs_and_batched_shots_agree(seiswg, threads):
     o = O.Oracle(p, 4, stf=stf.astype(np.float64))
     obs, _ = o.forward(2, 0)
     compare(c.obs(2, O.OBS_X), obs[O.OBS_X], "batched shot 2 ux")
+
+
+@pytest.mark.parametrize("phys", ["sh", "spin", "all"])
+@pytest.mark.parametrize("threads,group", [(128, 2), (256, 2), (128, 4)])
+def test_grouped_shots_equal_single_runs(seiswg, phys, threads, group):
+    """Every physics with several shots per workgroup against one shot per run."""
+    sh, psv, spin = phys in ("sh", "all"), phys in ("spin", "all"), phys in ("spin", "all")
+    nx, nz, nt = 50, 140, 30
+    p, stf = random_problem(nx, nz, nt, sh, psv, spin, seed=17)
+    print(f"case phys={phys} threads={threads} group={group}", flush=True)
+    c = gpu_context(p, stf, seiswg)
+    c.set_tiling(threads, 11, group)
+    comps = ([O.OBS_Y] if sh or spin else []) + ([O.OBS_X, O.OBS_Z] if psv else []) + \
+            ([O.OBS_YI, O.OBS_YC] if spin else [])
+    fields = (STATE["sh"] if sh else []) + (STATE["psv"] if psv else []) + (STATE["spin"] if spin else [])
+    runs = [0, 1, 2, 3]
+    single = []
+    for r in runs:
+        c.forward(r)
+        single.append(([c.obs(0, k).copy() for k in comps],
+                       [c.field(getattr(seiswg.abi, "F_" + n)).copy() for n in fields]))
+    c.forward_batch(runs)
+    eff = c.tiling()[2]
+    assert eff == (2 if spin and threads * group > 512 else group)
+    for q in range(len(runs)):
+        for k, ref in zip(comps, single[q][0]):
+            assert np.array_equal(c.obs(q, k).view(np.uint32), ref.view(np.uint32)), (q, k)
+        for n, ref in zip(fields, single[q][1]):
+            assert np.array_equal(c.field(getattr(seiswg.abi, "F_" + n), shot=q).view(np.uint32),
+                                  ref.view(np.uint32)), (q, n)
 
 
 def load_example(tmp_path, ex, ini_name="config.ini"):

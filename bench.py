@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gradient", action="store_true", help="skip the batched-vs-plain gradient measurement")
     ap.add_argument("--gradient-nt", type=int, default=0, help="time steps of the gradient workload (0: 3000)")
+    ap.add_argument("--gradient-only", action="store_true",
+                    help="run only the batched-vs-plain gradient measurement (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB copy kernel (calibrates the HBM counters of a profiler run)")
     a = ap.parse_args()
@@ -135,6 +137,9 @@ def main():
     from seiswg_amd import synth
     if a.calibrate_copy:
         seiswg.bench_copy(1 << 30, 5)
+    if a.gradient_only:
+        print(json.dumps({"gradient": gradient_bench(seiswg, synth, a.gradient_nt)}), flush=True)
+        return 0
     nt = max(synth.C2["nt"], a.warmup + a.steps)
     ctx, model, geom, src_idx, rec_idx = build_context(seiswg, synth, nt)
     nshots = synth.C2["nshots"]

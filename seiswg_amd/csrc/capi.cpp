@@ -209,18 +209,24 @@ int effective_group(int nt, int requested, int nshots, bool spin) {
   return g;
 }
 
-// Tile width from the column count.  Shots per workgroup: the largest of
-// 4, 2, 1 that divides the batch and whose resident workgroups still give a
-// CU at least 16 waves (enough to keep its memory pipeline busy); more shots
-// per workgroup read each model row fewer times.  Segments: as many as fill
-// every CU's resident slots once (whole workgroups per CU, no second round),
-// but at least 8 rows each.
+// Tile width: 256 threads from 400 columns up, 128 from 100 (wide tiles
+// measured faster even where they leave more lanes beyond nz idle), narrower
+// while even 8-row segments would leave CUs without a workgroup.
+// Shots per workgroup: the largest of 4, 2, 1 that divides the batch and
+// whose resident workgroups still give a CU at least 16 waves; more shots per
+// workgroup read each model row fewer times.  Segments: as many as fill every
+// CU's resident slots once (no second round), but at least 8 rows each.
 void heuristic_tiling(sw_ctx* c, int nshots) {
   Tiling& t = c->tile;
-  const int nt = c->g.nz >= 400 ? 256 : c->g.nz >= 100 ? 128 : 64;
+  const int64_t ncu = c->ncu > 0 ? c->ncu : 256;
+  auto strips_of = [&](int w) { return int64_t((c->g.nz + (w - 6) - 1) / (w - 6)); };
+  int nt = c->g.nz >= 400 ? 256 : c->g.nz >= 100 ? 128 : 64;
+  const int64_t max_segs = std::max(1, c->g.nx / 8);
+  for (int w : {128, 64})
+    if (w < nt && int64_t(nshots) * strips_of(nt) * max_segs < ncu) nt = w;
   const bool spin = uses_spin(c);
   const int kind = uses_psv(c) ? (spin ? MK_SPIN : MK_PSV) : MK_SH;
-  const int64_t ncu = c->ncu > 0 ? c->ncu : 256, nstrips = (c->g.nz + (nt - 6) - 1) / (nt - 6);
+  const int64_t nstrips = strips_of(nt);
   int group = 1;
   int64_t resident = std::max(step_blocks_per_cu(kind, nt, 1), 1);
   for (int g : {4, 2, 1}) {

@@ -49,13 +49,43 @@ __device__ __forceinline__ float div_exact(float x, Div d) {
   return x / d.y;
 }
 
-// fd2d.wgsl:151-164 (backward) and 166-180 (forward) differences, with the
-// reference's 1/8 weight on the 3-cell difference; d.y = 8*h, d.r = 1/(8*h).
-__device__ __forceinline__ float dminus(float fm2, float fm1, float f0, float fp1, Div d) {
-  return div_exact(9.0f * (f0 - fm1) - (fp1 - fm2), d);
-}
+// fd2d.wgsl:166-180 forward difference, with the reference's 1/8 weight on
+// the 3-cell difference; d.y = 8*h, d.r = 1/(8*h).
 __device__ __forceinline__ float dplus(float fm1, float f0, float fp1, float fp2, Div d) {
   return div_exact(9.0f * (fp1 - f0) - (fp2 - fm1), d);
+}
+// numerators of the backward (fd2d.wgsl:151-164) and forward differences
+__device__ __forceinline__ float nminus(float fm2, float fm1, float f0, float fp1) {
+  return 9.0f * (f0 - fm1) - (fp1 - fm2);
+}
+__device__ __forceinline__ float nplus(float fm1, float f0, float fp1, float fp2) {
+  return 9.0f * (fp1 - f0) - (fp2 - fm1);
+}
+
+// A group of divisions q[k] = x[k] / d[k].y sharing one range test: the
+// reciprocal form for all of them, and the IEEE division for the lanes where
+// any numerator is outside {0} U [2^-100, 2^100] or a divisor is outside the
+// fast range.  The test runs on the integer k = bits(x) << 1 (sign dropped):
+// min(k - 1) >= bits(2^-100) << 1 - 1 (zero wraps to the maximum) and
+// max(k) <= bits(2^100) << 1, accumulated in registers, so a group costs one
+// branch instead of one per division.
+template <int N>
+__device__ __forceinline__ void div_group(float (&q)[N], const float (&x)[N], const Div (&d)[N], bool fast) {
+  constexpr uint32_t kLo = (127u - 100u) << 24, kHi = (127u + 100u) << 24;
+  uint32_t lo = 0xffffffffu, hi = 0u;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const float q0 = x[k] * d[k].r;
+    const float e = __builtin_fmaf(q0, d[k].y, -x[k]);
+    q[k] = __builtin_fmaf(-e, d[k].r, q0);
+    const uint32_t b = __builtin_bit_cast(uint32_t, x[k]) << 1;
+    lo = min(lo, b - 1u);
+    hi = max(hi, b);
+  }
+  if (__builtin_expect(!(fast && lo >= kLo - 1u && hi <= kHi), 0)) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) q[k] = x[k] / d[k].y;
+  }
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Map them so that
@@ -158,6 +188,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const bool fast_c = (a.fast_div & FD_CONST) != 0;
   const bool fast_rho = (a.fast_div & FD_RHO) != 0;
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
             tp2 = g.base + min(t + 2, NT - 1);
@@ -182,10 +213,12 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     szy[q] = bload(rin, jc4, o_szy + g.row(i0 - 3 + q));
   }
   sxy[3] = szy[3] = 0.0f;
-  // 2-deep prefetch FIFO, slot (m - (i0-1)) & 1: stress row m+1, velocity and
-  // density row m, shear modulus row m-2 of iteration m
-  float pf_sxy[2], pf_szy[2], pf_vy[2], pf_rho[2], pf_irho[2], pf_mu[2];
-  for (int q = 0; q < 2; ++q) {
+  // D-deep prefetch FIFO, slot (m - (i0-1)) % D: stress row m+1, velocity and
+  // density row m, shear modulus row m-2 of iteration m.  Four shots per
+  // workgroup leave one workgroup per CU, with registers to spare for D = 4.
+  constexpr int D = S >= 4 ? 4 : 2;
+  float pf_sxy[D], pf_szy[D], pf_vy[D], pf_rho[D], pf_irho[D], pf_mu[D];
+  for (int q = 0; q < D; ++q) {
     pf_sxy[q] = bload(rin, jc4, o_sxy + g.row(i0 + q));
     pf_szy[q] = bload(rin, jc4, o_szy + g.row(i0 + q));
     pf_vy[q] = bload(rin, jc4, o_vy + g.row(i0 - 1 + q));
@@ -193,16 +226,24 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     pf_irho[q] = bload(rmod, jc4, m_irho + g.row(i0 - 1 + q));
     pf_mu[q] = bload(rmod, jc4, m_mu + g.row(i0 - 3 + q));
   }
+  // One barrier per row: iteration m reads the z-neighbours of stress row m
+  // from lds_s[m & 1] (written by iteration m-1, or here for the first row),
+  // writes new-velocity row m to lds_v[m & 1] and stress row m+1 to
+  // lds_s[(m+1) & 1], then synchronises once before reading velocity
+  // neighbours.  Parity double-buffering keeps every write two barriers away
+  // from the reads of the previous contents.
+  lds_s[0][lane] = szy[2];
+  __syncthreads();
 
   auto body = [&](auto phase, int m) __attribute__((always_inline)) {
     constexpr int R = decltype(phase)::value;
     constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
-    constexpr int f = R & 1;
+    constexpr int f = R % D;
     sxy[s3] = pf_sxy[f];       // row m+1
     szy[s3] = pf_szy[f];
     const float vy_o = pf_vy[f], rho = pf_rho[f], irho = pf_irho[f], mu = pf_mu[f];
     {
-      const uint32_t r3 = g.row(m + 3), r2 = g.row(m + 2), r0 = g.row(m);
+      const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
       pf_sxy[f] = bload(rin, jc4, o_sxy + r3);
       pf_szy[f] = bload(rin, jc4, o_szy + r3);
       pf_vy[f] = bload(rin, jc4, o_vy + r2);
@@ -213,11 +254,14 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // velocity half-step at row m
-    lds_s[par][lane] = szy[s2];
-    __syncthreads();
     const bool xm_ok = m >= 2 && m < nx - 2;
-    float dsy = (xm_ok ? dminus(sxy[s0], sxy[s1], sxy[s2], sxy[s3], Dx) : 0.0f) +
-                (g.zm_ok ? dminus(lds_s[par][tm2], lds_s[par][tm1], szy[s2], lds_s[par][tp1], Dz) : 0.0f);
+    float qa[2];
+    {
+      const float xa[2] = {xm_ok ? nminus(sxy[s0], sxy[s1], sxy[s2], sxy[s3]) : 0.0f,
+                           g.zm_ok ? nminus(lds_s[par][tm2], lds_s[par][tm1], szy[s2], lds_s[par][tp1]) : 0.0f};
+      div_group(qa, xa, {Dx, Dz}, fast_c);
+    }
+    float dsy = qa[0] + qa[1];
     if (m == inj_row) {
       for (; ip < ie; ++ip) {
         const int4 e = a.inj[ip];
@@ -227,15 +271,30 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
       inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;
     }
     const float b = (bxm * bzb) * bzt;
-    const float vyn = b * (vy_o + div_exact(dt * dsy, Div{rho, irho, fast_rho}));
+    float qb[1];
+    {
+      const float xb[1] = {dt * dsy};
+      div_group(qb, xb, {Div{rho, irho, true}}, fast_rho);
+    }
+    const float vyn = b * (vy_o + qb[0]);
     if (g.jst && m >= i0 && m < i1) {
       bstore(vyn, rout, jc4, o_vy + uint32_t(m) * g.rowb);
       if constexpr (AUX) bstore(dsy, raux, jc4, A_DSY * npt4 + uint32_t(m) * g.rowb);
     }
     vy[s3] = vyn;              // new-velocity ring: rows m-3..m in slots s0..s3
     lds_v[par][lane] = vyn;
+    lds_s[par ^ 1][lane] = szy[s3];
     __syncthreads();
-    dvydz[s2] = g.zp_ok ? dplus(lds_v[par][tm1], vyn, lds_v[par][tp1], lds_v[par][tp2], Dz) : 0.0f;
+    // d+z of the new vy at row m and d+x at row m-2 (for the stress half-step)
+    const int i = m - 2;
+    const bool xp_ok = i >= 1 && i < nx - 2;
+    float qc[2];
+    {
+      const float xc[2] = {g.zp_ok ? nplus(lds_v[par][tm1], vyn, lds_v[par][tp1], lds_v[par][tp2]) : 0.0f,
+                           xp_ok ? nplus(vy[s0], vy[s1], vy[s2], vy[s3]) : 0.0f};
+      div_group(qc, xc, {Dz, Dx}, fast_c);
+    }
+    dvydz[s2] = qc[0];
 
     // receivers on row m: uy += vy*dt, trace = uy
     if (m == rec_row) {
@@ -253,11 +312,9 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     }
 
     // stress half-step at row i = m-2
-    const int i = m - 2;
     if (i >= i0) {
-      const bool xp_ok = i >= 1 && i < nx - 2;
       const float mu_dt = mu * dt;
-      const float dvydx = xp_ok ? dplus(vy[s0], vy[s1], vy[s2], vy[s3], Dx) : 0.0f;
+      const float dvydx = qc[1];
       const float nsxy = sxy[s0] + mu_dt * dvydx;
       const float nszy = szy[s0] + mu_dt * dvydz[s0];
       if (g.jst) {
@@ -321,6 +378,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const bool fast_c = (a.fast_div & FD_CONST) != 0;
   const bool fast_rho = (a.fast_div & FD_RHO) != 0, fast_j = (a.fast_div & FD_J) != 0;
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
             tp2 = g.base + min(t + 2, NT - 1);
@@ -355,12 +413,13 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   }
   sxx[3] = sxz[3] = szz[3] = syx[3] = syy[3] = syz[3] = 0.0f;
   if constexpr (!SPIN) syx[0] = syx[1] = syx[2] = syy[0] = syy[1] = syy[2] = syz[0] = syz[1] = syz[2] = 0.0f;
-  // 2-deep prefetch FIFO, slot (m - (i0-1)) & 1: stress rows m+1, velocity /
+  // D-deep prefetch FIFO, slot (m - (i0-1)) % D: stress rows m+1, velocity /
   // density / J rows m, moduli rows m-2 of iteration m
-  float pf_sxx[2], pf_sxz[2], pf_szz[2], pf_vx[2], pf_vz[2], pf_rho[2], pf_irho[2], pf_lam[2], pf_mu[2];
-  float pf_syx[2] = {0, 0}, pf_syy[2] = {0, 0}, pf_syz[2] = {0, 0}, pf_vyc[2] = {0, 0}, pf_jj[2] = {1, 1},
-        pf_ij[2] = {1, 1}, pf_nu[2] = {0, 0}, pf_muc[2] = {0, 0}, pf_nuc[2] = {0, 0};
-  for (int q = 0; q < 2; ++q) {
+  constexpr int D = S >= 4 ? 4 : 2;
+  float pf_sxx[D], pf_sxz[D], pf_szz[D], pf_vx[D], pf_vz[D], pf_rho[D], pf_irho[D], pf_lam[D], pf_mu[D];
+  float pf_syx[D] = {}, pf_syy[D] = {}, pf_syz[D] = {}, pf_vyc[D] = {}, pf_jj[D] = {}, pf_ij[D] = {}, pf_nu[D] = {},
+        pf_muc[D] = {}, pf_nuc[D] = {};
+  for (int q = 0; q < D; ++q) {
     const uint32_t r1 = g.row(i0 + q), r0 = g.row(i0 - 1 + q), rm2 = g.row(i0 - 3 + q);
     pf_sxx[q] = bload(rin, jc4, o_sxx + r1);
     pf_sxz[q] = bload(rin, jc4, o_sxz + r1);
@@ -384,16 +443,22 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     }
   }
 
+  // one barrier per row, as in the SH step: stress row i0-1 goes to lds_s[0]
+  lds_s[0][0][lane] = sxz[2];
+  lds_s[0][1][lane] = szz[2];
+  if constexpr (SPIN) { lds_s[0][2][lane] = syz[2]; lds_s[0][3][lane] = syy[2]; }
+  __syncthreads();
+
   auto body = [&](auto phase, int m) __attribute__((always_inline)) {
     constexpr int R = decltype(phase)::value;
     constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
-    constexpr int f = R & 1;
+    constexpr int f = R % D;
     sxx[s3] = pf_sxx[f]; sxz[s3] = pf_sxz[f]; szz[s3] = pf_szz[f];     // row m+1
     if constexpr (SPIN) { syx[s3] = pf_syx[f]; syy[s3] = pf_syy[f]; syz[s3] = pf_syz[f]; }
     const float vx_o = pf_vx[f], vz_o = pf_vz[f], rho = pf_rho[f], irho = pf_irho[f], lam = pf_lam[f], mu = pf_mu[f];
     const float vyc_o = pf_vyc[f], jj = pf_jj[f], ij = pf_ij[f], nu = pf_nu[f], muc = pf_muc[f], nuc = pf_nuc[f];
     {
-      const uint32_t r3 = g.row(m + 3), r2 = g.row(m + 2), r0 = g.row(m);
+      const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
       pf_sxx[f] = bload(rin, jc4, o_sxx + r3);
       pf_sxz[f] = bload(rin, jc4, o_sxz + r3);
       pf_szz[f] = bload(rin, jc4, o_szz + r3);
@@ -418,26 +483,34 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // ---- velocity half-step at row m
-    lds_s[par][0][lane] = sxz[s2];
-    lds_s[par][1][lane] = szz[s2];
-    if constexpr (SPIN) { lds_s[par][2][lane] = syz[s2]; lds_s[par][3][lane] = syy[s2]; }
-    __syncthreads();
     const bool xm_ok = m >= 2 && m < nx - 2;
     const bool zm_ok = g.zm_ok;
-    float dsx = (xm_ok ? dminus(sxx[s0], sxx[s1], sxx[s2], sxx[s3], Dx) : 0.0f) +
-                (zm_ok ? dminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[s2], lds_s[par][0][tp1], Dz)
-                       : 0.0f);
-    float dsz = (xm_ok ? dminus(sxz[s0], sxz[s1], sxz[s2], sxz[s3], Dx) : 0.0f) +
-                (zm_ok ? dminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[s2], lds_s[par][1][tp1], Dz)
-                       : 0.0f);
+    // D- of the stresses: one group of 4 (8 with spin) divisions
+    constexpr int NA = SPIN ? 8 : 4;
+    float qa[NA];
+    {
+      float xa[NA];
+      xa[0] = xm_ok ? nminus(sxx[s0], sxx[s1], sxx[s2], sxx[s3]) : 0.0f;
+      xa[1] = zm_ok ? nminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[s2], lds_s[par][0][tp1]) : 0.0f;
+      xa[2] = xm_ok ? nminus(sxz[s0], sxz[s1], sxz[s2], sxz[s3]) : 0.0f;
+      xa[3] = zm_ok ? nminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[s2], lds_s[par][1][tp1]) : 0.0f;
+      if constexpr (SPIN) {
+        xa[4] = xm_ok ? nminus(syx[s0], syx[s1], syx[s2], syx[s3]) : 0.0f;
+        xa[5] = zm_ok ? nminus(lds_s[par][2][tm2], lds_s[par][2][tm1], syz[s2], lds_s[par][2][tp1]) : 0.0f;
+        xa[6] = zm_ok ? nminus(lds_s[par][3][tm2], lds_s[par][3][tm1], syy[s2], lds_s[par][3][tp1]) : 0.0f;
+        xa[7] = xm_ok ? nminus(syy[s0], syy[s1], syy[s2], syy[s3]) : 0.0f;
+        div_group(qa, xa, {Dx, Dz, Dx, Dz, Dx, Dz, Dz, Dx}, fast_c);
+      } else {
+        div_group(qa, xa, {Dx, Dz, Dx, Dz}, fast_c);
+      }
+    }
+    float dsx = qa[0] + qa[1];
+    float dsz = qa[2] + qa[3];
     float dsyc = 0.0f;
     if constexpr (SPIN) {
-      dsyc = (xm_ok ? dminus(syx[s0], syx[s1], syx[s2], syx[s3], Dx) : 0.0f) +
-             (zm_ok ? dminus(lds_s[par][2][tm2], lds_s[par][2][tm1], syz[s2], lds_s[par][2][tp1], Dz)
-                    : 0.0f);
-      dsx = dsx + (zm_ok ? dminus(lds_s[par][3][tm2], lds_s[par][3][tm1], syy[s2], lds_s[par][3][tp1], Dz)
-                         : 0.0f);
-      dsz = dsz + (-(xm_ok ? dminus(syy[s0], syy[s1], syy[s2], syy[s3], Dx) : 0.0f));
+      dsyc = qa[4] + qa[5];
+      dsx = dsx + qa[6];
+      dsz = dsz + (-qa[7]);
       dsyc = dsyc + 2.0f * syy[s2];
     }
     if (m == inj_row) {
@@ -452,10 +525,20 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
       inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;
     }
     const float b = (bxm * bzb) * bzt;
-    const float vxn = b * (vx_o + div_exact(dt * dsx, Div{rho, irho, fast_rho}));
-    const float vzn = b * (vz_o + div_exact(dt * dsz, Div{rho, irho, fast_rho}));
+    // divisions by rho (and J)
+    constexpr int NB = SPIN ? 3 : 2;
+    float qb[NB];
+    if constexpr (SPIN) {
+      const float xb[3] = {dt * dsx, dt * dsz, dt * dsyc};
+      div_group(qb, xb, {Div{rho, irho, true}, Div{rho, irho, true}, Div{jj, ij, true}}, fast_rho && fast_j);
+    } else {
+      const float xb[2] = {dt * dsx, dt * dsz};
+      div_group(qb, xb, {Div{rho, irho, true}, Div{rho, irho, true}}, fast_rho);
+    }
+    const float vxn = b * (vx_o + qb[0]);
+    const float vzn = b * (vz_o + qb[1]);
     float vycn = 0.0f;
-    if constexpr (SPIN) vycn = b * (vyc_o + div_exact(dt * dsyc, Div{jj, ij, fast_j}));
+    if constexpr (SPIN) vycn = b * (vyc_o + qb[2]);
     if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
       bstore(vxn, rout, jc4, o_vx + ro);
@@ -472,12 +555,33 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     lds_v[par][0][lane] = vxn;
     lds_v[par][1][lane] = vzn;
     if constexpr (SPIN) lds_v[par][2][lane] = vycn;
+    lds_s[par ^ 1][0][lane] = sxz[s3];
+    lds_s[par ^ 1][1][lane] = szz[s3];
+    if constexpr (SPIN) { lds_s[par ^ 1][2][lane] = syz[s3]; lds_s[par ^ 1][3][lane] = syy[s3]; }
     __syncthreads();
+    // D+ of the new velocities: z at row m, x at row i = m-2 (stress half-step)
     const bool zp_ok = g.zp_ok;
-    dvxdz[s2] = zp_ok ? dplus(lds_v[par][0][tm1], vxn, lds_v[par][0][tp1], lds_v[par][0][tp2], Dz) : 0.0f;
-    dvzdz[s2] = zp_ok ? dplus(lds_v[par][1][tm1], vzn, lds_v[par][1][tp1], lds_v[par][1][tp2], Dz) : 0.0f;
-    if constexpr (SPIN)
-      dvycdz[s2] = zp_ok ? dplus(lds_v[par][2][tm1], vycn, lds_v[par][2][tp1], lds_v[par][2][tp2], Dz) : 0.0f;
+    const int i = m - 2;
+    const bool xp_ok = i >= 1 && i < nx - 2;
+    constexpr int NC = SPIN ? 6 : 4;
+    float qc[NC];
+    {
+      float xc[NC];
+      xc[0] = zp_ok ? nplus(lds_v[par][0][tm1], vxn, lds_v[par][0][tp1], lds_v[par][0][tp2]) : 0.0f;
+      xc[1] = zp_ok ? nplus(lds_v[par][1][tm1], vzn, lds_v[par][1][tp1], lds_v[par][1][tp2]) : 0.0f;
+      xc[2] = xp_ok ? nplus(vx[s0], vx[s1], vx[s2], vx[s3]) : 0.0f;
+      xc[3] = xp_ok ? nplus(vz[s0], vz[s1], vz[s2], vz[s3]) : 0.0f;
+      if constexpr (SPIN) {
+        xc[4] = zp_ok ? nplus(lds_v[par][2][tm1], vycn, lds_v[par][2][tp1], lds_v[par][2][tp2]) : 0.0f;
+        xc[5] = xp_ok ? nplus(vyc[s0], vyc[s1], vyc[s2], vyc[s3]) : 0.0f;
+        div_group(qc, xc, {Dz, Dz, Dx, Dx, Dz, Dx}, fast_c);
+      } else {
+        div_group(qc, xc, {Dz, Dz, Dx, Dx}, fast_c);
+      }
+    }
+    dvxdz[s2] = qc[0];
+    dvzdz[s2] = qc[1];
+    if constexpr (SPIN) dvycdz[s2] = qc[4];
 
     // ---- receivers: u = sum v*dt at the cells the traces need
     if constexpr (!SPIN) {
@@ -541,11 +645,9 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     }
 
     // ---- stress half-step at row i = m-2
-    const int i = m - 2;
     if (i >= i0) {
-      const bool xp_ok = i >= 1 && i < nx - 2;
-      const float dvxdx = xp_ok ? dplus(vx[s0], vx[s1], vx[s2], vx[s3], Dx) : 0.0f;
-      const float dvzdx = xp_ok ? dplus(vz[s0], vz[s1], vz[s2], vz[s3], Dx) : 0.0f;
+      const float dvxdx = qc[2];
+      const float dvzdx = qc[3];
       const float l2m = lam + 2.0f * mu;
       const float nsxx = sxx[s0] + dt * (l2m * dvxdx + lam * dvzdz[s0]);
       const float nszz = szz[s0] + dt * (l2m * dvzdz[s0] + lam * dvxdx);
@@ -562,7 +664,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
         }
       }
       if constexpr (SPIN) {
-        const float dvycdx = xp_ok ? dplus(vyc[s0], vyc[s1], vyc[s2], vyc[s3], Dx) : 0.0f;
+        const float dvycdx = qc[5];
         const float nsyy = syy[s0] + 2.0f * dt * nu * (-vyc[s1] - 0.5f * (dvzdx - dvxdz[s0]));
         const float cc = dt * (muc + nuc);
         const float nsyx = syx[s0] + cc * dvycdx;
@@ -605,11 +707,20 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
 // outside the rows/columns on which the forward operator is active (D+: 1 <=
 // i < n-2, D-: 2 <= i < n-2).  ds and the strains come from the aux fields
 // of the recomputed forward step.  Same marching/tiling as the forward step.
+// The adjoint is not a reference path (its parity is against the FP64
+// adjoint, to a tolerance): its divisions use the reciprocal-and-correction
+// form without the range test, correctly rounded except for subnormal or
+// tiny quotients, where they are within one unit.
+__device__ __forceinline__ float div_rc(float x, float y, float r) {
+  const float q0 = x * r;
+  const float e = __builtin_fmaf(q0, y, -x);
+  return __builtin_fmaf(-e, r, q0);
+}
 __device__ __forceinline__ float ndm(float fm2, float fm1, float f0, float fp1, Div d) {
-  return -div_exact(9.0f * (f0 - fm1) - (fp1 - fm2), d);
+  return -div_rc(9.0f * (f0 - fm1) - (fp1 - fm2), d.y, d.r);
 }
 __device__ __forceinline__ float ndp(float fm1, float f0, float fp1, float fp2, Div d) {
-  return -div_exact(9.0f * (fp1 - f0) - (fp2 - fm1), d);
+  return -div_rc(9.0f * (fp1 - f0) - (fp2 - fm1), d.y, d.r);
 }
 
 template <int NT>
@@ -629,13 +740,15 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   const auto raux =
       make_rsrc(a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt), NAUX_SH * npt4);
   const auto rk = make_rsrc(a.kgrad + int64_t(g.shot) * NK * int64_t(a.npt), NK * npt4);
+  const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
   const uint32_t o_vy = 4u * a.slot_off[S_VY], o_sxy = 4u * a.slot_off[S_SXY], o_szy = 4u * a.slot_off[S_SZY];
-  const uint32_t m_rho = MF_RHO * npt4, m_mu = MF_MU * npt4;
+  const uint32_t m_rho = MF_RHO * npt4, m_irho = MF_IRHO * npt4, m_mu = MF_MU * npt4;
+  const uint32_t k_rho = K_RHO * npt4, k_mu = K_MU * npt4;
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
-  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const Div Dx{a.h8x, a.r8x, true}, Dz{a.h8z, a.r8z, true};
   const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
   const bool zt = j >= 1 && j < nz - 2, zq = j >= 2 && j < nz - 2;
   auto xt = [&](int r) { return r >= 1 && r < nx - 2; };
@@ -658,40 +771,58 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     if (k == 2) tz_cur = zt ? dt * mu * sbz[k] : 0.0f;
   }
   sbx[3] = sbz[3] = tx[3] = 0.0f;
-  float pf_sbx = bload(rin, jc4, o_sxy + g.row(i0)), pf_sbz = bload(rin, jc4, o_szy + g.row(i0));
-  float pf_mu = bload(rmod, jc4, m_mu + g.row(i0));
-  float pf_vb = bload(rin, jc4, o_vy + g.row(i0 - 1));
-  float pf_rho = bload(rmod, jc4, m_rho + g.row(i0 - 1));
-  float pf_ds = bload(raux, jc4, A_DSY * npt4 + g.row(i0 - 1));
-  float pf_ex = bload(raux, jc4, A_EXY * npt4 + g.row(i0 - 3));
-  float pf_ez = bload(raux, jc4, A_EZY * npt4 + g.row(i0 - 3));
+  // 2-deep prefetch FIFO, slot (m - (i0-1)) & 1: sigbar and mu rows m+1;
+  // vbar, rho, 1/rho, ds and K_rho rows m; strains and K_mu rows m-2
+  float pf_sbx[2], pf_sbz[2], pf_mu[2], pf_vb[2], pf_rho[2], pf_irho[2], pf_ds[2], pf_ex[2], pf_ez[2], pf_kr[2],
+      pf_km[2];
+  for (int k = 0; k < 2; ++k) {
+    const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k), rm2 = g.row(i0 - 3 + k);
+    pf_sbx[k] = bload(rin, jc4, o_sxy + r1);
+    pf_sbz[k] = bload(rin, jc4, o_szy + r1);
+    pf_mu[k] = bload(rmod, jc4, m_mu + r1);
+    pf_vb[k] = bload(rin, jc4, o_vy + r0);
+    pf_rho[k] = bload(rmod, jc4, m_rho + r0);
+    pf_irho[k] = bload(rmod, jc4, m_irho + r0);
+    pf_ds[k] = bload(raux, jc4, A_DSY * npt4 + r0);
+    pf_kr[k] = bload(rk, jc4, k_rho + r0);
+    pf_ex[k] = bload(raux, jc4, A_EXY * npt4 + rm2);
+    pf_ez[k] = bload(raux, jc4, A_EZY * npt4 + rm2);
+    pf_km[k] = bload(rk, jc4, k_mu + rm2);
+  }
+
+  // one barrier per row (as in the forward step): t of row i0-1 goes to
+  // lds_t[0]; iteration m writes q of row m and t of row m+1 before it
+  lds_t[0][t] = tz_cur;
+  __syncthreads();
 
   auto body = [&](auto phase, int m) __attribute__((always_inline)) {
     constexpr int R = decltype(phase)::value;
-    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
+    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1, f = R & 1;
     // row m+1 enters the rings
-    sbx[s3] = pf_sbx;
-    sbz[s3] = pf_sbz;
-    const float mu1 = pf_mu;
-    tx[s3] = xt(m + 1) ? dt * mu1 * pf_sbx : 0.0f;
-    const float tz_next = zt ? dt * mu1 * pf_sbz : 0.0f;
-    const float vb_o = pf_vb, rho = pf_rho, ds = pf_ds, ex = pf_ex, ez = pf_ez;
+    sbx[s3] = pf_sbx[f];
+    sbz[s3] = pf_sbz[f];
+    const float mu1 = pf_mu[f];
+    tx[s3] = xt(m + 1) ? dt * mu1 * sbx[s3] : 0.0f;
+    const float tz_next = zt ? dt * mu1 * sbz[s3] : 0.0f;
+    const float vb_o = pf_vb[f], rho = pf_rho[f], irho = pf_irho[f], ds = pf_ds[f], ex = pf_ex[f], ez = pf_ez[f];
+    const float kr = pf_kr[f], km = pf_km[f];
     {
-      const uint32_t r2 = g.row(m + 2), r1 = g.row(m + 1), rm1 = g.row(m - 1);
-      pf_sbx = bload(rin, jc4, o_sxy + r2);
-      pf_sbz = bload(rin, jc4, o_szy + r2);
-      pf_mu = bload(rmod, jc4, m_mu + r2);
-      pf_vb = bload(rin, jc4, o_vy + r1);
-      pf_rho = bload(rmod, jc4, m_rho + r1);
-      pf_ds = bload(raux, jc4, A_DSY * npt4 + r1);
-      pf_ex = bload(raux, jc4, A_EXY * npt4 + rm1);
-      pf_ez = bload(raux, jc4, A_EZY * npt4 + rm1);
+      const uint32_t r3 = g.row(m + 3), r2 = g.row(m + 2), r0 = g.row(m);
+      pf_sbx[f] = bload(rin, jc4, o_sxy + r3);
+      pf_sbz[f] = bload(rin, jc4, o_szy + r3);
+      pf_mu[f] = bload(rmod, jc4, m_mu + r3);
+      pf_vb[f] = bload(rin, jc4, o_vy + r2);
+      pf_rho[f] = bload(rmod, jc4, m_rho + r2);
+      pf_irho[f] = bload(rmod, jc4, m_irho + r2);
+      pf_ds[f] = bload(raux, jc4, A_DSY * npt4 + r2);
+      pf_kr[f] = bload(rk, jc4, k_rho + r2);
+      pf_ex[f] = bload(raux, jc4, A_EXY * npt4 + r0);
+      pf_ez[f] = bload(raux, jc4, A_EZY * npt4 + r0);
+      pf_km[f] = bload(rk, jc4, k_mu + r0);
     }
-    const float bxm = a.bx[min(max(m, 0), nx - 1)];
+    const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // adjoint "velocity" at row m
-    lds_t[par][t] = tz_cur;
-    __syncthreads();
     float vb = vb_o;
     while (ip < ie) {
       const int4 e = a.inj[ip];
@@ -703,17 +834,16 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
                ndm(lds_t[par][tm2], lds_t[par][tm1], tz_cur, lds_t[par][tp1], Dz));
     const float b = (bxm * bzb) * bzt;
     const float gv = b * vb;
-    const bool own_m = g.jst && m >= i0 && m < i1;
-    if (own_m) {
+    const float qv = div_rc(dt * gv, rho, irho);     // dt g / rho
+    if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
       bstore(gv, rout, jc4, o_vy + ro);
-      const uint32_t ko = K_RHO * npt4 + ro;
-      bstore(bload(rk, jc4, ko) + (-dt * gv * ds / (rho * rho)), rk, jc4, ko);
+      bstore(kr + (-qv * ds) * irho, rk, jc4, k_rho + ro);      // K_rho += -dt g ds / rho^2
     }
-    const float qv = dt * gv / rho;
     q[s3] = xq(m) ? qv : 0.0f;
     const float qz = zq ? qv : 0.0f;
     lds_q[par][t] = qz;
+    lds_t[par ^ 1][t] = tz_next;
     __syncthreads();
     dzq[s2] = ndp(lds_q[par][tm1], qz, lds_q[par][tp1], lds_q[par][tp2], Dz);
     tz_cur = tz_next;
@@ -726,8 +856,7 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
       const float nsz = sbz[s0] + dzq[s0];
       bstore(nsx, rout, jc4, o_sxy + ro);
       bstore(nsz, rout, jc4, o_szy + ro);
-      const uint32_t ko = K_MU * npt4 + ro;
-      bstore(bload(rk, jc4, ko) + dt * (sbx[s0] * ex + sbz[s0] * ez), rk, jc4, ko);
+      bstore(km + dt * (sbx[s0] * ex + sbz[s0] * ez), rk, jc4, k_mu + ro);
     }
   };
 
@@ -762,14 +891,16 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   const auto raux = make_rsrc(a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt),
                               NAUX_PSV * npt4);
   const auto rk = make_rsrc(a.kgrad + int64_t(g.shot) * NK * int64_t(a.npt), NK * npt4);
+  const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
   const uint32_t o_vx = 4u * a.slot_off[S_VX], o_vz = 4u * a.slot_off[S_VZ];
   const uint32_t o_sxx = 4u * a.slot_off[S_SXX], o_szz = 4u * a.slot_off[S_SZZ], o_sxz = 4u * a.slot_off[S_SXZ];
-  const uint32_t m_rho = MF_RHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4;
+  const uint32_t m_rho = MF_RHO * npt4, m_irho = MF_IRHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4;
+  const uint32_t k_rho = K_RHO * npt4, k_lam = K_LAM * npt4, k_mu = K_MU * npt4;
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
-  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const Div Dx{a.h8x, a.r8x, true}, Dz{a.h8z, a.r8z, true};
   const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
   const bool zt = j >= 1 && j < nz - 2, zq = j >= 2 && j < nz - 2;
   auto xt = [&](int r) { return r >= 1 && r < nx - 2; };
@@ -800,48 +931,72 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     if (k == 2) { tzxz_cur = zt ? a_xz : 0.0f; tzzz_cur = zt ? a_zz : 0.0f; }
   }
   sbxx[3] = sbzz[3] = sbxz[3] = txx[3] = txz[3] = 0.0f;
-  float pf_sxx = bload(rin, jc4, o_sxx + g.row(i0)), pf_szz = bload(rin, jc4, o_szz + g.row(i0)),
-        pf_sxz = bload(rin, jc4, o_sxz + g.row(i0));
-  float pf_lam = bload(rmod, jc4, m_lam + g.row(i0)), pf_mu = bload(rmod, jc4, m_mu + g.row(i0));
-  float pf_vbx = bload(rin, jc4, o_vx + g.row(i0 - 1)), pf_vbz = bload(rin, jc4, o_vz + g.row(i0 - 1));
-  float pf_rho = bload(rmod, jc4, m_rho + g.row(i0 - 1));
-  float pf_dsx = bload(raux, jc4, A_DSX * npt4 + g.row(i0 - 1)), pf_dsz = bload(raux, jc4, A_DSZ * npt4 + g.row(i0 - 1));
-  float pf_exx = bload(raux, jc4, A_EXX * npt4 + g.row(i0 - 3)), pf_ezz = bload(raux, jc4, A_EZZ * npt4 + g.row(i0 - 3)),
-        pf_exz = bload(raux, jc4, A_EXZ * npt4 + g.row(i0 - 3));
+  // 2-deep prefetch FIFO, slot (m - (i0-1)) & 1: sigbar, lambda, mu rows
+  // m+1; vbar, rho, 1/rho, ds and K_rho rows m; strains, K_lambda and K_mu
+  // rows m-2
+  float pf_sxx[2], pf_szz[2], pf_sxz[2], pf_lam[2], pf_mu[2], pf_vbx[2], pf_vbz[2], pf_rho[2], pf_irho[2];
+  float pf_dsx[2], pf_dsz[2], pf_exx[2], pf_ezz[2], pf_exz[2], pf_kr[2], pf_kl[2], pf_km[2];
+  for (int k = 0; k < 2; ++k) {
+    const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k), rm2 = g.row(i0 - 3 + k);
+    pf_sxx[k] = bload(rin, jc4, o_sxx + r1);
+    pf_szz[k] = bload(rin, jc4, o_szz + r1);
+    pf_sxz[k] = bload(rin, jc4, o_sxz + r1);
+    pf_lam[k] = bload(rmod, jc4, m_lam + r1);
+    pf_mu[k] = bload(rmod, jc4, m_mu + r1);
+    pf_vbx[k] = bload(rin, jc4, o_vx + r0);
+    pf_vbz[k] = bload(rin, jc4, o_vz + r0);
+    pf_rho[k] = bload(rmod, jc4, m_rho + r0);
+    pf_irho[k] = bload(rmod, jc4, m_irho + r0);
+    pf_dsx[k] = bload(raux, jc4, A_DSX * npt4 + r0);
+    pf_dsz[k] = bload(raux, jc4, A_DSZ * npt4 + r0);
+    pf_kr[k] = bload(rk, jc4, k_rho + r0);
+    pf_exx[k] = bload(raux, jc4, A_EXX * npt4 + rm2);
+    pf_ezz[k] = bload(raux, jc4, A_EZZ * npt4 + rm2);
+    pf_exz[k] = bload(raux, jc4, A_EXZ * npt4 + rm2);
+    pf_kl[k] = bload(rk, jc4, k_lam + rm2);
+    pf_km[k] = bload(rk, jc4, k_mu + rm2);
+  }
+
+  // one barrier per row: t of row i0-1 goes to lds_t[0]
+  lds_t[0][0][t] = tzxz_cur;
+  lds_t[0][1][t] = tzzz_cur;
+  __syncthreads();
 
   auto body = [&](auto phase, int m) __attribute__((always_inline)) {
     constexpr int R = decltype(phase)::value;
-    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
-    sbxx[s3] = pf_sxx; sbzz[s3] = pf_szz; sbxz[s3] = pf_sxz;     // row m+1
+    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1, f = R & 1;
+    sbxx[s3] = pf_sxx[f]; sbzz[s3] = pf_szz[f]; sbxz[s3] = pf_sxz[f];     // row m+1
     float n_xx, n_zz, n_xz;
-    tvals(pf_lam, pf_mu, pf_sxx, pf_szz, pf_sxz, n_xx, n_zz, n_xz);
+    tvals(pf_lam[f], pf_mu[f], sbxx[s3], sbzz[s3], sbxz[s3], n_xx, n_zz, n_xz);
     txx[s3] = xt(m + 1) ? n_xx : 0.0f;
     txz[s3] = xt(m + 1) ? n_xz : 0.0f;
     const float tzxz_next = zt ? n_xz : 0.0f, tzzz_next = zt ? n_zz : 0.0f;
-    const float vbx_o = pf_vbx, vbz_o = pf_vbz, rho = pf_rho, dsx = pf_dsx, dsz = pf_dsz;
-    const float exx = pf_exx, ezz = pf_ezz, exz = pf_exz;
+    const float vbx_o = pf_vbx[f], vbz_o = pf_vbz[f], rho = pf_rho[f], irho = pf_irho[f], dsx = pf_dsx[f],
+                dsz = pf_dsz[f];
+    const float exx = pf_exx[f], ezz = pf_ezz[f], exz = pf_exz[f], kr = pf_kr[f], kl = pf_kl[f], km = pf_km[f];
     {
-      const uint32_t r2 = g.row(m + 2), r1 = g.row(m + 1), rm1 = g.row(m - 1);
-      pf_sxx = bload(rin, jc4, o_sxx + r2);
-      pf_szz = bload(rin, jc4, o_szz + r2);
-      pf_sxz = bload(rin, jc4, o_sxz + r2);
-      pf_lam = bload(rmod, jc4, m_lam + r2);
-      pf_mu = bload(rmod, jc4, m_mu + r2);
-      pf_vbx = bload(rin, jc4, o_vx + r1);
-      pf_vbz = bload(rin, jc4, o_vz + r1);
-      pf_rho = bload(rmod, jc4, m_rho + r1);
-      pf_dsx = bload(raux, jc4, A_DSX * npt4 + r1);
-      pf_dsz = bload(raux, jc4, A_DSZ * npt4 + r1);
-      pf_exx = bload(raux, jc4, A_EXX * npt4 + rm1);
-      pf_ezz = bload(raux, jc4, A_EZZ * npt4 + rm1);
-      pf_exz = bload(raux, jc4, A_EXZ * npt4 + rm1);
+      const uint32_t r3 = g.row(m + 3), r2 = g.row(m + 2), r0 = g.row(m);
+      pf_sxx[f] = bload(rin, jc4, o_sxx + r3);
+      pf_szz[f] = bload(rin, jc4, o_szz + r3);
+      pf_sxz[f] = bload(rin, jc4, o_sxz + r3);
+      pf_lam[f] = bload(rmod, jc4, m_lam + r3);
+      pf_mu[f] = bload(rmod, jc4, m_mu + r3);
+      pf_vbx[f] = bload(rin, jc4, o_vx + r2);
+      pf_vbz[f] = bload(rin, jc4, o_vz + r2);
+      pf_rho[f] = bload(rmod, jc4, m_rho + r2);
+      pf_irho[f] = bload(rmod, jc4, m_irho + r2);
+      pf_dsx[f] = bload(raux, jc4, A_DSX * npt4 + r2);
+      pf_dsz[f] = bload(raux, jc4, A_DSZ * npt4 + r2);
+      pf_kr[f] = bload(rk, jc4, k_rho + r2);
+      pf_exx[f] = bload(raux, jc4, A_EXX * npt4 + r0);
+      pf_ezz[f] = bload(raux, jc4, A_EZZ * npt4 + r0);
+      pf_exz[f] = bload(raux, jc4, A_EXZ * npt4 + r0);
+      pf_kl[f] = bload(rk, jc4, k_lam + r0);
+      pf_km[f] = bload(rk, jc4, k_mu + r0);
     }
-    const float bxm = a.bx[min(max(m, 0), nx - 1)];
+    const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // adjoint "velocity" at row m
-    lds_t[par][0][t] = tzxz_cur;
-    lds_t[par][1][t] = tzzz_cur;
-    __syncthreads();
     float vbx = vbx_o, vbz = vbz_o;
     while (ip < ie) {
       const int4 e = a.inj[ip];
@@ -858,19 +1013,20 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
                  ndm(txz[s0], txz[s1], txz[s2], txz[s3], Dx));
     const float b = (bxm * bzb) * bzt;
     const float gx = b * vbx, gz = b * vbz;
+    const float qxv = div_rc(dt * gx, rho, irho), qzv = div_rc(dt * gz, rho, irho);   // dt g / rho
     if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
       bstore(gx, rout, jc4, o_vx + ro);
       bstore(gz, rout, jc4, o_vz + ro);
-      const uint32_t ko = K_RHO * npt4 + ro;
-      bstore(bload(rk, jc4, ko) + (-dt * (gx * dsx + gz * dsz) / (rho * rho)), rk, jc4, ko);
+      bstore(kr + (-(qxv * dsx + qzv * dsz)) * irho, rk, jc4, k_rho + ro);   // K_rho += -dt g.ds / rho^2
     }
-    const float qxv = dt * gx / rho, qzv = dt * gz / rho;
     qx[s3] = xq(m) ? qxv : 0.0f;
     qz[s3] = xq(m) ? qzv : 0.0f;
     const float qxz = zq ? qxv : 0.0f, qzz = zq ? qzv : 0.0f;
     lds_q[par][0][t] = qxz;
     lds_q[par][1][t] = qzz;
+    lds_t[par ^ 1][0][t] = tzxz_next;
+    lds_t[par ^ 1][1][t] = tzzz_next;
     __syncthreads();
     dzqx[s2] = ndp(lds_q[par][0][tm1], qxz, lds_q[par][0][tp1], lds_q[par][0][tp2], Dz);
     dzqz[s2] = ndp(lds_q[par][1][tm1], qzz, lds_q[par][1][tp1], lds_q[par][1][tp2], Dz);
@@ -887,9 +1043,8 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
       bstore(nxx, rout, jc4, o_sxx + ro);
       bstore(nzz, rout, jc4, o_szz + ro);
       bstore(nxz, rout, jc4, o_sxz + ro);
-      const uint32_t kl = K_LAM * npt4 + ro, km = K_MU * npt4 + ro;
-      bstore(bload(rk, jc4, kl) + dt * (sbxx[s0] + sbzz[s0]) * (exx + ezz), rk, jc4, kl);
-      bstore(bload(rk, jc4, km) + dt * (2.0f * sbxx[s0] * exx + 2.0f * sbzz[s0] * ezz + sbxz[s0] * exz), rk, jc4, km);
+      bstore(kl + dt * (sbxx[s0] + sbzz[s0]) * (exx + ezz), rk, jc4, k_lam + ro);
+      bstore(km + dt * (2.0f * sbxx[s0] * exx + 2.0f * sbzz[s0] * ezz + sbxz[s0] * exz), rk, jc4, k_mu + ro);
     }
   };
 
@@ -912,9 +1067,12 @@ template <int NT, int S, bool AUX>
 __global__ __launch_bounds__(NT * S) void k_step_sh(const StepArgs a) {
   step_sh<NT, S, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
-// P-SV without spin fits 80 VGPRs: ask for 6 waves per SIMD at every group size
+// P-SV without spin fits 80 VGPRs with the 2-deep prefetch: 6 waves per
+// SIMD; four shots per workgroup (16 waves, one workgroup per CU) trade
+// occupancy for the 4-deep prefetch
 template <int NT, int S, bool AUX>
-__global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(6))) void k_step_psv(const StepArgs a) {
+__global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(S >= 4 ? 4 : 6))) void k_step_psv(
+    const StepArgs a) {
   step_psv<NT, S, false, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 template <int NT, int S>
@@ -925,8 +1083,9 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_adj_sh(const StepArgs a) {
   adj_sh<NT>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
+// the P-SV adjoint keeps 17 fields in flight per row: 4 waves per SIMD
 template <int NT>
-__global__ __launch_bounds__(NT) void k_adj_psv(const StepArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_psv(const StepArgs a) {
   adj_psv<NT>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 // multi-problem entry point: one launch advances several problems

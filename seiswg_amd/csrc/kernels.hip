@@ -122,6 +122,34 @@ __device__ __forceinline__ void advance_clock(int* clock, int it, int dir, int n
 template <int R>
 using Phase = std::integral_constant<int, R>;
 
+// A register copy the compiler cannot fold away.  Stress rows stay in the
+// register ring for four rows after leaving the prefetch FIFO; by their last
+// use more than 63 vector-memory operations (the counter's range) have been
+// issued after their load, and the compiler would then wait for nearly all
+// outstanding prefetches.  Copying a row as it enters the ring waits for its
+// load once, at a point where the count is still exact.
+__device__ __forceinline__ float settle(float x) {
+  float y;
+  asm("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+  return y;
+}
+
+// Row of CSR entry q of a per-tile list (INT_MAX past the end), read into a
+// scalar register where it is loaded: the wait for the load stays on the
+// rare path that reads the list, instead of reaching the per-row compare
+// (and, through it, the prefetched rows) of every later iteration.
+__device__ __forceinline__ int next_row(const int4* list, int q, int end) {
+  return __builtin_amdgcn_readfirstlane(q < end ? list[q].x : 0x7fffffff);
+}
+
+// Uniform read through the scalar cache (constant address space): the x
+// taper factor of a row.  Scalar loads leave the vector-memory counter to
+// the prefetched rows (at most 63 vector loads and stores can be pending).
+__device__ __forceinline__ float taper_x(const float* bx, int row, int nx) {
+  const auto* c = (const __attribute__((address_space(4))) float*)bx;
+  return c[min(max(row, 0), nx - 1)];
+}
+
 // Geometry shared by the step kernels.  Rows are clamped to the grid and the
 // column to [0, nz): loads never branch and never leave the slice; values read
 // for rows/columns outside the grid only feed halo lanes whose results are
@@ -185,7 +213,6 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
-  const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
   const bool fast_c = (a.fast_div & FD_CONST) != 0;
@@ -193,12 +220,15 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
             tp2 = g.base + min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;
-  int ip = a.inj_off[g.shot * tps + g.tile];
-  const int ie = a.inj_off[g.shot * tps + g.tile + 1];
-  int inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;     // row of the next injection entry
+  int ip = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile]);
+  const int ie = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile + 1]);
+  int inj_row = next_row(a.inj, ip, ie);     // row of the next injection entry
   int rp = 0, re = 0;
-  if (a.record) { rp = a.rec_off[g.tile]; re = a.rec_off[g.tile + 1]; }
-  int rec_row = rp < re ? a.rec[rp].x : 0x7fffffff;     // row of the next receiver entry
+  if (a.record) {
+    rp = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile]);
+    re = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile + 1]);
+  }
+  int rec_row = next_row(a.rec, rp, re);     // row of the next receiver entry
   float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
   float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const auto raux =
@@ -209,8 +239,8 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   // (r - (i0-4)) & 3; d+z of new vy, row r, in slot (r - (i0-3)) & 3
   float sxy[4], szy[4], vy[4] = {0, 0, 0, 0}, dvydz[4] = {0, 0, 0, 0};
   for (int q = 0; q < 3; ++q) {
-    sxy[q] = bload(rin, jc4, o_sxy + g.row(i0 - 3 + q));
-    szy[q] = bload(rin, jc4, o_szy + g.row(i0 - 3 + q));
+    sxy[q] = settle(bload(rin, jc4, o_sxy + g.row(i0 - 3 + q)));
+    szy[q] = settle(bload(rin, jc4, o_szy + g.row(i0 - 3 + q)));
   }
   sxy[3] = szy[3] = 0.0f;
   // D-deep prefetch FIFO, slot (m - (i0-1)) % D: stress row m+1, velocity and
@@ -218,6 +248,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   // workgroup leave one workgroup per CU, with registers to spare for D = 4.
   constexpr int D = S >= 4 ? 4 : 2;
   float pf_sxy[D], pf_szy[D], pf_vy[D], pf_rho[D], pf_irho[D], pf_mu[D];
+  float bx_next = taper_x(a.bx, i0 - 1, nx);     // taper factor of the next velocity row
   for (int q = 0; q < D; ++q) {
     pf_sxy[q] = bload(rin, jc4, o_sxy + g.row(i0 + q));
     pf_szy[q] = bload(rin, jc4, o_szy + g.row(i0 + q));
@@ -239,9 +270,10 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     constexpr int R = decltype(phase)::value;
     constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
     constexpr int f = R % D;
-    sxy[s3] = pf_sxy[f];       // row m+1
-    szy[s3] = pf_szy[f];
-    const float vy_o = pf_vy[f], rho = pf_rho[f], irho = pf_irho[f], mu = pf_mu[f];
+    sxy[s3] = settle(pf_sxy[f]);       // row m+1
+    szy[s3] = settle(pf_szy[f]);
+    const float vy_o = pf_vy[f], rho = pf_rho[f], irho = pf_irho[f], mu = pf_mu[f], bxm = bx_next;
+    bx_next = taper_x(a.bx, m + 1, nx);
     {
       const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
       pf_sxy[f] = bload(rin, jc4, o_sxy + r3);
@@ -251,7 +283,6 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
       pf_irho[f] = bload(rmod, jc4, m_irho + r2);
       pf_mu[f] = bload(rmod, jc4, m_mu + r0);
     }
-    const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // velocity half-step at row m
     const bool xm_ok = m >= 2 && m < nx - 2;
@@ -268,7 +299,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
         if (e.x != m) break;
         if (e.y == j) dsy = dsy + a.val_y[int64_t(e.z) * a.nt_val + it];
       }
-      inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;
+      inj_row = next_row(a.inj, ip, ie);
     }
     const float b = (bxm * bzb) * bzt;
     float qb[1];
@@ -308,7 +339,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
           obs[(1 * a.nrec + e.z) * a.nt + it] = uy;
         }
       }
-      rec_row = rp < re ? a.rec[rp].x : 0x7fffffff;
+      rec_row = next_row(a.rec, rp, re);
     }
 
     // stress half-step at row i = m-2
@@ -328,16 +359,44 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     }
   };
 
+  // The first four rows run before the loop, which then repeats 8 phases:
+  // a FIFO slot refilled in phase k is read in phase k+D, so the registers
+  // carried around the loop are the same every trip (no copies of rows still
+  // in flight at the back-edge), and on entering the loop every slot already
+  // holds a row loaded in that steady pattern, so the compiler's wait counts
+  // for the loop body are those of the steady state rather than those of the
+  // prologue's load order.
   const int mlast = i1 + 1;
-  for (int m = i0 - 1;;) {
-    body(Phase<0>{}, m);
-    if (++m > mlast) break;
+  int m = i0 - 1;
+  body(Phase<0>{}, m);
+  if (++m <= mlast) {
     body(Phase<1>{}, m);
-    if (++m > mlast) break;
-    body(Phase<2>{}, m);
-    if (++m > mlast) break;
-    body(Phase<3>{}, m);
-    if (++m > mlast) break;
+    if (++m <= mlast) {
+      body(Phase<2>{}, m);
+      if (++m <= mlast) {
+        body(Phase<3>{}, m);
+        if (++m <= mlast) {
+          for (;;) {
+            body(Phase<4>{}, m);
+            if (++m > mlast) break;
+            body(Phase<5>{}, m);
+            if (++m > mlast) break;
+            body(Phase<6>{}, m);
+            if (++m > mlast) break;
+            body(Phase<7>{}, m);
+            if (++m > mlast) break;
+            body(Phase<0>{}, m);
+            if (++m > mlast) break;
+            body(Phase<1>{}, m);
+            if (++m > mlast) break;
+            body(Phase<2>{}, m);
+            if (++m > mlast) break;
+            body(Phase<3>{}, m);
+            if (++m > mlast) break;
+          }
+        }
+      }
+    }
   }
   if (a.advance) advance_clock(a.clock, it, 1, a.nblocks);
 }
@@ -375,7 +434,6 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
-  const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
   const bool fast_c = (a.fast_div & FD_CONST) != 0;
@@ -383,12 +441,15 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
             tp2 = g.base + min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;
-  int ip = a.inj_off[g.shot * tps + g.tile];
-  const int ie = a.inj_off[g.shot * tps + g.tile + 1];
-  int inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;
+  int ip = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile]);
+  const int ie = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile + 1]);
+  int inj_row = next_row(a.inj, ip, ie);
   int rp = 0, re = 0;
-  if (a.record) { rp = a.rec_off[g.tile]; re = a.rec_off[g.tile + 1]; }
-  int rec_row = rp < re ? a.rec[rp].x : 0x7fffffff;   // first row of the pending receivers
+  if (a.record) {
+    rp = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile]);
+    re = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile + 1]);
+  }
+  int rec_row = next_row(a.rec, rp, re);   // first row of the pending receivers
   float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
   float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const int nobs = a.nrec * a.nt;
@@ -402,13 +463,13 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   float dvxdz[4] = {0, 0, 0, 0}, dvzdz[4] = {0, 0, 0, 0}, dvycdz[4] = {0, 0, 0, 0};
   for (int q = 0; q < 3; ++q) {
     const uint32_t r = g.row(i0 - 3 + q);
-    sxx[q] = bload(rin, jc4, o_sxx + r);
-    sxz[q] = bload(rin, jc4, o_sxz + r);
-    szz[q] = bload(rin, jc4, o_szz + r);
+    sxx[q] = settle(bload(rin, jc4, o_sxx + r));
+    sxz[q] = settle(bload(rin, jc4, o_sxz + r));
+    szz[q] = settle(bload(rin, jc4, o_szz + r));
     if constexpr (SPIN) {
-      syx[q] = bload(rin, jc4, o_syx + r);
-      syy[q] = bload(rin, jc4, o_syy + r);
-      syz[q] = bload(rin, jc4, o_syz + r);
+      syx[q] = settle(bload(rin, jc4, o_syx + r));
+      syy[q] = settle(bload(rin, jc4, o_syy + r));
+      syz[q] = settle(bload(rin, jc4, o_syz + r));
     }
   }
   sxx[3] = sxz[3] = szz[3] = syx[3] = syy[3] = syz[3] = 0.0f;
@@ -417,6 +478,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   // density / J rows m, moduli rows m-2 of iteration m
   constexpr int D = S >= 4 ? 4 : 2;
   float pf_sxx[D], pf_sxz[D], pf_szz[D], pf_vx[D], pf_vz[D], pf_rho[D], pf_irho[D], pf_lam[D], pf_mu[D];
+  float bx_next = taper_x(a.bx, i0 - 1, nx);     // taper factor of the next velocity row
   float pf_syx[D] = {}, pf_syy[D] = {}, pf_syz[D] = {}, pf_vyc[D] = {}, pf_jj[D] = {}, pf_ij[D] = {}, pf_nu[D] = {},
         pf_muc[D] = {}, pf_nuc[D] = {};
   for (int q = 0; q < D; ++q) {
@@ -453,9 +515,11 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     constexpr int R = decltype(phase)::value;
     constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
     constexpr int f = R % D;
-    sxx[s3] = pf_sxx[f]; sxz[s3] = pf_sxz[f]; szz[s3] = pf_szz[f];     // row m+1
-    if constexpr (SPIN) { syx[s3] = pf_syx[f]; syy[s3] = pf_syy[f]; syz[s3] = pf_syz[f]; }
+    sxx[s3] = settle(pf_sxx[f]); sxz[s3] = settle(pf_sxz[f]); szz[s3] = settle(pf_szz[f]);     // row m+1
+    if constexpr (SPIN) { syx[s3] = settle(pf_syx[f]); syy[s3] = settle(pf_syy[f]); syz[s3] = settle(pf_syz[f]); }
     const float vx_o = pf_vx[f], vz_o = pf_vz[f], rho = pf_rho[f], irho = pf_irho[f], lam = pf_lam[f], mu = pf_mu[f];
+    const float bxm = bx_next;
+    bx_next = taper_x(a.bx, m + 1, nx);
     const float vyc_o = pf_vyc[f], jj = pf_jj[f], ij = pf_ij[f], nu = pf_nu[f], muc = pf_muc[f], nuc = pf_nuc[f];
     {
       const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
@@ -480,7 +544,6 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
         pf_nuc[f] = bload(rmod, jc4, m_nuc + r0);
       }
     }
-    const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // ---- velocity half-step at row m
     const bool xm_ok = m >= 2 && m < nx - 2;
@@ -522,7 +585,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
           dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it];
         }
       }
-      inj_row = ip < ie ? a.inj[ip].x : 0x7fffffff;
+      inj_row = next_row(a.inj, ip, ie);
     }
     const float b = (bxm * bzb) * bzt;
     // divisions by rho (and J)
@@ -598,7 +661,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
             obs[2 * nobs + e.z * a.nt + it] = uz;
           }
         }
-        rec_row = rp < re ? a.rec[rp].x : 0x7fffffff;
+        rec_row = next_row(a.rec, rp, re);
       }
     } else {
       // a receiver on row ir needs uz on rows ir-1..ir+2 and ux on columns j-1..j+2
@@ -640,7 +703,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
           }
         }
         while (rp < re && a.rec[rp].x + 2 <= m) ++rp;
-        rec_row = rp < re ? a.rec[rp].x : 0x7fffffff;
+        rec_row = next_row(a.rec, rp, re);
       }
     }
 
@@ -678,16 +741,44 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     }
   };
 
+  // The first four rows run before the loop, which then repeats 8 phases:
+  // a FIFO slot refilled in phase k is read in phase k+D, so the registers
+  // carried around the loop are the same every trip (no copies of rows still
+  // in flight at the back-edge), and on entering the loop every slot already
+  // holds a row loaded in that steady pattern, so the compiler's wait counts
+  // for the loop body are those of the steady state rather than those of the
+  // prologue's load order.
   const int mlast = i1 + 1;
-  for (int m = i0 - 1;;) {
-    body(Phase<0>{}, m);
-    if (++m > mlast) break;
+  int m = i0 - 1;
+  body(Phase<0>{}, m);
+  if (++m <= mlast) {
     body(Phase<1>{}, m);
-    if (++m > mlast) break;
-    body(Phase<2>{}, m);
-    if (++m > mlast) break;
-    body(Phase<3>{}, m);
-    if (++m > mlast) break;
+    if (++m <= mlast) {
+      body(Phase<2>{}, m);
+      if (++m <= mlast) {
+        body(Phase<3>{}, m);
+        if (++m <= mlast) {
+          for (;;) {
+            body(Phase<4>{}, m);
+            if (++m > mlast) break;
+            body(Phase<5>{}, m);
+            if (++m > mlast) break;
+            body(Phase<6>{}, m);
+            if (++m > mlast) break;
+            body(Phase<7>{}, m);
+            if (++m > mlast) break;
+            body(Phase<0>{}, m);
+            if (++m > mlast) break;
+            body(Phase<1>{}, m);
+            if (++m > mlast) break;
+            body(Phase<2>{}, m);
+            if (++m > mlast) break;
+            body(Phase<3>{}, m);
+            if (++m > mlast) break;
+          }
+        }
+      }
+    }
   }
   if (a.advance) advance_clock(a.clock, it, 1, a.nblocks);
 }
@@ -740,7 +831,6 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   const auto raux =
       make_rsrc(a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt), NAUX_SH * npt4);
   const auto rk = make_rsrc(a.kgrad + int64_t(g.shot) * NK * int64_t(a.npt), NK * npt4);
-  const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
   const uint32_t o_vy = 4u * a.slot_off[S_VY], o_sxy = 4u * a.slot_off[S_SXY], o_szy = 4u * a.slot_off[S_SZY];
   const uint32_t m_rho = MF_RHO * npt4, m_irho = MF_IRHO * npt4, m_mu = MF_MU * npt4;
   const uint32_t k_rho = K_RHO * npt4, k_mu = K_MU * npt4;
@@ -754,8 +844,9 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   auto xt = [&](int r) { return r >= 1 && r < nx - 2; };
   auto xq = [&](int r) { return r >= 2 && r < nx - 2; };
   const int tps = a.nstrips * a.nsegs;
-  int ip = a.inj_off[g.shot * tps + g.tile];
-  const int ie = a.inj_off[g.shot * tps + g.tile + 1];
+  int ip = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile]);
+  const int ie = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile + 1]);
+  int inj_row = next_row(a.inj, ip, ie);     // row of the next adjoint-source entry
 
   // rings (as in the forward): sigbar rows m-2..m+1 and their x-masked t;
   // q (x-masked) rows m-3..m; -D+z(q^) rows m-2..m
@@ -764,8 +855,8 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   for (int k = 0; k < 3; ++k) {
     const int r = i0 - 3 + k;
     const uint32_t ro = g.row(r);
-    sbx[k] = bload(rin, jc4, o_sxy + ro);
-    sbz[k] = bload(rin, jc4, o_szy + ro);
+    sbx[k] = settle(bload(rin, jc4, o_sxy + ro));
+    sbz[k] = settle(bload(rin, jc4, o_szy + ro));
     const float mu = bload(rmod, jc4, m_mu + ro);
     tx[k] = xt(r) ? dt * mu * sbx[k] : 0.0f;
     if (k == 2) tz_cur = zt ? dt * mu * sbz[k] : 0.0f;
@@ -775,6 +866,7 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   // vbar, rho, 1/rho, ds and K_rho rows m; strains and K_mu rows m-2
   float pf_sbx[2], pf_sbz[2], pf_mu[2], pf_vb[2], pf_rho[2], pf_irho[2], pf_ds[2], pf_ex[2], pf_ez[2], pf_kr[2],
       pf_km[2];
+  float bx_next = taper_x(a.bx, i0 - 1, nx);
   for (int k = 0; k < 2; ++k) {
     const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k), rm2 = g.row(i0 - 3 + k);
     pf_sbx[k] = bload(rin, jc4, o_sxy + r1);
@@ -799,13 +891,14 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     constexpr int R = decltype(phase)::value;
     constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1, f = R & 1;
     // row m+1 enters the rings
-    sbx[s3] = pf_sbx[f];
-    sbz[s3] = pf_sbz[f];
+    sbx[s3] = settle(pf_sbx[f]);
+    sbz[s3] = settle(pf_sbz[f]);
     const float mu1 = pf_mu[f];
     tx[s3] = xt(m + 1) ? dt * mu1 * sbx[s3] : 0.0f;
     const float tz_next = zt ? dt * mu1 * sbz[s3] : 0.0f;
     const float vb_o = pf_vb[f], rho = pf_rho[f], irho = pf_irho[f], ds = pf_ds[f], ex = pf_ex[f], ez = pf_ez[f];
-    const float kr = pf_kr[f], km = pf_km[f];
+    const float kr = pf_kr[f], km = pf_km[f], bxm = bx_next;
+    bx_next = taper_x(a.bx, m + 1, nx);
     {
       const uint32_t r3 = g.row(m + 3), r2 = g.row(m + 2), r0 = g.row(m);
       pf_sbx[f] = bload(rin, jc4, o_sxy + r3);
@@ -820,15 +913,16 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
       pf_ez[f] = bload(raux, jc4, A_EZY * npt4 + r0);
       pf_km[f] = bload(rk, jc4, k_mu + r0);
     }
-    const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // adjoint "velocity" at row m
     float vb = vb_o;
-    while (ip < ie) {
-      const int4 e = a.inj[ip];
-      if (e.x > m) break;
-      if (e.x == m && e.y == j) vb = vb + a.val_y[int64_t(e.z) * a.nt_val + it];
-      ++ip;
+    if (m >= inj_row) {
+      for (; ip < ie; ++ip) {
+        const int4 e = a.inj[ip];
+        if (e.x > m) break;
+        if (e.x == m && e.y == j) vb = vb + a.val_y[int64_t(e.z) * a.nt_val + it];
+      }
+      inj_row = next_row(a.inj, ip, ie);
     }
     vb = vb + (ndm(tx[s0], tx[s1], tx[s2], tx[s3], Dx) +
                ndm(lds_t[par][tm2], lds_t[par][tm1], tz_cur, lds_t[par][tp1], Dz));
@@ -891,7 +985,6 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   const auto raux = make_rsrc(a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt),
                               NAUX_PSV * npt4);
   const auto rk = make_rsrc(a.kgrad + int64_t(g.shot) * NK * int64_t(a.npt), NK * npt4);
-  const auto rbx = make_rsrc(a.bx, 4u * uint32_t(nx));
   const uint32_t o_vx = 4u * a.slot_off[S_VX], o_vz = 4u * a.slot_off[S_VZ];
   const uint32_t o_sxx = 4u * a.slot_off[S_SXX], o_szz = 4u * a.slot_off[S_SZZ], o_sxz = 4u * a.slot_off[S_SXZ];
   const uint32_t m_rho = MF_RHO * npt4, m_irho = MF_IRHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4;
@@ -906,8 +999,9 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   auto xt = [&](int r) { return r >= 1 && r < nx - 2; };
   auto xq = [&](int r) { return r >= 2 && r < nx - 2; };
   const int tps = a.nstrips * a.nsegs;
-  int ip = a.inj_off[g.shot * tps + g.tile];
-  const int ie = a.inj_off[g.shot * tps + g.tile + 1];
+  int ip = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile]);
+  const int ie = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile + 1]);
+  int inj_row = next_row(a.inj, ip, ie);     // row of the next adjoint-source entry
 
   float sbxx[4], sbzz[4], sbxz[4], txx[4], txz[4];
   float qx[4] = {0, 0, 0, 0}, qz[4] = {0, 0, 0, 0}, dzqx[4] = {0, 0, 0, 0}, dzqz[4] = {0, 0, 0, 0};
@@ -921,9 +1015,9 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   for (int k = 0; k < 3; ++k) {
     const int r = i0 - 3 + k;
     const uint32_t ro = g.row(r);
-    sbxx[k] = bload(rin, jc4, o_sxx + ro);
-    sbzz[k] = bload(rin, jc4, o_szz + ro);
-    sbxz[k] = bload(rin, jc4, o_sxz + ro);
+    sbxx[k] = settle(bload(rin, jc4, o_sxx + ro));
+    sbzz[k] = settle(bload(rin, jc4, o_szz + ro));
+    sbxz[k] = settle(bload(rin, jc4, o_sxz + ro));
     float a_xx, a_zz, a_xz;
     tvals(bload(rmod, jc4, m_lam + ro), bload(rmod, jc4, m_mu + ro), sbxx[k], sbzz[k], sbxz[k], a_xx, a_zz, a_xz);
     txx[k] = xt(r) ? a_xx : 0.0f;
@@ -936,6 +1030,7 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   // rows m-2
   float pf_sxx[2], pf_szz[2], pf_sxz[2], pf_lam[2], pf_mu[2], pf_vbx[2], pf_vbz[2], pf_rho[2], pf_irho[2];
   float pf_dsx[2], pf_dsz[2], pf_exx[2], pf_ezz[2], pf_exz[2], pf_kr[2], pf_kl[2], pf_km[2];
+  float bx_next = taper_x(a.bx, i0 - 1, nx);
   for (int k = 0; k < 2; ++k) {
     const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k), rm2 = g.row(i0 - 3 + k);
     pf_sxx[k] = bload(rin, jc4, o_sxx + r1);
@@ -965,7 +1060,7 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   auto body = [&](auto phase, int m) __attribute__((always_inline)) {
     constexpr int R = decltype(phase)::value;
     constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1, f = R & 1;
-    sbxx[s3] = pf_sxx[f]; sbzz[s3] = pf_szz[f]; sbxz[s3] = pf_sxz[f];     // row m+1
+    sbxx[s3] = settle(pf_sxx[f]); sbzz[s3] = settle(pf_szz[f]); sbxz[s3] = settle(pf_sxz[f]);     // row m+1
     float n_xx, n_zz, n_xz;
     tvals(pf_lam[f], pf_mu[f], sbxx[s3], sbzz[s3], sbxz[s3], n_xx, n_zz, n_xz);
     txx[s3] = xt(m + 1) ? n_xx : 0.0f;
@@ -974,6 +1069,8 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     const float vbx_o = pf_vbx[f], vbz_o = pf_vbz[f], rho = pf_rho[f], irho = pf_irho[f], dsx = pf_dsx[f],
                 dsz = pf_dsz[f];
     const float exx = pf_exx[f], ezz = pf_ezz[f], exz = pf_exz[f], kr = pf_kr[f], kl = pf_kl[f], km = pf_km[f];
+    const float bxm = bx_next;
+    bx_next = taper_x(a.bx, m + 1, nx);
     {
       const uint32_t r3 = g.row(m + 3), r2 = g.row(m + 2), r0 = g.row(m);
       pf_sxx[f] = bload(rin, jc4, o_sxx + r3);
@@ -994,18 +1091,19 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
       pf_kl[f] = bload(rk, jc4, k_lam + r0);
       pf_km[f] = bload(rk, jc4, k_mu + r0);
     }
-    const float bxm = bload(rbx, 0u, 4u * uint32_t(min(max(m, 0), nx - 1)));
 
     // adjoint "velocity" at row m
     float vbx = vbx_o, vbz = vbz_o;
-    while (ip < ie) {
-      const int4 e = a.inj[ip];
-      if (e.x > m) break;
-      if (e.x == m && e.y == j) {
-        vbx = vbx + a.val_x[int64_t(e.z) * a.nt_val + it];
-        vbz = vbz + a.val_z[int64_t(e.z) * a.nt_val + it];
+    if (m >= inj_row) {
+      for (; ip < ie; ++ip) {
+        const int4 e = a.inj[ip];
+        if (e.x > m) break;
+        if (e.x == m && e.y == j) {
+          vbx = vbx + a.val_x[int64_t(e.z) * a.nt_val + it];
+          vbz = vbz + a.val_z[int64_t(e.z) * a.nt_val + it];
+        }
       }
-      ++ip;
+      inj_row = next_row(a.inj, ip, ie);
     }
     vbx = vbx + (ndm(txx[s0], txx[s1], txx[s2], txx[s3], Dx) +
                  ndm(lds_t[par][0][tm2], lds_t[par][0][tm1], tzxz_cur, lds_t[par][0][tp1], Dz));

@@ -113,6 +113,135 @@ def gradient_bench(seiswg, synth, nt_override=0):
             "misfit_total": float(mis_b.sum())}
 
 
+def small_config_bench(seiswg, synth, which, with_cpu=True):
+    """One forward run of the plain SH configuration (C0: 200x200, 2000
+    steps, launch-bound) or the P-SV + spin configuration (C1: 500x500,
+    3000 steps, with a stable micro-inertia, see synth.c1_problem): GPU
+    wall time of the whole time loop after a warm-up run, and the oracle's
+    FP32 CPU restatement of the same run on the host cores."""
+    abi = seiswg.abi
+    spin = which == "c1"
+    q = synth.c1_problem() if spin else synth.c0_problem()
+    nx, nz, h, nt, dt, g = q["nx"], q["nz"], q["h"], q["nt"], q["dt"], q["geom"]
+    ctx = seiswg.Context(nx, nz, nt, h, h, dt, sh=not spin, psv=spin, spin=spin, abs_left=True, abs_right=True,
+                         abs_bottom=True, abs_top=True, abs_width=q["abs_width"], abs_alpha=0.015)
+    m = q["model"]
+    if spin:
+        ctx.set_model(abi.MODEL_LAME, [m[k] for k in ("rho", "lambda", "mu", "nu", "j", "lambda_c", "mu_c", "nu_c")])
+    else:
+        ctx.set_model(abi.MODEL_VPVS, [m["rho"], m["vp"], m["vs"]])
+    rows = g.src_rows()
+    src_idx = [seiswg.grid_index(r[0], r[1], h, h, nz) for r in rows]
+    rec_idx = [seiswg.grid_index(x, z, h, h, nz) for x, z in zip(g.rec_x, g.rec_z)]
+    ctx.set_sources(src_idx, seiswg.ricker_table([[r[3], r[4], r[5], r[6]] for r in rows], nt, dt))
+    ctx.set_receivers(rec_idx)
+    ctx.forward_batch([-1])
+    t0 = time.perf_counter()
+    ctx.forward_batch([-1])
+    wall = time.perf_counter() - t0
+    tm = ctx.timing()
+    tiling = list(ctx.tiling())
+    ctx.close()
+    cells = nx * nz * nt
+    out = {"workload": ("plain SH forward, 200x200, nt=2000, 25 receivers, taper on all edges" if not spin else
+                        "P-SV + micropolar spin forward, 500x500, nt=3000, j=1e4, 7 receivers"),
+           "cell_updates_per_s": cells / wall, "s_total": wall, "us_per_step": wall / nt * 1e6,
+           "kernel_loop_ms": tm.loop_ms, "launches": int(tm.step_launches), "tiling": tiling}
+    if with_cpu:
+        from oracle import oracle as O
+        p = O.Problem(nx=nx, nz=nz, nt=nt, dx=h, dz=h, dt=dt, sh=not spin, psv=spin, spin=spin, abs_left=True,
+                      abs_right=True, abs_bottom=True, abs_top=True, abs_width=q["abs_width"], abs_alpha=0.015,
+                      model=m, src_rows=rows, src_idx=src_idx, rec_idx=rec_idx)
+        o = O.Oracle(p, 4)
+        t0 = time.perf_counter()
+        o.forward(-1, 0)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cells / el, "unit": "cell-updates/s", "cores": int(O.lib().or_num_threads()),
+                               "kind": "port", "sample": f"oracle FP32 restatement, the full run ({nt} steps) in "
+                                                         f"{el:.2f} s"}
+    return out
+
+
+def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0):
+    """Uneven batch of 48 independent SH / P-SV gradient problems (grids
+    384..3072 x 384..1536, 4-16 shots, 100-300 receivers, nt scaled from
+    1500..4000 by nt_scale, checkpoint every 50): total shots/s with the
+    problems merged into shared launches (sw_multi_gradient, in waves that
+    fit budget_gb of device memory) vs one problem per launch (sw_gradient
+    per problem, its shots batched)."""
+    import numpy as np
+    abi = seiswg.abi
+    probs = synth.c4_problems(nt_scale)
+
+    def make_ctx(p, model):
+        sh = p["kind"] == "sh"
+        x = seiswg.Context(p["nx"], p["nz"], p["nt"], p["h"], p["h"], p["dt"], sh=sh, psv=not sh, abs_left=True,
+                           abs_right=True, abs_bottom=True, abs_top=False, abs_width=40, abs_alpha=0.015)
+        x.set_model(abi.MODEL_VPVS, [model["rho"], model["vp"], model["vs"]])
+        g = p["geom"]
+        rows = g.src_rows()
+        x.set_sources([seiswg.grid_index(r[0], r[1], p["h"], p["h"], p["nz"]) for r in rows],
+                      seiswg.ricker_table([[r[3], r[4], r[5], r[6]] for r in rows], p["nt"], p["dt"]))
+        x.set_receivers([seiswg.grid_index(a, b, p["h"], p["h"], p["nz"]) for a, b in zip(g.rec_x, g.rec_z)])
+        return x
+
+    comps = {"sh": [abi.OBS_Y], "psv": [abi.OBS_X, abi.OBS_Z]}
+    obs, shots = [], []
+    for p in probs:
+        n = len(p["geom"].src_x)
+        c = make_ctx(p, p["true"])
+        c.forward_batch(list(range(n)))
+        obs.append(np.stack([np.stack([c.obs(s, k) for k in comps[p["kind"]]]) for s in range(n)]))
+        shots.append(list(range(n)))
+        c.close()
+
+    def mem_bytes(p, n):
+        nslot = 3 if p["kind"] == "sh" else 5
+        npt = p["nx"] * p["nz"]
+        nck = -(-p["nt"] // synth.C4["ckpt"])
+        return 4.0 * npt * n * (4 * nslot + (nck + 1) * nslot + synth.C4["ckpt"] * nslot + 3) + 4.0 * n * 5 * 300 * p["nt"]
+
+    total_shots = sum(len(x) for x in shots)
+    t_plain, ref0 = 0.0, None
+    for q, (p, o, sh) in enumerate(zip(probs, obs, shots)):
+        c = make_ctx(p, p["init"])
+        t0 = time.perf_counter()
+        r = c.gradient(sh, o, ckpt_every=synth.C4["ckpt"])
+        t_plain += time.perf_counter() - t0
+        if q == 0:
+            ref0 = r
+        c.close()
+    waves, cur, used = [], [], 0.0
+    for q, p in enumerate(probs):
+        b = mem_bytes(p, len(shots[q]))
+        if cur and used + b > budget_gb * 1e9:
+            waves.append(cur)
+            cur, used = [], 0.0
+        cur.append(q)
+        used += b
+    waves.append(cur)
+    t_batched, got0 = 0.0, None
+    for w in waves:
+        ctxs = [make_ctx(probs[q], probs[q]["init"]) for q in w]
+        m = seiswg.Multi(ctxs)
+        t0 = time.perf_counter()
+        res = m.gradient([shots[q] for q in w], [obs[q] for q in w], ckpt_every=synth.C4["ckpt"])
+        t_batched += time.perf_counter() - t0
+        if w[0] == 0:
+            got0 = res[0]
+        m.close()
+        for c in ctxs:
+            c.close()
+    same = all(np.array_equal(a.view(np.uint64 if a.dtype == np.float64 else np.uint32),
+                              b.view(np.uint64 if b.dtype == np.float64 else np.uint32)) for a, b in zip(got0, ref0))
+    return {"workload": f"uneven batch of {len(probs)} SH/P-SV gradient problems (grids 384..3072 x 384..1536, "
+                        f"4-16 shots, nt = {nt_scale} x 1500..4000, checkpoint every {synth.C4['ckpt']})",
+            "problems": len(probs), "shots": total_shots, "waves": len(waves),
+            "shots_per_s_batched": total_shots / t_batched, "shots_per_s_plain": total_shots / t_plain,
+            "speedup": t_plain / t_batched, "s_batched": t_batched, "s_plain": t_plain,
+            "problem0_bit_identical": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +255,9 @@ def main():
     ap.add_argument("--gradient-nt", type=int, default=0, help="time steps of the gradient workload (0: 3000)")
     ap.add_argument("--gradient-only", action="store_true",
                     help="run only the batched-vs-plain gradient measurement (for profiler runs)")
+    ap.add_argument("--uneven", action="store_true", help="also measure the uneven batch of 48 gradient problems")
+    ap.add_argument("--no-small", action="store_true", help="skip the 200x200 SH and 500x500 spin forward runs")
+    ap.add_argument("--uneven-nt-scale", type=float, default=0.04, help="nt of the uneven batch, as a fraction")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB copy kernel (calibrates the HBM counters of a profiler run)")
     a = ap.parse_args()
@@ -177,6 +309,11 @@ def main():
     ctx.close()
     if not a.no_gradient:
         out["gradient"] = gradient_bench(seiswg, synth, a.gradient_nt)
+    if not a.no_small:
+        out["c0_sh_forward"] = small_config_bench(seiswg, synth, "c0", not a.no_cpu_baseline)
+        out["c1_spin_forward"] = small_config_bench(seiswg, synth, "c1", not a.no_cpu_baseline)
+    if a.uneven:
+        out["uneven_batch"] = uneven_bench(seiswg, synth, a.uneven_nt_scale)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(synth, model, geom, src_idx, rec_idx)
     print(json.dumps(out), flush=True)

@@ -130,3 +130,69 @@ def c3_problem():
     geom = line_geometry(c["nx"], c["h"], c["nshots"], c["nrec"], rec_first=2, rec_step=step, f0=c["f0"],
                          t0=c["t0"], ang=c["ang"])
     return init, true, geom
+
+
+C4 = dict(nproblems=48, seed=4, nx=(384, 3072), nz=(384, 1536), shots=(4, 16), nrec=(100, 300), nt=(1500, 4000),
+          f0=(5.0, 15.0), h=10.0, cfl=0.5, abs_width=40, ckpt=50)
+
+
+def c4_problems(nt_scale: float = 1.0, nproblems: int | None = None):
+    """Uneven batch of independent SH and P-SV gradient problems (seeded):
+    grid nx in [384, 3072], nz in [384, 1536], each with its own
+    layered-plus-random-field model (the C2 recipe, seed 100+q) and a true
+    model with Gaussian vs anomalies; 4-16 shots at 50 m depth, 100-300
+    receivers at 10 m; nt in [1500, 4000] (times nt_scale, at least 60);
+    dt from CFL 0.5 on the model's largest vp; Ricker f0 in [5, 15] Hz,
+    t0 = 1.2 / f0.  Even problems are SH, odd ones P-SV.  Returns a list of
+    dicts: kind, nx, nz, h, dt, nt, init, true, geom."""
+    c = C4
+    rng = np.random.default_rng(c["seed"])
+    out = []
+    for q in range(nproblems or c["nproblems"]):
+        nx = int(rng.integers(c["nx"][0] // 8, c["nx"][1] // 8 + 1)) * 8
+        nz = int(rng.integers(c["nz"][0] // 8, c["nz"][1] // 8 + 1)) * 8
+        nshots = int(rng.integers(c["shots"][0], c["shots"][1] + 1))
+        nrec = int(rng.integers(c["nrec"][0], c["nrec"][1] + 1))
+        nt = max(60, int(round(int(rng.integers(c["nt"][0], c["nt"][1] + 1)) * nt_scale)))
+        f0 = float(rng.uniform(*c["f0"]))
+        init = layered_random_model(nx, nz, c["h"], 100 + q)
+        true = gaussian_anomalies(init, nx, nz, c["h"], 200 + q)
+        vmax = float(max(init["vp"].max(), true["vp"].max()))
+        dt = float(np.float32(c["cfl"] * c["h"] / vmax))
+        step = max((nx - 4) // nrec, 1)
+        geom = line_geometry(nx, c["h"], nshots, nrec, rec_first=2, rec_step=step, f0=f0, t0=1.2 / f0, ang=90.0)
+        out.append(dict(kind="sh" if q % 2 == 0 else "psv", nx=nx, nz=nz, h=c["h"], dt=dt, nt=nt, init=init,
+                        true=true, geom=geom))
+    return out
+
+
+def c0_problem():
+    """Plain SH forward: homogeneous 200x200, dx = dz = 100 m, vs = 3000,
+    rho = 2600 (vp = 5200 stored), nt = 2000, dt = 0.006 s; one Ricker source
+    at the centre (f0 = 2 Hz, t0 = 0.6 s, angle 0, amplitude 1e10); 25
+    receivers at iz = 40, ix = 28 + 6r; absorbing on all edges, width 20."""
+    nx = nz = 200
+    model = {"rho": np.full(nx * nz, 2600.0, np.float32), "vp": np.full(nx * nz, 5200.0, np.float32),
+             "vs": np.full(nx * nz, 3000.0, np.float32)}
+    g = Geometry(src_x=[10000.0], src_z=[10000.0], rec_x=[float((28 + 6 * r) * 100) for r in range(25)],
+                 rec_z=[4000.0] * 25, f0=2.0, t0=0.6, ang=0.0, amp=1e10)
+    return dict(nx=nx, nz=nz, h=100.0, nt=2000, dt=0.006, abs_width=20, model=model, geom=g)
+
+
+def c1_problem(j_value: float = 1.0e4):
+    """P-SV + micropolar spin forward: homogeneous 500x500, dx = dz = 5 m,
+    lambda = 2e10, mu = 1.5e10, nu = 2e9, lambda_c = mu_c = nu_c = 1e5,
+    rho = 2500, nt = 3000, dt = 4e-4; Ricker at the centre (f0 = 25 Hz,
+    t0 = 0.05 s, 45 degrees, 1e10); 7 receivers on the 45-degree ray
+    (ix = iz = 270 + 20n); absorbing on all edges, width 40.  The
+    micro-inertia j: with the stated j = 1e-2 the rotation mode has
+    omega*dt ~ 358 and the explicit scheme diverges; j_value (default 1e4,
+    omega*dt ~ 0.36) keeps it stable."""
+    nx = nz = 500
+    n = nx * nz
+    f = lambda v: np.full(n, v, np.float32)
+    model = {"rho": f(2500.0), "lambda": f(2.0e10), "mu": f(1.5e10), "nu": f(2.0e9), "j": f(j_value),
+             "lambda_c": f(1.0e5), "mu_c": f(1.0e5), "nu_c": f(1.0e5)}
+    g = Geometry(src_x=[1250.0], src_z=[1250.0], rec_x=[float((270 + 20 * k) * 5) for k in range(7)],
+                 rec_z=[float((270 + 20 * k) * 5) for k in range(7)], f0=25.0, t0=0.05, ang=45.0, amp=1e10)
+    return dict(nx=nx, nz=nz, h=5.0, nt=3000, dt=4e-4, abs_width=40, model=model, geom=g)

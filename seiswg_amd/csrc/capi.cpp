@@ -423,8 +423,8 @@ StepArgs make_args(sw_ctx* c, StepMode mode) {
   a.st[1] = c->state[1].p;
   a.it_lo = -0x7fffffff;
   a.it_hi = 0x7fffffff;
-  // forward steps run tile.group shots per workgroup, adjoint steps one
-  a.nblocks = (mode == M_ADJ ? c->nshots : c->nshots / c->tile.group) * c->tile.nstrips * c->tile.nsegs;
+  // forward and adjoint steps run tile.group shots per workgroup
+  a.nblocks = c->nshots / c->tile.group * c->tile.nstrips * c->tile.nsegs;
   a.advance = 1;
   if (mode == M_ADJ) {
     a.inj_off = c->ainj_off.p; a.inj = c->ainj.p;

@@ -814,12 +814,12 @@ __device__ __forceinline__ float ndp(float fm1, float f0, float fp1, float fp2, 
   return -div_rc(9.0f * (fp1 - f0) - (fp2 - fm1), d.y, d.r);
 }
 
-template <int NT>
+template <int NT, int S>
 __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
-  __shared__ float lds_t[2][NT];
-  __shared__ float lds_q[2][NT];
-  const TileGeom g = tile_geom<NT>(a, logical);
-  const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
+  __shared__ float lds_t[2][S * NT];
+  __shared__ float lds_q[2][S * NT];
+  const TileGeom g = tile_geom<NT, S>(a, logical);
+  const int t = g.t, lane = g.lane, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (it < a.it_lo || it >= a.it_hi) return;
   const int64_t so = int64_t(g.shot) * a.shot_floats;
@@ -839,7 +839,8 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, true}, Dz{a.h8z, a.r8z, true};
-  const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
+  const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
+            tp2 = g.base + min(t + 2, NT - 1);
   const bool zt = j >= 1 && j < nz - 2, zq = j >= 2 && j < nz - 2;
   auto xt = [&](int r) { return r >= 1 && r < nx - 2; };
   auto xq = [&](int r) { return r >= 2 && r < nx - 2; };
@@ -862,12 +863,14 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     if (k == 2) tz_cur = zt ? dt * mu * sbz[k] : 0.0f;
   }
   sbx[3] = sbz[3] = tx[3] = 0.0f;
-  // 2-deep prefetch FIFO, slot (m - (i0-1)) & 1: sigbar and mu rows m+1;
-  // vbar, rho, 1/rho, ds and K_rho rows m; strains and K_mu rows m-2
-  float pf_sbx[2], pf_sbz[2], pf_mu[2], pf_vb[2], pf_rho[2], pf_irho[2], pf_ds[2], pf_ex[2], pf_ez[2], pf_kr[2],
-      pf_km[2];
+  // D-deep prefetch FIFO, slot (m - (i0-1)) % D: sigbar and mu rows m+1;
+  // vbar, rho, 1/rho, ds and K_rho rows m; strains and K_mu rows m-2 of
+  // iteration m.  D = 2, or 1 with several shots per workgroup (registers).
+  constexpr int D = S > 1 ? 1 : 2;
+  float pf_sbx[D], pf_sbz[D], pf_mu[D], pf_vb[D], pf_rho[D], pf_irho[D], pf_ds[D], pf_ex[D], pf_ez[D], pf_kr[D],
+      pf_km[D];
   float bx_next = taper_x(a.bx, i0 - 1, nx);
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < D; ++k) {
     const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k), rm2 = g.row(i0 - 3 + k);
     pf_sbx[k] = bload(rin, jc4, o_sxy + r1);
     pf_sbz[k] = bload(rin, jc4, o_szy + r1);
@@ -884,12 +887,12 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
 
   // one barrier per row (as in the forward step): t of row i0-1 goes to
   // lds_t[0]; iteration m writes q of row m and t of row m+1 before it
-  lds_t[0][t] = tz_cur;
+  lds_t[0][lane] = tz_cur;
   __syncthreads();
 
   auto body = [&](auto phase, int m) __attribute__((always_inline)) {
     constexpr int R = decltype(phase)::value;
-    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1, f = R & 1;
+    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1, f = R % D;
     // row m+1 enters the rings
     sbx[s3] = settle(pf_sbx[f]);
     sbz[s3] = settle(pf_sbz[f]);
@@ -900,7 +903,7 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     const float kr = pf_kr[f], km = pf_km[f], bxm = bx_next;
     bx_next = taper_x(a.bx, m + 1, nx);
     {
-      const uint32_t r3 = g.row(m + 3), r2 = g.row(m + 2), r0 = g.row(m);
+      const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
       pf_sbx[f] = bload(rin, jc4, o_sxy + r3);
       pf_sbz[f] = bload(rin, jc4, o_szy + r3);
       pf_mu[f] = bload(rmod, jc4, m_mu + r3);
@@ -936,8 +939,8 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     }
     q[s3] = xq(m) ? qv : 0.0f;
     const float qz = zq ? qv : 0.0f;
-    lds_q[par][t] = qz;
-    lds_t[par ^ 1][t] = tz_next;
+    lds_q[par][lane] = qz;
+    lds_t[par ^ 1][lane] = tz_next;
     __syncthreads();
     dzq[s2] = ndp(lds_q[par][tm1], qz, lds_q[par][tp1], lds_q[par][tp2], Dz);
     tz_cur = tz_next;
@@ -968,12 +971,12 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   if (a.advance) advance_clock(a.clock, it, -1, a.nblocks);
 }
 
-template <int NT>
+template <int NT, int S>
 __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
-  __shared__ float lds_t[2][2][NT];   // z-masked t_xz, t_zz of row m
-  __shared__ float lds_q[2][2][NT];   // z-masked qx, qz of row m
-  const TileGeom g = tile_geom<NT>(a, logical);
-  const int t = g.t, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
+  __shared__ float lds_t[2][2][S * NT];   // z-masked t_xz, t_zz of row m
+  __shared__ float lds_q[2][2][S * NT];   // z-masked qx, qz of row m
+  const TileGeom g = tile_geom<NT, S>(a, logical);
+  const int t = g.t, lane = g.lane, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (it < a.it_lo || it >= a.it_hi) return;
   const int64_t so = int64_t(g.shot) * a.shot_floats;
@@ -994,7 +997,8 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, true}, Dz{a.h8z, a.r8z, true};
-  const int tm2 = max(t - 2, 0), tm1 = max(t - 1, 0), tp1 = min(t + 1, NT - 1), tp2 = min(t + 2, NT - 1);
+  const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
+            tp2 = g.base + min(t + 2, NT - 1);
   const bool zt = j >= 1 && j < nz - 2, zq = j >= 2 && j < nz - 2;
   auto xt = [&](int r) { return r >= 1 && r < nx - 2; };
   auto xq = [&](int r) { return r >= 2 && r < nx - 2; };
@@ -1025,13 +1029,14 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     if (k == 2) { tzxz_cur = zt ? a_xz : 0.0f; tzzz_cur = zt ? a_zz : 0.0f; }
   }
   sbxx[3] = sbzz[3] = sbxz[3] = txx[3] = txz[3] = 0.0f;
-  // 2-deep prefetch FIFO, slot (m - (i0-1)) & 1: sigbar, lambda, mu rows
+  // D-deep prefetch FIFO, slot (m - (i0-1)) % D: sigbar, lambda, mu rows
   // m+1; vbar, rho, 1/rho, ds and K_rho rows m; strains, K_lambda and K_mu
-  // rows m-2
-  float pf_sxx[2], pf_szz[2], pf_sxz[2], pf_lam[2], pf_mu[2], pf_vbx[2], pf_vbz[2], pf_rho[2], pf_irho[2];
-  float pf_dsx[2], pf_dsz[2], pf_exx[2], pf_ezz[2], pf_exz[2], pf_kr[2], pf_kl[2], pf_km[2];
+  // rows m-2 of iteration m.  D = 2, or 1 with several shots per workgroup.
+  constexpr int D = S > 1 ? 1 : 2;
+  float pf_sxx[D], pf_szz[D], pf_sxz[D], pf_lam[D], pf_mu[D], pf_vbx[D], pf_vbz[D], pf_rho[D], pf_irho[D];
+  float pf_dsx[D], pf_dsz[D], pf_exx[D], pf_ezz[D], pf_exz[D], pf_kr[D], pf_kl[D], pf_km[D];
   float bx_next = taper_x(a.bx, i0 - 1, nx);
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < D; ++k) {
     const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k), rm2 = g.row(i0 - 3 + k);
     pf_sxx[k] = bload(rin, jc4, o_sxx + r1);
     pf_szz[k] = bload(rin, jc4, o_szz + r1);
@@ -1053,13 +1058,13 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   }
 
   // one barrier per row: t of row i0-1 goes to lds_t[0]
-  lds_t[0][0][t] = tzxz_cur;
-  lds_t[0][1][t] = tzzz_cur;
+  lds_t[0][0][lane] = tzxz_cur;
+  lds_t[0][1][lane] = tzzz_cur;
   __syncthreads();
 
   auto body = [&](auto phase, int m) __attribute__((always_inline)) {
     constexpr int R = decltype(phase)::value;
-    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1, f = R & 1;
+    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1, f = R % D;
     sbxx[s3] = settle(pf_sxx[f]); sbzz[s3] = settle(pf_szz[f]); sbxz[s3] = settle(pf_sxz[f]);     // row m+1
     float n_xx, n_zz, n_xz;
     tvals(pf_lam[f], pf_mu[f], sbxx[s3], sbzz[s3], sbxz[s3], n_xx, n_zz, n_xz);
@@ -1072,7 +1077,7 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     const float bxm = bx_next;
     bx_next = taper_x(a.bx, m + 1, nx);
     {
-      const uint32_t r3 = g.row(m + 3), r2 = g.row(m + 2), r0 = g.row(m);
+      const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
       pf_sxx[f] = bload(rin, jc4, o_sxx + r3);
       pf_szz[f] = bload(rin, jc4, o_szz + r3);
       pf_sxz[f] = bload(rin, jc4, o_sxz + r3);
@@ -1121,10 +1126,10 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     qx[s3] = xq(m) ? qxv : 0.0f;
     qz[s3] = xq(m) ? qzv : 0.0f;
     const float qxz = zq ? qxv : 0.0f, qzz = zq ? qzv : 0.0f;
-    lds_q[par][0][t] = qxz;
-    lds_q[par][1][t] = qzz;
-    lds_t[par ^ 1][0][t] = tzxz_next;
-    lds_t[par ^ 1][1][t] = tzzz_next;
+    lds_q[par][0][lane] = qxz;
+    lds_q[par][1][lane] = qzz;
+    lds_t[par ^ 1][0][lane] = tzxz_next;
+    lds_t[par ^ 1][1][lane] = tzzz_next;
     __syncthreads();
     dzqx[s2] = ndp(lds_q[par][0][tm1], qxz, lds_q[par][0][tp1], lds_q[par][0][tp2], Dz);
     dzqz[s2] = ndp(lds_q[par][1][tm1], qzz, lds_q[par][1][tp1], lds_q[par][1][tp2], Dz);
@@ -1177,14 +1182,14 @@ template <int NT, int S>
 __global__ __launch_bounds__(NT * S) void k_step_spin(const StepArgs a) {
   step_psv<NT, S, true, false>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
-template <int NT>
-__global__ __launch_bounds__(NT) void k_adj_sh(const StepArgs a) {
-  adj_sh<NT>(a, xcd_remap(blockIdx.x, gridDim.x));
+template <int NT, int S>
+__global__ __launch_bounds__(NT * S) void k_adj_sh(const StepArgs a) {
+  adj_sh<NT, S>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 // the P-SV adjoint keeps 17 fields in flight per row: 4 waves per SIMD
-template <int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_psv(const StepArgs a) {
-  adj_psv<NT>(a, xcd_remap(blockIdx.x, gridDim.x));
+template <int NT, int S>
+__global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_psv(const StepArgs a) {
+  adj_psv<NT, S>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 // multi-problem entry point: one launch advances several problems
 template <int NT, int KIND>
@@ -1196,8 +1201,8 @@ __global__ __launch_bounds__(NT) void k_multi(const StepArgs* __restrict__ args,
   else if constexpr (KIND == MK_SPIN) step_psv<NT, 1, true, false>(a, w.y);
   else if constexpr (KIND == MK_SH_AUX) step_sh<NT, 1, true>(a, w.y);
   else if constexpr (KIND == MK_PSV_AUX) step_psv<NT, 1, false, true>(a, w.y);
-  else if constexpr (KIND == MK_ADJ_SH) adj_sh<NT>(a, w.y);
-  else adj_psv<NT>(a, w.y);
+  else if constexpr (KIND == MK_ADJ_SH) adj_sh<NT, 1>(a, w.y);
+  else adj_psv<NT, 1>(a, w.y);
 }
 
 template <int NT>
@@ -1429,10 +1434,8 @@ int blocks_per_cu_ns(int kind) {
     case MK_SH_AUX: return occupancy(k_step_sh<NT, S, true>, NT * S);
     case MK_PSV_AUX: return occupancy(k_step_psv<NT, S, true>, NT * S);
   }
-  if constexpr (S == 1) {
-    if (kind == MK_ADJ_SH) return occupancy(k_adj_sh<NT>, NT);
-    if (kind == MK_ADJ_PSV) return occupancy(k_adj_psv<NT>, NT);
-  }
+  if (kind == MK_ADJ_SH) return occupancy(k_adj_sh<NT, S>, NT * S);
+  if (kind == MK_ADJ_PSV) return occupancy(k_adj_psv<NT, S>, NT * S);
   return 0;
 }
 }  // namespace
@@ -1469,26 +1472,28 @@ hipError_t launch_multi(int kind, int nt_threads, const StepArgs* args, const in
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_adj_sh(const StepArgs& a, const Tiling& t, hipStream_t s) {
-  const int nblk = a.nshots * t.nstrips * t.nsegs;
-  switch (t.nt_threads) {
-    case 64: hipLaunchKernelGGL(k_adj_sh<64>, dim3(nblk), dim3(64), 0, s, a); break;
-    case 128: hipLaunchKernelGGL(k_adj_sh<128>, dim3(nblk), dim3(128), 0, s, a); break;
-    case 256: hipLaunchKernelGGL(k_adj_sh<256>, dim3(nblk), dim3(256), 0, s, a); break;
-    default: return hipErrorInvalidValue;
-  }
+template <int NT, int S>
+hipError_t launch_adj_ns(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s) {
+  const int nblk = a.nshots / S * t.nstrips * t.nsegs;
+  if (sh) hipLaunchKernelGGL((k_adj_sh<NT, S>), dim3(nblk), dim3(NT * S), 0, s, a);
+  else hipLaunchKernelGGL((k_adj_psv<NT, S>), dim3(nblk), dim3(NT * S), 0, s, a);
   return hipGetLastError();
 }
-hipError_t launch_adj_psv(const StepArgs& a, const Tiling& t, hipStream_t s) {
-  const int nblk = a.nshots * t.nstrips * t.nsegs;
-  switch (t.nt_threads) {
-    case 64: hipLaunchKernelGGL(k_adj_psv<64>, dim3(nblk), dim3(64), 0, s, a); break;
-    case 128: hipLaunchKernelGGL(k_adj_psv<128>, dim3(nblk), dim3(128), 0, s, a); break;
-    case 256: hipLaunchKernelGGL(k_adj_psv<256>, dim3(nblk), dim3(256), 0, s, a); break;
-    default: return hipErrorInvalidValue;
+hipError_t launch_adj(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s) {
+  if (t.group < 1 || a.nshots % t.group != 0) return hipErrorInvalidValue;
+  switch (t.nt_threads * 8 + t.group) {
+    case 64 * 8 + 1: return launch_adj_ns<64, 1>(sh, a, t, s);
+    case 128 * 8 + 1: return launch_adj_ns<128, 1>(sh, a, t, s);
+    case 128 * 8 + 2: return launch_adj_ns<128, 2>(sh, a, t, s);
+    case 128 * 8 + 4: return launch_adj_ns<128, 4>(sh, a, t, s);
+    case 256 * 8 + 1: return launch_adj_ns<256, 1>(sh, a, t, s);
+    case 256 * 8 + 2: return launch_adj_ns<256, 2>(sh, a, t, s);
+    case 256 * 8 + 4: return launch_adj_ns<256, 4>(sh, a, t, s);
   }
-  return hipGetLastError();
+  return hipErrorInvalidValue;
 }
+hipError_t launch_adj_sh(const StepArgs& a, const Tiling& t, hipStream_t s) { return launch_adj(true, a, t, s); }
+hipError_t launch_adj_psv(const StepArgs& a, const Tiling& t, hipStream_t s) { return launch_adj(false, a, t, s); }
 hipError_t launch_adjsrc(const float* obs_syn, const float* obs_ref, const float* taper, float* adj, double* misfit,
                          int nshots, int ncomp, const int* comps, int nrec, int nt, float dt, hipStream_t s) {
   const int total = nshots * ncomp * nrec;

@@ -171,7 +171,12 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0):
     per problem, its shots batched)."""
     import numpy as np
     abi = seiswg.abi
+
+    def progress(msg):
+        print(f"[uneven batch] {msg}", file=sys.stderr, flush=True)
+
     probs = synth.c4_problems(nt_scale)
+    progress(f"{len(probs)} problems generated")
 
     def make_ctx(p, model):
         sh = p["kind"] == "sh"
@@ -194,6 +199,7 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0):
         obs.append(np.stack([np.stack([c.obs(s, k) for k in comps[p["kind"]]]) for s in range(n)]))
         shots.append(list(range(n)))
         c.close()
+    progress("observed data computed")
 
     def mem_bytes(p, n):
         nslot = 3 if p["kind"] == "sh" else 5
@@ -211,6 +217,8 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0):
         if q == 0:
             ref0 = r
         c.close()
+        if q % 8 == 7:
+            progress(f"plain: {q + 1} problems, {t_plain:.1f} s")
     waves, cur, used = [], [], 0.0
     for q, p in enumerate(probs):
         b = mem_bytes(p, len(shots[q]))
@@ -232,6 +240,7 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0):
         m.close()
         for c in ctxs:
             c.close()
+        progress(f"batched: wave of {len(w)} problems, {t_batched:.1f} s so far")
     same = all(np.array_equal(a.view(np.uint64 if a.dtype == np.float64 else np.uint32),
                               b.view(np.uint64 if b.dtype == np.float64 else np.uint32)) for a, b in zip(got0, ref0))
     return {"workload": f"uneven batch of {len(probs)} SH/P-SV gradient problems (grids 384..3072 x 384..1536, "
@@ -250,6 +259,8 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="tile width (64/128/256); 0 = library default")
     ap.add_argument("--seglen", type=int, default=0)
     ap.add_argument("--group", type=int, default=0, help="shots per workgroup (1, 2, 4); 0 = library default")
+    ap.add_argument("--steps-per-launch", type=int, default=0,
+                    help="P-SV forward steps per launch (1 or 2); 0 = library default (2)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gradient", action="store_true", help="skip the batched-vs-plain gradient measurement")
     ap.add_argument("--gradient-nt", type=int, default=0, help="time steps of the gradient workload (0: 3000)")
@@ -275,6 +286,8 @@ def main():
     nt = max(synth.C2["nt"], a.warmup + a.steps)
     ctx, model, geom, src_idx, rec_idx = build_context(seiswg, synth, nt)
     nshots = synth.C2["nshots"]
+    if a.steps_per_launch:
+        ctx.set_steps_per_launch(a.steps_per_launch)
     ctx.prepare(list(range(nshots)))
     if a.threads or a.seglen or a.group:
         t, s, g = ctx.tiling()

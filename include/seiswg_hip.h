@@ -180,6 +180,14 @@ SW_API int sw_smooth(sw_ctx* ctx, int32_t field, float sigma);
  * sw_get_tiling reports the shape of the last prepared batch. */
 SW_API int sw_set_tiling(sw_ctx* ctx, int32_t threads, int32_t seglen, int32_t shots_per_block);
 SW_API int sw_get_tiling(const sw_ctx* ctx, int32_t* threads, int32_t* seglen, int32_t* shots_per_block);
+/* Time steps per launch of the P-SV forward step without spin (no
+ * counterpart in the reference, which dispatches every half-step
+ * separately): 2 (the default) runs steps in pairs of launches that each
+ * advance two steps, reading the state once and writing it once per launch;
+ * 1 runs one step per launch.  Results are the same bit for bit.  SH, spin
+ * and the gradient's recompute/adjoint steps always run one step per
+ * launch.  A change to 2 takes effect at the next prepared batch. */
+SW_API int sw_set_steps_per_launch(sw_ctx* ctx, int32_t steps);
 /* Autotuner: times `steps` forward steps of `nshots` shots for a set of
  * tile shapes, keeps the fastest (as sw_set_tiling does) and returns it with
  * its ms per step.  Tile shapes never change results. */

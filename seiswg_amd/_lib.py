@@ -48,6 +48,7 @@ SNAP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.PO
 EXPORTS = ["sw_last_error", "sw_create", "sw_destroy", "sw_set_model", "sw_set_sources", "sw_set_receivers",
            "sw_forward", "sw_forward_batch", "sw_read_obs", "sw_read_field", "sw_write_field", "sw_prepare",
            "sw_advance", "sw_adjoint_plain", "sw_smooth", "sw_set_tiling", "sw_get_tiling", "sw_get_timing",
+           "sw_set_steps_per_launch",
            "sw_bench_copy", "sw_gradient", "sw_autotune", "sw_multi_create", "sw_multi_destroy",
            "sw_multi_forward", "sw_multi_gradient", "sw_multi_timing",
            "seiswg_run", "seiswg_ricker_table", "seiswg_infer_grid", "seiswg_grid_index"]
@@ -80,6 +81,7 @@ def load():
     L.sw_adjoint_plain.argtypes = [vp, C.c_int32, ip, fp, C.c_int32, C.c_float, dp]
     L.sw_smooth.argtypes = [vp, C.c_int32, C.c_float]
     L.sw_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+    L.sw_set_steps_per_launch.argtypes = [vp, C.c_int32]
     L.sw_get_tiling.argtypes = [vp, ip, ip, ip]
     L.sw_get_timing.argtypes = [vp, C.POINTER(sw_timing)]
     L.sw_multi_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
@@ -216,6 +218,11 @@ class Context:
 
     def set_tiling(self, threads: int, seglen: int, shots_per_block: int = 0):
         _check(self.L.sw_set_tiling(self.h, threads, seglen, shots_per_block))
+
+    def set_steps_per_launch(self, steps: int):
+        """P-SV forward steps per launch (2: pairs of two-step launches, the
+        default; 1: one step per launch).  Same results either way."""
+        _check(self.L.sw_set_steps_per_launch(self.h, steps))
 
     def autotune(self, nshots: int, steps: int = 10):
         """Fastest tile shape for nshots shots: (threads, seglen, shots per block, ms per step)."""

@@ -112,6 +112,15 @@ struct sw_ctx {
   DBuf<int> clock;
   DBuf<int> inj_off, rec_off;
   DBuf<int4> inj, rec;
+  // two-step launches (P-SV without spin): their tiling (6-column halos),
+  // per-tile lists and the third state buffer they write between the two
+  // ping-pong buffers (a pair of launches returns the state to state[cur])
+  int steps_per_launch = 2;   // requested: 1 or 2
+  bool two_ok = false;        // the prepared batch can use them
+  Tiling tile2;
+  DBuf<int> inj2_off, rec2_off;
+  DBuf<int4> inj2, rec2;
+  DBuf<float> spare;
   int64_t steps_done = 0;
   // injection values of the current batch (forward: stf; adjoint: adjoint source)
   const float* val_x = nullptr;
@@ -128,7 +137,7 @@ struct sw_ctx {
   struct GraphEntry { uint64_t key; hipGraphExec_t exec; };
   std::vector<GraphEntry> graphs;
   uint64_t graph_gen = 0;
-  int graph_steps = 16;       // steps per captured block (even)
+  int graph_steps = 16;       // steps per captured block (a multiple of 4)
 
   // plain adjoint
   DBuf<float> kmu, k_lam, k_rho, gsum, gtmp, wtab, sumx, sumz, fwd_store, adj_src, ref, taper;
@@ -254,6 +263,10 @@ int check_tiling(sw_ctx* c, int nshots) {
   t.nstrips = (c->g.nz + W - 1) / W;
   t.nsegs = (c->g.nx + t.seglen - 1) / t.seglen;
   t.group = effective_group(t.nt_threads, c->tile_fixed ? c->group_req : t.group, nshots, uses_spin(c));
+  // two-step launches: same tile width, segments and shots per workgroup
+  c->tile2 = t;
+  c->tile2.nstrips = (c->g.nz + (t.nt_threads - 12) - 1) / (t.nt_threads - 12);
+  c->two_ok = c->steps_per_launch == 2 && uses_psv(c) && !uses_spin(c) && !uses_sh(c);
   return 0;
 }
 
@@ -263,12 +276,12 @@ int check_tiling(sw_ctx* c, int nshots) {
 // stable, so duplicates of one cell stay in index order.
 struct Entry { int i, j, q; };
 
-void tile_bounds(const sw_ctx* c, int tile, int& i0, int& i1, int& c0) {
-  const Tiling& t = c->tile;
+void tile_bounds(const sw_ctx* c, int tile, int& i0, int& i1, int& c0, int halo = 3) {
+  const Tiling& t = halo == 6 ? c->tile2 : c->tile;
   const int strip = tile % t.nstrips, seg = tile / t.nstrips;
   i0 = seg * t.seglen;
   i1 = std::min(i0 + t.seglen, c->g.nx);
-  c0 = strip * (t.nt_threads - 6) - 3;
+  c0 = strip * (t.nt_threads - 2 * halo) - halo;
 }
 
 int upload_inj(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj, DBuf<int>& off_d, DBuf<int4>& data_d) {
@@ -297,9 +310,14 @@ int upload_inj(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj, DBuf<
   return 0;
 }
 
-int build_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj) {
-  const int ntile = c->tile.nstrips * c->tile.nsegs;
-  const int NT = c->tile.nt_threads;
+// Lists of one tiling: halo 3 (one step per launch) or 6 (two steps; the
+// first stage computes velocities on rows [i0-4, i1+4]).
+int build_tile_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj, int halo, DBuf<int>& off_d,
+                     DBuf<int4>& data_d, DBuf<int>& roff_d, DBuf<int4>& rdata_d) {
+  const Tiling& tl = halo == 6 ? c->tile2 : c->tile;
+  const int ntile = tl.nstrips * tl.nsegs;
+  const int NT = tl.nt_threads;
+  const int before = halo == 6 ? 4 : 1, after = halo == 6 ? 5 : 2;
   std::vector<int> off(size_t(c->nshots) * ntile + 1, 0);
   std::vector<int4> data;
   for (int s = 0; s < c->nshots; ++s) {
@@ -307,10 +325,11 @@ int build_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj) {
     std::stable_sort(pts.begin(), pts.end(), [](const Entry& a, const Entry& b) { return a.i < b.i; });
     for (int t = 0; t < ntile; ++t) {
       int i0, i1, c0;
-      tile_bounds(c, t, i0, i1, c0);
+      tile_bounds(c, t, i0, i1, c0, halo);
       off[size_t(s) * ntile + t] = int(data.size());
       for (const Entry& e : pts)
-        if (e.i >= i0 - 1 && e.i < i1 + 2 && e.j >= c0 && e.j < c0 + NT) data.push_back(make_int4(e.i, e.j, e.q, 0));
+        if (e.i >= i0 - before && e.i < i1 + after && e.j >= c0 && e.j < c0 + NT)
+          data.push_back(make_int4(e.i, e.j, e.q, 0));
     }
   }
   off.back() = int(data.size());
@@ -326,25 +345,31 @@ int build_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj) {
   std::vector<int4> rdata;
   for (int t = 0; t < ntile; ++t) {
     int i0, i1, c0;
-    tile_bounds(c, t, i0, i1, c0);
+    tile_bounds(c, t, i0, i1, c0, halo);
     roff[t] = int(rdata.size());
     for (const Entry& e : recs)
-      if (e.i >= i0 && e.i < i1 && e.j >= c0 + 3 && e.j < c0 + NT - 3 && e.j < c->g.nz)
+      if (e.i >= i0 && e.i < i1 && e.j >= c0 + halo && e.j < c0 + NT - halo && e.j < c->g.nz)
         rdata.push_back(make_int4(e.i, e.j, e.q, 0));
   }
   roff[ntile] = int(rdata.size());
-  HIPCHK(c->inj_off.ensure(off.size()));
-  HIPCHK(hipMemcpyAsync(c->inj_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c->inj.ensure(std::max<size_t>(data.size(), 1)));
+  HIPCHK(off_d.ensure(off.size()));
+  HIPCHK(hipMemcpyAsync(off_d.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(data_d.ensure(std::max<size_t>(data.size(), 1)));
   if (!data.empty())
-    HIPCHK(hipMemcpyAsync(c->inj.p, data.data(), data.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c->rec_off.ensure(roff.size()));
-  HIPCHK(hipMemcpyAsync(c->rec_off.p, roff.data(), roff.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c->rec.ensure(std::max<size_t>(rdata.size(), 1)));
+    HIPCHK(hipMemcpyAsync(data_d.p, data.data(), data.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(roff_d.ensure(roff.size()));
+  HIPCHK(hipMemcpyAsync(roff_d.p, roff.data(), roff.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(rdata_d.ensure(std::max<size_t>(rdata.size(), 1)));
   if (!rdata.empty())
-    HIPCHK(hipMemcpyAsync(c->rec.p, rdata.data(), rdata.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(rdata_d.p, rdata.data(), rdata.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));   // host vectors go out of scope
   return 0;
+}
+
+int build_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj) {
+  int rc = build_tile_lists(c, shot_inj, 3, c->inj_off, c->inj, c->rec_off, c->rec);
+  if (rc || !c->two_ok) return rc;
+  return build_tile_lists(c, shot_inj, 6, c->inj2_off, c->inj2, c->rec2_off, c->rec2);
 }
 
 std::vector<Entry> source_entries(const sw_ctx* c, int isrc) {
@@ -445,6 +470,27 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   const bool aux = mode == M_FWD_AUX;
   StepArgs a_sh = a;
   a_sh.advance = psv ? 0 : 1;   // with both physics the P-SV kernel advances the clock
+  // Four steps as two two-step launches: state[cur] -> spare -> state[cur],
+  // so the buffer of every step stays that of its parity.
+  const bool pairs = mode == M_FWD && c->two_ok && c->steps_per_launch == 2 && nsteps >= 4;
+  StepArgs a2a{}, a2b{};
+  if (pairs) {
+    HIPCHK(c->spare.ensure(size_t(c->nslots) * size_t(c->nshots) * size_t(c->npt)));
+    a2a = a;
+    a2a.nstrips = c->tile2.nstrips;
+    a2a.inj_off = c->inj2_off.p; a2a.inj = c->inj2.p;
+    a2a.rec_off = c->rec2_off.p; a2a.rec = c->rec2.p;
+    a2a.nblocks = c->nshots / c->tile2.group * c->tile2.nstrips * c->tile2.nsegs;
+    a2b = a2a;
+    a2a.st[0] = c->state[c->cur].p; a2a.st[1] = c->spare.p;
+    a2b.st[0] = c->spare.p; a2b.st[1] = c->state[c->cur].p;
+  }
+  auto enqueue4 = [&]() -> int {
+    HIPCHK(launch_step_psv2(a2a, c->tile2, c->stream));
+    HIPCHK(launch_step_psv2(a2b, c->tile2, c->stream));
+    c->launches += 2;
+    return 0;
+  };
   // one time step: the fused kernel(s) of the enabled physics
   auto enqueue = [&]() -> int {
     if (mode == M_ADJ) {
@@ -477,7 +523,7 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     return 0;
   }
   int done = 0;
-  const int G = c->graph_steps;
+  const int G = c->graph_steps;   // a multiple of 4
   if (G >= 2 && nsteps >= 2 * G && !getenv("SEISWG_NO_GRAPHS")) {
     if (c->graph_gen != c->gen) {
       for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.exec);
@@ -485,7 +531,7 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
       c->graph_gen = c->gen;
     }
     const uint64_t key = uint64_t(mode) | (uint64_t(a.record) << 4) | (uint64_t(c->sh_only) << 5) |
-                         (uint64_t(G) << 8);
+                         (uint64_t(pairs) << 6) | (uint64_t(c->cur) << 7) | (uint64_t(G) << 8);
     hipGraphExec_t exec = nullptr;
     for (auto& e : c->graphs)
       if (e.key == key) exec = e.exec;
@@ -494,7 +540,10 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
       const int64_t l0 = c->launches;
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       int rc = 0;
-      for (int q = 0; q < G && !rc; ++q) rc = enqueue();
+      if (pairs)
+        for (int q = 0; q < G && !rc; q += 4) rc = enqueue4();
+      else
+        for (int q = 0; q < G && !rc; ++q) rc = enqueue();
       hipError_t e_end = hipStreamEndCapture(c->stream, &graph);
       if (rc) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -507,13 +556,19 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
       c->graphs.push_back({key, exec});
       c->launches = l0;   // captured, not launched
     }
-    const int64_t per = (mode != M_ADJ && sh && psv) ? 2 : 1;
+    const int64_t per_block = pairs ? G / 2 : (mode != M_ADJ && sh && psv) ? 2 * G : G;
     for (; done + G <= nsteps; done += G) {
       HIPCHK(hipGraphLaunch(exec, c->stream));
-      c->launches += per * G;
+      c->launches += per_block;
       count_step(G);
     }
   }
+  if (pairs)
+    for (; done + 4 <= nsteps; done += 4) {
+      int rc = enqueue4();
+      if (rc) return rc;
+      count_step(4);
+    }
   for (; done < nsteps; ++done) {
     int rc = enqueue();
     if (rc) return rc;
@@ -827,6 +882,13 @@ SW_API int sw_write_field(sw_ctx* c, int32_t shot, int32_t field, const float* i
     c->j_fast = all_fast_divisors(in, size_t(c->npt));
   }
   HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+SW_API int sw_set_steps_per_launch(sw_ctx* c, int32_t steps) {
+  if (!c || (steps != 1 && steps != 2)) return fail(-1, "steps per launch must be 1 or 2");
+  c->steps_per_launch = steps;
+  if (steps == 1) c->two_ok = false;
   return 0;
 }
 

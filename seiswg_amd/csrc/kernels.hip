@@ -166,9 +166,10 @@ struct TileGeom {
 // NT-thread group per shot (S divides the shot count): the groups share the
 // barriers, so their model-field loads of a row come close together in time
 // and all but the first are served by the caches.
-template <int NT, int S = 1>
+// H is the halo width in columns: 3 for one step per launch, 6 for two.
+template <int NT, int S = 1, int H = 3>
 __device__ __forceinline__ TileGeom tile_geom(const StepArgs& a, int logical) {
-  constexpr int W = NT - 6;
+  constexpr int W = NT - 2 * H;
   TileGeom g;
   const int tps = a.nstrips * a.nsegs;
   const int grp = logical / tps;
@@ -180,11 +181,11 @@ __device__ __forceinline__ TileGeom tile_geom(const StepArgs& a, int logical) {
   const int strip = g.tile % a.nstrips, seg = g.tile / a.nstrips;
   g.t = g.lane - g.base;
   g.nx = a.nx;
-  g.j = strip * W - 3 + g.t;
+  g.j = strip * W - H + g.t;
   g.i0 = seg * a.seglen;
   g.i1 = min(g.i0 + a.seglen, a.nx);
   g.jin = g.j >= 0 && g.j < a.nz;
-  g.jst = g.jin && g.t >= 3 && g.t < NT - 3;
+  g.jst = g.jin && g.t >= H && g.t < NT - H;
   g.zm_ok = g.j >= 2 && g.j < a.nz - 2;
   g.zp_ok = g.j >= 1 && g.j < a.nz - 2;
   g.jc4 = 4u * uint32_t(min(max(g.j, 0), a.nz - 1));
@@ -783,6 +784,348 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   if (a.advance) advance_clock(a.clock, it, 1, a.nblocks);
 }
 
+// ------------------------------------------- two P-SV steps per launch
+// Steps it and it+1 of P-SV without spin in one march.  A second pipeline
+// stage trails the first by three rows and takes the stresses of step it+1
+// from registers and LDS as the first stage produces them, so the state of
+// step it+1 never goes to memory: each workgroup reads the state of step it
+// once and writes that of step it+2 once, half the HBM traffic of two
+// one-step launches.  The price is a 6-column halo (W = NT-12) and 4-5 rows of
+// recomputation at each end of a segment.  Iteration m runs
+//   stage 1 (step it):   velocity row m | barrier | stress row m-2
+//   stage 2 (step it+1): velocity row m-3, D+z of velocity row m-4, stress row m-5
+// Stage-2 z-neighbours go through LDS written after the barrier of one
+// iteration and read after the barrier of the next.  Every cell's arithmetic
+// is that of step_psv, operation for operation, so the result equals two
+// one-step launches bit for bit.  The host passes the buffer to read in st[0]
+// and the one to write in st[1].
+template <int NT, int S>
+__device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
+  __shared__ float lds_s[2][2][S * NT];    // stage 1: sxz, szz of step it
+  __shared__ float lds_v[2][2][S * NT];    // stage 1: vx, vz of step it+1
+  __shared__ float lds_s2[2][2][S * NT];   // stage 2: sxz, szz of step it+1
+  __shared__ float lds_v2[2][2][S * NT];   // stage 2: vx, vz of step it+2
+  const TileGeom g = tile_geom<NT, S, 6>(a, logical);
+  const int t = g.t, lane = g.lane, j = g.j, nx = a.nx, i0 = g.i0, i1 = g.i1;
+  const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int64_t so = int64_t(g.shot) * a.shot_floats;
+  const uint32_t sbytes = 4u * uint32_t(a.shot_floats);
+  const auto rin = make_rsrc(a.st[0] + so, sbytes);
+  const auto rout = make_rsrc(a.st[1] + so, sbytes);
+  const uint32_t npt4 = 4u * uint32_t(a.npt);
+  const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
+  const uint32_t o_vx = 4u * a.slot_off[S_VX], o_vz = 4u * a.slot_off[S_VZ];
+  const uint32_t o_sxx = 4u * a.slot_off[S_SXX], o_szz = 4u * a.slot_off[S_SZZ], o_sxz = 4u * a.slot_off[S_SXZ];
+  const uint32_t m_rho = MF_RHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4, m_irho = MF_IRHO * npt4;
+  const uint32_t jc4 = g.jc4;
+  const int jc = int(jc4 >> 2);
+  const float bzb = a.bzb[jc], bzt = a.bzt[jc];
+  const float dt = a.dt;
+  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const bool fast_c = (a.fast_div & FD_CONST) != 0;
+  const bool fast_rho = (a.fast_div & FD_RHO) != 0;
+  const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
+            tp2 = g.base + min(t + 2, NT - 1);
+  const int tps = a.nstrips * a.nsegs;
+  // one injection list per tile, walked by a cursor per stage
+  int ip = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile]);
+  const int ie = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile + 1]);
+  int ip2 = ip;
+  int inj_row = next_row(a.inj, ip, ie);
+  int inj_row2 = inj_row;
+  int rp = 0, re = 0;
+  if (a.record) {
+    rp = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile]);
+    re = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile + 1]);
+  }
+  int rp2 = rp;
+  int rec_row = next_row(a.rec, rp, re);
+  int rec_row2 = rec_row;
+  float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
+  float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
+  const int nobs = a.nrec * a.nt;
+  const int m0 = i0 - 4;   // first velocity row of stage 1
+
+  // stage 1 rings: stresses of step it (rows m-2..m+1), new velocities
+  // (rows m-3..m), D+z of the new velocities (row m at slot s2, used at s0)
+  float sxx[4], sxz[4], szz[4];
+  float vx[4] = {0, 0, 0, 0}, vz[4] = {0, 0, 0, 0};
+  float dvxdz[4] = {0, 0, 0, 0}, dvzdz[4] = {0, 0, 0, 0};
+  // stage 2 rings: stresses of step it+1 (rows m-5..m-2, row m-2 at s0),
+  // velocities of step it+2 (rows m-6..m-3, row m-3 at s0), D+z of those
+  // velocities (row m-4 at [par], row m-5 at [par^1]), and the model values /
+  // taper factor stage 1 used three rows earlier (written at s0, read at s1)
+  float t1xx[4] = {0, 0, 0, 0}, t1xz[4] = {0, 0, 0, 0}, t1zz[4] = {0, 0, 0, 0};
+  float u2x[4] = {0, 0, 0, 0}, u2z[4] = {0, 0, 0, 0};
+  float dz2x[2] = {0, 0}, dz2z[2] = {0, 0};
+  float drho[4] = {1, 1, 1, 1}, dirho[4] = {1, 1, 1, 1}, dlam[4] = {0, 0, 0, 0}, dmu[4] = {0, 0, 0, 0};
+  float dbx[4] = {0, 0, 0, 0};
+  for (int q = 0; q < 3; ++q) {
+    const uint32_t r = g.row(m0 - 2 + q);
+    sxx[q] = settle(bload(rin, jc4, o_sxx + r));
+    sxz[q] = settle(bload(rin, jc4, o_sxz + r));
+    szz[q] = settle(bload(rin, jc4, o_szz + r));
+  }
+  sxx[3] = sxz[3] = szz[3] = 0.0f;
+  // D-deep prefetch FIFO, slot (m - m0) % D: stress rows m+1, velocity /
+  // density rows m, moduli rows m-2 of iteration m
+  constexpr int D = 2;
+  float pf_sxx[D], pf_sxz[D], pf_szz[D], pf_vx[D], pf_vz[D], pf_rho[D], pf_irho[D], pf_lam[D], pf_mu[D];
+  float bx_next = taper_x(a.bx, m0, nx);
+  for (int q = 0; q < D; ++q) {
+    const uint32_t r1 = g.row(m0 + 1 + q), r0 = g.row(m0 + q), rm2 = g.row(m0 - 2 + q);
+    pf_sxx[q] = bload(rin, jc4, o_sxx + r1);
+    pf_sxz[q] = bload(rin, jc4, o_sxz + r1);
+    pf_szz[q] = bload(rin, jc4, o_szz + r1);
+    pf_vx[q] = bload(rin, jc4, o_vx + r0);
+    pf_vz[q] = bload(rin, jc4, o_vz + r0);
+    pf_rho[q] = bload(rmod, jc4, m_rho + r0);
+    pf_irho[q] = bload(rmod, jc4, m_irho + r0);
+    pf_lam[q] = bload(rmod, jc4, m_lam + rm2);
+    pf_mu[q] = bload(rmod, jc4, m_mu + rm2);
+  }
+  lds_s[0][0][lane] = sxz[2];
+  lds_s[0][1][lane] = szz[2];
+  __syncthreads();
+
+  const bool zm_ok = g.zm_ok, zp_ok = g.zp_ok;
+  auto body = [&](auto phase, int m) __attribute__((always_inline)) {
+    constexpr int R = decltype(phase)::value;
+    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
+    constexpr int f = R % D;
+    sxx[s3] = settle(pf_sxx[f]); sxz[s3] = settle(pf_sxz[f]); szz[s3] = settle(pf_szz[f]);     // row m+1
+    const float vx_o = pf_vx[f], vz_o = pf_vz[f], rho = pf_rho[f], irho = pf_irho[f], lam = pf_lam[f], mu = pf_mu[f];
+    const float bxm = bx_next;
+    bx_next = taper_x(a.bx, m + 1, nx);
+    {
+      const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
+      pf_sxx[f] = bload(rin, jc4, o_sxx + r3);
+      pf_sxz[f] = bload(rin, jc4, o_sxz + r3);
+      pf_szz[f] = bload(rin, jc4, o_szz + r3);
+      pf_vx[f] = bload(rin, jc4, o_vx + r2);
+      pf_vz[f] = bload(rin, jc4, o_vz + r2);
+      pf_rho[f] = bload(rmod, jc4, m_rho + r2);
+      pf_irho[f] = bload(rmod, jc4, m_irho + r2);
+      pf_lam[f] = bload(rmod, jc4, m_lam + r0);
+      pf_mu[f] = bload(rmod, jc4, m_mu + r0);
+    }
+
+    // ---- stage 1: velocity of step it+1 at row m (not stored)
+    {
+      const bool xm_ok = m >= 2 && m < nx - 2;
+      float qa[4];
+      {
+        float xa[4];
+        xa[0] = xm_ok ? nminus(sxx[s0], sxx[s1], sxx[s2], sxx[s3]) : 0.0f;
+        xa[1] = zm_ok ? nminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[s2], lds_s[par][0][tp1]) : 0.0f;
+        xa[2] = xm_ok ? nminus(sxz[s0], sxz[s1], sxz[s2], sxz[s3]) : 0.0f;
+        xa[3] = zm_ok ? nminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[s2], lds_s[par][1][tp1]) : 0.0f;
+        div_group(qa, xa, {Dx, Dz, Dx, Dz}, fast_c);
+      }
+      float dsx = qa[0] + qa[1];
+      float dsz = qa[2] + qa[3];
+      if (m == inj_row) {
+        for (; ip < ie; ++ip) {
+          const int4 e = a.inj[ip];
+          if (e.x != m) break;
+          if (e.y == j) {
+            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it];
+            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it];
+          }
+        }
+        inj_row = next_row(a.inj, ip, ie);
+      }
+      const float b = (bxm * bzb) * bzt;
+      float qb[2];
+      const float xb[2] = {dt * dsx, dt * dsz};
+      div_group(qb, xb, {Div{rho, irho, true}, Div{rho, irho, true}}, fast_rho);
+      vx[s3] = b * (vx_o + qb[0]);
+      vz[s3] = b * (vz_o + qb[1]);
+    }
+    lds_v[par][0][lane] = vx[s3];
+    lds_v[par][1][lane] = vz[s3];
+    lds_s[par ^ 1][0][lane] = sxz[s3];
+    lds_s[par ^ 1][1][lane] = szz[s3];
+    __syncthreads();
+
+    // ---- stage 1: D+ of the new velocities (z at row m, x at row m-2)
+    {
+      const int i = m - 2;
+      const bool xp_ok = i >= 1 && i < nx - 2;
+      float qc[4];
+      {
+        float xc[4];
+        xc[0] = zp_ok ? nplus(lds_v[par][0][tm1], vx[s3], lds_v[par][0][tp1], lds_v[par][0][tp2]) : 0.0f;
+        xc[1] = zp_ok ? nplus(lds_v[par][1][tm1], vz[s3], lds_v[par][1][tp1], lds_v[par][1][tp2]) : 0.0f;
+        xc[2] = xp_ok ? nplus(vx[s0], vx[s1], vx[s2], vx[s3]) : 0.0f;
+        xc[3] = xp_ok ? nplus(vz[s0], vz[s1], vz[s2], vz[s3]) : 0.0f;
+        div_group(qc, xc, {Dz, Dz, Dx, Dx}, fast_c);
+      }
+      dvxdz[s2] = qc[0];
+      dvzdz[s2] = qc[1];
+      // receivers of step it (row m)
+      if (m == rec_row) {
+        for (; rp < re; ++rp) {
+          const int4 e = a.rec[rp];
+          if (e.x != m) break;
+          if (e.y == j) {
+            float* u = uacc + e.z * N_UACC;
+            const float ux = u[U_UX] + vx[s3] * dt, uz = u[U_UZ] + vz[s3] * dt;
+            u[U_UX] = ux;
+            u[U_UZ] = uz;
+            obs[0 * nobs + e.z * a.nt + it] = ux;
+            obs[2 * nobs + e.z * a.nt + it] = uz;
+          }
+        }
+        rec_row = next_row(a.rec, rp, re);
+      }
+      // stress of step it+1 at row m-2: into the stage-2 ring and LDS
+      const float dvxdx = qc[2];
+      const float dvzdx = qc[3];
+      const float l2m = lam + 2.0f * mu;
+      t1xx[s0] = sxx[s0] + dt * (l2m * dvxdx + lam * dvzdz[s0]);
+      t1zz[s0] = szz[s0] + dt * (l2m * dvzdz[s0] + lam * dvxdx);
+      t1xz[s0] = sxz[s0] + dt * mu * (dvxdz[s0] + dvzdx);
+      lds_s2[par][0][lane] = t1xz[s0];
+      lds_s2[par][1][lane] = t1zz[s0];
+    }
+    drho[s0] = rho;
+    dirho[s0] = irho;
+    dlam[s0] = lam;
+    dmu[s0] = mu;
+    dbx[s0] = bxm;
+
+    // ---- stage 2: velocity of step it+2 at row m-3
+    {
+      const int m2 = m - 3;
+      const bool xm_ok = m2 >= 2 && m2 < nx - 2;
+      float qa[4];
+      {
+        float xa[4];
+        xa[0] = xm_ok ? nminus(t1xx[s1], t1xx[s2], t1xx[s3], t1xx[s0]) : 0.0f;
+        xa[1] = zm_ok ? nminus(lds_s2[par ^ 1][0][tm2], lds_s2[par ^ 1][0][tm1], t1xz[s3], lds_s2[par ^ 1][0][tp1])
+                      : 0.0f;
+        xa[2] = xm_ok ? nminus(t1xz[s1], t1xz[s2], t1xz[s3], t1xz[s0]) : 0.0f;
+        xa[3] = zm_ok ? nminus(lds_s2[par ^ 1][1][tm2], lds_s2[par ^ 1][1][tm1], t1zz[s3], lds_s2[par ^ 1][1][tp1])
+                      : 0.0f;
+        div_group(qa, xa, {Dx, Dz, Dx, Dz}, fast_c);
+      }
+      float dsx = qa[0] + qa[1];
+      float dsz = qa[2] + qa[3];
+      if (m2 == inj_row2) {
+        for (; ip2 < ie; ++ip2) {
+          const int4 e = a.inj[ip2];
+          if (e.x != m2) break;
+          if (e.y == j) {
+            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it + 1];
+            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it + 1];
+          }
+        }
+        inj_row2 = next_row(a.inj, ip2, ie);
+      }
+      const float b = (dbx[s1] * bzb) * bzt;
+      float qb[2];
+      const float xb[2] = {dt * dsx, dt * dsz};
+      div_group(qb, xb, {Div{drho[s1], dirho[s1], true}, Div{drho[s1], dirho[s1], true}}, fast_rho);
+      const float vxn = b * (vx[s0] + qb[0]);
+      const float vzn = b * (vz[s0] + qb[1]);
+      if (g.jst && m2 >= i0 && m2 < i1) {
+        const uint32_t ro = uint32_t(m2) * g.rowb;
+        bstore(vxn, rout, jc4, o_vx + ro);
+        bstore(vzn, rout, jc4, o_vz + ro);
+      }
+      u2x[s0] = vxn;
+      u2z[s0] = vzn;
+      // receivers of step it+1 (row m-3)
+      if (m2 == rec_row2) {
+        for (; rp2 < re; ++rp2) {
+          const int4 e = a.rec[rp2];
+          if (e.x != m2) break;
+          if (e.y == j) {
+            float* u = uacc + e.z * N_UACC;
+            const float ux = u[U_UX] + vxn * dt, uz = u[U_UZ] + vzn * dt;
+            u[U_UX] = ux;
+            u[U_UZ] = uz;
+            obs[0 * nobs + e.z * a.nt + it + 1] = ux;
+            obs[2 * nobs + e.z * a.nt + it + 1] = uz;
+          }
+        }
+        rec_row2 = next_row(a.rec, rp2, re);
+      }
+    }
+
+    // ---- stage 2: D+z of velocity row m-4 (LDS of the previous iteration),
+    // D+x at row m-5, stress of step it+2 at row m-5
+    {
+      const int i2 = m - 5;
+      const bool xp_ok = i2 >= 1 && i2 < nx - 2;
+      float qc[4];
+      {
+        float xc[4];
+        xc[0] = zp_ok ? nplus(lds_v2[par ^ 1][0][tm1], u2x[s3], lds_v2[par ^ 1][0][tp1], lds_v2[par ^ 1][0][tp2])
+                      : 0.0f;
+        xc[1] = zp_ok ? nplus(lds_v2[par ^ 1][1][tm1], u2z[s3], lds_v2[par ^ 1][1][tp1], lds_v2[par ^ 1][1][tp2])
+                      : 0.0f;
+        xc[2] = xp_ok ? nplus(u2x[s1], u2x[s2], u2x[s3], u2x[s0]) : 0.0f;
+        xc[3] = xp_ok ? nplus(u2z[s1], u2z[s2], u2z[s3], u2z[s0]) : 0.0f;
+        div_group(qc, xc, {Dz, Dz, Dx, Dx}, fast_c);
+      }
+      dz2x[par] = qc[0];
+      dz2z[par] = qc[1];
+      const float lam2 = dlam[s1], mu2 = dmu[s1];
+      const float dvxdx = qc[2];
+      const float dvzdx = qc[3];
+      const float l2m = lam2 + 2.0f * mu2;
+      const float nsxx = t1xx[s1] + dt * (l2m * dvxdx + lam2 * dz2z[par ^ 1]);
+      const float nszz = t1zz[s1] + dt * (l2m * dz2z[par ^ 1] + lam2 * dvxdx);
+      const float nsxz = t1xz[s1] + dt * mu2 * (dz2x[par ^ 1] + dvzdx);
+      if (g.jst && i2 >= i0 && i2 < i1) {
+        const uint32_t ro = uint32_t(i2) * g.rowb;
+        bstore(nsxx, rout, jc4, o_sxx + ro);
+        bstore(nszz, rout, jc4, o_szz + ro);
+        bstore(nsxz, rout, jc4, o_sxz + ro);
+      }
+    }
+    lds_v2[par][0][lane] = u2x[s0];
+    lds_v2[par][1][lane] = u2z[s0];
+  };
+
+  // as in step_psv: four peeled rows, then 8 phases per loop trip
+  const int mlast = i1 + 4;
+  int m = m0;
+  body(Phase<0>{}, m);
+  if (++m <= mlast) {
+    body(Phase<1>{}, m);
+    if (++m <= mlast) {
+      body(Phase<2>{}, m);
+      if (++m <= mlast) {
+        body(Phase<3>{}, m);
+        if (++m <= mlast) {
+          for (;;) {
+            body(Phase<4>{}, m);
+            if (++m > mlast) break;
+            body(Phase<5>{}, m);
+            if (++m > mlast) break;
+            body(Phase<6>{}, m);
+            if (++m > mlast) break;
+            body(Phase<7>{}, m);
+            if (++m > mlast) break;
+            body(Phase<0>{}, m);
+            if (++m > mlast) break;
+            body(Phase<1>{}, m);
+            if (++m > mlast) break;
+            body(Phase<2>{}, m);
+            if (++m > mlast) break;
+            body(Phase<3>{}, m);
+            if (++m > mlast) break;
+          }
+        }
+      }
+    }
+  }
+  if (a.advance) advance_clock(a.clock, it, 2, a.nblocks);
+}
+
 // ------------------------------------------------------ adjoint time steps
 // Exact transpose of one forward step (SH or P-SV), run in reverse time.
 // With the forward step
@@ -1179,6 +1522,11 @@ __global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(S >= 4 ?
   step_psv<NT, S, false, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 template <int NT, int S>
+__global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(4))) void k_step_psv2(
+    const StepArgs a) {
+  step_psv2<NT, S>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+template <int NT, int S>
 __global__ __launch_bounds__(NT * S) void k_step_spin(const StepArgs a) {
   step_psv<NT, S, true, false>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
@@ -1455,6 +1803,26 @@ int step_blocks_per_cu(int kind, int nt_threads, int group) {
 
 hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
   return launch_fwd(aux ? MK_SH_AUX : MK_SH, a, t, s);
+}
+
+template <int NT, int S>
+hipError_t launch_psv2_ns(const StepArgs& a, const Tiling& t, hipStream_t s) {
+  const int nblk = a.nshots / S * t.nstrips * t.nsegs;
+  hipLaunchKernelGGL((k_step_psv2<NT, S>), dim3(nblk), dim3(NT * S), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s) {
+  if (t.group < 1 || a.nshots % t.group != 0) return hipErrorInvalidValue;
+  switch (t.nt_threads * 8 + t.group) {
+    case 64 * 8 + 1: return launch_psv2_ns<64, 1>(a, t, s);
+    case 128 * 8 + 1: return launch_psv2_ns<128, 1>(a, t, s);
+    case 128 * 8 + 2: return launch_psv2_ns<128, 2>(a, t, s);
+    case 128 * 8 + 4: return launch_psv2_ns<128, 4>(a, t, s);
+    case 256 * 8 + 1: return launch_psv2_ns<256, 1>(a, t, s);
+    case 256 * 8 + 2: return launch_psv2_ns<256, 2>(a, t, s);
+    case 256 * 8 + 4: return launch_psv2_ns<256, 4>(a, t, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStream_t s, bool aux) {

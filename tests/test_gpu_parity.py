@@ -181,6 +181,42 @@ def test_grouped_shots_equal_single_runs(seiswg, phys, threads, group):
                                   ref.view(np.uint32)), (q, n)
 
 
+@pytest.mark.parametrize("threads,seglen,group", [(64, 5, 1), (128, 16, 2), (256, 9, 4), (128, 3, 4), (64, 200, 1)])
+@pytest.mark.parametrize("nt", [11, 37])
+def test_two_step_launches_equal_one_step_launches(seiswg, threads, seglen, group, nt):
+    """P-SV forward with steps in pairs of two-step launches (the default)
+    against one step per launch: state and traces bit for bit, for short
+    segments (recomputed rows at both ends), several shots per workgroup,
+    sources and receivers on edges, halos and duplicated cells, an odd step
+    count (single steps after the pairs) and, with 37 steps, captured graphs."""
+    nx, nz = 61, 150
+    p, stf = random_problem(nx, nz, nt, False, True, False, seed=29)
+    print(f"case threads={threads} seglen={seglen} group={group} nt={nt}", flush=True)
+    comps = [O.OBS_X, O.OBS_Z]
+    fields = STATE["psv"]
+    runs = [0, 1, 2, 3]
+    out = {}
+    for spl in (1, 2):
+        c = gpu_context(p, stf, seiswg)
+        c.set_tiling(threads, seglen, group)
+        c.set_steps_per_launch(spl)
+        c.forward_batch(runs)
+        tm = c.timing()
+        out[spl] = ([[c.obs(q, k).copy() for k in comps] for q in range(len(runs))],
+                    [[c.field(getattr(seiswg.abi, "F_" + n), shot=q).copy() for n in fields]
+                     for q in range(len(runs))], tm.step_launches)
+        c.close()
+    assert out[1][2] == nt and out[2][2] == nt // 4 * 2 + nt % 4, (out[1][2], out[2][2])
+    for q in range(len(runs)):
+        for k, a, b in zip(comps, out[1][0][q], out[2][0][q]):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (q, k)
+        for n, a, b in zip(fields, out[1][1][q], out[2][1][q]):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (q, n)
+    o = O.Oracle(p, 4, stf=stf.astype(np.float64))
+    obs, _ = o.forward(1, 0)
+    compare(out[2][0][1][0], obs[O.OBS_X], "two-step shot 1 ux")
+
+
 def load_example(tmp_path, ex, ini_name="config.ini"):
     d = E.materialise(ex, str(tmp_path))
     ini = E.ini_values(os.path.join(d, ini_name))

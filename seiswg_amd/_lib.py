@@ -48,7 +48,7 @@ SNAP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.PO
 EXPORTS = ["sw_last_error", "sw_create", "sw_destroy", "sw_set_model", "sw_set_sources", "sw_set_receivers",
            "sw_forward", "sw_forward_batch", "sw_read_obs", "sw_read_field", "sw_write_field", "sw_prepare",
            "sw_advance", "sw_adjoint_plain", "sw_smooth", "sw_set_tiling", "sw_get_tiling", "sw_get_timing",
-           "sw_set_steps_per_launch",
+           "sw_set_steps_per_launch", "sw_set_tiling2", "sw_get_tiling2",
            "sw_bench_copy", "sw_gradient", "sw_autotune", "sw_multi_create", "sw_multi_destroy",
            "sw_multi_forward", "sw_multi_gradient", "sw_multi_timing",
            "seiswg_run", "seiswg_ricker_table", "seiswg_infer_grid", "seiswg_grid_index"]
@@ -82,6 +82,7 @@ def load():
     L.sw_smooth.argtypes = [vp, C.c_int32, C.c_float]
     L.sw_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
     L.sw_set_steps_per_launch.argtypes = [vp, C.c_int32]
+    L.sw_set_tiling2.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
     L.sw_get_tiling.argtypes = [vp, ip, ip, ip]
     L.sw_get_timing.argtypes = [vp, C.POINTER(sw_timing)]
     L.sw_multi_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
@@ -223,6 +224,17 @@ class Context:
         """P-SV forward steps per launch (2: pairs of two-step launches, the
         default; 1: one step per launch).  Same results either way."""
         _check(self.L.sw_set_steps_per_launch(self.h, steps))
+
+    def set_tiling2(self, threads: int, seglen: int, shots_per_block: int = 0):
+        """Tile shape of the two-step P-SV launches."""
+        _check(self.L.sw_set_tiling2(self.h, threads, seglen, shots_per_block))
+
+    def tiling2(self):
+        """(threads, seglen, shots per block) of the two-step launches of the
+        last prepared batch; zeros when it runs one step per launch."""
+        a, b, g = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(self.L.sw_get_tiling2(self.h, C.byref(a), C.byref(b), C.byref(g)))
+        return a.value, b.value, g.value
 
     def autotune(self, nshots: int, steps: int = 10):
         """Fastest tile shape for nshots shots: (threads, seglen, shots per block, ms per step)."""

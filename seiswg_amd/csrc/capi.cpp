@@ -118,6 +118,8 @@ struct sw_ctx {
   int steps_per_launch = 2;   // requested: 1 or 2
   bool two_ok = false;        // the prepared batch can use them
   Tiling tile2;
+  bool tile2_fixed = false;   // set by sw_set_tiling2
+  int group2_req = 0;
   DBuf<int> inj2_off, rec2_off;
   DBuf<int4> inj2, rec2;
   DBuf<float> spare;
@@ -217,6 +219,40 @@ int effective_group(int nt, int requested, int nshots, bool spin) {
   while (g > 1 && (!group_supported(nt, g) || nshots % g != 0 || (spin && nt * g > 512))) g /= 2;
   return g;
 }
+int effective_group2(int nt, int requested, int nshots) {
+  int g = requested > 0 ? requested : 1;
+  while (g > 1 && (!group2_supported(nt, g) || nshots % g != 0)) g /= 2;
+  return g;
+}
+
+// Shape of the two-step launches (6-column halos, W = threads-12): the tile
+// width that wastes the fewest lanes on the last strip (the wider on ties),
+// shots per workgroup as requested (else 1), and as many segments as fill the
+// resident slots of every CU once, at least 32 rows each.
+void heuristic_tiling2(sw_ctx* c, int nshots) {
+  Tiling& t = c->tile2;
+  const int nz = c->g.nz;
+  int best = 64;
+  double best_eff = -1.0;
+  for (int nt : {512, 256, 128, 64}) {
+    const int strips = (nz + nt - 12 - 1) / (nt - 12);
+    const double eff = double(nz) / (double(strips) * nt);
+    if (eff > best_eff + 1e-9) {
+      best_eff = eff;
+      best = nt;
+    }
+  }
+  const int group = effective_group2(best, c->group2_req, nshots);
+  const int64_t ncu = c->ncu > 0 ? c->ncu : 256;
+  const int64_t resident = std::max(step2_blocks_per_cu(best, group), 1);
+  const int64_t strips = (nz + best - 12 - 1) / (best - 12);
+  const int64_t base = int64_t(nshots / group) * strips;
+  const int64_t nsegs = std::max<int64_t>(1, (ncu * resident) / base);
+  int seg = int(std::max<int64_t>((c->g.nx + nsegs - 1) / nsegs, 32));
+  t.nt_threads = best;
+  t.seglen = std::min(seg, c->g.nx);
+  t.group = group;
+}
 
 // Tile width: 256 threads from 400 columns up, 128 from 100 (wide tiles
 // measured faster even where they leave more lanes beyond nz idle), narrower
@@ -263,10 +299,19 @@ int check_tiling(sw_ctx* c, int nshots) {
   t.nstrips = (c->g.nz + W - 1) / W;
   t.nsegs = (c->g.nx + t.seglen - 1) / t.seglen;
   t.group = effective_group(t.nt_threads, c->tile_fixed ? c->group_req : t.group, nshots, uses_spin(c));
-  // two-step launches: same tile width, segments and shots per workgroup
-  c->tile2 = t;
-  c->tile2.nstrips = (c->g.nz + (t.nt_threads - 12) - 1) / (t.nt_threads - 12);
+  // two-step launches: their own shape if set or chosen by the heuristic,
+  // else that of the one-step launches
   c->two_ok = c->steps_per_launch == 2 && uses_psv(c) && !uses_spin(c) && !uses_sh(c);
+  if (c->two_ok) {
+    Tiling& t2 = c->tile2;
+    if (!c->tile2_fixed && c->tile_fixed) {
+      t2 = t;
+      c->group2_req = c->group_req;
+    }
+    t2.nstrips = (c->g.nz + (t2.nt_threads - 12) - 1) / (t2.nt_threads - 12);
+    t2.nsegs = (c->g.nx + t2.seglen - 1) / t2.seglen;
+    t2.group = effective_group2(t2.nt_threads, c->group2_req > 0 ? c->group2_req : t2.group, nshots);
+  }
   return 0;
 }
 
@@ -390,6 +435,7 @@ int prepare_batch(sw_ctx* c, int nshots, const std::vector<std::vector<Entry>>& 
   ++c->gen;
   if (nshots < 1) return fail(-2, "nshots must be >= 1");
   if (!c->tile_fixed) heuristic_tiling(c, nshots);
+  if (!c->tile2_fixed && !c->tile_fixed) heuristic_tiling2(c, nshots);
   int rc = check_tiling(c, nshots);
   if (rc) return rc;
   c->nshots = nshots;
@@ -478,6 +524,8 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     HIPCHK(c->spare.ensure(size_t(c->nslots) * size_t(c->nshots) * size_t(c->npt)));
     a2a = a;
     a2a.nstrips = c->tile2.nstrips;
+    a2a.nsegs = c->tile2.nsegs;
+    a2a.seglen = c->tile2.seglen;
     a2a.inj_off = c->inj2_off.p; a2a.inj = c->inj2.p;
     a2a.rec_off = c->rec2_off.p; a2a.rec = c->rec2.p;
     a2a.nblocks = c->nshots / c->tile2.group * c->tile2.nstrips * c->tile2.nsegs;
@@ -956,6 +1004,29 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
   return 0;
 }
 
+SW_API int sw_set_tiling2(sw_ctx* c, int32_t threads, int32_t seglen, int32_t shots_per_block) {
+  if (!c) return fail(-1, "null ctx");
+  if (!tiling2_supported(threads)) return fail(-2, "threads must be 64, 128, 256 or 512");
+  if (seglen < 1) return fail(-2, "seglen must be >= 1");
+  if (shots_per_block != 0 && !group2_supported(threads, shots_per_block))
+    return fail(-2, "shots_per_block must be 0 (automatic), 1, or 2 or 4 with threads >= 128 and at most "
+                    "1024 threads per workgroup");
+  c->tile2.nt_threads = threads;
+  c->tile2.seglen = seglen;
+  c->tile2.group = shots_per_block > 0 ? shots_per_block : 1;
+  c->group2_req = shots_per_block;
+  c->tile2_fixed = true;
+  ++c->gen;
+  c->nshots = 0;   // lists must be rebuilt
+  return 0;
+}
+SW_API int sw_get_tiling2(const sw_ctx* c, int32_t* threads, int32_t* seglen, int32_t* shots_per_block) {
+  if (!c || !threads || !seglen) return fail(-1, "null argument");
+  *threads = c->two_ok ? c->tile2.nt_threads : 0;
+  *seglen = c->two_ok ? c->tile2.seglen : 0;
+  if (shots_per_block) *shots_per_block = c->two_ok ? c->tile2.group : 0;
+  return 0;
+}
 SW_API int sw_get_tiling(const sw_ctx* c, int32_t* threads, int32_t* seglen, int32_t* shots_per_block) {
   if (!c || !threads || !seglen) return fail(-1, "null argument");
   *threads = c->tile.nt_threads;

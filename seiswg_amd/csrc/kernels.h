@@ -105,6 +105,11 @@ hipError_t launch_reduce_shots(const float* kshot, float* kout, int nshots, int6
                                hipStream_t s);
 bool tiling_supported(int nt_threads);
 bool group_supported(int nt_threads, int group);
+// two-step P-SV launches: tile widths 64..512, 2 or 4 shots per workgroup up
+// to 1024 threads; resident workgroups per CU of a shape
+bool tiling2_supported(int nt_threads);
+bool group2_supported(int nt_threads, int group);
+int step2_blocks_per_cu(int nt_threads, int group);
 // resident workgroups per CU of the step kernel of a MultiKind at a tile width
 // and shots per workgroup (adjoint kinds: group 1 only)
 int step_blocks_per_cu(int kind, int nt_threads, int group);

@@ -61,6 +61,9 @@ __device__ __forceinline__ float nminus(float fm2, float fm1, float f0, float fp
 __device__ __forceinline__ float nplus(float fm1, float f0, float fp1, float fp2) {
   return 9.0f * (fp1 - f0) - (fp2 - fm1);
 }
+// v where an FD operator is active, +0 elsewhere (the operators' edge rule),
+// as a select of a value computed on every lane rather than a branch around it
+__device__ __forceinline__ float keep(bool active, float v) { return active ? v : 0.0f; }
 
 // A group of divisions q[k] = x[k] / d[k].y sharing one range test: the
 // reciprocal form for all of them, and the IEEE division for the lanes where
@@ -554,15 +557,15 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     float qa[NA];
     {
       float xa[NA];
-      xa[0] = xm_ok ? nminus(sxx[s0], sxx[s1], sxx[s2], sxx[s3]) : 0.0f;
-      xa[1] = zm_ok ? nminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[s2], lds_s[par][0][tp1]) : 0.0f;
-      xa[2] = xm_ok ? nminus(sxz[s0], sxz[s1], sxz[s2], sxz[s3]) : 0.0f;
-      xa[3] = zm_ok ? nminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[s2], lds_s[par][1][tp1]) : 0.0f;
+      xa[0] = keep(xm_ok, nminus(sxx[s0], sxx[s1], sxx[s2], sxx[s3]));
+      xa[1] = keep(zm_ok, nminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[s2], lds_s[par][0][tp1]));
+      xa[2] = keep(xm_ok, nminus(sxz[s0], sxz[s1], sxz[s2], sxz[s3]));
+      xa[3] = keep(zm_ok, nminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[s2], lds_s[par][1][tp1]));
       if constexpr (SPIN) {
-        xa[4] = xm_ok ? nminus(syx[s0], syx[s1], syx[s2], syx[s3]) : 0.0f;
-        xa[5] = zm_ok ? nminus(lds_s[par][2][tm2], lds_s[par][2][tm1], syz[s2], lds_s[par][2][tp1]) : 0.0f;
-        xa[6] = zm_ok ? nminus(lds_s[par][3][tm2], lds_s[par][3][tm1], syy[s2], lds_s[par][3][tp1]) : 0.0f;
-        xa[7] = xm_ok ? nminus(syy[s0], syy[s1], syy[s2], syy[s3]) : 0.0f;
+        xa[4] = keep(xm_ok, nminus(syx[s0], syx[s1], syx[s2], syx[s3]));
+        xa[5] = keep(zm_ok, nminus(lds_s[par][2][tm2], lds_s[par][2][tm1], syz[s2], lds_s[par][2][tp1]));
+        xa[6] = keep(zm_ok, nminus(lds_s[par][3][tm2], lds_s[par][3][tm1], syy[s2], lds_s[par][3][tp1]));
+        xa[7] = keep(xm_ok, nminus(syy[s0], syy[s1], syy[s2], syy[s3]));
         div_group(qa, xa, {Dx, Dz, Dx, Dz, Dx, Dz, Dz, Dx}, fast_c);
       } else {
         div_group(qa, xa, {Dx, Dz, Dx, Dz}, fast_c);
@@ -631,13 +634,13 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     float qc[NC];
     {
       float xc[NC];
-      xc[0] = zp_ok ? nplus(lds_v[par][0][tm1], vxn, lds_v[par][0][tp1], lds_v[par][0][tp2]) : 0.0f;
-      xc[1] = zp_ok ? nplus(lds_v[par][1][tm1], vzn, lds_v[par][1][tp1], lds_v[par][1][tp2]) : 0.0f;
-      xc[2] = xp_ok ? nplus(vx[s0], vx[s1], vx[s2], vx[s3]) : 0.0f;
-      xc[3] = xp_ok ? nplus(vz[s0], vz[s1], vz[s2], vz[s3]) : 0.0f;
+      xc[0] = keep(zp_ok, nplus(lds_v[par][0][tm1], vxn, lds_v[par][0][tp1], lds_v[par][0][tp2]));
+      xc[1] = keep(zp_ok, nplus(lds_v[par][1][tm1], vzn, lds_v[par][1][tp1], lds_v[par][1][tp2]));
+      xc[2] = keep(xp_ok, nplus(vx[s0], vx[s1], vx[s2], vx[s3]));
+      xc[3] = keep(xp_ok, nplus(vz[s0], vz[s1], vz[s2], vz[s3]));
       if constexpr (SPIN) {
-        xc[4] = zp_ok ? nplus(lds_v[par][2][tm1], vycn, lds_v[par][2][tp1], lds_v[par][2][tp2]) : 0.0f;
-        xc[5] = xp_ok ? nplus(vyc[s0], vyc[s1], vyc[s2], vyc[s3]) : 0.0f;
+        xc[4] = keep(zp_ok, nplus(lds_v[par][2][tm1], vycn, lds_v[par][2][tp1], lds_v[par][2][tp2]));
+        xc[5] = keep(xp_ok, nplus(vyc[s0], vyc[s1], vyc[s2], vyc[s3]));
         div_group(qc, xc, {Dz, Dz, Dx, Dx, Dz, Dx}, fast_c);
       } else {
         div_group(qc, xc, {Dz, Dz, Dx, Dx}, fast_c);
@@ -869,9 +872,13 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   sxx[3] = sxz[3] = szz[3] = 0.0f;
   // D-deep prefetch FIFO, slot (m - m0) % D: stress rows m+1, velocity /
   // density rows m, moduli rows m-2 of iteration m
-  constexpr int D = 2;
+  constexpr int D = 1;   // an iteration of two stages outlasts a load's latency
   float pf_sxx[D], pf_sxz[D], pf_szz[D], pf_vx[D], pf_vz[D], pf_rho[D], pf_irho[D], pf_lam[D], pf_mu[D];
   float bx_next = taper_x(a.bx, m0, nx);
+  // clamped row offsets, slot (row - m0) & 3: each iteration computes the
+  // newest (m+D+1) and finds rows m+D and m+D-2 among the earlier ones
+  uint32_t rofs[4];
+  for (int k = D - 2; k <= D; ++k) rofs[k & 3] = g.row(m0 + k);
   for (int q = 0; q < D; ++q) {
     const uint32_t r1 = g.row(m0 + 1 + q), r0 = g.row(m0 + q), rm2 = g.row(m0 - 2 + q);
     pf_sxx[q] = bload(rin, jc4, o_sxx + r1);
@@ -898,7 +905,9 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
     const float bxm = bx_next;
     bx_next = taper_x(a.bx, m + 1, nx);
     {
-      const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
+      constexpr int k1 = (R + D + 1) & 3, k0 = (R + D) & 3, km2 = (R + D + 2) & 3;
+      rofs[k1] = g.row(m + D + 1);
+      const uint32_t r3 = rofs[k1], r2 = rofs[k0], r0 = rofs[km2];
       pf_sxx[f] = bload(rin, jc4, o_sxx + r3);
       pf_sxz[f] = bload(rin, jc4, o_sxz + r3);
       pf_szz[f] = bload(rin, jc4, o_szz + r3);
@@ -915,11 +924,11 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       const bool xm_ok = m >= 2 && m < nx - 2;
       float qa[4];
       {
-        float xa[4];
-        xa[0] = xm_ok ? nminus(sxx[s0], sxx[s1], sxx[s2], sxx[s3]) : 0.0f;
-        xa[1] = zm_ok ? nminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[s2], lds_s[par][0][tp1]) : 0.0f;
-        xa[2] = xm_ok ? nminus(sxz[s0], sxz[s1], sxz[s2], sxz[s3]) : 0.0f;
-        xa[3] = zm_ok ? nminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[s2], lds_s[par][1][tp1]) : 0.0f;
+        const float n0 = nminus(sxx[s0], sxx[s1], sxx[s2], sxx[s3]);
+        const float n1 = nminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[s2], lds_s[par][0][tp1]);
+        const float n2 = nminus(sxz[s0], sxz[s1], sxz[s2], sxz[s3]);
+        const float n3 = nminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[s2], lds_s[par][1][tp1]);
+        const float xa[4] = {keep(xm_ok, n0), keep(zm_ok, n1), keep(xm_ok, n2), keep(zm_ok, n3)};
         div_group(qa, xa, {Dx, Dz, Dx, Dz}, fast_c);
       }
       float dsx = qa[0] + qa[1];
@@ -954,11 +963,11 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       const bool xp_ok = i >= 1 && i < nx - 2;
       float qc[4];
       {
-        float xc[4];
-        xc[0] = zp_ok ? nplus(lds_v[par][0][tm1], vx[s3], lds_v[par][0][tp1], lds_v[par][0][tp2]) : 0.0f;
-        xc[1] = zp_ok ? nplus(lds_v[par][1][tm1], vz[s3], lds_v[par][1][tp1], lds_v[par][1][tp2]) : 0.0f;
-        xc[2] = xp_ok ? nplus(vx[s0], vx[s1], vx[s2], vx[s3]) : 0.0f;
-        xc[3] = xp_ok ? nplus(vz[s0], vz[s1], vz[s2], vz[s3]) : 0.0f;
+        const float n0 = nplus(lds_v[par][0][tm1], vx[s3], lds_v[par][0][tp1], lds_v[par][0][tp2]);
+        const float n1 = nplus(lds_v[par][1][tm1], vz[s3], lds_v[par][1][tp1], lds_v[par][1][tp2]);
+        const float n2 = nplus(vx[s0], vx[s1], vx[s2], vx[s3]);
+        const float n3 = nplus(vz[s0], vz[s1], vz[s2], vz[s3]);
+        const float xc[4] = {keep(zp_ok, n0), keep(zp_ok, n1), keep(xp_ok, n2), keep(xp_ok, n3)};
         div_group(qc, xc, {Dz, Dz, Dx, Dx}, fast_c);
       }
       dvxdz[s2] = qc[0];
@@ -1001,13 +1010,11 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       const bool xm_ok = m2 >= 2 && m2 < nx - 2;
       float qa[4];
       {
-        float xa[4];
-        xa[0] = xm_ok ? nminus(t1xx[s1], t1xx[s2], t1xx[s3], t1xx[s0]) : 0.0f;
-        xa[1] = zm_ok ? nminus(lds_s2[par ^ 1][0][tm2], lds_s2[par ^ 1][0][tm1], t1xz[s3], lds_s2[par ^ 1][0][tp1])
-                      : 0.0f;
-        xa[2] = xm_ok ? nminus(t1xz[s1], t1xz[s2], t1xz[s3], t1xz[s0]) : 0.0f;
-        xa[3] = zm_ok ? nminus(lds_s2[par ^ 1][1][tm2], lds_s2[par ^ 1][1][tm1], t1zz[s3], lds_s2[par ^ 1][1][tp1])
-                      : 0.0f;
+        const float n0 = nminus(t1xx[s1], t1xx[s2], t1xx[s3], t1xx[s0]);
+        const float n1 = nminus(lds_s2[par ^ 1][0][tm2], lds_s2[par ^ 1][0][tm1], t1xz[s3], lds_s2[par ^ 1][0][tp1]);
+        const float n2 = nminus(t1xz[s1], t1xz[s2], t1xz[s3], t1xz[s0]);
+        const float n3 = nminus(lds_s2[par ^ 1][1][tm2], lds_s2[par ^ 1][1][tm1], t1zz[s3], lds_s2[par ^ 1][1][tp1]);
+        const float xa[4] = {keep(xm_ok, n0), keep(zm_ok, n1), keep(xm_ok, n2), keep(zm_ok, n3)};
         div_group(qa, xa, {Dx, Dz, Dx, Dz}, fast_c);
       }
       float dsx = qa[0] + qa[1];
@@ -1061,13 +1068,11 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       const bool xp_ok = i2 >= 1 && i2 < nx - 2;
       float qc[4];
       {
-        float xc[4];
-        xc[0] = zp_ok ? nplus(lds_v2[par ^ 1][0][tm1], u2x[s3], lds_v2[par ^ 1][0][tp1], lds_v2[par ^ 1][0][tp2])
-                      : 0.0f;
-        xc[1] = zp_ok ? nplus(lds_v2[par ^ 1][1][tm1], u2z[s3], lds_v2[par ^ 1][1][tp1], lds_v2[par ^ 1][1][tp2])
-                      : 0.0f;
-        xc[2] = xp_ok ? nplus(u2x[s1], u2x[s2], u2x[s3], u2x[s0]) : 0.0f;
-        xc[3] = xp_ok ? nplus(u2z[s1], u2z[s2], u2z[s3], u2z[s0]) : 0.0f;
+        const float n0 = nplus(lds_v2[par ^ 1][0][tm1], u2x[s3], lds_v2[par ^ 1][0][tp1], lds_v2[par ^ 1][0][tp2]);
+        const float n1 = nplus(lds_v2[par ^ 1][1][tm1], u2z[s3], lds_v2[par ^ 1][1][tp1], lds_v2[par ^ 1][1][tp2]);
+        const float n2 = nplus(u2x[s1], u2x[s2], u2x[s3], u2x[s0]);
+        const float n3 = nplus(u2z[s1], u2z[s2], u2z[s3], u2z[s0]);
+        const float xc[4] = {keep(zp_ok, n0), keep(zp_ok, n1), keep(xp_ok, n2), keep(xp_ok, n3)};
         div_group(qc, xc, {Dz, Dz, Dx, Dx}, fast_c);
       }
       dz2x[par] = qc[0];
@@ -1517,7 +1522,7 @@ __global__ __launch_bounds__(NT * S) void k_step_sh(const StepArgs a) {
 // SIMD; four shots per workgroup (16 waves, one workgroup per CU) trade
 // occupancy for the 4-deep prefetch
 template <int NT, int S, bool AUX>
-__global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(S >= 4 ? 4 : 6))) void k_step_psv(
+__global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(S >= 4 ? 4 : S == 2 ? 5 : 6))) void k_step_psv(
     const StepArgs a) {
   step_psv<NT, S, false, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
@@ -1728,8 +1733,17 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
 
 // forward step launch for a tile width NT and S shots per workgroup; kind is
 // MK_SH, MK_SH_AUX, MK_PSV, MK_PSV_AUX or MK_SPIN
+// The grid and the tile geometry the kernels derive from the arguments must
+// describe the same tiles: a mismatch would map workgroups to shots past the
+// batch.
+bool args_match(const StepArgs& a, const Tiling& t, int S) {
+  return a.nstrips == t.nstrips && a.nsegs == t.nsegs && a.seglen == t.seglen && S >= 1 && a.nshots % S == 0 &&
+         a.nblocks == a.nshots / S * t.nstrips * t.nsegs;
+}
+
 template <int NT, int S>
 hipError_t launch_fwd_ns(int kind, const StepArgs& a, const Tiling& t, hipStream_t s) {
+  if (!args_match(a, t, S)) return hipErrorInvalidValue;
   const int nblk = a.nshots / S * t.nstrips * t.nsegs;
   const dim3 grid(nblk), block(NT * S);
   switch (kind) {
@@ -1807,6 +1821,7 @@ hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, boo
 
 template <int NT, int S>
 hipError_t launch_psv2_ns(const StepArgs& a, const Tiling& t, hipStream_t s) {
+  if (!args_match(a, t, S)) return hipErrorInvalidValue;
   const int nblk = a.nshots / S * t.nstrips * t.nsegs;
   hipLaunchKernelGGL((k_step_psv2<NT, S>), dim3(nblk), dim3(NT * S), 0, s, a);
   return hipGetLastError();
@@ -1821,8 +1836,28 @@ hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s) {
     case 256 * 8 + 1: return launch_psv2_ns<256, 1>(a, t, s);
     case 256 * 8 + 2: return launch_psv2_ns<256, 2>(a, t, s);
     case 256 * 8 + 4: return launch_psv2_ns<256, 4>(a, t, s);
+    case 512 * 8 + 1: return launch_psv2_ns<512, 1>(a, t, s);
+    case 512 * 8 + 2: return launch_psv2_ns<512, 2>(a, t, s);
   }
   return hipErrorInvalidValue;
+}
+bool tiling2_supported(int nt) { return nt == 64 || nt == 128 || nt == 256 || nt == 512; }
+bool group2_supported(int nt, int group) {
+  return tiling2_supported(nt) && (group == 1 || (nt >= 128 && (group == 2 || group == 4) && nt * group <= 1024));
+}
+int step2_blocks_per_cu(int nt, int group) {
+  switch (nt * 8 + group) {
+    case 64 * 8 + 1: return occupancy(k_step_psv2<64, 1>, 64);
+    case 128 * 8 + 1: return occupancy(k_step_psv2<128, 1>, 128);
+    case 128 * 8 + 2: return occupancy(k_step_psv2<128, 2>, 256);
+    case 128 * 8 + 4: return occupancy(k_step_psv2<128, 4>, 512);
+    case 256 * 8 + 1: return occupancy(k_step_psv2<256, 1>, 256);
+    case 256 * 8 + 2: return occupancy(k_step_psv2<256, 2>, 512);
+    case 256 * 8 + 4: return occupancy(k_step_psv2<256, 4>, 1024);
+    case 512 * 8 + 1: return occupancy(k_step_psv2<512, 1>, 512);
+    case 512 * 8 + 2: return occupancy(k_step_psv2<512, 2>, 1024);
+  }
+  return 0;
 }
 
 hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStream_t s, bool aux) {
@@ -1842,6 +1877,7 @@ hipError_t launch_multi(int kind, int nt_threads, const StepArgs* args, const in
 
 template <int NT, int S>
 hipError_t launch_adj_ns(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s) {
+  if (!args_match(a, t, S)) return hipErrorInvalidValue;
   const int nblk = a.nshots / S * t.nstrips * t.nsegs;
   if (sh) hipLaunchKernelGGL((k_adj_sh<NT, S>), dim3(nblk), dim3(NT * S), 0, s, a);
   else hipLaunchKernelGGL((k_adj_psv<NT, S>), dim3(nblk), dim3(NT * S), 0, s, a);

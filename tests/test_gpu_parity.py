@@ -217,6 +217,32 @@ def test_two_step_launches_equal_one_step_launches(seiswg, threads, seglen, grou
     compare(out[2][0][1][0], obs[O.OBS_X], "two-step shot 1 ux")
 
 
+@pytest.mark.parametrize("tiling2", [(512, 7, 1), (512, 13, 2), (256, 20, 4), (128, 5, 2), (64, 9, 1)])
+def test_two_step_shape_set_apart_from_one_step_shape(seiswg, tiling2):
+    """Two-step launches with a tile shape of their own (sw_set_tiling2),
+    other segment lengths and tile widths than the one-step launches of the
+    same batch, against one step per launch, bit for bit."""
+    nx, nz, nt = 61, 150, 23
+    p, stf = random_problem(nx, nz, nt, False, True, False, seed=31)
+    print(f"case tiling2={tiling2}", flush=True)
+    runs = [0, 1, 2, 3]
+    out = {}
+    for spl in (1, 2):
+        c = gpu_context(p, stf, seiswg)
+        c.set_steps_per_launch(spl)
+        if spl == 2:
+            c.set_tiling2(*tiling2)
+        c.forward_batch(runs)
+        if spl == 2:
+            assert c.tiling2() == tiling2
+        out[spl] = [[c.field(getattr(seiswg.abi, "F_" + n), shot=q).copy() for n in STATE["psv"]] +
+                    [c.obs(q, O.OBS_X).copy(), c.obs(q, O.OBS_Z).copy()] for q in range(len(runs))]
+        c.close()
+    for q in range(len(runs)):
+        for k, (a, b) in enumerate(zip(out[1][q], out[2][q])):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (q, k)
+
+
 def load_example(tmp_path, ex, ini_name="config.ini"):
     d = E.materialise(ex, str(tmp_path))
     ini = E.ini_values(os.path.join(d, ini_name))

@@ -60,18 +60,22 @@ def cpu_baseline(synth, model, geom, src_idx, rec_idx, budget_s=12.0):
     o = O.Oracle(p, 4)
     o.clear()
     o.step(0, 0)     # first-touch
-    n, t0 = 0, time.perf_counter()
+    n, shot, it, t0 = 0, 0, 1, time.perf_counter()
     while True:
-        o.step(1 + n, 0)
+        o.step(it, shot)
         n += 1
+        it += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or n >= 3000:
+        if el >= budget_s:
             break
+        if it == p.nt:          # next shot of the batch, from a cleared state
+            shot, it = shot + 1, 0
+            o.clear()
     cells = c["nx"] * c["nz"]
     return {"value": cells * n / el, "unit": "cell-updates/s", "cores": int(O.lib().or_num_threads()),
             "kind": "port",
-            "sample": f"oracle FP32 restatement of the reference kernels, 1 shot of the 2000x1000 P-SV workload, "
-                      f"{n} time steps in {el:.1f} s"}
+            "sample": f"oracle FP32 restatement of the reference kernels on the 2000x1000 P-SV workload: "
+                      f"{n} shot-steps ({shot + 1} shot(s) in order, nt=4000 each) in {el:.1f} s"}
 
 
 def gradient_bench(seiswg, synth, nt_override=0):
@@ -165,26 +169,30 @@ def small_config_bench(seiswg, synth, which, with_cpu=True):
     return out
 
 
-def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0):
+def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
     """Uneven batch of 48 independent SH / P-SV gradient problems (grids
     384..3072 x 384..1536, 4-16 shots, 100-300 receivers, nt scaled from
-    1500..4000 by nt_scale, checkpoint every 50): total shots/s with the
-    problems merged into shared launches (sw_multi_gradient, in waves that
-    fit budget_gb of device memory) vs one problem per launch (sw_gradient
-    per problem, its shots batched)."""
+    1500..4000 by nt_scale, checkpoint every 50; `spec` = synth.C4_SMALL: grids
+    128..448, 2-6 shots): total shots/s through sw_multi_gradient (problems
+    below its merge limit share merged launches, the others run on their own
+    launches; in waves that fit budget_gb of device memory) vs one problem per
+    launch (sw_gradient per problem, its shots batched)."""
     import numpy as np
     abi = seiswg.abi
+    spec = spec or synth.C4
+    tag = "uneven batch" if spec is synth.C4 else "small uneven batch"
 
     def progress(msg):
-        print(f"[uneven batch] {msg}", file=sys.stderr, flush=True)
+        print(f"[{tag}] {msg}", file=sys.stderr, flush=True)
 
-    probs = synth.c4_problems(nt_scale)
+    probs = synth.c4_problems(nt_scale, spec=spec)
     progress(f"{len(probs)} problems generated")
 
     def make_ctx(p, model):
         sh = p["kind"] == "sh"
         x = seiswg.Context(p["nx"], p["nz"], p["nt"], p["h"], p["h"], p["dt"], sh=sh, psv=not sh, abs_left=True,
-                           abs_right=True, abs_bottom=True, abs_top=False, abs_width=40, abs_alpha=0.015)
+                           abs_right=True, abs_bottom=True, abs_top=False, abs_width=spec["abs_width"],
+                           abs_alpha=0.015)
         x.set_model(abi.MODEL_VPVS, [model["rho"], model["vp"], model["vs"]])
         g = p["geom"]
         rows = g.src_rows()
@@ -246,9 +254,12 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0):
         progress(f"batched: wave of {len(w)} problems, {t_batched:.1f} s so far")
     same = all(np.array_equal(a.view(np.uint64 if a.dtype == np.float64 else np.uint32),
                               b.view(np.uint64 if b.dtype == np.float64 else np.uint32)) for a, b in zip(got0, ref0))
-    return {"workload": f"uneven batch of {len(probs)} SH/P-SV gradient problems (grids 384..3072 x 384..1536, "
-                        f"4-16 shots, nt = {nt_scale} x 1500..4000, checkpoint every {synth.C4['ckpt']})",
+    merged = sum(1 for p, sh in zip(probs, shots) if len(sh) * p["nx"] * p["nz"] < (1 << 19))
+    return {"workload": f"{tag} of {len(probs)} SH/P-SV gradient problems (grids {spec['nx'][0]}..{spec['nx'][1]} x "
+                        f"{spec['nz'][0]}..{spec['nz'][1]}, {spec['shots'][0]}-{spec['shots'][1]} shots, "
+                        f"nt = {nt_scale} x 1500..4000, checkpoint every {synth.C4['ckpt']})",
             "problems": len(probs), "shots": total_shots, "waves": len(waves),
+            "problems_merged": merged, "problems_own_launches": len(probs) - merged,
             "shots_per_s_batched": total_shots / t_batched, "shots_per_s_plain": total_shots / t_plain,
             "speedup": t_plain / t_batched, "s_batched": t_batched, "s_plain": t_plain,
             "problem0_bit_identical": bool(same)}
@@ -313,7 +324,10 @@ def main():
     traffic = None
     tf = os.path.join(ROOT, "profiles", "psv_c2_traffic.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+        prof = json.load(open(tf))
+        if steps_per_launch < 1.5:
+            prof = prof.get("one_step_kernel", {})
+        traffic = prof.get("hbm_bytes_per_launch")
     out = {
         "metric": METRIC, "value": cu / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -336,6 +350,7 @@ def main():
         out["c1_spin_forward"] = small_config_bench(seiswg, synth, "c1", not a.no_cpu_baseline)
     if a.uneven:
         out["uneven_batch"] = uneven_bench(seiswg, synth, a.uneven_nt_scale)
+        out["small_uneven_batch"] = uneven_bench(seiswg, synth, 0.1, spec=synth.C4_SMALL)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(synth, model, geom, src_idx, rec_idx)
     print(json.dumps(out), flush=True)

@@ -161,10 +161,16 @@ SW_API void sw_multi_destroy(sw_multi* m);
 SW_API int sw_multi_forward(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc);
 /* Gradients (as sw_gradient, all shots of a problem in one batch):
  * obs_ref[q] is [nshots[q]][ncomp][nrec][nt] of problem q; misfit[q] may be
- * NULL.  A common checkpoint interval; segments of all problems are aligned. */
+ * NULL.  A common checkpoint interval; segments of the merged problems are
+ * aligned.  Problems with at least `merge limit` shot-cells per step
+ * (nshots * nx * nz; default 2^19) fill the GPU on their own and run on their
+ * own launches, one problem after the other; the smaller ones advance
+ * together in merged launches.  Results do not depend on the split. */
 SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc,
                              const float* const* obs_ref, int32_t ckpt_every, double* const* misfit);
-/* Device time (HIP events) of the last multi call and merged launches issued. */
+SW_API int sw_multi_set_merge_limit(sw_multi* m, int64_t shot_cells);
+/* Elapsed time of the last multi call (HIP events for merged forward runs,
+ * host clock for gradients) and merged launches issued. */
 SW_API int sw_multi_timing(const sw_multi* m, double* ms, int64_t* launches);
 
 /* Gaussian smoothing of a kernel field in place (smooth_gradient,

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -140,6 +141,7 @@ struct sw_ctx {
   std::vector<GraphEntry> graphs;
   uint64_t graph_gen = 0;
   int graph_steps = 16;       // steps per captured block (a multiple of 4)
+  bool graphs_off = false;    // set after a failed capture
 
   // plain adjoint
   DBuf<float> kmu, k_lam, k_rho, gsum, gtmp, wtab, sumx, sumz, fwd_store, adj_src, ref, taper;
@@ -228,7 +230,9 @@ int effective_group2(int nt, int requested, int nshots) {
 // Shape of the two-step launches (6-column halos, W = threads-12): the tile
 // width that wastes the fewest lanes on the last strip (the wider on ties),
 // shots per workgroup as requested (else 1), and as many segments as fill the
-// resident slots of every CU once, at least 32 rows each.
+// resident slots of every CU once, at least 8 rows each (a small batch is
+// latency-bound: many short segments beat few long ones despite the 9
+// recomputed rows per segment).
 void heuristic_tiling2(sw_ctx* c, int nshots) {
   Tiling& t = c->tile2;
   const int nz = c->g.nz;
@@ -248,7 +252,7 @@ void heuristic_tiling2(sw_ctx* c, int nshots) {
   const int64_t strips = (nz + best - 12 - 1) / (best - 12);
   const int64_t base = int64_t(nshots / group) * strips;
   const int64_t nsegs = std::max<int64_t>(1, (ncu * resident) / base);
-  int seg = int(std::max<int64_t>((c->g.nx + nsegs - 1) / nsegs, 32));
+  int seg = int(std::max<int64_t>((c->g.nx + nsegs - 1) / nsegs, 8));
   t.nt_threads = best;
   t.seglen = std::min(seg, c->g.nx);
   t.group = group;
@@ -572,7 +576,7 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   }
   int done = 0;
   const int G = c->graph_steps;   // a multiple of 4
-  if (G >= 2 && nsteps >= 2 * G && !getenv("SEISWG_NO_GRAPHS")) {
+  if (G >= 2 && nsteps >= 2 * G && !c->graphs_off && !getenv("SEISWG_NO_GRAPHS")) {
     if (c->graph_gen != c->gen) {
       for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.exec);
       c->graphs.clear();
@@ -584,28 +588,34 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     for (auto& e : c->graphs)
       if (e.key == key) exec = e.exec;
     if (!exec) {
+      // Capture G steps.  Should the capture fail (HIP can invalidate a
+      // capture because of work other host threads submit meanwhile), nothing
+      // of it ran: this context stops using graphs and launches directly.
       hipGraph_t graph = nullptr;
       const int64_t l0 = c->launches;
-      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      int rc = 0;
-      if (pairs)
-        for (int q = 0; q < G && !rc; q += 4) rc = enqueue4();
-      else
-        for (int q = 0; q < G && !rc; ++q) rc = enqueue();
-      hipError_t e_end = hipStreamEndCapture(c->stream, &graph);
-      if (rc) {
+      bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+      if (ok) {
+        int rc = 0;
+        if (pairs)
+          for (int q = 0; q < G && !rc; q += 4) rc = enqueue4();
+        else
+          for (int q = 0; q < G && !rc; ++q) rc = enqueue();
+        const hipError_t e_end = hipStreamEndCapture(c->stream, &graph);
+        ok = rc == 0 && e_end == hipSuccess && graph != nullptr &&
+             hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
         if (graph) (void)hipGraphDestroy(graph);
-        return rc;
       }
-      HIPCHK(e_end);
-      hipError_t e_inst = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      HIPCHK(e_inst);
-      c->graphs.push_back({key, exec});
-      c->launches = l0;   // captured, not launched
+      c->launches = l0;   // captured (or discarded), not launched
+      if (ok) {
+        c->graphs.push_back({key, exec});
+      } else {
+        exec = nullptr;
+        c->graphs_off = true;
+        (void)hipGetLastError();
+      }
     }
     const int64_t per_block = pairs ? G / 2 : (mode != M_ADJ && sh && psv) ? 2 * G : G;
-    for (; done + G <= nsteps; done += G) {
+    for (; exec && done + G <= nsteps; done += G) {
       HIPCHK(hipGraphLaunch(exec, c->stream));
       c->launches += per_block;
       count_step(G);
@@ -1288,6 +1298,11 @@ struct sw_multi {
   std::vector<sw_ctx*> p;
   int device = 0;
   hipStream_t stream = nullptr;
+  // gradients: problems with at least this many shot-cells per step run on
+  // their own launches; the smaller ones share merged launches (measured:
+  // merged launches win below ~0.3 M shot-cells per step, own launches from
+  // ~0.6 M on)
+  int64_t merge_limit = int64_t(1) << 19;
   DBuf<StepArgs> args;
   DBuf<int2> work;
   double loop_ms = 0;
@@ -1431,9 +1446,9 @@ SW_API int sw_multi_forward(sw_multi* m, const int32_t* nshots, const int32_t* c
   return 0;
 }
 
-SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc,
-                             const float* const* obs_ref, int32_t ckpt_every, double* const* misfit) {
-  if (!m || !nshots || !isrc || !obs_ref) return fail(-1, "bad arguments");
+namespace {
+int multi_gradient_merged(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc,
+                          const float* const* obs_ref, int32_t ckpt_every, double* const* misfit) {
   HIPCHK(hipSetDevice(m->device));
   SharedStream guard(m);
   const int n = int(m->p.size());
@@ -1500,6 +1515,54 @@ SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* 
   for (int q = 0; q < n; ++q)
     if (misfit && misfit[q])
       for (int s = 0; s < nshots[q]; ++s) misfit[q][s] = m->p[q]->gp.mis[s];
+  return 0;
+}
+}  // namespace
+
+SW_API int sw_multi_set_merge_limit(sw_multi* m, int64_t shot_cells) {
+  if (!m || shot_cells < 0) return fail(-1, "bad arguments");
+  m->merge_limit = shot_cells;
+  return 0;
+}
+
+SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc,
+                             const float* const* obs_ref, int32_t ckpt_every, double* const* misfit) {
+  if (!m || !nshots || !isrc || !obs_ref) return fail(-1, "bad arguments");
+  HIPCHK(hipSetDevice(m->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  const int n = int(m->p.size());
+  std::vector<int> merged;
+  int rc = 0;
+  // problems that fill the GPU on their own: their own launches (own tile
+  // shapes, two-step pairs, captured graphs), one problem after the other
+  for (int q = 0; q < n; ++q) {
+    if (int64_t(nshots[q]) * m->p[q]->npt < m->merge_limit) {
+      merged.push_back(q);
+      continue;
+    }
+    if ((rc = sw_gradient(m->p[q], nshots[q], isrc[q], obs_ref[q], ckpt_every, 0, misfit ? misfit[q] : nullptr)))
+      return rc;
+  }
+  if (!merged.empty()) {
+    // the others in merged launches, over the subset
+    std::vector<sw_ctx*> all = m->p, sub;
+    std::vector<int32_t> ns;
+    std::vector<const int32_t*> is;
+    std::vector<const float*> ob;
+    std::vector<double*> mi;
+    for (int q : merged) {
+      sub.push_back(all[q]);
+      ns.push_back(nshots[q]);
+      is.push_back(isrc[q]);
+      ob.push_back(obs_ref[q]);
+      mi.push_back(misfit ? misfit[q] : nullptr);
+    }
+    m->p = sub;
+    rc = multi_gradient_merged(m, ns.data(), is.data(), ob.data(), ckpt_every, mi.data());
+    m->p = all;
+    if (rc) return rc;
+  }
+  m->loop_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
 

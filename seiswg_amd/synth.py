@@ -136,16 +136,22 @@ C4 = dict(nproblems=48, seed=4, nx=(384, 3072), nz=(384, 1536), shots=(4, 16), n
           f0=(5.0, 15.0), h=10.0, cfl=0.5, abs_width=40, ckpt=50)
 
 
-def c4_problems(nt_scale: float = 1.0, nproblems: int | None = None):
+# a batch of small uneven problems (same recipe, grids 128-448): the sizes at
+# which merging problems into shared launches pays
+C4_SMALL = dict(C4, seed=5, nx=(128, 448), nz=(128, 448), shots=(2, 6), nrec=(40, 100), abs_width=20)
+
+
+def c4_problems(nt_scale: float = 1.0, nproblems: int | None = None, spec: dict | None = None):
     """Uneven batch of independent SH and P-SV gradient problems (seeded):
     grid nx in [384, 3072], nz in [384, 1536], each with its own
     layered-plus-random-field model (the C2 recipe, seed 100+q) and a true
     model with Gaussian vs anomalies; 4-16 shots at 50 m depth, 100-300
     receivers at 10 m; nt in [1500, 4000] (times nt_scale, at least 60);
     dt from CFL 0.5 on the model's largest vp; Ricker f0 in [5, 15] Hz,
-    t0 = 1.2 / f0.  Even problems are SH, odd ones P-SV.  Returns a list of
+    t0 = 1.2 / f0.  Even problems are SH, odd ones P-SV.  `spec` replaces
+    these ranges (C4_SMALL: the small-problem batch).  Returns a list of
     dicts: kind, nx, nz, h, dt, nt, init, true, geom."""
-    c = C4
+    c = spec or C4
     rng = np.random.default_rng(c["seed"])
     out = []
     for q in range(nproblems or c["nproblems"]):

@@ -143,13 +143,19 @@ def test_multi_problem_forward_equals_single_runs(seiswg):
 
 
 @pytest.mark.gpu
-def test_multi_problem_gradient_equals_single_gradients(seiswg):
+@pytest.mark.parametrize("merge_limit", [None, 0, 3000])
+def test_multi_problem_gradient_equals_single_gradients(seiswg, merge_limit):
+    """Merged launches (default: all four problems are small), every problem
+    on its own launches (limit 0), and a split (limit 3000 shot-cells: two
+    problems merged, two on their own) give the single-problem gradients."""
     probs = multi_problems(seiswg)
     runs = [[0, -1], [1, 2], [0], [2, 0, 1]]
     refs = [(np.random.default_rng(q).standard_normal((len(r), nc, len(p.rec_idx), p.nt)) * 1e-3).astype(np.float32)
             for q, ((p, stf, nc, c), r) in enumerate(zip(probs, runs))]
     single = [c.gradient(r, o, ckpt_every=20) for (p, stf, nc, c), r, o in zip(probs, runs, refs)]
     m = seiswg.Multi([c for *_, c in probs])
+    if merge_limit is not None:
+        m.set_merge_limit(merge_limit)
     got = m.gradient(runs, refs, ckpt_every=20)
     for a, b in zip(got, single):
         assert np.array_equal(a[0], b[0])

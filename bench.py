@@ -283,6 +283,8 @@ def main():
     ap.add_argument("--uneven", action="store_true", help="also measure the uneven batch of 48 gradient problems")
     ap.add_argument("--no-small", action="store_true", help="skip the 200x200 SH and 500x500 spin forward runs")
     ap.add_argument("--uneven-nt-scale", type=float, default=0.04, help="nt of the uneven batch, as a fraction")
+    ap.add_argument("--uneven-budget-gb", type=float, default=200.0,
+                    help="device memory per wave of the uneven batch's multi-problem calls")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB copy kernel (calibrates the HBM counters of a profiler run)")
     a = ap.parse_args()
@@ -349,7 +351,7 @@ def main():
         out["c0_sh_forward"] = small_config_bench(seiswg, synth, "c0", not a.no_cpu_baseline)
         out["c1_spin_forward"] = small_config_bench(seiswg, synth, "c1", not a.no_cpu_baseline)
     if a.uneven:
-        out["uneven_batch"] = uneven_bench(seiswg, synth, a.uneven_nt_scale)
+        out["uneven_batch"] = uneven_bench(seiswg, synth, a.uneven_nt_scale, a.uneven_budget_gb)
         out["small_uneven_batch"] = uneven_bench(seiswg, synth, 0.1, spec=synth.C4_SMALL)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(synth, model, geom, src_idx, rec_idx)

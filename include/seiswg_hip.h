@@ -218,6 +218,14 @@ typedef struct sw_timing {
 } sw_timing;
 SW_API int sw_get_timing(const sw_ctx* ctx, sw_timing* out);
 
+/* Device memory of freed buffers is kept for reuse by later buffers of up to
+ * the same size (no counterpart in the reference): runs over many problems
+ * then do not wait on fresh allocations.  This returns the cached memory of
+ * `hip_device` (-1: every device) to the runtime, e.g. before another
+ * library allocates; an allocation that fails empties the cache and retries
+ * by itself.  Environment SEISWG_NO_MEMORY_CACHE=1 disables the cache. */
+SW_API int sw_release_cached_memory(int32_t hip_device, int64_t* bytes_freed);
+
 /* Measurement aid with no counterpart in the reference: `iters` launches of
  * a plain float4 copy kernel moving nbytes (read) + nbytes (write) on
  * `hip_device`; average kernel time in ms from HIP events.  Used to calibrate

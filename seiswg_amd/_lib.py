@@ -48,7 +48,7 @@ SNAP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.PO
 EXPORTS = ["sw_last_error", "sw_create", "sw_destroy", "sw_set_model", "sw_set_sources", "sw_set_receivers",
            "sw_forward", "sw_forward_batch", "sw_read_obs", "sw_read_field", "sw_write_field", "sw_prepare",
            "sw_advance", "sw_adjoint_plain", "sw_smooth", "sw_set_tiling", "sw_get_tiling", "sw_get_timing",
-           "sw_set_steps_per_launch", "sw_set_tiling2", "sw_get_tiling2",
+           "sw_set_steps_per_launch", "sw_set_tiling2", "sw_get_tiling2", "sw_release_cached_memory",
            "sw_bench_copy", "sw_gradient", "sw_autotune", "sw_multi_create", "sw_multi_destroy",
            "sw_multi_forward", "sw_multi_gradient", "sw_multi_timing", "sw_multi_set_merge_limit",
            "seiswg_run", "seiswg_ricker_table", "seiswg_infer_grid", "seiswg_grid_index"]
@@ -83,6 +83,7 @@ def load():
     L.sw_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
     L.sw_set_steps_per_launch.argtypes = [vp, C.c_int32]
     L.sw_set_tiling2.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+    L.sw_release_cached_memory.argtypes = [C.c_int32, C.POINTER(C.c_int64)]
     L.sw_get_tiling.argtypes = [vp, ip, ip, ip]
     L.sw_get_timing.argtypes = [vp, C.POINTER(sw_timing)]
     L.sw_multi_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
@@ -153,6 +154,13 @@ def bench_copy(nbytes: int, iters: int = 20, device: int = 0) -> float:
     ms = C.c_double()
     _check(load().sw_bench_copy(device, nbytes, iters, C.byref(ms)))
     return ms.value
+
+
+def release_cached_memory(device: int = -1) -> int:
+    """Return the library's cached device memory to the runtime; bytes freed."""
+    n = C.c_int64()
+    _check(load().sw_release_cached_memory(device, C.byref(n)))
+    return n.value
 
 
 def run_config(path: str) -> int:

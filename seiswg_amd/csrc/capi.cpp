@@ -12,7 +12,10 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -41,22 +44,106 @@ int fail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail(-3, "%s: %s", #expr, hipGetErrorString(e_));     \
   } while (0)
 
+// Device-memory cache.  Runs over many problems (gradients of an uneven
+// batch, one context per problem) allocate and free buffers of tens of GB in
+// turn; hipMalloc of memory that was just freed can then wait for the driver
+// (measured: single gradient calls taking seconds instead of tens of ms).
+// Freed blocks are kept per device and handed out again for requests of up
+// to their size (at most twice the request); on an allocation failure the
+// cache of that device is emptied and the allocation retried.  A free waits
+// for the device to go idle first, as hipFree does, so a cached block is
+// never still in use by a kernel.  SEISWG_NO_MEMORY_CACHE=1 turns it off.
+struct MemCache {
+  std::mutex mu;
+  std::multimap<std::pair<int, size_t>, void*> blocks;   // (device, bytes) -> block
+  bool on = getenv("SEISWG_NO_MEMORY_CACHE") == nullptr;
+};
+MemCache& mem_cache() {
+  static MemCache* c = new MemCache;   // never destroyed: usable until exit
+  return *c;
+}
+size_t cache_round(size_t bytes) {
+  const size_t g = bytes >= (size_t(1) << 21) ? (size_t(1) << 21) : 4096;
+  return (bytes + g - 1) / g * g;
+}
+size_t cache_trim(int dev) {
+  MemCache& c = mem_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  size_t freed = 0;
+  for (auto it = c.blocks.begin(); it != c.blocks.end();) {
+    if (dev >= 0 && it->first.first != dev) { ++it; continue; }
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(it->first.first);
+    (void)hipFree(it->second);
+    (void)hipSetDevice(cur);
+    freed += it->first.second;
+    it = c.blocks.erase(it);
+  }
+  return freed;
+}
+hipError_t cache_alloc(void** p, size_t& bytes, int dev) {
+  MemCache& c = mem_cache();
+  bytes = cache_round(bytes);
+  if (c.on) {
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.blocks.lower_bound({dev, bytes});
+    if (it != c.blocks.end() && it->first.first == dev && it->first.second <= 2 * bytes) {
+      *p = it->second;
+      bytes = it->first.second;
+      c.blocks.erase(it);
+      return hipSuccess;
+    }
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && c.on) {
+    (void)hipGetLastError();
+    cache_trim(dev);
+    e = hipMalloc(p, bytes);
+  }
+  return e;
+}
+void cache_free(void* p, size_t bytes, int dev) {
+  MemCache& c = mem_cache();
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  if (cur != dev) (void)hipSetDevice(dev);
+  if (c.on) {
+    (void)hipDeviceSynchronize();
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.blocks.insert({{dev, bytes}, p});
+  } else {
+    (void)hipFree(p);
+  }
+  if (cur != dev) (void)hipSetDevice(cur);
+}
+
 template <typename T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  size_t bytes = 0;
+  int dev = 0;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) cache_free(p, bytes, dev);
     p = nullptr;
     n = 0;
+    bytes = 0;
   }
   hipError_t ensure(size_t count) {
     if (count <= n && p) return hipSuccess;
     release();
     if (count == 0) return hipSuccess;
-    hipError_t e = hipMalloc(&p, count * sizeof(T));
-    if (e == hipSuccess) n = count;
+    (void)hipGetDevice(&dev);
+    size_t b = count * sizeof(T);
+    void* q = nullptr;
+    hipError_t e = cache_alloc(&q, b, dev);
+    if (e == hipSuccess) {
+      p = static_cast<T*>(q);
+      n = count;
+      bytes = b;
+    }
     return e;
   }
 };
@@ -522,7 +609,7 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   a_sh.advance = psv ? 0 : 1;   // with both physics the P-SV kernel advances the clock
   // Four steps as two two-step launches: state[cur] -> spare -> state[cur],
   // so the buffer of every step stays that of its parity.
-  const bool pairs = mode == M_FWD && c->two_ok && c->steps_per_launch == 2 && nsteps >= 4;
+  const bool pairs = (mode == M_FWD || mode == M_FWD_AUX) && c->two_ok && c->steps_per_launch == 2 && nsteps >= 4;
   StepArgs a2a{}, a2b{};
   if (pairs) {
     HIPCHK(c->spare.ensure(size_t(c->nslots) * size_t(c->nshots) * size_t(c->npt)));
@@ -538,8 +625,8 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     a2b.st[0] = c->spare.p; a2b.st[1] = c->state[c->cur].p;
   }
   auto enqueue4 = [&]() -> int {
-    HIPCHK(launch_step_psv2(a2a, c->tile2, c->stream));
-    HIPCHK(launch_step_psv2(a2b, c->tile2, c->stream));
+    HIPCHK(launch_step_psv2(a2a, c->tile2, c->stream, aux));
+    HIPCHK(launch_step_psv2(a2b, c->tile2, c->stream, aux));
     c->launches += 2;
     return 0;
   };
@@ -1045,6 +1132,12 @@ SW_API int sw_get_tiling(const sw_ctx* c, int32_t* threads, int32_t* seglen, int
   return 0;
 }
 
+SW_API int sw_release_cached_memory(int32_t hip_device, int64_t* bytes_freed) {
+  const size_t n = cache_trim(hip_device);
+  if (bytes_freed) *bytes_freed = int64_t(n);
+  return 0;
+}
+
 SW_API int sw_bench_copy(int32_t device, int64_t nbytes, int32_t iters, double* avg_ms) {
   if (nbytes <= 0 || nbytes % 16 || iters < 1 || !avg_ms) return fail(-1, "bad arguments");
   HIPCHK(hipSetDevice(device));
@@ -1542,6 +1635,11 @@ SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* 
     }
     if ((rc = sw_gradient(m->p[q], nshots[q], isrc[q], obs_ref[q], ckpt_every, 0, misfit ? misfit[q] : nullptr)))
       return rc;
+    // its kernels are in K_RHO / K_MU / K_LAM: give the device memory of the
+    // run (checkpoints, aux fields, adjoint state, per-shot kernels) back
+    // before the next problem allocates its own
+    sw_ctx* c = m->p[q];
+    for (DBuf<float>* b : {&c->aux, &c->ckpt, &c->adj[0], &c->adj[1], &c->kshot, &c->spare}) b->release();
   }
   if (!merged.empty()) {
     // the others in merged launches, over the subset

@@ -86,9 +86,10 @@ enum KField : int { K_RHO = 0, K_MU, K_LAM, NK };
 // launches (kernels.hip)
 hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false);
 hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStream_t s, bool aux = false);
-// two P-SV steps (no spin, no aux fields) per launch: reads st[0], writes
-// st[1] (a buffer distinct from st[0]); tiles of nt_threads-12 columns
-hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s);
+// two P-SV steps (no spin) per launch: reads st[0], writes st[1] (a buffer
+// distinct from st[0]); tiles of nt_threads-12 columns; aux: also the aux
+// fields of both steps (forward recompute of the gradient)
+hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false);
 hipError_t launch_adj_sh(const StepArgs& a, const Tiling& t, hipStream_t s);
 hipError_t launch_adj_psv(const StepArgs& a, const Tiling& t, hipStream_t s);
 

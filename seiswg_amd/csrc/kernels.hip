@@ -802,7 +802,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
 // is that of step_psv, operation for operation, so the result equals two
 // one-step launches bit for bit.  The host passes the buffer to read in st[0]
 // and the one to write in st[1].
-template <int NT, int S>
+template <int NT, int S, bool AUX>
 __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   __shared__ float lds_s[2][2][S * NT];    // stage 1: sxz, szz of step it
   __shared__ float lds_v[2][2][S * NT];    // stage 1: vx, vz of step it+1
@@ -848,6 +848,13 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const int nobs = a.nrec * a.nt;
   const int m0 = i0 - 4;   // first velocity row of stage 1
+  // forward recompute of the gradient: the aux fields of steps it and it+1
+  auto aux_rsrc = [&](int step) {
+    return make_rsrc(
+        AUX ? a.aux + (int64_t(step % a.aux_period) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt) : a.model,
+        AUX ? uint32_t(NAUX_PSV) * npt4 : 0u);
+  };
+  const auto raux1 = aux_rsrc(it), raux2 = aux_rsrc(it + 1);
 
   // stage 1 rings: stresses of step it (rows m-2..m+1), new velocities
   // (rows m-3..m), D+z of the new velocities (row m at slot s2, used at s0)
@@ -950,6 +957,13 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       div_group(qb, xb, {Div{rho, irho, true}, Div{rho, irho, true}}, fast_rho);
       vx[s3] = b * (vx_o + qb[0]);
       vz[s3] = b * (vz_o + qb[1]);
+      if constexpr (AUX) {
+        if (g.jst && m >= i0 && m < i1) {
+          const uint32_t ro = uint32_t(m) * g.rowb;
+          bstore(dsx, raux1, jc4, A_DSX * npt4 + ro);
+          bstore(dsz, raux1, jc4, A_DSZ * npt4 + ro);
+        }
+      }
     }
     lds_v[par][0][lane] = vx[s3];
     lds_v[par][1][lane] = vz[s3];
@@ -995,6 +1009,15 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       t1xx[s0] = sxx[s0] + dt * (l2m * dvxdx + lam * dvzdz[s0]);
       t1zz[s0] = szz[s0] + dt * (l2m * dvzdz[s0] + lam * dvxdx);
       t1xz[s0] = sxz[s0] + dt * mu * (dvxdz[s0] + dvzdx);
+      if constexpr (AUX) {
+        const int i = m - 2;
+        if (g.jst && i >= i0 && i < i1) {
+          const uint32_t ro = uint32_t(i) * g.rowb;
+          bstore(dvxdx, raux1, jc4, A_EXX * npt4 + ro);
+          bstore(dvzdz[s0], raux1, jc4, A_EZZ * npt4 + ro);
+          bstore(dvxdz[s0] + dvzdx, raux1, jc4, A_EXZ * npt4 + ro);
+        }
+      }
       lds_s2[par][0][lane] = t1xz[s0];
       lds_s2[par][1][lane] = t1zz[s0];
     }
@@ -1040,6 +1063,10 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
         const uint32_t ro = uint32_t(m2) * g.rowb;
         bstore(vxn, rout, jc4, o_vx + ro);
         bstore(vzn, rout, jc4, o_vz + ro);
+        if constexpr (AUX) {
+          bstore(dsx, raux2, jc4, A_DSX * npt4 + ro);
+          bstore(dsz, raux2, jc4, A_DSZ * npt4 + ro);
+        }
       }
       u2x[s0] = vxn;
       u2z[s0] = vzn;
@@ -1089,6 +1116,11 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
         bstore(nsxx, rout, jc4, o_sxx + ro);
         bstore(nszz, rout, jc4, o_szz + ro);
         bstore(nsxz, rout, jc4, o_sxz + ro);
+        if constexpr (AUX) {
+          bstore(dvxdx, raux2, jc4, A_EXX * npt4 + ro);
+          bstore(dz2z[par ^ 1], raux2, jc4, A_EZZ * npt4 + ro);
+          bstore(dz2x[par ^ 1] + dvzdx, raux2, jc4, A_EXZ * npt4 + ro);
+        }
       }
     }
     lds_v2[par][0][lane] = u2x[s0];
@@ -1526,10 +1558,10 @@ __global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(S >= 4 ?
     const StepArgs a) {
   step_psv<NT, S, false, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
-template <int NT, int S>
+template <int NT, int S, bool AUX>
 __global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(4))) void k_step_psv2(
     const StepArgs a) {
-  step_psv2<NT, S>(a, xcd_remap(blockIdx.x, gridDim.x));
+  step_psv2<NT, S, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 template <int NT, int S>
 __global__ __launch_bounds__(NT * S) void k_step_spin(const StepArgs a) {
@@ -1820,24 +1852,25 @@ hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, boo
 }
 
 template <int NT, int S>
-hipError_t launch_psv2_ns(const StepArgs& a, const Tiling& t, hipStream_t s) {
+hipError_t launch_psv2_ns(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
   if (!args_match(a, t, S)) return hipErrorInvalidValue;
   const int nblk = a.nshots / S * t.nstrips * t.nsegs;
-  hipLaunchKernelGGL((k_step_psv2<NT, S>), dim3(nblk), dim3(NT * S), 0, s, a);
+  if (aux) hipLaunchKernelGGL((k_step_psv2<NT, S, true>), dim3(nblk), dim3(NT * S), 0, s, a);
+  else hipLaunchKernelGGL((k_step_psv2<NT, S, false>), dim3(nblk), dim3(NT * S), 0, s, a);
   return hipGetLastError();
 }
-hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s) {
+hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
   if (t.group < 1 || a.nshots % t.group != 0) return hipErrorInvalidValue;
   switch (t.nt_threads * 8 + t.group) {
-    case 64 * 8 + 1: return launch_psv2_ns<64, 1>(a, t, s);
-    case 128 * 8 + 1: return launch_psv2_ns<128, 1>(a, t, s);
-    case 128 * 8 + 2: return launch_psv2_ns<128, 2>(a, t, s);
-    case 128 * 8 + 4: return launch_psv2_ns<128, 4>(a, t, s);
-    case 256 * 8 + 1: return launch_psv2_ns<256, 1>(a, t, s);
-    case 256 * 8 + 2: return launch_psv2_ns<256, 2>(a, t, s);
-    case 256 * 8 + 4: return launch_psv2_ns<256, 4>(a, t, s);
-    case 512 * 8 + 1: return launch_psv2_ns<512, 1>(a, t, s);
-    case 512 * 8 + 2: return launch_psv2_ns<512, 2>(a, t, s);
+    case 64 * 8 + 1: return launch_psv2_ns<64, 1>(a, t, s, aux);
+    case 128 * 8 + 1: return launch_psv2_ns<128, 1>(a, t, s, aux);
+    case 128 * 8 + 2: return launch_psv2_ns<128, 2>(a, t, s, aux);
+    case 128 * 8 + 4: return launch_psv2_ns<128, 4>(a, t, s, aux);
+    case 256 * 8 + 1: return launch_psv2_ns<256, 1>(a, t, s, aux);
+    case 256 * 8 + 2: return launch_psv2_ns<256, 2>(a, t, s, aux);
+    case 256 * 8 + 4: return launch_psv2_ns<256, 4>(a, t, s, aux);
+    case 512 * 8 + 1: return launch_psv2_ns<512, 1>(a, t, s, aux);
+    case 512 * 8 + 2: return launch_psv2_ns<512, 2>(a, t, s, aux);
   }
   return hipErrorInvalidValue;
 }
@@ -1847,15 +1880,15 @@ bool group2_supported(int nt, int group) {
 }
 int step2_blocks_per_cu(int nt, int group) {
   switch (nt * 8 + group) {
-    case 64 * 8 + 1: return occupancy(k_step_psv2<64, 1>, 64);
-    case 128 * 8 + 1: return occupancy(k_step_psv2<128, 1>, 128);
-    case 128 * 8 + 2: return occupancy(k_step_psv2<128, 2>, 256);
-    case 128 * 8 + 4: return occupancy(k_step_psv2<128, 4>, 512);
-    case 256 * 8 + 1: return occupancy(k_step_psv2<256, 1>, 256);
-    case 256 * 8 + 2: return occupancy(k_step_psv2<256, 2>, 512);
-    case 256 * 8 + 4: return occupancy(k_step_psv2<256, 4>, 1024);
-    case 512 * 8 + 1: return occupancy(k_step_psv2<512, 1>, 512);
-    case 512 * 8 + 2: return occupancy(k_step_psv2<512, 2>, 1024);
+    case 64 * 8 + 1: return occupancy(k_step_psv2<64, 1, false>, 64);
+    case 128 * 8 + 1: return occupancy(k_step_psv2<128, 1, false>, 128);
+    case 128 * 8 + 2: return occupancy(k_step_psv2<128, 2, false>, 256);
+    case 128 * 8 + 4: return occupancy(k_step_psv2<128, 4, false>, 512);
+    case 256 * 8 + 1: return occupancy(k_step_psv2<256, 1, false>, 256);
+    case 256 * 8 + 2: return occupancy(k_step_psv2<256, 2, false>, 512);
+    case 256 * 8 + 4: return occupancy(k_step_psv2<256, 4, false>, 1024);
+    case 512 * 8 + 1: return occupancy(k_step_psv2<512, 1, false>, 512);
+    case 512 * 8 + 2: return occupancy(k_step_psv2<512, 2, false>, 1024);
   }
   return 0;
 }

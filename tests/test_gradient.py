@@ -111,6 +111,27 @@ def test_gpu_gradient_independent_of_checkpoint_interval_and_batching(seiswg, sh
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (ck, spl)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("ck", [7, 25, 90])
+def test_gpu_gradient_two_step_launches_equal_one_step_launches(seiswg, ck):
+    """P-SV gradient with the forward and the aux-field recompute in two-step
+    launches (the default) against one step per launch: misfits and kernels
+    bit for bit, for segments of 7 (one pair and three single steps), 25 and
+    90 steps (captured graphs)."""
+    p, stf, nc = problem(False, 50, 44, 90, seed=6)
+    runs = [0, 1, 2, -1]
+    obs = (np.random.default_rng(9).standard_normal((len(runs), nc, len(p.rec_idx), p.nt)) * 1e-3).astype(np.float32)
+    out = []
+    for spl in (1, 2):
+        c = gpu_problem(seiswg, p, stf)
+        c.set_steps_per_launch(spl)
+        out.append(c.gradient(runs, obs, ckpt_every=ck))
+        c.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1:], out[1][1:]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), ck
+
+
 def multi_problems(seiswg):
     """Uneven problems: SH and P-SV, different grids, nt, dt and tile shapes."""
     specs = [(True, 40, 36, 70, 0.1, (64, 7)), (False, 52, 30, 90, 0.08, (128, 16)),

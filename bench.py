@@ -254,7 +254,7 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
         progress(f"batched: wave of {len(w)} problems, {t_batched:.1f} s so far")
     same = all(np.array_equal(a.view(np.uint64 if a.dtype == np.float64 else np.uint32),
                               b.view(np.uint64 if b.dtype == np.float64 else np.uint32)) for a, b in zip(got0, ref0))
-    merged = sum(1 for p, sh in zip(probs, shots) if len(sh) * p["nx"] * p["nz"] < (1 << 19))
+    merged = sum(1 for p, sh in zip(probs, shots) if len(sh) * p["nx"] * p["nz"] < (3 << 17))
     return {"workload": f"{tag} of {len(probs)} SH/P-SV gradient problems (grids {spec['nx'][0]}..{spec['nx'][1]} x "
                         f"{spec['nz'][0]}..{spec['nz'][1]}, {spec['shots'][0]}-{spec['shots'][1]} shots, "
                         f"nt = {nt_scale} x 1500..4000, checkpoint every {synth.C4['ckpt']})",

@@ -163,7 +163,7 @@ SW_API int sw_multi_forward(sw_multi* m, const int32_t* nshots, const int32_t* c
  * obs_ref[q] is [nshots[q]][ncomp][nrec][nt] of problem q; misfit[q] may be
  * NULL.  A common checkpoint interval; segments of the merged problems are
  * aligned.  Problems with at least `merge limit` shot-cells per step
- * (nshots * nx * nz; default 2^19) fill the GPU on their own and run on their
+ * (nshots * nx * nz; default 3 * 2^17) fill the GPU on their own and run on their
  * own launches, one problem after the other; the smaller ones advance
  * together in merged launches.  Results do not depend on the split. */
 SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc,

@@ -315,7 +315,8 @@ int effective_group2(int nt, int requested, int nshots) {
 }
 
 // Shape of the two-step launches (6-column halos, W = threads-12): the tile
-// width that wastes the fewest lanes on the last strip (the wider on ties),
+// width that wastes the fewest lanes on the last strip (the wider on ties;
+// for grids too small to fill the CUs, the one that gives more workgroups),
 // shots per workgroup as requested (else 1), and as many segments as fill the
 // resident slots of every CU once, at least 8 rows each (a small batch is
 // latency-bound: many short segments beat few long ones despite the 9
@@ -323,18 +324,22 @@ int effective_group2(int nt, int requested, int nshots) {
 void heuristic_tiling2(sw_ctx* c, int nshots) {
   Tiling& t = c->tile2;
   const int nz = c->g.nz;
+  const int64_t ncu = c->ncu > 0 ? c->ncu : 256;
   int best = 64;
-  double best_eff = -1.0;
+  double best_score = -1.0;
   for (int nt : {512, 256, 128, 64}) {
-    const int strips = (nz + nt - 12 - 1) / (nt - 12);
+    const int64_t strips = (nz + nt - 12 - 1) / (nt - 12);
     const double eff = double(nz) / (double(strips) * nt);
-    if (eff > best_eff + 1e-9) {
-      best_eff = eff;
+    // a grid too small to give every CU a workgroup even with 8-row
+    // segments prefers the narrower tile that gives more of them
+    const int64_t most = int64_t(nshots) * strips * std::max(1, (c->g.nx + 7) / 8);
+    const double score = eff * std::min(1.0, double(most) / double(ncu));
+    if (score > best_score + 1e-9) {
+      best_score = score;
       best = nt;
     }
   }
   const int group = effective_group2(best, c->group2_req, nshots);
-  const int64_t ncu = c->ncu > 0 ? c->ncu : 256;
   const int64_t resident = std::max(step2_blocks_per_cu(best, group), 1);
   const int64_t strips = (nz + best - 12 - 1) / (best - 12);
   const int64_t base = int64_t(nshots / group) * strips;
@@ -392,7 +397,7 @@ int check_tiling(sw_ctx* c, int nshots) {
   t.group = effective_group(t.nt_threads, c->tile_fixed ? c->group_req : t.group, nshots, uses_spin(c));
   // two-step launches: their own shape if set or chosen by the heuristic,
   // else that of the one-step launches
-  c->two_ok = c->steps_per_launch == 2 && uses_psv(c) && !uses_spin(c) && !uses_sh(c);
+  c->two_ok = c->steps_per_launch == 2 && uses_psv(c) != uses_sh(c) && !uses_spin(c);
   if (c->two_ok) {
     Tiling& t2 = c->tile2;
     if (!c->tile2_fixed && c->tile_fixed) {
@@ -625,8 +630,13 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     a2b.st[0] = c->spare.p; a2b.st[1] = c->state[c->cur].p;
   }
   auto enqueue4 = [&]() -> int {
-    HIPCHK(launch_step_psv2(a2a, c->tile2, c->stream, aux));
-    HIPCHK(launch_step_psv2(a2b, c->tile2, c->stream, aux));
+    if (sh) {
+      HIPCHK(launch_step_sh2(a2a, c->tile2, c->stream, aux));
+      HIPCHK(launch_step_sh2(a2b, c->tile2, c->stream, aux));
+    } else {
+      HIPCHK(launch_step_psv2(a2a, c->tile2, c->stream, aux));
+      HIPCHK(launch_step_psv2(a2b, c->tile2, c->stream, aux));
+    }
     c->launches += 2;
     return 0;
   };
@@ -1393,9 +1403,9 @@ struct sw_multi {
   hipStream_t stream = nullptr;
   // gradients: problems with at least this many shot-cells per step run on
   // their own launches; the smaller ones share merged launches (measured:
-  // merged launches win below ~0.3 M shot-cells per step, own launches from
-  // ~0.6 M on)
-  int64_t merge_limit = int64_t(1) << 19;
+  // merged launches ahead up to ~0.3 M shot-cells per step, own launches
+  // from ~0.6 M on)
+  int64_t merge_limit = int64_t(3) << 17;
   DBuf<StepArgs> args;
   DBuf<int2> work;
   double loop_ms = 0;

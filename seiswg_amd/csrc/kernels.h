@@ -90,6 +90,8 @@ hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStr
 // distinct from st[0]); tiles of nt_threads-12 columns; aux: also the aux
 // fields of both steps (forward recompute of the gradient)
 hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false);
+// the same for SH
+hipError_t launch_step_sh2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false);
 hipError_t launch_adj_sh(const StepArgs& a, const Tiling& t, hipStream_t s);
 hipError_t launch_adj_psv(const StepArgs& a, const Tiling& t, hipStream_t s);
 

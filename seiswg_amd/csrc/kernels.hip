@@ -1163,6 +1163,299 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   if (a.advance) advance_clock(a.clock, it, 2, a.nblocks);
 }
 
+// ------------------------------------------------- two SH steps per launch
+// Steps it and it+1 of SH in one march, pipelined as step_psv2 (stage 2
+// trails stage 1 by three rows; the state of step it+1 stays in registers
+// and LDS); every cell's arithmetic is that of step_sh.  The host passes the
+// buffer to read in st[0] and the one to write in st[1].
+template <int NT, int S, bool AUX>
+__device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
+  __shared__ float lds_s[2][S * NT];    // stage 1: szy of step it
+  __shared__ float lds_v[2][S * NT];    // stage 1: vy of step it+1
+  __shared__ float lds_s2[2][S * NT];   // stage 2: szy of step it+1
+  __shared__ float lds_v2[2][S * NT];   // stage 2: vy of step it+2
+  const TileGeom g = tile_geom<NT, S, 6>(a, logical);
+  const int t = g.t, lane = g.lane, j = g.j, nx = a.nx, i0 = g.i0, i1 = g.i1;
+  const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int64_t so = int64_t(g.shot) * a.shot_floats;
+  const uint32_t sbytes = 4u * uint32_t(a.shot_floats);
+  const auto rin = make_rsrc(a.st[0] + so, sbytes);
+  const auto rout = make_rsrc(a.st[1] + so, sbytes);
+  const uint32_t npt4 = 4u * uint32_t(a.npt);
+  const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
+  const uint32_t o_vy = 4u * a.slot_off[S_VY], o_sxy = 4u * a.slot_off[S_SXY], o_szy = 4u * a.slot_off[S_SZY];
+  const uint32_t m_rho = MF_RHO * npt4, m_mu = MF_MU * npt4, m_irho = MF_IRHO * npt4;
+  const uint32_t jc4 = g.jc4;
+  const int jc = int(jc4 >> 2);
+  const float bzb = a.bzb[jc], bzt = a.bzt[jc];
+  const float dt = a.dt;
+  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const bool fast_c = (a.fast_div & FD_CONST) != 0;
+  const bool fast_rho = (a.fast_div & FD_RHO) != 0;
+  const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
+            tp2 = g.base + min(t + 2, NT - 1);
+  const int tps = a.nstrips * a.nsegs;
+  int ip = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile]);
+  const int ie = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile + 1]);
+  int ip2 = ip;
+  int inj_row = next_row(a.inj, ip, ie);
+  int inj_row2 = inj_row;
+  int rp = 0, re = 0;
+  if (a.record) {
+    rp = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile]);
+    re = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile + 1]);
+  }
+  int rp2 = rp;
+  int rec_row = next_row(a.rec, rp, re);
+  int rec_row2 = rec_row;
+  float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
+  float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
+  const int m0 = i0 - 4;   // first velocity row of stage 1
+  auto aux_rsrc = [&](int step) {
+    return make_rsrc(
+        AUX ? a.aux + (int64_t(step % a.aux_period) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt) : a.model,
+        AUX ? uint32_t(NAUX_SH) * npt4 : 0u);
+  };
+  const auto raux1 = aux_rsrc(it), raux2 = aux_rsrc(it + 1);
+
+  // stage 1 rings: stresses of step it (rows m-2..m+1), new vy (rows m-3..m),
+  // D+z of the new vy (row m at s2, used at s0); stage 2 rings: stresses of
+  // step it+1 (rows m-5..m-2, row m-2 at s0), vy of step it+2 (rows m-6..m-3,
+  // row m-3 at s0), its D+z (row m-4 at [par], row m-5 at [par^1]); model
+  // values and taper factor of three rows back (written at s0, read at s1)
+  float sxy[4], szy[4], vy[4] = {0, 0, 0, 0}, dvydz[4] = {0, 0, 0, 0};
+  float t1xy[4] = {0, 0, 0, 0}, t1zy[4] = {0, 0, 0, 0}, u2[4] = {0, 0, 0, 0}, dz2[2] = {0, 0};
+  float drho[4] = {1, 1, 1, 1}, dirho[4] = {1, 1, 1, 1}, dmu[4] = {0, 0, 0, 0}, dbx[4] = {0, 0, 0, 0};
+  for (int q = 0; q < 3; ++q) {
+    sxy[q] = settle(bload(rin, jc4, o_sxy + g.row(m0 - 2 + q)));
+    szy[q] = settle(bload(rin, jc4, o_szy + g.row(m0 - 2 + q)));
+  }
+  sxy[3] = szy[3] = 0.0f;
+  constexpr int D = 2;
+  float pf_sxy[D], pf_szy[D], pf_vy[D], pf_rho[D], pf_irho[D], pf_mu[D];
+  float bx_next = taper_x(a.bx, m0, nx);
+  uint32_t rofs[4];
+  for (int k = D - 2; k <= D; ++k) rofs[k & 3] = g.row(m0 + k);
+  for (int q = 0; q < D; ++q) {
+    pf_sxy[q] = bload(rin, jc4, o_sxy + g.row(m0 + 1 + q));
+    pf_szy[q] = bload(rin, jc4, o_szy + g.row(m0 + 1 + q));
+    pf_vy[q] = bload(rin, jc4, o_vy + g.row(m0 + q));
+    pf_rho[q] = bload(rmod, jc4, m_rho + g.row(m0 + q));
+    pf_irho[q] = bload(rmod, jc4, m_irho + g.row(m0 + q));
+    pf_mu[q] = bload(rmod, jc4, m_mu + g.row(m0 - 2 + q));
+  }
+  lds_s[0][lane] = szy[2];
+  __syncthreads();
+
+  const bool zm_ok = g.zm_ok, zp_ok = g.zp_ok;
+  auto body = [&](auto phase, int m) __attribute__((always_inline)) {
+    constexpr int R = decltype(phase)::value;
+    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
+    constexpr int f = R % D;
+    sxy[s3] = settle(pf_sxy[f]);       // row m+1
+    szy[s3] = settle(pf_szy[f]);
+    const float vy_o = pf_vy[f], rho = pf_rho[f], irho = pf_irho[f], mu = pf_mu[f], bxm = bx_next;
+    bx_next = taper_x(a.bx, m + 1, nx);
+    {
+      constexpr int k1 = (R + D + 1) & 3, k0 = (R + D) & 3, km2 = (R + D + 2) & 3;
+      rofs[k1] = g.row(m + D + 1);
+      const uint32_t r3 = rofs[k1], r2 = rofs[k0], r0 = rofs[km2];
+      pf_sxy[f] = bload(rin, jc4, o_sxy + r3);
+      pf_szy[f] = bload(rin, jc4, o_szy + r3);
+      pf_vy[f] = bload(rin, jc4, o_vy + r2);
+      pf_rho[f] = bload(rmod, jc4, m_rho + r2);
+      pf_irho[f] = bload(rmod, jc4, m_irho + r2);
+      pf_mu[f] = bload(rmod, jc4, m_mu + r0);
+    }
+
+    // ---- stage 1: vy of step it+1 at row m (not stored)
+    {
+      const bool xm_ok = m >= 2 && m < nx - 2;
+      float qa[2];
+      {
+        const float n0 = nminus(sxy[s0], sxy[s1], sxy[s2], sxy[s3]);
+        const float n1 = nminus(lds_s[par][tm2], lds_s[par][tm1], szy[s2], lds_s[par][tp1]);
+        const float xa[2] = {keep(xm_ok, n0), keep(zm_ok, n1)};
+        div_group(qa, xa, {Dx, Dz}, fast_c);
+      }
+      float dsy = qa[0] + qa[1];
+      if (m == inj_row) {
+        for (; ip < ie; ++ip) {
+          const int4 e = a.inj[ip];
+          if (e.x != m) break;
+          if (e.y == j) dsy = dsy + a.val_y[int64_t(e.z) * a.nt_val + it];
+        }
+        inj_row = next_row(a.inj, ip, ie);
+      }
+      const float b = (bxm * bzb) * bzt;
+      float qb[1];
+      {
+        const float xb[1] = {dt * dsy};
+        div_group(qb, xb, {Div{rho, irho, true}}, fast_rho);
+      }
+      vy[s3] = b * (vy_o + qb[0]);
+      if constexpr (AUX) {
+        if (g.jst && m >= i0 && m < i1) bstore(dsy, raux1, jc4, A_DSY * npt4 + uint32_t(m) * g.rowb);
+      }
+    }
+    lds_v[par][lane] = vy[s3];
+    lds_s[par ^ 1][lane] = szy[s3];
+    __syncthreads();
+
+    // ---- stage 1: D+ of the new vy (z at row m, x at row m-2), receivers of
+    // step it, stress of step it+1 at row m-2 into the stage-2 ring and LDS
+    {
+      const int i = m - 2;
+      const bool xp_ok = i >= 1 && i < nx - 2;
+      float qc[2];
+      {
+        const float n0 = nplus(lds_v[par][tm1], vy[s3], lds_v[par][tp1], lds_v[par][tp2]);
+        const float n1 = nplus(vy[s0], vy[s1], vy[s2], vy[s3]);
+        const float xc[2] = {keep(zp_ok, n0), keep(xp_ok, n1)};
+        div_group(qc, xc, {Dz, Dx}, fast_c);
+      }
+      dvydz[s2] = qc[0];
+      if (m == rec_row) {
+        for (; rp < re; ++rp) {
+          const int4 e = a.rec[rp];
+          if (e.x != m) break;
+          if (e.y == j) {
+            float* u = uacc + e.z * N_UACC;
+            const float uy = u[U_UY] + vy[s3] * dt;
+            u[U_UY] = uy;
+            obs[(1 * a.nrec + e.z) * a.nt + it] = uy;
+          }
+        }
+        rec_row = next_row(a.rec, rp, re);
+      }
+      const float mu_dt = mu * dt;
+      const float dvydx = qc[1];
+      t1xy[s0] = sxy[s0] + mu_dt * dvydx;
+      t1zy[s0] = szy[s0] + mu_dt * dvydz[s0];
+      if constexpr (AUX) {
+        if (g.jst && i >= i0 && i < i1) {
+          bstore(dvydx, raux1, jc4, A_EXY * npt4 + uint32_t(i) * g.rowb);
+          bstore(dvydz[s0], raux1, jc4, A_EZY * npt4 + uint32_t(i) * g.rowb);
+        }
+      }
+      lds_s2[par][lane] = t1zy[s0];
+    }
+    drho[s0] = rho;
+    dirho[s0] = irho;
+    dmu[s0] = mu;
+    dbx[s0] = bxm;
+
+    // ---- stage 2: vy of step it+2 at row m-3
+    {
+      const int m2 = m - 3;
+      const bool xm_ok = m2 >= 2 && m2 < nx - 2;
+      float qa[2];
+      {
+        const float n0 = nminus(t1xy[s1], t1xy[s2], t1xy[s3], t1xy[s0]);
+        const float n1 = nminus(lds_s2[par ^ 1][tm2], lds_s2[par ^ 1][tm1], t1zy[s3], lds_s2[par ^ 1][tp1]);
+        const float xa[2] = {keep(xm_ok, n0), keep(zm_ok, n1)};
+        div_group(qa, xa, {Dx, Dz}, fast_c);
+      }
+      float dsy = qa[0] + qa[1];
+      if (m2 == inj_row2) {
+        for (; ip2 < ie; ++ip2) {
+          const int4 e = a.inj[ip2];
+          if (e.x != m2) break;
+          if (e.y == j) dsy = dsy + a.val_y[int64_t(e.z) * a.nt_val + it + 1];
+        }
+        inj_row2 = next_row(a.inj, ip2, ie);
+      }
+      const float b = (dbx[s1] * bzb) * bzt;
+      float qb[1];
+      {
+        const float xb[1] = {dt * dsy};
+        div_group(qb, xb, {Div{drho[s1], dirho[s1], true}}, fast_rho);
+      }
+      const float vyn = b * (vy[s0] + qb[0]);
+      if (g.jst && m2 >= i0 && m2 < i1) {
+        bstore(vyn, rout, jc4, o_vy + uint32_t(m2) * g.rowb);
+        if constexpr (AUX) bstore(dsy, raux2, jc4, A_DSY * npt4 + uint32_t(m2) * g.rowb);
+      }
+      u2[s0] = vyn;
+      if (m2 == rec_row2) {
+        for (; rp2 < re; ++rp2) {
+          const int4 e = a.rec[rp2];
+          if (e.x != m2) break;
+          if (e.y == j) {
+            float* u = uacc + e.z * N_UACC;
+            const float uy = u[U_UY] + vyn * dt;
+            u[U_UY] = uy;
+            obs[(1 * a.nrec + e.z) * a.nt + it + 1] = uy;
+          }
+        }
+        rec_row2 = next_row(a.rec, rp2, re);
+      }
+    }
+
+    // ---- stage 2: D+z of vy row m-4 (LDS of the previous iteration), D+x at
+    // row m-5, stress of step it+2 at row m-5
+    {
+      const int i2 = m - 5;
+      const bool xp_ok = i2 >= 1 && i2 < nx - 2;
+      float qc[2];
+      {
+        const float n0 = nplus(lds_v2[par ^ 1][tm1], u2[s3], lds_v2[par ^ 1][tp1], lds_v2[par ^ 1][tp2]);
+        const float n1 = nplus(u2[s1], u2[s2], u2[s3], u2[s0]);
+        const float xc[2] = {keep(zp_ok, n0), keep(xp_ok, n1)};
+        div_group(qc, xc, {Dz, Dx}, fast_c);
+      }
+      dz2[par] = qc[0];
+      const float mu_dt = dmu[s1] * dt;
+      const float dvydx = qc[1];
+      const float nsxy = t1xy[s1] + mu_dt * dvydx;
+      const float nszy = t1zy[s1] + mu_dt * dz2[par ^ 1];
+      if (g.jst && i2 >= i0 && i2 < i1) {
+        bstore(nsxy, rout, jc4, o_sxy + uint32_t(i2) * g.rowb);
+        bstore(nszy, rout, jc4, o_szy + uint32_t(i2) * g.rowb);
+        if constexpr (AUX) {
+          bstore(dvydx, raux2, jc4, A_EXY * npt4 + uint32_t(i2) * g.rowb);
+          bstore(dz2[par ^ 1], raux2, jc4, A_EZY * npt4 + uint32_t(i2) * g.rowb);
+        }
+      }
+    }
+    lds_v2[par][lane] = u2[s0];
+  };
+
+  const int mlast = i1 + 4;
+  int m = m0;
+  body(Phase<0>{}, m);
+  if (++m <= mlast) {
+    body(Phase<1>{}, m);
+    if (++m <= mlast) {
+      body(Phase<2>{}, m);
+      if (++m <= mlast) {
+        body(Phase<3>{}, m);
+        if (++m <= mlast) {
+          for (;;) {
+            body(Phase<4>{}, m);
+            if (++m > mlast) break;
+            body(Phase<5>{}, m);
+            if (++m > mlast) break;
+            body(Phase<6>{}, m);
+            if (++m > mlast) break;
+            body(Phase<7>{}, m);
+            if (++m > mlast) break;
+            body(Phase<0>{}, m);
+            if (++m > mlast) break;
+            body(Phase<1>{}, m);
+            if (++m > mlast) break;
+            body(Phase<2>{}, m);
+            if (++m > mlast) break;
+            body(Phase<3>{}, m);
+            if (++m > mlast) break;
+          }
+        }
+      }
+    }
+  }
+  if (a.advance) advance_clock(a.clock, it, 2, a.nblocks);
+}
+
 // ------------------------------------------------------ adjoint time steps
 // Exact transpose of one forward step (SH or P-SV), run in reverse time.
 // With the forward step
@@ -1563,6 +1856,10 @@ __global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(4))) voi
     const StepArgs a) {
   step_psv2<NT, S, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
+template <int NT, int S, bool AUX>
+__global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(4))) void k_step_sh2(const StepArgs a) {
+  step_sh2<NT, S, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
 template <int NT, int S>
 __global__ __launch_bounds__(NT * S) void k_step_spin(const StepArgs a) {
   step_psv<NT, S, true, false>(a, xcd_remap(blockIdx.x, gridDim.x));
@@ -1852,27 +2149,36 @@ hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, boo
 }
 
 template <int NT, int S>
-hipError_t launch_psv2_ns(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
+hipError_t launch_two_ns(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
   if (!args_match(a, t, S)) return hipErrorInvalidValue;
   const int nblk = a.nshots / S * t.nstrips * t.nsegs;
-  if (aux) hipLaunchKernelGGL((k_step_psv2<NT, S, true>), dim3(nblk), dim3(NT * S), 0, s, a);
-  else hipLaunchKernelGGL((k_step_psv2<NT, S, false>), dim3(nblk), dim3(NT * S), 0, s, a);
+  const dim3 grid(nblk), block(NT * S);
+  if (sh && aux) hipLaunchKernelGGL((k_step_sh2<NT, S, true>), grid, block, 0, s, a);
+  else if (sh) hipLaunchKernelGGL((k_step_sh2<NT, S, false>), grid, block, 0, s, a);
+  else if (aux) hipLaunchKernelGGL((k_step_psv2<NT, S, true>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((k_step_psv2<NT, S, false>), grid, block, 0, s, a);
   return hipGetLastError();
 }
-hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
+hipError_t launch_two(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
   if (t.group < 1 || a.nshots % t.group != 0) return hipErrorInvalidValue;
   switch (t.nt_threads * 8 + t.group) {
-    case 64 * 8 + 1: return launch_psv2_ns<64, 1>(a, t, s, aux);
-    case 128 * 8 + 1: return launch_psv2_ns<128, 1>(a, t, s, aux);
-    case 128 * 8 + 2: return launch_psv2_ns<128, 2>(a, t, s, aux);
-    case 128 * 8 + 4: return launch_psv2_ns<128, 4>(a, t, s, aux);
-    case 256 * 8 + 1: return launch_psv2_ns<256, 1>(a, t, s, aux);
-    case 256 * 8 + 2: return launch_psv2_ns<256, 2>(a, t, s, aux);
-    case 256 * 8 + 4: return launch_psv2_ns<256, 4>(a, t, s, aux);
-    case 512 * 8 + 1: return launch_psv2_ns<512, 1>(a, t, s, aux);
-    case 512 * 8 + 2: return launch_psv2_ns<512, 2>(a, t, s, aux);
+    case 64 * 8 + 1: return launch_two_ns<64, 1>(sh, a, t, s, aux);
+    case 128 * 8 + 1: return launch_two_ns<128, 1>(sh, a, t, s, aux);
+    case 128 * 8 + 2: return launch_two_ns<128, 2>(sh, a, t, s, aux);
+    case 128 * 8 + 4: return launch_two_ns<128, 4>(sh, a, t, s, aux);
+    case 256 * 8 + 1: return launch_two_ns<256, 1>(sh, a, t, s, aux);
+    case 256 * 8 + 2: return launch_two_ns<256, 2>(sh, a, t, s, aux);
+    case 256 * 8 + 4: return launch_two_ns<256, 4>(sh, a, t, s, aux);
+    case 512 * 8 + 1: return launch_two_ns<512, 1>(sh, a, t, s, aux);
+    case 512 * 8 + 2: return launch_two_ns<512, 2>(sh, a, t, s, aux);
   }
   return hipErrorInvalidValue;
+}
+hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
+  return launch_two(false, a, t, s, aux);
+}
+hipError_t launch_step_sh2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
+  return launch_two(true, a, t, s, aux);
 }
 bool tiling2_supported(int nt) { return nt == 64 || nt == 128 || nt == 256 || nt == 512; }
 bool group2_supported(int nt, int group) {

@@ -183,17 +183,20 @@ def test_grouped_shots_equal_single_runs(seiswg, phys, threads, group):
 
 @pytest.mark.parametrize("threads,seglen,group", [(64, 5, 1), (128, 16, 2), (256, 9, 4), (128, 3, 4), (64, 200, 1)])
 @pytest.mark.parametrize("nt", [11, 37])
-def test_two_step_launches_equal_one_step_launches(seiswg, threads, seglen, group, nt):
-    """P-SV forward with steps in pairs of two-step launches (the default)
-    against one step per launch: state and traces bit for bit, for short
-    segments (recomputed rows at both ends), several shots per workgroup,
-    sources and receivers on edges, halos and duplicated cells, an odd step
-    count (single steps after the pairs) and, with 37 steps, captured graphs."""
+@pytest.mark.parametrize("phys", ["psv", "sh"])
+def test_two_step_launches_equal_one_step_launches(seiswg, threads, seglen, group, nt, phys):
+    """P-SV and SH forward with steps in pairs of two-step launches (the
+    default) against one step per launch: state and traces bit for bit, for
+    short segments (recomputed rows at both ends), several shots per
+    workgroup, sources and receivers on edges, halos and duplicated cells, an
+    odd step count (single steps after the pairs) and, with 37 steps,
+    captured graphs."""
     nx, nz = 61, 150
-    p, stf = random_problem(nx, nz, nt, False, True, False, seed=29)
-    print(f"case threads={threads} seglen={seglen} group={group} nt={nt}", flush=True)
-    comps = [O.OBS_X, O.OBS_Z]
-    fields = STATE["psv"]
+    sh = phys == "sh"
+    p, stf = random_problem(nx, nz, nt, sh, not sh, False, seed=29)
+    print(f"case phys={phys} threads={threads} seglen={seglen} group={group} nt={nt}", flush=True)
+    comps = [O.OBS_Y] if sh else [O.OBS_X, O.OBS_Z]
+    fields = STATE[phys]
     runs = [0, 1, 2, 3]
     out = {}
     for spl in (1, 2):
@@ -214,7 +217,7 @@ def test_two_step_launches_equal_one_step_launches(seiswg, threads, seglen, grou
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (q, n)
     o = O.Oracle(p, 4, stf=stf.astype(np.float64))
     obs, _ = o.forward(1, 0)
-    compare(out[2][0][1][0], obs[O.OBS_X], "two-step shot 1 ux")
+    compare(out[2][0][1][0], obs[comps[0]], "two-step shot 1 trace")
 
 
 @pytest.mark.parametrize("tiling2", [(512, 7, 1), (512, 13, 2), (256, 20, 4), (128, 5, 2), (64, 9, 1)])

@@ -182,6 +182,9 @@ struct Config {
   bool combine = false;
   uint32_t save_snapshot = 0, adj_interval = 10;
   float smooth = 5.0f;
+  // workflow = inversion (section [inversion]; no reference counterpart)
+  uint32_t inv_niter = 3, inv_tries = 4, inv_ckpt = 50;
+  float inv_step = 0.02f;
   fs::path output, output_traces, model_true, model_init, sources, stations, traces;
   bool has_output_traces = false, has_model_true = false, has_model_init = false, has_traces = false;
 };
@@ -236,6 +239,14 @@ Config load_config(const fs::path& path) {
   c.adj_interval = get_u32_or("adj_interval", 10);
   c.smooth = get_f32_or("smooth", 5.0f);
   std::string v;
+  {
+    uint32_t u;
+    float f;
+    if (ini.get("inversion", "niter", v) && parse_u32(trim(v), u)) c.inv_niter = u;
+    if (ini.get("inversion", "linesearch", v) && parse_u32(trim(v), u)) c.inv_tries = std::max(u, 1u);
+    if (ini.get("inversion", "checkpoint", v) && parse_u32(trim(v), u)) c.inv_ckpt = std::max(u, 1u);
+    if (ini.get("inversion", "step", v) && parse_f32(trim(v), f)) c.inv_step = f;
+  }
   c.output = resolve(required("path", "output"));
   if (ini.get("path", "output_traces", v)) { c.output_traces = resolve(v); c.has_output_traces = true; }
   if (ini.get("path", "model_true", v)) { c.model_true = resolve(v); c.has_model_true = true; }
@@ -349,6 +360,8 @@ struct Problem {
   int32_t nx = 0, nz = 0;
   float dx = 0, dz = 0;
   int64_t npt = 0;
+  std::vector<std::vector<float>> model;   // the loaded model fields (rho, vp, vs / Lame set)
+  std::vector<float> x, z;                 // coordinates of the model files
   std::vector<uint32_t> src_k, rec_k;
   sw_ctx* ctx = nullptr;
   ~Problem() { sw_destroy(ctx); }
@@ -421,6 +434,9 @@ void solver_new(Problem& P) {
   std::vector<const float*> fp;
   for (auto& f : fields) fp.push_back(f.data());
   check(sw_set_model(P.ctx, spin ? SW_MODEL_LAME : SW_MODEL_VPVS, fp.data()));
+  P.model = std::move(fields);
+  P.x = std::move(x);
+  P.z = std::move(z);
   std::vector<int32_t> si(P.src_k.begin(), P.src_k.end()), ri(P.rec_k.begin(), P.rec_k.end());
   check(sw_set_sources(P.ctx, nsrc, si.data(), stf.data()));
   check(sw_set_receivers(P.ctx, int32_t(ri.size()), ri.data()));
@@ -545,6 +561,153 @@ void run_adjoint(Problem& P) {
   log_info("Adjoint: gradient saved to " + kp.string());
 }
 
+// workflow = inversion.  The reference lists it as planned (README.md:61) and
+// stops on unknown workflows (main.rs:94); this is a steepest-descent loop on
+// top of the batched exact gradient: per iteration the misfit gradient of all
+// runs (sw_gradient, checkpointed), Gaussian smoothing of K_mu / K_lambda as
+// in the adjoint workflow, the chain rule through vps2lm to vs (and vp for
+// P-SV; rho stays), a step of at most `step` times the largest velocity, a
+// backtracking line search (halving, `linesearch` trials) on the misfit of a
+// forward run, and the accepted model written in the proc000000_*.bin format
+// to output/iter_NNN.  The misfit is that of the gradient: the sum over runs,
+// components, receivers and samples of (w_n (d - obs))^2, w the cosine taper
+// of solver.rs:441-455.
+double run_misfit(Problem& P, const std::vector<int32_t>& runs, const std::vector<int>& comps,
+                  const std::vector<float>& ref) {
+  const Config& c = P.cfg;
+  const int nrec = int(P.rec_k.size()), nt = int(c.nt);
+  check(sw_forward_batch(P.ctx, int32_t(runs.size()), runs.data(), 0, nullptr, nullptr));
+  const float PI_F = 3.14159265358979323846f, dt = c.dt;
+  const float t_end = float(nt - 1) * dt, tw = t_end / 10.0f, t_min = tw, t_max = t_end - tw;
+  std::vector<float> w(static_cast<size_t>(nt));
+  for (int n = 0; n < nt; ++n) {
+    const float t = float(n) * dt;
+    w[n] = t <= t_min ? 0.5f + 0.5f * std::cos(PI_F * (t_min - t) / tw)
+           : t >= t_max ? 0.5f + 0.5f * std::cos(PI_F * (t_max - t) / tw) : 1.0f;
+  }
+  std::vector<float> d(size_t(nrec) * nt);
+  double J = 0.0;
+  const size_t per_comp = size_t(nrec) * nt;
+  for (size_t r = 0; r < runs.size(); ++r)
+    for (size_t k = 0; k < comps.size(); ++k) {
+      check(sw_read_obs(P.ctx, int32_t(r), comps[k], d.data()));
+      const float* o = ref.data() + (r * comps.size() + k) * per_comp;
+      for (int q = 0; q < nrec; ++q)
+        for (int n = 0; n < nt; ++n) {
+          const double e = double(w[n] * (d[size_t(q) * nt + n] - o[size_t(q) * nt + n]));
+          J += e * e;
+        }
+    }
+  return J;
+}
+
+void write_model_dir(const Problem& P, const fs::path& dir) {
+  std::error_code ec;
+  fs::create_directories(dir, ec);
+  if (ec) throw Error(fmt("Cannot create %s: %s", dir.string().c_str(), ec.message().c_str()));
+  const int32_t hdr = int32_t(P.npt);
+  const char* names[] = {"rho", "vp", "vs"};
+  for (int q = 0; q < 3; ++q)
+    write_raw(dir / (std::string("proc000000_") + names[q] + ".bin"), P.model[q].data(), size_t(P.npt), &hdr);
+  write_raw(dir / "proc000000_x.bin", P.x.data(), size_t(P.npt), &hdr);
+  write_raw(dir / "proc000000_z.bin", P.z.data(), size_t(P.npt), &hdr);
+}
+
+void run_inversion(Problem& P) {
+  const Config& c = P.cfg;
+  if (c.spin || c.sh == c.psv) throw Error("the inversion needs exactly one of sh / psv, without spin");
+  if (!c.has_traces) throw Error("config [path] traces is required for the inversion workflow");
+  const int nsrc = int(P.src_k.size()), nrec = int(P.rec_k.size()), nt = int(c.nt);
+  const size_t npt = size_t(P.npt);
+  std::error_code ec;
+  fs::create_directories(c.output, ec);
+  if (ec) throw Error(fmt("Cannot create %s: %s", c.output.string().c_str(), ec.message().c_str()));
+  std::vector<int32_t> runs;
+  if (c.combine) runs.push_back(-1);
+  else for (int s = 0; s < nsrc; ++s) runs.push_back(s);
+  const std::vector<int> comps = c.sh ? std::vector<int>{SW_OBS_Y} : std::vector<int>{SW_OBS_X, SW_OBS_Z};
+  const char* comp_names[] = {"ux", "uy", "uz"};
+  const size_t nobs = size_t(nrec) * nt;
+  std::vector<float> ref(runs.size() * comps.size() * nobs, 0.0f);
+  for (size_t r = 0; r < runs.size(); ++r)
+    for (size_t k = 0; k < comps.size(); ++k) {
+      const int idx = c.combine ? 0 : int(r);
+      fs::path p = c.traces / fmt("%s_%06d.npy", comp_names[comps[k] == SW_OBS_X ? 0 : comps[k] == SW_OBS_Y ? 1 : 2],
+                                  idx);
+      std::vector<float> v = load_npy(p);
+      if (v.size() < nobs)
+        throw Error(fmt("observed traces %s hold %zu values, expected %zu", p.string().c_str(), v.size(), nobs));
+      std::copy(v.begin(), v.begin() + nobs, ref.begin() + (r * comps.size() + k) * nobs);
+    }
+  std::vector<float>& rho = P.model[0];
+  std::vector<float>& vp = P.model[1];
+  std::vector<float>& vs = P.model[2];
+  auto set_model = [&](const std::vector<float>& a, const std::vector<float>& b) {
+    const float* f[3] = {rho.data(), a.data(), b.data()};
+    check(sw_set_model(P.ctx, SW_MODEL_VPVS, f));
+  };
+  std::ofstream mlog(c.output / "misfit.txt", std::ios::trunc);
+  double J0 = run_misfit(P, runs, comps, ref);
+  log_info(fmt("Inversion: %zu run(s), initial misfit %.6e", runs.size(), J0));
+  mlog << 0 << " " << fmt("%.9e", J0) << "\n";
+  std::vector<double> mis(runs.size());
+  std::vector<float> kmu(npt), klam(npt), gvs(npt), gvp(npt), vs1(npt), vp1(npt);
+  for (uint32_t iter = 1; iter <= c.inv_niter; ++iter) {
+    check(sw_gradient(P.ctx, int32_t(runs.size()), runs.data(), ref.data(), int32_t(c.inv_ckpt), 0, mis.data()));
+    check(sw_smooth(P.ctx, SW_F_K_MU, c.smooth));
+    check(sw_read_field(P.ctx, 0, SW_F_K_MU, kmu.data()));
+    if (c.psv) {
+      check(sw_smooth(P.ctx, SW_F_K_LAM, c.smooth));
+      check(sw_read_field(P.ctx, 0, SW_F_K_LAM, klam.data()));
+    } else {
+      std::fill(klam.begin(), klam.end(), 0.0f);
+    }
+    // chain rule through mu = rho vs^2, lambda = rho (vp^2 - 2 vs^2) (0 where vp <= vs)
+    float gmax_s = 0.0f, gmax_p = 0.0f, vs_max = 0.0f, vp_max = 0.0f;
+    for (size_t k = 0; k < npt; ++k) {
+      const bool lam_on = vp[k] > vs[k];
+      gvs[k] = kmu[k] * 2.0f * rho[k] * vs[k] + (lam_on ? klam[k] * (-4.0f) * rho[k] * vs[k] : 0.0f);
+      gvp[k] = lam_on ? klam[k] * 2.0f * rho[k] * vp[k] : 0.0f;
+      gmax_s = std::max(gmax_s, std::fabs(gvs[k]));
+      gmax_p = std::max(gmax_p, std::fabs(gvp[k]));
+      vs_max = std::max(vs_max, vs[k]);
+      vp_max = std::max(vp_max, vp[k]);
+    }
+    if (!(gmax_s > 0.0f) && !(gmax_p > 0.0f)) {
+      log_info("Inversion: zero gradient; stopping");
+      break;
+    }
+    float step = c.inv_step;
+    bool accepted = false;
+    double J1 = J0;
+    for (uint32_t tr = 0; tr < c.inv_tries && !accepted; ++tr, step *= 0.5f) {
+      const float as = gmax_s > 0.0f ? step * vs_max / gmax_s : 0.0f;
+      const float ap = (c.psv && gmax_p > 0.0f) ? step * vp_max / gmax_p : 0.0f;
+      for (size_t k = 0; k < npt; ++k) {
+        vs1[k] = std::max(vs[k] - as * gvs[k], 0.0f);
+        vp1[k] = c.psv ? std::max(vp[k] - ap * gvp[k], 0.0f) : vp[k];
+      }
+      set_model(vp1, vs1);
+      J1 = run_misfit(P, runs, comps, ref);
+      log_info(fmt("Inversion: iteration %u, step %.4g: misfit %.6e (from %.6e)", iter, step, J1, J0));
+      accepted = J1 < J0;
+    }
+    if (!accepted) {
+      set_model(vp, vs);
+      log_info("Inversion: no step lowers the misfit; stopping");
+      break;
+    }
+    vs.swap(vs1);
+    vp.swap(vp1);
+    J0 = J1;
+    write_model_dir(P, c.output / fmt("iter_%03u", iter));
+    mlog << iter << " " << fmt("%.9e", J0) << "\n";
+    mlog.flush();
+    log_info(fmt("Inversion: iteration %u accepted, misfit %.6e; model written to %s", iter, J0,
+                 (c.output / fmt("iter_%03u", iter)).string().c_str()));
+  }
+}
+
 }  // namespace
 
 namespace sw {
@@ -631,6 +794,7 @@ SW_API int seiswg_run(const char* config_path) {
     }
     if (P.cfg.workflow == "forward") run_forward(P);
     else if (P.cfg.workflow == "adjoint") run_adjoint(P);
+    else if (P.cfg.workflow == "inversion") run_inversion(P);
     else throw Error(fmt("Workflow '%s' is not yet implemented in seiswg", P.cfg.workflow.c_str()));
     log_info("Done.");
     return 0;

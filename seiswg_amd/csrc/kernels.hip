@@ -1127,7 +1127,9 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
     lds_v2[par][1][lane] = u2z[s0];
   };
 
-  // as in step_psv: four peeled rows, then 8 phases per loop trip
+  // four peeled rows, then 4 phases per loop trip (with a 1-deep FIFO every
+  // ring repeats with period 4; half the code of an 8-phase trip)
+  static_assert(D == 1, "the 4-phase loop assumes a 1-deep prefetch");
   const int mlast = i1 + 4;
   int m = m0;
   body(Phase<0>{}, m);
@@ -1146,14 +1148,6 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
             body(Phase<6>{}, m);
             if (++m > mlast) break;
             body(Phase<7>{}, m);
-            if (++m > mlast) break;
-            body(Phase<0>{}, m);
-            if (++m > mlast) break;
-            body(Phase<1>{}, m);
-            if (++m > mlast) break;
-            body(Phase<2>{}, m);
-            if (++m > mlast) break;
-            body(Phase<3>{}, m);
             if (++m > mlast) break;
           }
         }

@@ -31,8 +31,8 @@ METRIC = "cell-updates/s + HBM GB/s % peak; batched gradient shots/s vs plain"
 
 
 def build_context(seiswg, synth, nt):
-    c = synth.C2
-    model, geom = synth.c2_problem()
+    c = synth.PSV_BATCH
+    model, geom = synth.psv_batch_problem()
     ctx = seiswg.Context(c["nx"], c["nz"], nt, c["h"], c["h"], c["dt"], sh=False, psv=True, spin=False,
                          abs_left=True, abs_right=True, abs_bottom=True, abs_top=False, abs_width=c["abs_width"],
                          abs_alpha=0.015)
@@ -52,7 +52,7 @@ def cpu_baseline(synth, model, geom, src_idx, rec_idx, budget_s=12.0):
     loop per WGSL kernel, OpenMP over cells), one shot of the same workload,
     timed for about budget_s seconds."""
     from oracle import oracle as O
-    c = synth.C2
+    c = synth.PSV_BATCH
     p = O.Problem(nx=c["nx"], nz=c["nz"], nt=4000, dx=c["h"], dz=c["h"], dt=c["dt"], sh=False, psv=True,
                   spin=False, abs_left=True, abs_right=True, abs_bottom=True, abs_top=False,
                   abs_width=c["abs_width"], abs_alpha=0.015, model=model, src_rows=geom.src_rows(),
@@ -83,8 +83,8 @@ def gradient_bench(seiswg, synth, nt_override=0):
     checkpoint every 50 steps): shots/s with all shots batched per launch vs
     the plain one-shot-per-launch path on the same GPU, same kernels, same
     results (bit-identical kernels are checked)."""
-    c = synth.C3
-    init, true, geom = synth.c3_problem()
+    c = synth.PSV_GRADIENT
+    init, true, geom = synth.gradient_problem()
     nt = nt_override or c["nt"]
 
     def ctx_for(model):
@@ -121,14 +121,14 @@ def gradient_bench(seiswg, synth, nt_override=0):
 
 
 def small_config_bench(seiswg, synth, which, with_cpu=True):
-    """One forward run of the plain SH configuration (C0: 200x200, 2000
-    steps, launch-bound) or the P-SV + spin configuration (C1: 500x500,
-    3000 steps, with a stable micro-inertia, see synth.c1_problem): GPU
+    """One forward run of the plain SH configuration (200x200, 2000
+    steps, launch-bound) or the P-SV + spin configuration (500x500,
+    3000 steps, with a stable micro-inertia, see synth.spin_forward_problem): GPU
     wall time of the whole time loop after a warm-up run, and the oracle's
     FP32 CPU restatement of the same run on the host cores."""
     abi = seiswg.abi
-    spin = which == "c1"
-    q = synth.c1_problem() if spin else synth.c0_problem()
+    spin = which == "spin"
+    q = synth.spin_forward_problem() if spin else synth.sh_forward_problem()
     nx, nz, h, nt, dt, g = q["nx"], q["nz"], q["h"], q["nt"], q["dt"], q["geom"]
     ctx = seiswg.Context(nx, nz, nt, h, h, dt, sh=not spin, psv=spin, spin=spin, abs_left=True, abs_right=True,
                          abs_bottom=True, abs_top=True, abs_width=q["abs_width"], abs_alpha=0.015)
@@ -172,20 +172,20 @@ def small_config_bench(seiswg, synth, which, with_cpu=True):
 def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
     """Uneven batch of 48 independent SH / P-SV gradient problems (grids
     384..3072 x 384..1536, 4-16 shots, 100-300 receivers, nt scaled from
-    1500..4000 by nt_scale, checkpoint every 50; `spec` = synth.C4_SMALL: grids
+    1500..4000 by nt_scale, checkpoint every 50; `spec` = synth.UNEVEN_SMALL: grids
     128..448, 2-6 shots): total shots/s through sw_multi_gradient (problems
     below its merge limit share merged launches, the others run on their own
     launches; in waves that fit budget_gb of device memory) vs one problem per
     launch (sw_gradient per problem, its shots batched)."""
     import numpy as np
     abi = seiswg.abi
-    spec = spec or synth.C4
-    tag = "uneven batch" if spec is synth.C4 else "small uneven batch"
+    spec = spec or synth.UNEVEN
+    tag = "uneven batch" if spec is synth.UNEVEN else "small uneven batch"
 
     def progress(msg):
         print(f"[{tag}] {msg}", file=sys.stderr, flush=True)
 
-    probs = synth.c4_problems(nt_scale, spec=spec)
+    probs = synth.uneven_problems(nt_scale, spec=spec)
     progress(f"{len(probs)} problems generated")
 
     def make_ctx(p, model):
@@ -215,15 +215,15 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
     def mem_bytes(p, n):
         nslot = 3 if p["kind"] == "sh" else 5
         npt = p["nx"] * p["nz"]
-        nck = -(-p["nt"] // synth.C4["ckpt"])
-        return 4.0 * npt * n * (4 * nslot + (nck + 1) * nslot + synth.C4["ckpt"] * nslot + 3) + 4.0 * n * 5 * 300 * p["nt"]
+        nck = -(-p["nt"] // synth.UNEVEN["ckpt"])
+        return 4.0 * npt * n * (4 * nslot + (nck + 1) * nslot + synth.UNEVEN["ckpt"] * nslot + 3) + 4.0 * n * 5 * 300 * p["nt"]
 
     total_shots = sum(len(x) for x in shots)
     t_plain, ref0 = 0.0, None
     for q, (p, o, sh) in enumerate(zip(probs, obs, shots)):
         c = make_ctx(p, p["init"])
         t0 = time.perf_counter()
-        r = c.gradient(sh, o, ckpt_every=synth.C4["ckpt"])
+        r = c.gradient(sh, o, ckpt_every=synth.UNEVEN["ckpt"])
         t_plain += time.perf_counter() - t0
         if q == 0:
             ref0 = r
@@ -244,7 +244,7 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
         ctxs = [make_ctx(probs[q], probs[q]["init"]) for q in w]
         m = seiswg.Multi(ctxs)
         t0 = time.perf_counter()
-        res = m.gradient([shots[q] for q in w], [obs[q] for q in w], ckpt_every=synth.C4["ckpt"])
+        res = m.gradient([shots[q] for q in w], [obs[q] for q in w], ckpt_every=synth.UNEVEN["ckpt"])
         t_batched += time.perf_counter() - t0
         if w[0] == 0:
             got0 = res[0]
@@ -257,7 +257,7 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
     merged = sum(1 for p, sh in zip(probs, shots) if len(sh) * p["nx"] * p["nz"] < (3 << 17))
     return {"workload": f"{tag} of {len(probs)} SH/P-SV gradient problems (grids {spec['nx'][0]}..{spec['nx'][1]} x "
                         f"{spec['nz'][0]}..{spec['nz'][1]}, {spec['shots'][0]}-{spec['shots'][1]} shots, "
-                        f"nt = {nt_scale} x 1500..4000, checkpoint every {synth.C4['ckpt']})",
+                        f"nt = {nt_scale} x 1500..4000, checkpoint every {synth.UNEVEN['ckpt']})",
             "problems": len(probs), "shots": total_shots, "waves": len(waves),
             "problems_merged": merged, "problems_own_launches": len(probs) - merged,
             "shots_per_s_batched": total_shots / t_batched, "shots_per_s_plain": total_shots / t_plain,
@@ -299,9 +299,9 @@ def main():
     if a.gradient_only:
         print(json.dumps({"gradient": gradient_bench(seiswg, synth, a.gradient_nt)}), flush=True)
         return 0
-    nt = max(synth.C2["nt"], a.warmup + a.steps)
+    nt = max(synth.PSV_BATCH["nt"], a.warmup + a.steps)
     ctx, model, geom, src_idx, rec_idx = build_context(seiswg, synth, nt)
-    nshots = synth.C2["nshots"]
+    nshots = synth.PSV_BATCH["nshots"]
     if a.steps_per_launch:
         ctx.set_steps_per_launch(a.steps_per_launch)
     ctx.prepare(list(range(nshots)))
@@ -315,7 +315,7 @@ def main():
     ctx.advance(a.steps)          # returns after the stream has drained
     wall = time.perf_counter() - t0
     tm = ctx.timing()
-    cells = synth.C2["nx"] * synth.C2["nz"]
+    cells = synth.PSV_BATCH["nx"] * synth.PSV_BATCH["nz"]
     cu = cells * nshots * a.steps
     avg_launch_s = tm.loop_ms * 1e-3 / max(tm.step_launches, 1)
     steps_per_launch = tm.steps / max(tm.step_launches, 1)
@@ -324,7 +324,7 @@ def main():
     algo_bytes = steps_per_launch * nshots * cells * 4.0 * (2 * 5 + 3.0 / nshots)
     achieved = algo_bytes / avg_launch_s / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "psv_c2_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "psv_2000x1000_traffic.json")
     if os.path.exists(tf):
         prof = json.load(open(tf))
         if steps_per_launch < 1.5:
@@ -348,11 +348,11 @@ def main():
     if not a.no_gradient:
         out["gradient"] = gradient_bench(seiswg, synth, a.gradient_nt)
     if not a.no_small:
-        out["c0_sh_forward"] = small_config_bench(seiswg, synth, "c0", not a.no_cpu_baseline)
-        out["c1_spin_forward"] = small_config_bench(seiswg, synth, "c1", not a.no_cpu_baseline)
+        out["sh_forward_200x200"] = small_config_bench(seiswg, synth, "sh", not a.no_cpu_baseline)
+        out["spin_forward_500x500"] = small_config_bench(seiswg, synth, "spin", not a.no_cpu_baseline)
     if a.uneven:
         out["uneven_batch"] = uneven_bench(seiswg, synth, a.uneven_nt_scale, a.uneven_budget_gb)
-        out["small_uneven_batch"] = uneven_bench(seiswg, synth, 0.1, spec=synth.C4_SMALL)
+        out["small_uneven_batch"] = uneven_bench(seiswg, synth, 0.1, spec=synth.UNEVEN_SMALL)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(synth, model, geom, src_idx, rec_idx)
     print(json.dumps(out), flush=True)

@@ -99,31 +99,31 @@ def write_tables(d: str, geom: Geometry):
             f.write(f"{x!r} {z!r}\n")
 
 
-# BASELINE workloads ---------------------------------------------------------
-C2 = dict(nx=2000, nz=1000, h=10.0, nshots=64, nrec=400, nt=4000, dt=1e-3, f0=8.0, t0=0.15, ang=90.0,
+# benchmark workloads -------------------------------------------------------
+PSV_BATCH = dict(nx=2000, nz=1000, h=10.0, nshots=64, nrec=400, nt=4000, dt=1e-3, f0=8.0, t0=0.15, ang=90.0,
           abs_width=40, seed=2)
 
 
-def c2_problem():
+def psv_batch_problem():
     """Batched multi-shot PSV forward: 2000x1000, 64 shots at 50 m depth,
     400 receivers at 10 m (ix = 5r + 2), taper left/right/bottom width 40."""
-    c = C2
+    c = PSV_BATCH
     model = layered_random_model(c["nx"], c["nz"], c["h"], c["seed"])
     geom = line_geometry(c["nx"], c["h"], c["nshots"], c["nrec"], rec_first=2, rec_step=5, f0=c["f0"], t0=c["t0"],
                          ang=c["ang"])
     return model, geom
 
 
-C3 = dict(nx=1000, nz=600, h=10.0, nshots=16, nrec=200, nt=3000, dt=1e-3, f0=8.0, t0=0.15, ang=90.0,
+PSV_GRADIENT = dict(nx=1000, nz=600, h=10.0, nshots=16, nrec=200, nt=3000, dt=1e-3, f0=8.0, t0=0.15, ang=90.0,
           abs_width=40, seed=2, anomaly_seed=3, ckpt=50)
 
 
-def c3_problem():
-    """Checkpointed P-SV gradient workload: 1000x600 initial model (the C2
+def gradient_problem():
+    """Checkpointed P-SV gradient workload: 1000x600 initial model (the PSV_BATCH
     recipe, seed 2), true model with three +/-8% vs Gaussian anomalies of
     radius 150 m (seed 3); 16 shots at 50 m depth and 200 receivers at 10 m,
     evenly spaced."""
-    c = C3
+    c = PSV_GRADIENT
     init = layered_random_model(c["nx"], c["nz"], c["h"], c["seed"])
     true = gaussian_anomalies(init, c["nx"], c["nz"], c["h"], c["anomaly_seed"])
     step = (c["nx"] - 4) // c["nrec"]
@@ -132,26 +132,26 @@ def c3_problem():
     return init, true, geom
 
 
-C4 = dict(nproblems=48, seed=4, nx=(384, 3072), nz=(384, 1536), shots=(4, 16), nrec=(100, 300), nt=(1500, 4000),
+UNEVEN = dict(nproblems=48, seed=4, nx=(384, 3072), nz=(384, 1536), shots=(4, 16), nrec=(100, 300), nt=(1500, 4000),
           f0=(5.0, 15.0), h=10.0, cfl=0.5, abs_width=40, ckpt=50)
 
 
 # a batch of small uneven problems (same recipe, grids 128-448): the sizes at
 # which merging problems into shared launches pays
-C4_SMALL = dict(C4, seed=5, nx=(128, 448), nz=(128, 448), shots=(2, 6), nrec=(40, 100), abs_width=20)
+UNEVEN_SMALL = dict(UNEVEN, seed=5, nx=(128, 448), nz=(128, 448), shots=(2, 6), nrec=(40, 100), abs_width=20)
 
 
-def c4_problems(nt_scale: float = 1.0, nproblems: int | None = None, spec: dict | None = None):
+def uneven_problems(nt_scale: float = 1.0, nproblems: int | None = None, spec: dict | None = None):
     """Uneven batch of independent SH and P-SV gradient problems (seeded):
     grid nx in [384, 3072], nz in [384, 1536], each with its own
-    layered-plus-random-field model (the C2 recipe, seed 100+q) and a true
+    layered-plus-random-field model (the PSV_BATCH recipe, seed 100+q) and a true
     model with Gaussian vs anomalies; 4-16 shots at 50 m depth, 100-300
     receivers at 10 m; nt in [1500, 4000] (times nt_scale, at least 60);
     dt from CFL 0.5 on the model's largest vp; Ricker f0 in [5, 15] Hz,
     t0 = 1.2 / f0.  Even problems are SH, odd ones P-SV.  `spec` replaces
-    these ranges (C4_SMALL: the small-problem batch).  Returns a list of
+    these ranges (UNEVEN_SMALL: the small-problem batch).  Returns a list of
     dicts: kind, nx, nz, h, dt, nt, init, true, geom."""
-    c = spec or C4
+    c = spec or UNEVEN
     rng = np.random.default_rng(c["seed"])
     out = []
     for q in range(nproblems or c["nproblems"]):
@@ -172,7 +172,7 @@ def c4_problems(nt_scale: float = 1.0, nproblems: int | None = None, spec: dict 
     return out
 
 
-def c0_problem():
+def sh_forward_problem():
     """Plain SH forward: homogeneous 200x200, dx = dz = 100 m, vs = 3000,
     rho = 2600 (vp = 5200 stored), nt = 2000, dt = 0.006 s; one Ricker source
     at the centre (f0 = 2 Hz, t0 = 0.6 s, angle 0, amplitude 1e10); 25
@@ -185,7 +185,7 @@ def c0_problem():
     return dict(nx=nx, nz=nz, h=100.0, nt=2000, dt=0.006, abs_width=20, model=model, geom=g)
 
 
-def c1_problem(j_value: float = 1.0e4):
+def spin_forward_problem(j_value: float = 1.0e4):
     """P-SV + micropolar spin forward: homogeneous 500x500, dx = dz = 5 m,
     lambda = 2e10, mu = 1.5e10, nu = 2e9, lambda_c = mu_c = nu_c = 1e5,
     rho = 2500, nt = 3000, dt = 4e-4; Ricker at the centre (f0 = 25 Hz,

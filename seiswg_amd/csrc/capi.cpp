@@ -318,9 +318,9 @@ int effective_group2(int nt, int requested, int nshots) {
 // width that wastes the fewest lanes on the last strip (the wider on ties;
 // for grids too small to fill the CUs, the one that gives more workgroups),
 // shots per workgroup as requested (else 1), and as many segments as fill the
-// resident slots of every CU once, at least 8 rows each (a small batch is
-// latency-bound: many short segments beat few long ones despite the 9
-// recomputed rows per segment).
+// resident slots of every CU once, at least 8 rows each, 4 on grids too small
+// to fill the CUs (latency-bound: many short segments beat few long ones
+// despite the 9 recomputed rows per segment).
 void heuristic_tiling2(sw_ctx* c, int nshots) {
   Tiling& t = c->tile2;
   const int nz = c->g.nz;
@@ -344,27 +344,27 @@ void heuristic_tiling2(sw_ctx* c, int nshots) {
   const int64_t strips = (nz + best - 12 - 1) / (best - 12);
   const int64_t base = int64_t(nshots / group) * strips;
   const int64_t nsegs = std::max<int64_t>(1, (ncu * resident) / base);
-  int seg = int(std::max<int64_t>((c->g.nx + nsegs - 1) / nsegs, 8));
+  const int min_seg = base * ((c->g.nx + 7) / 8) < ncu ? 4 : 8;
+  int seg = int(std::max<int64_t>((c->g.nx + nsegs - 1) / nsegs, min_seg));
   t.nt_threads = best;
   t.seglen = std::min(seg, c->g.nx);
   t.group = group;
 }
 
 // Tile width: 256 threads from 400 columns up, 128 from 100 (wide tiles
-// measured faster even where they leave more lanes beyond nz idle), narrower
-// while even 8-row segments would leave CUs without a workgroup.
+// measured faster even where they leave more lanes beyond nz idle, and on
+// grids too small to fill the CUs: 500x500 P-SV+spin, one shot, 23.3 us per
+// step with 256 threads against 26.9 with 128).
 // Shots per workgroup: the largest of 4, 2, 1 that divides the batch and
 // whose resident workgroups still give a CU at least 16 waves; more shots per
 // workgroup read each model row fewer times.  Segments: as many as fill every
-// CU's resident slots once (no second round), but at least 8 rows each.
+// CU's resident slots once (no second round), but at least 8 rows each (4 on
+// grids too small to fill the CUs).
 void heuristic_tiling(sw_ctx* c, int nshots) {
   Tiling& t = c->tile;
   const int64_t ncu = c->ncu > 0 ? c->ncu : 256;
   auto strips_of = [&](int w) { return int64_t((c->g.nz + (w - 6) - 1) / (w - 6)); };
-  int nt = c->g.nz >= 400 ? 256 : c->g.nz >= 100 ? 128 : 64;
-  const int64_t max_segs = std::max(1, c->g.nx / 8);
-  for (int w : {128, 64})
-    if (w < nt && int64_t(nshots) * strips_of(nt) * max_segs < ncu) nt = w;
+  const int nt = c->g.nz >= 400 ? 256 : c->g.nz >= 100 ? 128 : 64;
   const bool spin = uses_spin(c);
   const int kind = uses_psv(c) ? (spin ? MK_SPIN : MK_PSV) : MK_SH;
   const int64_t nstrips = strips_of(nt);
@@ -380,7 +380,10 @@ void heuristic_tiling(sw_ctx* c, int nshots) {
   }
   const int64_t base = int64_t(nshots / group) * nstrips;
   const int64_t nsegs = std::max<int64_t>(1, (ncu * resident) / base);
-  int seg = int(std::max<int64_t>((c->g.nx + nsegs - 1) / nsegs, 8));
+  // a grid that 8-row segments cannot spread over every CU is latency-bound:
+  // 4-row segments (fewer rows marched per workgroup) measured faster there
+  const int min_seg = base * ((c->g.nx + 7) / 8) < ncu ? 4 : 8;
+  int seg = int(std::max<int64_t>((c->g.nx + nsegs - 1) / nsegs, min_seg));
   seg = std::min(seg, c->g.nx);
   t.nt_threads = nt;
   t.seglen = seg;

@@ -84,7 +84,10 @@ typedef struct sw_grid {
 /* Snapshot callback: called after step `it` (it > 0, it % snap_every == 0,
  * solver.rs:1035) once per field written by the reference's save_snapshot
  * (solver.rs:1144-1176: vy | vx, vz | vy_c) with a host copy of the field.
- * A non-zero return aborts the run. */
+ * Calls come in snapshot order but one snapshot late, while the following
+ * steps run on the GPU (the copy overlaps them); `data` is valid during the
+ * call only, and the callback must not call back into the context.  A
+ * non-zero return aborts the run. */
 typedef int (*sw_snapshot_fn)(void* user, int32_t shot, int32_t it, int32_t field, const float* data,
                               int64_t npt);
 

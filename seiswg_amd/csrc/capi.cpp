@@ -251,6 +251,14 @@ struct sw_ctx {
 
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // snapshots: device staging and pinned host buffers (two of each), a copy
+  // stream and its events, so a snapshot's transfer overlaps the next steps
+  DBuf<float> snap_stage[2];
+  float* snap_host[2] = {nullptr, nullptr};
+  size_t snap_host_bytes = 0;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t snap_ready[2] = {nullptr, nullptr}, snap_copied[2] = {nullptr, nullptr};
   sw_timing timing{};
   int64_t launches = 0;
 };
@@ -776,11 +784,57 @@ float* model_ptr(sw_ctx* c, int field) {
   return nullptr;
 }
 
+// The forward time loop with snapshots (save_snapshot, solver.rs:1035,
+// 1144-1176: after step it when it > 0 and it % snap == 0).  At a snapshot
+// the fields are copied on the device into a staging buffer, which a copy
+// stream moves to pinned host memory while the next steps run; the callbacks
+// of a snapshot are made once the following block of steps has been queued
+// (or at the end), in snapshot order.
 int run_with_snapshots(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user) {
   const int nt = c->g.nt;
-  std::vector<float> host;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   const int64_t l0 = c->launches;
+  const int fields[4] = {SW_F_VY, SW_F_VX, SW_F_VZ, SW_F_VY_C};
+  const bool on[4] = {uses_sh(c), uses_psv(c), uses_psv(c), uses_spin(c)};
+  int nf = 0;
+  for (int q = 0; q < 4; ++q) nf += on[q] ? 1 : 0;
+  const size_t per_field = size_t(c->npt);
+  const size_t snap_floats = size_t(c->nshots) * nf * per_field;
+  const bool snaps = snap > 0 && cb != nullptr && snap_floats > 0;
+  if (snaps) {
+    if (!c->copy_stream) {
+      HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+      for (int q = 0; q < 2; ++q) {
+        HIPCHK(hipEventCreateWithFlags(&c->snap_ready[q], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->snap_copied[q], hipEventDisableTiming));
+      }
+    }
+    if (c->snap_host_bytes < snap_floats * 4) {
+      for (int q = 0; q < 2; ++q) {
+        if (c->snap_host[q]) (void)hipHostFree(c->snap_host[q]);
+        c->snap_host[q] = nullptr;
+      }
+      c->snap_host_bytes = 0;
+      for (int q = 0; q < 2; ++q) HIPCHK(hipHostMalloc((void**)&c->snap_host[q], snap_floats * 4, hipHostMallocDefault));
+      c->snap_host_bytes = snap_floats * 4;
+    }
+    for (int q = 0; q < 2; ++q) {
+      HIPCHK(c->snap_stage[q].ensure(snap_floats));
+      HIPCHK(hipEventRecord(c->snap_copied[q], c->copy_stream));   // both slots free
+    }
+  }
+  int pending = -1, pending_it = 0, slot = 0;
+  auto deliver = [&](int q, int step) -> int {
+    HIPCHK(hipEventSynchronize(c->snap_copied[q]));
+    const float* h = c->snap_host[q];
+    for (int s = 0; s < c->nshots; ++s)
+      for (int f = 0; f < 4; ++f) {
+        if (!on[f]) continue;
+        if (cb(user, s, step, fields[f], h, c->npt) != 0) return fail(-4, "snapshot callback failed");
+        h += per_field;
+      }
+    return 0;
+  };
   int it = 0;
   while (it < nt) {
     int stop = nt - 1;
@@ -791,21 +845,32 @@ int run_with_snapshots(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user) {
     }
     int rc = launch_steps(c, stop - it + 1);
     if (rc) return rc;
-    if (snap > 0 && stop > 0 && stop % snap == 0 && cb) {
-      // save_snapshot (solver.rs:1144-1176)
-      const int fields[4] = {SW_F_VY, SW_F_VX, SW_F_VZ, SW_F_VY_C};
-      const bool on[4] = {uses_sh(c), uses_psv(c), uses_psv(c), uses_spin(c)};
-      host.resize(size_t(c->npt));
+    if (snaps && stop > 0 && stop % snap == 0) {
+      // staging slot: its previous transfer must be done (it was delivered)
+      HIPCHK(hipStreamWaitEvent(c->stream, c->snap_copied[slot], 0));
+      float* d = c->snap_stage[slot].p;
       for (int s = 0; s < c->nshots; ++s)
-        for (int q = 0; q < 4; ++q) {
-          if (!on[q]) continue;
-          HIPCHK(hipMemcpyAsync(host.data(), slot_ptr(c, field_slot(fields[q]), s), size_t(c->npt) * 4,
-                                hipMemcpyDeviceToHost, c->stream));
-          HIPCHK(hipStreamSynchronize(c->stream));
-          if (cb(user, s, stop, fields[q], host.data(), c->npt) != 0) return fail(-4, "snapshot callback failed");
+        for (int f = 0; f < 4; ++f) {
+          if (!on[f]) continue;
+          HIPCHK(hipMemcpyAsync(d, slot_ptr(c, field_slot(fields[f]), s), per_field * 4, hipMemcpyDeviceToDevice,
+                                c->stream));
+          d += per_field;
         }
+      HIPCHK(hipEventRecord(c->snap_ready[slot], c->stream));
+      HIPCHK(hipStreamWaitEvent(c->copy_stream, c->snap_ready[slot], 0));
+      HIPCHK(hipMemcpyAsync(c->snap_host[slot], c->snap_stage[slot].p, snap_floats * 4, hipMemcpyDeviceToHost,
+                            c->copy_stream));
+      HIPCHK(hipEventRecord(c->snap_copied[slot], c->copy_stream));
+      if (pending >= 0 && (rc = deliver(pending, pending_it))) return rc;
+      pending = slot;
+      pending_it = stop;
+      slot ^= 1;
     }
     it = stop + 1;
+  }
+  if (pending >= 0) {
+    int rc = deliver(pending, pending_it);
+    if (rc) return rc;
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   HIPCHK(hipEventSynchronize(c->ev1));
@@ -860,6 +925,13 @@ SW_API void sw_destroy(sw_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.exec);
+  if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+  for (int q = 0; q < 2; ++q) {
+    if (c->snap_host[q]) (void)hipHostFree(c->snap_host[q]);
+    if (c->snap_ready[q]) (void)hipEventDestroy(c->snap_ready[q]);
+    if (c->snap_copied[q]) (void)hipEventDestroy(c->snap_copied[q]);
+  }
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);

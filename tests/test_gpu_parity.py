@@ -246,6 +246,32 @@ def test_two_step_shape_set_apart_from_one_step_shape(seiswg, tiling2):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (q, k)
 
 
+@pytest.mark.parametrize("phys", ["psv", "sh"])
+def test_snapshots_equal_the_state_after_their_step(seiswg, phys):
+    """Snapshots of a batched run (copied asynchronously while later steps
+    run) against the velocity fields read after advancing to the same step:
+    bit for bit, every shot, every snapshot."""
+    nx, nz, nt, every = 48, 90, 23, 5
+    sh = phys == "sh"
+    p, stf = random_problem(nx, nz, nt, sh, not sh, False, seed=41)
+    runs = [0, 1, 2]
+    c = gpu_context(p, stf, seiswg)
+    snaps = {}
+    c.forward_batch(runs, snap=every, snapshots=snaps)
+    fids = [seiswg.abi.F_VY] if sh else [seiswg.abi.F_VX, seiswg.abi.F_VZ]
+    steps = sorted({k[1] for k in snaps})
+    assert steps == list(range(every, nt, every)), steps
+    c.prepare(runs)
+    done = 0
+    for it in steps:
+        c.advance(it + 1 - done)
+        done = it + 1
+        for s in range(len(runs)):
+            for f in fids:
+                ref = c.field(f, shot=s)
+                assert np.array_equal(snaps[(s, it, f)].view(np.uint32), ref.view(np.uint32)), (s, it, f)
+
+
 def load_example(tmp_path, ex, ini_name="config.ini"):
     d = E.materialise(ex, str(tmp_path))
     ini = E.ini_values(os.path.join(d, ini_name))

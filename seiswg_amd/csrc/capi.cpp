@@ -325,10 +325,10 @@ int effective_group2(int nt, int requested, int nshots) {
 // Shape of the two-step launches (6-column halos, W = threads-12): the tile
 // width that wastes the fewest lanes on the last strip (the wider on ties;
 // for grids too small to fill the CUs, the one that gives more workgroups),
-// shots per workgroup as requested (else 1), and as many segments as fill the
-// resident slots of every CU once, at least 8 rows each, 4 on grids too small
-// to fill the CUs (latency-bound: many short segments beat few long ones
-// despite the 9 recomputed rows per segment).
+// shots per workgroup as requested (else 2 up to 256 threads, 1 above), and
+// as many segments as fill the resident slots of every CU once, at least 8
+// rows each, 4 on grids too small to fill the CUs (latency-bound: many short
+// segments beat few long ones despite the 9 recomputed rows per segment).
 void heuristic_tiling2(sw_ctx* c, int nshots) {
   Tiling& t = c->tile2;
   const int nz = c->g.nz;
@@ -347,7 +347,10 @@ void heuristic_tiling2(sw_ctx* c, int nshots) {
       best = nt;
     }
   }
-  const int group = effective_group2(best, c->group2_req, nshots);
+  // two shots per workgroup share their model rows; measured faster with
+  // tiles up to 256 wide (1000x600, 16 shots: 106.6 against 102.5 shots/s
+  // for the gradient), not with 512-wide tiles (2000x1000, 64 shots)
+  const int group = effective_group2(best, c->group2_req > 0 ? c->group2_req : (best <= 256 ? 2 : 1), nshots);
   const int64_t resident = std::max(step2_blocks_per_cu(best, group), 1);
   const int64_t strips = (nz + best - 12 - 1) / (best - 12);
   const int64_t base = int64_t(nshots / group) * strips;
@@ -790,7 +793,14 @@ float* model_ptr(sw_ctx* c, int field) {
 // stream moves to pinned host memory while the next steps run; the callbacks
 // of a snapshot are made once the following block of steps has been queued
 // (or at the end), in snapshot order.
+int run_snapshots_body(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user);
 int run_with_snapshots(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user) {
+  const int rc = run_snapshots_body(c, snap, cb, user);
+  // on an error, no transfer into the pinned buffers may still be running
+  if (rc && c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+  return rc;
+}
+int run_snapshots_body(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user) {
   const int nt = c->g.nt;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   const int64_t l0 = c->launches;

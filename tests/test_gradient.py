@@ -3,9 +3,10 @@ reference, parity with it is unpinned).
 
 Pinned instead by:
 * the oracle's FP64 exact discrete adjoint against FP64 central finite
-  differences of the same scheme (relative step 1e-4) on tiny grids - this
-  pins the adjoint derivation (a dropped term, a wrong sign or a transposed
-  stencil fails it);
+  differences of the same scheme (relative step 1e-4) on tiny grids, at
+  sampled cells and along a random direction through every cell at once
+  (<grad J, dm> = dJ/de) - this pins the adjoint derivation (a dropped term,
+  a wrong sign or a transposed stencil fails it);
 * the GPU's FP32 checkpointed adjoint against the oracle's FP64 adjoint, and
   its invariance to the checkpoint interval and to shot batching.
 """
@@ -70,6 +71,37 @@ def gpu_problem(seiswg, p, stf):
     c.set_sources(p.src_idx, stf)
     c.set_receivers(p.rec_idx)
     return c
+
+
+@pytest.mark.parametrize("sh", [True, False], ids=["sh", "psv"])
+def test_oracle_gradient_equals_directional_derivative(sh):
+    """<grad J, dm> against the central difference (J(m + e dm) - J(m - e dm))
+    / 2e for a random perturbation dm of every cell of rho, mu (and lambda)
+    at once: the dot-product form of the linearisation test, pinning the
+    whole kernel fields rather than sampled points."""
+    p, stf, nc = problem(sh, 16, 14, 50, seed=7)
+    o = O.Oracle(p, 8, stf=stf.astype(np.float64))
+    runs = [1, -1]
+    obs = np.random.default_rng(3).standard_normal((len(runs), nc, len(p.rec_idx), p.nt)) * 0.01
+    J, kr, km, kl = o.gradient(runs, obs)
+    rng = np.random.default_rng(11)
+    names = ["RHO", "MU"] + ([] if sh else ["LAM"])
+    grads = {"RHO": kr, "MU": km, "LAM": kl}
+    base = {n: o.field(O.F[n]).copy() for n in names}
+    dm = {n: rng.standard_normal(base[n].size) * np.abs(base[n]) for n in names}
+    ad = sum(float(np.dot(grads[n], dm[n])) for n in names)
+    eps = 1e-5
+
+    def j_at(sign):
+        for n in names:
+            o.set_field(O.F[n], base[n] + sign * eps * dm[n])
+        return o.objective(runs, obs)
+
+    fd = (j_at(1) - j_at(-1)) / (2 * eps)
+    for n in names:
+        o.set_field(O.F[n], base[n])
+    print(f"directional derivative: adjoint {ad:.12e} finite difference {fd:.12e}")
+    assert abs(fd - ad) <= 1e-7 * abs(ad)
 
 
 @pytest.mark.gpu

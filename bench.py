@@ -273,6 +273,8 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="tile width (64/128/256); 0 = library default")
     ap.add_argument("--seglen", type=int, default=0)
     ap.add_argument("--group", type=int, default=0, help="shots per workgroup (1, 2, 4); 0 = library default")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="measure with the heuristic tile shapes instead of autotuned ones")
     ap.add_argument("--steps-per-launch", type=int, default=0,
                     help="P-SV forward steps per launch (1 or 2); 0 = library default (2)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -305,9 +307,15 @@ def main():
     if a.steps_per_launch:
         ctx.set_steps_per_launch(a.steps_per_launch)
     ctx.prepare(list(range(nshots)))
+    autotune = None
     if a.threads or a.seglen or a.group:
         t, s, g = ctx.tiling()
         ctx.set_tiling(a.threads or t, a.seglen or s, a.group or g)
+        ctx.prepare(list(range(nshots)))
+    elif not a.no_autotune:
+        t0 = time.perf_counter()
+        _, _, _, tuned_ms = ctx.autotune(nshots, steps=8)
+        autotune = {"seconds": time.perf_counter() - t0, "ms_per_step_while_tuning": tuned_ms}
         ctx.prepare(list(range(nshots)))
     if a.warmup:
         ctx.advance(a.warmup)
@@ -337,6 +345,7 @@ def main():
         "config": {"workload": "batched multi-shot P-SV forward, 2000x1000 grid, dx=10 m, 64 shots, 400 receivers, "
                                "taper L/R/B 40, dt=1 ms", "nx": 2000, "nz": 1000, "shots": nshots,
                    "physics": "psv", "tiling": list(ctx.tiling()), "tiling_two_step": list(ctx.tiling2()),
+                   "autotuned": autotune is not None,
                    "parallelism": "1 GPU, shots batched per launch"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic,
@@ -344,6 +353,8 @@ def main():
                      "steps_per_launch": steps_per_launch,
                      "hbm_rate_GBps": (traffic / avg_launch_s / 1e9) if traffic else None},
     }
+    if autotune is not None:
+        out["autotune"] = autotune
     ctx.close()
     if not a.no_gradient:
         out["gradient"] = gradient_bench(seiswg, synth, a.gradient_nt)

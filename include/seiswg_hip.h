@@ -207,8 +207,11 @@ SW_API int sw_set_steps_per_launch(sw_ctx* ctx, int32_t steps);
 SW_API int sw_set_tiling2(sw_ctx* ctx, int32_t threads, int32_t seglen, int32_t shots_per_block);
 SW_API int sw_get_tiling2(const sw_ctx* ctx, int32_t* threads, int32_t* seglen, int32_t* shots_per_block);
 /* Autotuner: times `steps` forward steps of `nshots` shots for a set of
- * tile shapes, keeps the fastest (as sw_set_tiling does) and returns it with
- * its ms per step.  Tile shapes never change results. */
+ * tile shapes, keeps the fastest (as sw_set_tiling does) and returns it.
+ * When steps run two per launch (P-SV or SH without spin), the two-step
+ * shape is tuned too and kept (as sw_set_tiling2 does; read it with
+ * sw_get_tiling2); `ms` is the time per step of the configuration that runs
+ * (two-step launches where they apply).  Tile shapes never change results. */
 SW_API int sw_autotune(sw_ctx* ctx, int32_t nshots, int32_t steps, int32_t* threads, int32_t* seglen,
                        int32_t* shots_per_block, double* ms);
 

@@ -1151,31 +1151,48 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
                        int32_t* shots_per_block, double* ms) {
   if (!c || nshots < 1 || steps < 1) return fail(-1, "bad arguments");
   if (!c->model_set) return fail(-2, "model not set");
-  if (c->g.nt < steps + 2) return fail(-2, "nt=%d too short to time %d steps", c->g.nt, steps);
   HIPCHK(hipSetDevice(c->device));
   std::vector<int32_t> isrc(nshots);
   for (int s = 0; s < nshots; ++s) isrc[s] = c->nsrc > 0 ? s % c->nsrc : -1;
+  const int spl = c->steps_per_launch;
+  const bool two = spl == 2 && uses_psv(c) != uses_sh(c) && !uses_spin(c);
+  const int n2 = std::max(4, (steps + 3) / 4 * 4);   // whole pairs of two-step launches
+  if (c->g.nt < steps + 2 || (two && c->g.nt < n2 + 4))
+    return fail(-2, "nt=%d too short to time %d steps", c->g.nt, two ? n2 : steps);
+  // times `n` steps (after a warm-up) of the prepared batch, in ms
+  auto time_steps = [&](int warm, int n, float& t) -> int {
+    int rc = sw_prepare(c, nshots, isrc.data());
+    if (rc) return rc;
+    if ((rc = launch_steps(c, warm))) return rc;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if ((rc = launch_steps(c, n))) return rc;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    return 0;
+  };
+  const int segs[] = {8, 16, 32, 48, 64, 128, 256, 400, 500, 512};
+  // one-step launches (also used for the remainder steps, the gradient's
+  // adjoint and the physics without two-step kernels)
+  c->steps_per_launch = 1;
   double best = 1e300;
   Tiling bt = c->tile;
   int bg = c->group_req;
   for (int nt : {64, 128, 256}) {
     for (int grp : {1, 2, 4}) {
       if (!group_supported(nt, grp) || nshots % grp != 0) continue;
-      for (int seg : {32, 64, 128, 256, 400, 512}) {
+      for (int seg : segs) {
         if (seg > 2 * c->g.nx) continue;
         c->tile.nt_threads = nt;
         c->tile.seglen = seg;
         c->group_req = grp;
         c->tile_fixed = true;
-        int rc = sw_prepare(c, nshots, isrc.data());
-        if (rc) return rc;
-        if ((rc = launch_steps(c, 2))) return rc;       // warm-up
-        HIPCHK(hipEventRecord(c->ev0, c->stream));
-        if ((rc = launch_steps(c, steps))) return rc;
-        HIPCHK(hipEventRecord(c->ev1, c->stream));
-        HIPCHK(hipEventSynchronize(c->ev1));
         float t = 0;
-        HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+        const int rc = time_steps(2, steps, t);
+        if (rc) {
+          c->steps_per_launch = spl;
+          return rc;
+        }
         if (t < best) {
           best = t;
           bt = c->tile;
@@ -1184,15 +1201,48 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
       }
     }
   }
+  c->steps_per_launch = spl;
   c->tile = bt;
   c->group_req = bg;
   c->tile_fixed = true;
+  double best_ms = best / steps;
+  if (two) {
+    // two-step launches: their own widths (up to 512), groups and segments
+    double best2 = 1e300;
+    Tiling bt2 = c->tile2;
+    int bg2 = c->group2_req;
+    for (int nt : {64, 128, 256, 512}) {
+      for (int grp : {1, 2, 4}) {
+        if (!group2_supported(nt, grp) || nshots % grp != 0) continue;
+        for (int seg : segs) {
+          if (seg > 2 * c->g.nx) continue;
+          c->tile2.nt_threads = nt;
+          c->tile2.seglen = seg;
+          c->tile2.group = grp;
+          c->group2_req = grp;
+          c->tile2_fixed = true;
+          float t = 0;
+          const int rc = time_steps(4, n2, t);
+          if (rc) return rc;
+          if (t < best2) {
+            best2 = t;
+            bt2 = c->tile2;
+            bg2 = grp;
+          }
+        }
+      }
+    }
+    c->tile2 = bt2;
+    c->group2_req = bg2;
+    c->tile2_fixed = true;
+    best_ms = best2 / n2;
+  }
   c->nshots = 0;
   ++c->gen;
   if (threads) *threads = bt.nt_threads;
   if (seglen) *seglen = bt.seglen;
   if (shots_per_block) *shots_per_block = bt.group;
-  if (ms) *ms = best / steps;
+  if (ms) *ms = best_ms;
   return 0;
 }
 

@@ -272,6 +272,35 @@ def test_snapshots_equal_the_state_after_their_step(seiswg, phys):
                 assert np.array_equal(snaps[(s, it, f)].view(np.uint32), ref.view(np.uint32)), (s, it, f)
 
 
+@pytest.mark.parametrize("phys", ["psv", "sh", "spin"])
+def test_autotuned_shapes_keep_results(seiswg, phys):
+    """sw_autotune times one-step shapes and, where steps run two per launch,
+    two-step shapes; it keeps the fastest, and runs with them give the same
+    traces and states as the heuristic shapes, bit for bit."""
+    nx, nz, nt = 40, 150, 22
+    sh, psv, spin = phys == "sh", phys in ("psv", "spin"), phys == "spin"
+    p, stf = random_problem(nx, nz, nt, sh, psv, spin, seed=51)
+    runs = [0, 1, 2, 3]
+    comps = [O.OBS_Y] if sh or spin else [O.OBS_X, O.OBS_Z]
+    fields = STATE["sh"] if sh else STATE["psv"] + (STATE["spin"] if spin else [])
+    out = []
+    for tune in (False, True):
+        c = gpu_context(p, stf, seiswg)
+        if tune:
+            threads, seglen, group, ms = c.autotune(len(runs), steps=4)
+            print(f"case {phys}: tuned one-step {c.tiling()} two-step {c.tiling2()} ({ms:.4f} ms/step)")
+            assert ms > 0 and threads in (64, 128, 256)
+        c.forward_batch(runs)
+        if tune:
+            assert c.tiling() == (threads, seglen, group)
+            assert (c.tiling2()[0] > 0) == (not spin)
+        out.append([c.obs(q, k).copy() for q in range(len(runs)) for k in comps] +
+                   [c.field(getattr(seiswg.abi, "F_" + n), shot=q).copy() for q in range(len(runs)) for n in fields])
+        c.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def load_example(tmp_path, ex, ini_name="config.ini"):
     d = E.materialise(ex, str(tmp_path))
     ini = E.ini_values(os.path.join(d, ini_name))

@@ -363,7 +363,7 @@ def main():
         out["spin_forward_500x500"] = small_config_bench(seiswg, synth, "spin", not a.no_cpu_baseline)
     if a.uneven:
         out["uneven_batch"] = uneven_bench(seiswg, synth, a.uneven_nt_scale, a.uneven_budget_gb)
-        out["small_uneven_batch"] = uneven_bench(seiswg, synth, 0.1, spec=synth.UNEVEN_SMALL)
+        out["small_uneven_batch"] = uneven_bench(seiswg, synth, 1.0, spec=synth.UNEVEN_SMALL)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(synth, model, geom, src_idx, rec_idx)
     print(json.dumps(out), flush=True)

@@ -1001,7 +1001,12 @@ SW_API int sw_set_sources(sw_ctx* c, int32_t nsrc, const int32_t* idx, const flo
   ++c->gen;
   const size_t n = size_t(3) * nsrc * size_t(c->g.nt);
   HIPCHK(c->stf.ensure(std::max<size_t>(n, 1)));
-  if (n) HIPCHK(hipMemcpy(c->stf.p, stf, n * 4, hipMemcpyHostToDevice));
+  if (n) {
+    // on the context's stream: a copy on the legacy stream would have to wait
+    // for (and is refused while) other threads capture graphs
+    HIPCHK(hipMemcpyAsync(c->stf.p, stf, n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   return 0;
 }
 

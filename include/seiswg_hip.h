@@ -167,11 +167,15 @@ SW_API int sw_multi_forward(sw_multi* m, const int32_t* nshots, const int32_t* c
  * NULL.  A common checkpoint interval; segments of the merged problems are
  * aligned.  Problems with at least `merge limit` shot-cells per step
  * (nshots * nx * nz; default 3 * 2^17) fill the GPU on their own and run on their
- * own launches, one problem after the other; the smaller ones advance
- * together in merged launches.  Results do not depend on the split. */
+ * own launches (see sw_multi_set_workers); the smaller ones advance together
+ * in merged launches.  Results do not depend on the split. */
 SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc,
                              const float* const* obs_ref, int32_t ckpt_every, double* const* misfit);
 SW_API int sw_multi_set_merge_limit(sw_multi* m, int64_t shot_cells);
+/* Problems on their own launches run `workers` at a time (default 3), each
+ * from its own host thread on its context's stream, largest first; their
+ * graph capture is off while they run.  1 runs them one after the other. */
+SW_API int sw_multi_set_workers(sw_multi* m, int32_t workers);
 /* Elapsed time of the last multi call (HIP events for merged forward runs,
  * host clock for gradients) and merged launches issued. */
 SW_API int sw_multi_timing(const sw_multi* m, double* ms, int64_t* launches);

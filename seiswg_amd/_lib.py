@@ -51,6 +51,7 @@ EXPORTS = ["sw_last_error", "sw_create", "sw_destroy", "sw_set_model", "sw_set_s
            "sw_set_steps_per_launch", "sw_set_tiling2", "sw_get_tiling2", "sw_release_cached_memory",
            "sw_bench_copy", "sw_gradient", "sw_autotune", "sw_multi_create", "sw_multi_destroy",
            "sw_multi_forward", "sw_multi_gradient", "sw_multi_timing", "sw_multi_set_merge_limit",
+           "sw_multi_set_workers",
            "seiswg_run", "seiswg_ricker_table", "seiswg_infer_grid", "seiswg_grid_index"]
 
 _lib = None
@@ -93,6 +94,7 @@ def load():
     L.sw_multi_gradient.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(fp), C.c_int32, C.POINTER(dp)]
     L.sw_multi_timing.argtypes = [vp, dp, C.POINTER(C.c_int64)]
     L.sw_multi_set_merge_limit.argtypes = [vp, C.c_int64]
+    L.sw_multi_set_workers.argtypes = [vp, C.c_int32]
     L.sw_autotune.argtypes = [vp, C.c_int32, C.c_int32, ip, ip, ip, dp]
     L.sw_gradient.argtypes = [vp, C.c_int32, ip, fp, C.c_int32, C.c_int32, dp]
     L.sw_bench_copy.argtypes = [C.c_int32, C.c_int64, C.c_int32, dp]
@@ -360,6 +362,10 @@ class Multi:
         """Gradients: problems with at least this many shot-cells per step run
         on their own launches, smaller ones in merged launches."""
         _check(self.L.sw_multi_set_merge_limit(self.h, int(shot_cells)))
+
+    def set_workers(self, workers: int):
+        """Problems on their own launches run this many at a time."""
+        _check(self.L.sw_multi_set_workers(self.h, workers))
 
     def gradient(self, isrc_lists, obs_refs, ckpt_every: int = 50):
         """Returns per problem (misfit per shot, K_rho, K_mu, K_lambda)."""

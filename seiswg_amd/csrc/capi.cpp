@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -17,6 +18,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/seiswg_hip.h"
@@ -1546,6 +1548,9 @@ struct sw_multi {
   // merged launches ahead up to ~0.3 M shot-cells per step, own launches
   // from ~0.6 M on)
   int64_t merge_limit = int64_t(3) << 17;
+  // problems on their own launches run on this many host threads at once,
+  // each on its problem's stream (their launches and host work overlap)
+  int workers = 3;
   DBuf<StepArgs> args;
   DBuf<int2> work;
   double loop_ms = 0;
@@ -1794,28 +1799,70 @@ SW_API int sw_multi_set_merge_limit(sw_multi* m, int64_t shot_cells) {
   return 0;
 }
 
+SW_API int sw_multi_set_workers(sw_multi* m, int32_t workers) {
+  if (!m || workers < 1 || workers > 16) return fail(-1, "workers must be 1..16");
+  m->workers = workers;
+  return 0;
+}
+
 SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc,
                              const float* const* obs_ref, int32_t ckpt_every, double* const* misfit) {
   if (!m || !nshots || !isrc || !obs_ref) return fail(-1, "bad arguments");
   HIPCHK(hipSetDevice(m->device));
   const auto t0 = std::chrono::steady_clock::now();
   const int n = int(m->p.size());
-  std::vector<int> merged;
+  std::vector<int> merged, solo;
   int rc = 0;
+  for (int q = 0; q < n; ++q)
+    (int64_t(nshots[q]) * m->p[q]->npt < m->merge_limit ? merged : solo).push_back(q);
   // problems that fill the GPU on their own: their own launches (own tile
-  // shapes, two-step pairs, captured graphs), one problem after the other
-  for (int q = 0; q < n; ++q) {
-    if (int64_t(nshots[q]) * m->p[q]->npt < m->merge_limit) {
-      merged.push_back(q);
-      continue;
+  // shapes, two-step pairs), largest first, `workers` of them at a time on
+  // their own streams.  Graph capture is off meanwhile: a capture in one
+  // thread and another thread's allocations or synchronisations exclude each
+  // other in HIP (measured: no loss without graphs at these sizes).
+  std::stable_sort(solo.begin(), solo.end(), [&](int a, int b) {
+    return int64_t(nshots[a]) * m->p[a]->npt * m->p[a]->g.nt > int64_t(nshots[b]) * m->p[b]->npt * m->p[b]->g.nt;
+  });
+  if (!solo.empty()) {
+    std::atomic<size_t> next{0};
+    std::atomic<bool> failed{false};
+    std::mutex err_mu;
+    std::string first_err;
+    int first_rc = 0;
+    auto worker = [&]() {
+      if (hipSetDevice(m->device) != hipSuccess) return;
+      for (;;) {
+        const size_t i = next.fetch_add(1);
+        if (i >= solo.size() || failed.load()) return;
+        const int q = solo[i];
+        sw_ctx* c = m->p[q];
+        const bool graphs = c->graphs_off;
+        c->graphs_off = true;
+        const int r = sw_gradient(c, nshots[q], isrc[q], obs_ref[q], ckpt_every, 0, misfit ? misfit[q] : nullptr);
+        c->graphs_off = graphs;
+        if (r) {
+          std::lock_guard<std::mutex> lk(err_mu);
+          if (!failed.exchange(true)) {
+            first_rc = r;
+            first_err = g_err;
+          }
+          return;
+        }
+        // its kernels are in K_RHO / K_MU / K_LAM: give the device memory of
+        // the run (checkpoints, aux fields, adjoint state, per-shot kernels)
+        // back before the next problem allocates its own
+        for (DBuf<float>* b : {&c->aux, &c->ckpt, &c->adj[0], &c->adj[1], &c->kshot, &c->spare}) b->release();
+      }
+    };
+    const int nw = std::max(1, std::min<int>(m->workers, int(solo.size())));
+    if (nw == 1) {
+      worker();
+    } else {
+      std::vector<std::thread> th;
+      for (int w = 0; w < nw; ++w) th.emplace_back(worker);
+      for (auto& t : th) t.join();
     }
-    if ((rc = sw_gradient(m->p[q], nshots[q], isrc[q], obs_ref[q], ckpt_every, 0, misfit ? misfit[q] : nullptr)))
-      return rc;
-    // its kernels are in K_RHO / K_MU / K_LAM: give the device memory of the
-    // run (checkpoints, aux fields, adjoint state, per-shot kernels) back
-    // before the next problem allocates its own
-    sw_ctx* c = m->p[q];
-    for (DBuf<float>* b : {&c->aux, &c->ckpt, &c->adj[0], &c->adj[1], &c->kshot, &c->spare}) b->release();
+    if (failed.load()) return fail(first_rc, "%s", first_err.c_str());
   }
   if (!merged.empty()) {
     // the others in merged launches, over the subset

@@ -52,7 +52,9 @@ EXPORTS = ["sw_last_error", "sw_create", "sw_destroy", "sw_set_model", "sw_set_s
            "sw_bench_copy", "sw_gradient", "sw_autotune", "sw_multi_create", "sw_multi_destroy",
            "sw_multi_forward", "sw_multi_gradient", "sw_multi_timing", "sw_multi_set_merge_limit",
            "sw_multi_set_workers",
-           "seiswg_run", "seiswg_ricker_table", "seiswg_infer_grid", "seiswg_grid_index"]
+           "seiswg_run", "seiswg_ricker_table", "seiswg_infer_grid", "seiswg_grid_index",
+           "seiswg_vfs_new", "seiswg_vfs_free", "seiswg_vfs_put", "seiswg_vfs_get", "seiswg_vfs_count",
+           "seiswg_vfs_key", "seiswg_run_simulation"]
 
 _lib = None
 
@@ -103,8 +105,20 @@ def load():
     L.seiswg_infer_grid.argtypes = [C.c_int64, fp, fp, ip, ip, fp, fp, ip]
     L.seiswg_grid_index.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32]
     L.seiswg_grid_index.restype = C.c_uint32
+    L.seiswg_vfs_new.argtypes = []
+    L.seiswg_vfs_new.restype = vp
+    L.seiswg_vfs_free.argtypes = [vp]
+    L.seiswg_vfs_free.restype = None
+    L.seiswg_vfs_put.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_int64]
+    L.seiswg_vfs_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_void_p)]
+    L.seiswg_vfs_get.restype = C.c_int64
+    L.seiswg_vfs_count.argtypes = [vp]
+    L.seiswg_vfs_key.argtypes = [vp, C.c_int32]
+    L.seiswg_vfs_key.restype = C.c_char_p
+    L.seiswg_run_simulation.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, vp, vp]
     for n in EXPORTS:
-        if n not in ("sw_destroy", "sw_last_error", "seiswg_grid_index", "sw_multi_destroy"):
+        if n not in ("sw_destroy", "sw_last_error", "seiswg_grid_index", "sw_multi_destroy", "seiswg_vfs_new",
+                     "seiswg_vfs_free", "seiswg_vfs_get", "seiswg_vfs_key"):
             getattr(L, n).restype = C.c_int
     _lib = L
     return L
@@ -168,6 +182,60 @@ def release_cached_memory(device: int = -1) -> int:
 def run_config(path: str) -> int:
     """In-process `seiswg <config.ini>`."""
     return load().seiswg_run(path.encode())
+
+
+class Vfs:
+    """A path -> bytes map of the library (sw_vfs)."""
+
+    def __init__(self, files: dict | None = None):
+        L = load()
+        self._L = L
+        self.h = L.seiswg_vfs_new()
+        if not self.h:
+            raise SeiswgError("seiswg_vfs_new failed")
+        for k, v in (files or {}).items():
+            self[k] = v
+
+    def __setitem__(self, key: str, data: bytes):
+        data = bytes(data)
+        _check(self._L.seiswg_vfs_put(self.h, key.encode(), data, len(data)))
+
+    def __getitem__(self, key: str) -> bytes:
+        p = C.c_void_p()
+        n = self._L.seiswg_vfs_get(self.h, key.encode(), C.byref(p))
+        if n < 0:
+            raise KeyError(key)
+        return C.string_at(p, n) if n else b""
+
+    def keys(self) -> list:
+        return [self._L.seiswg_vfs_key(self.h, i).decode() for i in range(self._L.seiswg_vfs_count(self.h))]
+
+    def to_dict(self) -> dict:
+        return {k: self[k] for k in self.keys()}
+
+    def close(self):
+        if self.h:
+            self._L.seiswg_vfs_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def run_simulation(config_ini: str, sources_dat: str | None, stations_dat: str | None, files: dict,
+                   config_true_ini: str | None = None) -> dict:
+    """In-memory workflow (lib.rs:74-180): model files (and observed traces)
+    by path in `files`, outputs returned as a dict path -> bytes."""
+    L = load()
+    fin, fout = Vfs(files), Vfs()
+    try:
+        enc = lambda t: None if t is None else t.encode()
+        _check(L.seiswg_run_simulation(config_ini.encode(), enc(sources_dat), enc(stations_dat),
+                                       enc(config_true_ini), fin.h, fout.h))
+        return fout.to_dict()
+    finally:
+        fin.close()
+        fout.close()
 
 
 # ------------------------------------------------------------ device context

@@ -19,6 +19,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../../include/seiswg_host.h"
@@ -189,14 +190,10 @@ struct Config {
   bool has_output_traces = false, has_model_true = false, has_model_init = false, has_traces = false;
 };
 
-// config.rs:62-192
-Config load_config(const fs::path& path) {
-  std::ifstream f(path, std::ios::binary);
-  if (!f) throw Error(fmt("Cannot open %s: %s", path.string().c_str(), std::strerror(errno)));
-  std::stringstream ss;
-  ss << f.rdbuf();
-  Ini ini = parse_ini_text(ss.str());
-  fs::path base = path.parent_path();
+// config.rs:62-192; relative paths resolve against `base` (the INI's
+// directory, or "" for INI text given in memory as lib.rs:103 does)
+Config load_config_text(const std::string& text, const fs::path& base) {
+  Ini ini = parse_ini_text(text);
   auto required = [&](const char* s, const char* k) {
     std::string v;
     if (!ini.get(s, k, v)) throw Error(fmt("[%s] %s is required", s, k));
@@ -257,18 +254,68 @@ Config load_config(const fs::path& path) {
   return c;
 }
 
+Config load_config(const fs::path& path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw Error(fmt("Cannot open %s: %s", path.string().c_str(), std::strerror(errno)));
+  std::stringstream ss;
+  ss << f.rdbuf();
+  return load_config_text(ss.str(), path.parent_path());
+}
+
 // ---------------------------------------------------------------- files
-std::vector<char> read_all(const fs::path& p, bool& ok) {
+using Vfs = std::map<std::string, std::vector<char>>;
+
+// Where a workflow's files come from and go to.  The CLI reads and writes the
+// filesystem.  The in-memory entry point (lib.rs:74-180) looks its inputs up
+// in a path-keyed map first, by the resolved path or the path without a
+// leading "./" (solver.rs:236-266), and hands every output back in another
+// map keyed as the reference's Solver returns them (solver.rs:941-1119,
+// 1298-1529) without touching the filesystem.
+struct Io {
+  const Vfs* in = nullptr;   // inputs by path, looked up before the filesystem
+  Vfs* out = nullptr;        // outputs by path
+  bool disk = true;          // read inputs missing from `in` from, and write outputs to, the filesystem
+};
+
+std::vector<char> read_all(const Io& io, const fs::path& p, bool& ok) {
+  if (io.in) {
+    const std::string key = p.string();
+    auto it = io.in->find(key);
+    if (it == io.in->end() && key.rfind("./", 0) == 0) it = io.in->find(key.substr(2));
+    if (it != io.in->end()) {
+      ok = true;
+      return it->second;
+    }
+  }
+  ok = false;
+  if (!io.disk) return {};
   std::ifstream f(p, std::ios::binary);
   ok = bool(f);
   if (!ok) return {};
   return std::vector<char>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
 }
 
+void make_dirs(const Io& io, const fs::path& d) {
+  if (!io.disk) return;
+  std::error_code ec;
+  fs::create_directories(d, ec);
+  if (ec) throw Error(fmt("Cannot create %s: %s", d.string().c_str(), ec.message().c_str()));
+}
+
+// one output file: into the output map and/or onto the filesystem
+void emit(const Io& io, const fs::path& p, const std::vector<char>& bytes) {
+  if (io.out) (*io.out)[p.string()] = bytes;
+  if (!io.disk) return;
+  std::ofstream f(p, std::ios::binary | std::ios::trunc);
+  if (!f) throw Error(fmt("Cannot write %s", p.string().c_str()));
+  f.write(bytes.data(), std::streamsize(bytes.size()));
+  if (!f) throw Error(fmt("Cannot write %s", p.string().c_str()));
+}
+
 // load_model_field (solver.rs:300-325): i32 npt, then npt f32; extra bytes ignored
-std::vector<float> load_model_field(const fs::path& p) {
+std::vector<float> load_model_field(const Io& io, const fs::path& p) {
   bool ok;
-  std::vector<char> b = read_all(p, ok);
+  std::vector<char> b = read_all(io, p, ok);
   if (!ok) throw Error(fmt("Cannot read model file %s", p.string().c_str()));
   if (b.size() < 4) throw Error(fmt("Model file too small: %s", p.string().c_str()));
   int32_t n;
@@ -282,9 +329,9 @@ std::vector<float> load_model_field(const fs::path& p) {
 }
 
 // load_text_table (solver.rs:328-346)
-std::vector<std::vector<double>> load_table(const fs::path& p) {
+std::vector<std::vector<double>> load_table(const Io& io, const fs::path& p) {
   bool ok;
-  std::vector<char> b = read_all(p, ok);
+  std::vector<char> b = read_all(io, p, ok);
   if (!ok) throw Error(fmt("Cannot read %s", p.string().c_str()));
   std::vector<std::vector<double>> rows;
   std::istringstream in(std::string(b.begin(), b.end()));
@@ -304,35 +351,40 @@ std::vector<std::vector<double>> load_table(const fs::path& p) {
   return rows;
 }
 
-// write_npy_f32 / make_npy_bytes (solver.rs:267-285, 361-389)
-void write_npy(const fs::path& p, const float* data, size_t nrows, size_t ncols) {
+// make_npy_bytes (solver.rs:361-389): NumPy v1.0, header padded to a
+// multiple of 64 bytes
+std::vector<char> npy_bytes(const float* data, size_t nrows, size_t ncols) {
   std::string dict = fmt("{'descr': '<f4', 'fortran_order': False, 'shape': (%zu, %zu), }", nrows, ncols);
   const size_t hl = ((dict.size() + 1 + 63) / 64) * 64;
   std::string header = dict;
   while (header.size() < hl - 1) header.push_back(' ');
   header.push_back('\n');
-  std::ofstream f(p, std::ios::binary | std::ios::trunc);
-  if (!f) throw Error(fmt("Cannot create %s", p.string().c_str()));
-  f.write("\x93NUMPY\x01\x00", 8);
+  std::vector<char> b;
+  b.reserve(10 + hl + nrows * ncols * 4);
+  b.insert(b.end(), "\x93NUMPY\x01\x00", "\x93NUMPY\x01\x00" + 8);
   const uint16_t h16 = uint16_t(hl);
-  f.write(reinterpret_cast<const char*>(&h16), 2);
-  f.write(header.data(), std::streamsize(header.size()));
-  f.write(reinterpret_cast<const char*>(data), std::streamsize(nrows * ncols * 4));
-  if (!f) throw Error(fmt("Cannot write %s", p.string().c_str()));
+  b.push_back(char(h16 & 0xff));
+  b.push_back(char(h16 >> 8));
+  b.insert(b.end(), header.begin(), header.end());
+  const char* d = reinterpret_cast<const char*>(data);
+  b.insert(b.end(), d, d + nrows * ncols * 4);
+  return b;
 }
 
-void write_raw(const fs::path& p, const float* data, size_t n, const int32_t* header = nullptr) {
-  std::ofstream f(p, std::ios::binary | std::ios::trunc);
-  if (!f) throw Error(fmt("Cannot write %s", p.string().c_str()));
-  if (header) f.write(reinterpret_cast<const char*>(header), 4);
-  f.write(reinterpret_cast<const char*>(data), std::streamsize(n * 4));
-  if (!f) throw Error(fmt("Cannot write %s", p.string().c_str()));
+// raw little-endian f32, with the model format's i32 count in front if given
+std::vector<char> raw_bytes(const float* data, size_t n, const int32_t* header = nullptr) {
+  std::vector<char> b;
+  b.reserve(n * 4 + (header ? 4 : 0));
+  if (header) b.insert(b.end(), reinterpret_cast<const char*>(header), reinterpret_cast<const char*>(header) + 4);
+  const char* d = reinterpret_cast<const char*>(data);
+  b.insert(b.end(), d, d + n * 4);
+  return b;
 }
 
 // load_npy_f32 (solver.rs:401-428)
-std::vector<float> load_npy(const fs::path& p) {
+std::vector<float> load_npy(const Io& io, const fs::path& p) {
   bool ok;
-  std::vector<char> b = read_all(p, ok);
+  std::vector<char> b = read_all(io, p, ok);
   if (!ok) throw Error(fmt("Cannot load observed traces %s", p.string().c_str()));
   if (b.size() < 10 || std::memcmp(b.data(), "\x93NUMPY", 6) != 0)
     throw Error(fmt("Cannot load observed traces %s: Not a NumPy file", p.string().c_str()));
@@ -357,6 +409,7 @@ void check(int rc) {
 
 struct Problem {
   Config cfg;
+  Io io;
   int32_t nx = 0, nz = 0;
   float dx = 0, dz = 0;
   int64_t npt = 0;
@@ -375,10 +428,10 @@ void solver_new(Problem& P) {
   else if (c.has_model_true) model_dir = c.model_true;
   else throw Error("No model path in config");
   const bool spin = c.spin;
-  std::vector<float> rho = load_model_field(model_dir / "proc000000_rho.bin");
+  std::vector<float> rho = load_model_field(P.io, model_dir / "proc000000_rho.bin");
   const int64_t npt = int64_t(rho.size());
-  std::vector<float> x = load_model_field(model_dir / "proc000000_x.bin");
-  std::vector<float> z = load_model_field(model_dir / "proc000000_z.bin");
+  std::vector<float> x = load_model_field(P.io, model_dir / "proc000000_x.bin");
+  std::vector<float> z = load_model_field(P.io, model_dir / "proc000000_z.bin");
   if (int64_t(x.size()) != npt || int64_t(z.size()) != npt)
     throw Error(fmt("coordinate files hold %zu/%zu points, expected %lld", x.size(), z.size(), (long long)npt));
   int32_t fb = 0;
@@ -396,13 +449,13 @@ void solver_new(Problem& P) {
   const int nf = spin ? 8 : 3;
   for (int q = 1; q < nf; ++q) {
     const char* nm = spin ? names_sp[q] : names_vs[q];
-    fields.push_back(load_model_field(model_dir / (std::string("proc000000_") + nm + ".bin")));
+    fields.push_back(load_model_field(P.io, model_dir / (std::string("proc000000_") + nm + ".bin")));
     if (int64_t(fields.back().size()) < npt)
       throw Error(fmt("Model file proc000000_%s.bin holds %zu values, expected %lld", nm, fields.back().size(),
                       (long long)npt));
   }
   // sources (solver.rs:702-739)
-  auto src_table = load_table(c.sources);
+  auto src_table = load_table(P.io, c.sources);
   const int nsrc = int(src_table.size());
   std::vector<float> sp(size_t(nsrc) * 4);
   for (int s = 0; s < nsrc; ++s) {
@@ -418,7 +471,7 @@ void solver_new(Problem& P) {
   std::vector<float> stf(size_t(3) * nsrc * c.nt);
   check(seiswg_ricker_table(nsrc, int32_t(c.nt), c.dt, sp.data(), stf.data()));
   // stations (solver.rs:741-751)
-  auto rec_table = load_table(c.stations);
+  auto rec_table = load_table(P.io, c.stations);
   for (const auto& r : rec_table) {
     const float rx = float(r.size() > 0 ? r[0] : 0.0), rz = float(r.size() > 1 ? r[1] : 0.0);
     P.rec_k.push_back(seiswg_grid_index(rx, rz, P.dx, P.dz, uint32_t(P.nz)));
@@ -443,17 +496,24 @@ void solver_new(Problem& P) {
 }
 
 struct SnapWriter {
+  const Io* io;
   fs::path out_dir;
   int last_shot;
+  // frames by (shot, step, field), kept when outputs go to a map
+  std::map<std::tuple<int, int, int>, std::vector<float>> frames;
   std::string err;
 };
 
 int snap_cb(void* user, int32_t shot, int32_t it, int32_t field, const float* data, int64_t npt) {
   auto* w = static_cast<SnapWriter*>(user);
-  if (shot != w->last_shot) return 0;   // later runs overwrite the same files (solver.rs:1144-1176)
-  const char* nm = field == SW_F_VY ? "vy" : field == SW_F_VX ? "vx" : field == SW_F_VZ ? "vz" : "ry";
   try {
-    write_raw(w->out_dir / fmt("proc%06d_%s.bin", it, nm), data, size_t(npt));
+    if (w->io->out) w->frames[{shot, it, field}].assign(data, data + npt);
+    // later runs overwrite the same files (solver.rs:1144-1176)
+    if (w->io->disk && shot == w->last_shot) {
+      const char* nm = field == SW_F_VY ? "vy" : field == SW_F_VX ? "vx" : field == SW_F_VZ ? "vz" : "ry";
+      Io disk_only{nullptr, nullptr, true};
+      emit(disk_only, w->out_dir / fmt("proc%06d_%s.bin", it, nm), raw_bytes(data, size_t(npt)));
+    }
   } catch (const std::exception& e) {
     w->err = e.what();
     return 1;
@@ -466,20 +526,15 @@ void run_forward(Problem& P) {
   const Config& c = P.cfg;
   const int nsrc = int(P.src_k.size()), nrec = int(P.rec_k.size()), nt = int(c.nt);
   fs::path trace_dir = c.has_output_traces ? c.output_traces : c.output / "traces";
-  std::error_code ec;
-  fs::create_directories(trace_dir, ec);
-  if (ec) throw Error(fmt("Cannot create %s: %s", trace_dir.string().c_str(), ec.message().c_str()));
-  if (c.save_snapshot > 0) {
-    fs::create_directories(c.output, ec);
-    if (ec) throw Error(fmt("Cannot create %s: %s", c.output.string().c_str(), ec.message().c_str()));
-  }
+  make_dirs(P.io, trace_dir);
+  if (c.save_snapshot > 0) make_dirs(P.io, c.output);
   std::vector<int32_t> runs;
   if (c.combine) runs.push_back(-1);
   else for (int s = 0; s < nsrc; ++s) runs.push_back(s);
   if (runs.empty()) return;
   for (int32_t r : runs)
     log_info(fmt("Forward run  isrc=%s  (%d time steps)", r < 0 ? "all" : std::to_string(r).c_str(), nt));
-  SnapWriter w{c.output, int(runs.size()) - 1, ""};
+  SnapWriter w{&P.io, c.output, int(runs.size()) - 1, {}, ""};
   const int rc = sw_forward_batch(P.ctx, int32_t(runs.size()), runs.data(), int32_t(c.save_snapshot), snap_cb, &w);
   if (rc) throw Error(w.err.empty() ? std::string(sw_last_error()) : w.err);
   sw_timing tm{};
@@ -490,27 +545,48 @@ void run_forward(Problem& P) {
   for (size_t s = 0; s < runs.size(); ++s) {
     const int idx = runs[s] < 0 ? 0 : runs[s];
     auto get = [&](int cc) { check(sw_read_obs(P.ctx, int32_t(s), cc, comp.data())); };
+    auto npy = [&](const char* nm) { emit(P.io, trace_dir / fmt("%s_%06d.npy", nm, idx), npy_bytes(comp.data(), nrec, nt)); };
+    auto raw = [&](const char* nm) { emit(P.io, trace_dir / fmt("%s_%06d.npy", nm, idx), raw_bytes(comp.data(), comp.size())); };
     if (c.sh) {
       get(SW_OBS_Y);
-      write_npy(trace_dir / fmt("uy_%06d.npy", idx), comp.data(), nrec, nt);
-      write_raw(trace_dir / fmt("vy_%06d.npy", idx), comp.data(), comp.size());
+      npy("uy");
+      raw("vy");
     }
     if (c.psv) {
       get(SW_OBS_X);
-      write_npy(trace_dir / fmt("ux_%06d.npy", idx), comp.data(), nrec, nt);
-      write_raw(trace_dir / fmt("vx_%06d.npy", idx), comp.data(), comp.size());
+      npy("ux");
+      raw("vx");
       get(SW_OBS_Z);
-      write_npy(trace_dir / fmt("uz_%06d.npy", idx), comp.data(), nrec, nt);
-      write_raw(trace_dir / fmt("vz_%06d.npy", idx), comp.data(), comp.size());
+      npy("uz");
+      raw("vz");
       if (c.spin) {
         get(SW_OBS_Y);
-        write_npy(trace_dir / fmt("ry_%06d.npy", idx), comp.data(), nrec, nt);
+        npy("ry");
         get(SW_OBS_YI);
-        write_npy(trace_dir / fmt("yi_%06d.npy", idx), comp.data(), nrec, nt);
+        npy("yi");
       }
     }
   }
   log_info("  → traces saved to " + trace_dir.string());
+  // snapshot frames (solver.rs:1104-1117): "snapshot/nframes" = u32 count of
+  // snapshot steps over all runs; "snapshot/<field>.f32" = that field's
+  // frames, run after run, step after step
+  if (P.io.out && !w.frames.empty()) {
+    std::map<std::pair<int, int>, int> steps;
+    std::map<std::string, std::vector<char>> per_field;
+    for (const auto& f : w.frames) {
+      const int shot = std::get<0>(f.first), field = std::get<2>(f.first);
+      steps[{shot, std::get<1>(f.first)}] = 1;
+      const char* nm = field == SW_F_VY ? "vy" : field == SW_F_VX ? "vx" : field == SW_F_VZ ? "vz" : "ry";
+      std::vector<char>& b = per_field[nm];
+      const char* d = reinterpret_cast<const char*>(f.second.data());
+      b.insert(b.end(), d, d + f.second.size() * 4);
+    }
+    const uint32_t n = uint32_t(steps.size());
+    (*P.io.out)["snapshot/nframes"] = std::vector<char>(reinterpret_cast<const char*>(&n),
+                                                         reinterpret_cast<const char*>(&n) + 4);
+    for (auto& f : per_field) (*P.io.out)["snapshot/" + f.first + ".f32"] = std::move(f.second);
+  }
 }
 
 // run_adjoint (solver.rs:1298-1529)
@@ -520,9 +596,7 @@ void run_adjoint(Problem& P) {
   if (c.adj_interval == 0) throw Error("adj_interval must be > 0");
   const uint32_t nsa = c.nt / c.adj_interval;
   if (nsa == 0) throw Error(fmt("adj_interval (%u) >= nt (%u)", c.adj_interval, c.nt));
-  std::error_code ec;
-  fs::create_directories(c.output, ec);
-  if (ec) throw Error(fmt("Cannot create %s: %s", c.output.string().c_str(), ec.message().c_str()));
+  make_dirs(P.io, c.output);
   if (!c.has_traces) throw Error("config [path] traces is required for the adjoint workflow");
   std::vector<int32_t> runs;
   std::vector<int> ref_idx;
@@ -534,7 +608,7 @@ void run_adjoint(Problem& P) {
     for (size_t r = 0; r < runs.size(); ++r) {
       const int i = c.combine ? 0 : ref_idx[r];
       fs::path p = c.traces / fmt("uy_%06d.npy", i);
-      std::vector<float> v = load_npy(p);
+      std::vector<float> v = load_npy(P.io, p);
       if (v.size() < nobs)
         throw Error(fmt("observed traces %s hold %zu values, expected %zu", p.string().c_str(), v.size(), nobs));
       std::copy(v.begin(), v.begin() + nobs, ref.begin() + r * nobs);
@@ -556,8 +630,8 @@ void run_adjoint(Problem& P) {
   check(sw_read_field(P.ctx, 0, SW_F_K_MU, kmu.data()));
   const int32_t hdr = int32_t(P.npt);
   fs::path kp = c.output / "proc000000_kmu.bin";
-  write_raw(kp, kmu.data(), kmu.size());
-  write_raw(c.output / "gradient_kmu.bin", kmu.data(), kmu.size(), &hdr);
+  emit(P.io, kp, raw_bytes(kmu.data(), kmu.size()));
+  emit(P.io, c.output / "gradient_kmu.bin", raw_bytes(kmu.data(), kmu.size(), &hdr));
   log_info("Adjoint: gradient saved to " + kp.string());
 }
 
@@ -602,15 +676,13 @@ double run_misfit(Problem& P, const std::vector<int32_t>& runs, const std::vecto
 }
 
 void write_model_dir(const Problem& P, const fs::path& dir) {
-  std::error_code ec;
-  fs::create_directories(dir, ec);
-  if (ec) throw Error(fmt("Cannot create %s: %s", dir.string().c_str(), ec.message().c_str()));
+  make_dirs(P.io, dir);
   const int32_t hdr = int32_t(P.npt);
   const char* names[] = {"rho", "vp", "vs"};
   for (int q = 0; q < 3; ++q)
-    write_raw(dir / (std::string("proc000000_") + names[q] + ".bin"), P.model[q].data(), size_t(P.npt), &hdr);
-  write_raw(dir / "proc000000_x.bin", P.x.data(), size_t(P.npt), &hdr);
-  write_raw(dir / "proc000000_z.bin", P.z.data(), size_t(P.npt), &hdr);
+    emit(P.io, dir / (std::string("proc000000_") + names[q] + ".bin"), raw_bytes(P.model[q].data(), size_t(P.npt), &hdr));
+  emit(P.io, dir / "proc000000_x.bin", raw_bytes(P.x.data(), size_t(P.npt), &hdr));
+  emit(P.io, dir / "proc000000_z.bin", raw_bytes(P.z.data(), size_t(P.npt), &hdr));
 }
 
 void run_inversion(Problem& P) {
@@ -619,9 +691,7 @@ void run_inversion(Problem& P) {
   if (!c.has_traces) throw Error("config [path] traces is required for the inversion workflow");
   const int nsrc = int(P.src_k.size()), nrec = int(P.rec_k.size()), nt = int(c.nt);
   const size_t npt = size_t(P.npt);
-  std::error_code ec;
-  fs::create_directories(c.output, ec);
-  if (ec) throw Error(fmt("Cannot create %s: %s", c.output.string().c_str(), ec.message().c_str()));
+  make_dirs(P.io, c.output);
   std::vector<int32_t> runs;
   if (c.combine) runs.push_back(-1);
   else for (int s = 0; s < nsrc; ++s) runs.push_back(s);
@@ -634,7 +704,7 @@ void run_inversion(Problem& P) {
       const int idx = c.combine ? 0 : int(r);
       fs::path p = c.traces / fmt("%s_%06d.npy", comp_names[comps[k] == SW_OBS_X ? 0 : comps[k] == SW_OBS_Y ? 1 : 2],
                                   idx);
-      std::vector<float> v = load_npy(p);
+      std::vector<float> v = load_npy(P.io, p);
       if (v.size() < nobs)
         throw Error(fmt("observed traces %s hold %zu values, expected %zu", p.string().c_str(), v.size(), nobs));
       std::copy(v.begin(), v.begin() + nobs, ref.begin() + (r * comps.size() + k) * nobs);
@@ -646,10 +716,15 @@ void run_inversion(Problem& P) {
     const float* f[3] = {rho.data(), a.data(), b.data()};
     check(sw_set_model(P.ctx, SW_MODEL_VPVS, f));
   };
-  std::ofstream mlog(c.output / "misfit.txt", std::ios::trunc);
+  // misfit per accepted iteration, rewritten after each one
+  std::string mlog;
+  auto write_log = [&](uint32_t iter, double J) {
+    mlog += std::to_string(iter) + " " + fmt("%.9e", J) + "\n";
+    emit(P.io, c.output / "misfit.txt", std::vector<char>(mlog.begin(), mlog.end()));
+  };
   double J0 = run_misfit(P, runs, comps, ref);
   log_info(fmt("Inversion: %zu run(s), initial misfit %.6e", runs.size(), J0));
-  mlog << 0 << " " << fmt("%.9e", J0) << "\n";
+  write_log(0, J0);
   std::vector<double> mis(runs.size());
   std::vector<float> kmu(npt), klam(npt), gvs(npt), gvp(npt), vs1(npt), vp1(npt);
   for (uint32_t iter = 1; iter <= c.inv_niter; ++iter) {
@@ -701,14 +776,39 @@ void run_inversion(Problem& P) {
     vp.swap(vp1);
     J0 = J1;
     write_model_dir(P, c.output / fmt("iter_%03u", iter));
-    mlog << iter << " " << fmt("%.9e", J0) << "\n";
-    mlog.flush();
+    write_log(iter, J0);
     log_info(fmt("Inversion: iteration %u accepted, misfit %.6e; model written to %s", iter, J0,
                  (c.output / fmt("iter_%03u", iter)).string().c_str()));
   }
 }
 
+// Solver::new + the workflow of the config (main.rs:80-97)
+void run_workflow(Problem& P) {
+  log_info("Workflow: " + P.cfg.workflow);
+  try {
+    solver_new(P);
+  } catch (const Error& e) {
+    throw Error(std::string("Failed to initialise solver: ") + e.what());
+  }
+  if (P.cfg.workflow == "forward") run_forward(P);
+  else if (P.cfg.workflow == "adjoint") run_adjoint(P);
+  else if (P.cfg.workflow == "inversion") run_inversion(P);
+  else throw Error(fmt("Workflow '%s' is not yet implemented in seiswg", P.cfg.workflow.c_str()));
+  log_info("Done.");
+}
+
+void put_u32(Vfs& v, const char* key, uint32_t x) {
+  v[key] = std::vector<char>(reinterpret_cast<const char*>(&x), reinterpret_cast<const char*>(&x) + 4);
+}
+void put_f32(Vfs& v, const char* key, float x) {
+  v[key] = std::vector<char>(reinterpret_cast<const char*>(&x), reinterpret_cast<const char*>(&x) + 4);
+}
+
 }  // namespace
+
+struct sw_vfs {
+  Vfs files;
+};
 
 namespace sw {
 void set_last_error(const std::string& msg);   // capi.cpp
@@ -786,21 +886,95 @@ SW_API int seiswg_run(const char* config_path) {
     } catch (const Error& e) {
       throw Error(fmt("Failed to load %s: %s", config_path, e.what()));
     }
-    log_info("Workflow: " + P.cfg.workflow);
-    try {
-      solver_new(P);
-    } catch (const Error& e) {
-      throw Error(std::string("Failed to initialise solver: ") + e.what());
-    }
-    if (P.cfg.workflow == "forward") run_forward(P);
-    else if (P.cfg.workflow == "adjoint") run_adjoint(P);
-    else if (P.cfg.workflow == "inversion") run_inversion(P);
-    else throw Error(fmt("Workflow '%s' is not yet implemented in seiswg", P.cfg.workflow.c_str()));
-    log_info("Done.");
+    run_workflow(P);
     return 0;
   } catch (const std::exception& e) {
     sw::set_last_error(e.what());
     fprintf(stderr, "Error: %s\n", e.what());
+    return 1;
+  }
+}
+
+SW_API sw_vfs* seiswg_vfs_new(void) {
+  try {
+    return new sw_vfs;
+  } catch (const std::exception&) {
+    return nullptr;
+  }
+}
+
+SW_API void seiswg_vfs_free(sw_vfs* v) { delete v; }
+
+SW_API int seiswg_vfs_put(sw_vfs* v, const char* key, const void* data, int64_t nbytes) {
+  if (!v || !key || nbytes < 0 || (nbytes > 0 && !data)) {
+    sw::set_last_error("seiswg_vfs_put: bad arguments");
+    return -1;
+  }
+  try {
+    const char* d = static_cast<const char*>(data);
+    v->files[key] = std::vector<char>(d, d + nbytes);
+  } catch (const std::exception& e) {
+    sw::set_last_error(e.what());
+    return -3;
+  }
+  return 0;
+}
+
+SW_API int64_t seiswg_vfs_get(const sw_vfs* v, const char* key, const void** data) {
+  if (!v || !key) return -1;
+  auto it = v->files.find(key);
+  if (it == v->files.end()) return -1;
+  if (data) *data = it->second.data();
+  return int64_t(it->second.size());
+}
+
+SW_API int32_t seiswg_vfs_count(const sw_vfs* v) { return v ? int32_t(v->files.size()) : 0; }
+
+SW_API const char* seiswg_vfs_key(const sw_vfs* v, int32_t i) {
+  if (!v || i < 0 || size_t(i) >= v->files.size()) return nullptr;
+  return std::next(v->files.begin(), i)->first.c_str();
+}
+
+SW_API int seiswg_run_simulation(const char* config_ini, const char* sources_dat, const char* stations_dat,
+                                 const char* config_true_ini, const sw_vfs* files, sw_vfs* out) {
+  try {
+    if (!config_ini || !out) throw Error("seiswg_run_simulation: config text and output map are required");
+    Vfs vfs;
+    if (files) vfs = files->files;
+    Problem P;
+    P.cfg = load_config_text(config_ini, fs::path());
+    // the tables given as text stand for the files the configs name (lib.rs:97-98)
+    auto add_tables = [&](const Config& c) {
+      if (sources_dat) vfs[c.sources.string()] = std::vector<char>(sources_dat, sources_dat + std::strlen(sources_dat));
+      if (stations_dat)
+        vfs[c.stations.string()] = std::vector<char>(stations_dat, stations_dat + std::strlen(stations_dat));
+    };
+    add_tables(P.cfg);
+    // adjoint: a forward run on the true model first, whose traces join the
+    // inputs (lib.rs:106-113)
+    if (P.cfg.workflow == "adjoint" && config_true_ini) {
+      Problem T;
+      T.cfg = load_config_text(config_true_ini, fs::path());
+      add_tables(T.cfg);
+      Vfs obs;
+      T.io = Io{&vfs, &obs, false};
+      run_workflow(T);
+      for (auto& f : obs) vfs[f.first] = std::move(f.second);
+    }
+    Vfs result;
+    P.io = Io{&vfs, &result, false};
+    run_workflow(P);
+    // the scalars lib.rs:126-131 / 166-170 return beside the files
+    put_u32(result, "nt", P.cfg.nt);
+    put_f32(result, "dt", P.cfg.dt);
+    put_u32(result, "nrec", uint32_t(P.rec_k.size()));
+    put_u32(result, "npt", uint32_t(P.npt));
+    put_u32(result, "nx", uint32_t(P.nx));
+    put_u32(result, "nz", uint32_t(P.nz));
+    for (auto& f : result) out->files[f.first] = std::move(f.second);
+    return 0;
+  } catch (const std::exception& e) {
+    sw::set_last_error(e.what());
     return 1;
   }
 }

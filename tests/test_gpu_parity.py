@@ -388,6 +388,71 @@ def test_cli_adjoint_example_reproduces_kmu_fixture(seiswg, tmp_path):
     assert hdr == got.size
 
 
+def example_files(d, dirs):
+    """The files of the example's model directories, keyed by relative path."""
+    return {f"{sub}/{n}": open(os.path.join(d, sub, n), "rb").read()
+            for sub in dirs for n in sorted(os.listdir(os.path.join(d, sub)))}
+
+
+@pytest.mark.parametrize("ex,nframes,fields", [("forward", 11, ["vy"]), ("spin", 9, ["vx", "vz", "ry"])])
+def test_in_memory_run_equals_cli_files(seiswg, tmp_path, monkeypatch, ex, nframes, fields):
+    """seiswg_run_simulation on the forward and spin examples: every trace
+    file the CLI writes, byte for byte under the same relative path; the
+    snapshot frames equal to the CLI's proc<step>_<field>.bin files in step
+    order; the grid scalars; and nothing written to disk."""
+    d = E.materialise(ex, str(tmp_path))
+    run_cli(os.path.join(d, "config.ini"))
+    empty = tmp_path / "cwd"
+    empty.mkdir()
+    monkeypatch.chdir(empty)
+    text = lambda n: open(os.path.join(d, n)).read()
+    out = seiswg.run_simulation(text("config.ini"), text("sources.dat"), text("stations.dat"),
+                                example_files(d, ["model"]))
+    assert os.listdir(empty) == []
+    written = set()
+    for root, _, names in os.walk(os.path.join(d, "output")):
+        for n in names:
+            if not n.startswith("proc"):
+                written.add(os.path.relpath(os.path.join(root, n), d))
+    traces = {k for k in out if k.endswith(".npy")}
+    assert traces == written and traces
+    for k in traces:
+        assert out[k] == open(os.path.join(d, k), "rb").read(), k
+    assert np.frombuffer(out["snapshot/nframes"], "<u4")[0] == nframes
+    snap_dir = os.path.join(d, "output")
+    steps = sorted(int(n[4:10]) for n in os.listdir(snap_dir) if n.startswith("proc") and n.endswith(f"_{fields[0]}.bin"))
+    assert len(steps) == nframes
+    for f in fields:
+        cli = b"".join(open(os.path.join(snap_dir, f"proc{it:06d}_{f}.bin"), "rb").read() for it in steps)
+        assert out[f"snapshot/{f}.f32"] == cli, f
+    u32 = lambda k: int(np.frombuffer(out[k], "<u4")[0])
+    ini = E.ini_values(os.path.join(d, "config.ini"))
+    assert u32("nt") == ini["nt"] and np.frombuffer(out["dt"], "<f4")[0] == np.float32(ini["dt"])
+    rows = [ln for ln in text("stations.dat").splitlines() if ln.strip() and not ln.strip().startswith("#")]
+    assert u32("nx") * u32("nz") == u32("npt") and u32("nrec") == len(rows)
+
+
+def test_in_memory_adjoint_with_true_config_reproduces_kmu_fixture(seiswg, tmp_path, monkeypatch):
+    """The adjoint example in memory, as lib.rs runs it: the true-model
+    forward (config_true.ini) feeds its traces to the adjoint run; K_mu
+    against the reference fixture, the model-format copy, and nothing on disk."""
+    d = E.materialise("adjoint", str(tmp_path))
+    empty = tmp_path / "cwd"
+    empty.mkdir()
+    monkeypatch.chdir(empty)
+    text = lambda n: open(os.path.join(d, n)).read()
+    out = seiswg.run_simulation(text("config.ini"), text("sources.dat"), text("stations.dat"),
+                                example_files(d, ["model_init", "model_true"]), text("config_true.ini"))
+    assert os.listdir(empty) == []
+    got = np.frombuffer(out["output/proc000000_kmu.bin"], "<f4")
+    ref = E.fixture("adjoint", "proc000000_kmu.bin")
+    print(f"kmu rl2 {O.rel_l2(got, ref):.2e} max {O.max_rel_err(got, ref):.2e}")
+    assert O.max_rel_err(got, ref) <= 1e-4 and O.rel_l2(got, ref) <= 1e-5
+    g = out["output/gradient_kmu.bin"]
+    assert np.frombuffer(g[:4], "<i4")[0] == got.size and g[4:] == out["output/proc000000_kmu.bin"]
+    assert not any(k.startswith("obs_traces/") for k in out)
+
+
 def test_cli_inversion_lowers_the_misfit(seiswg, tmp_path):
     """workflow = inversion on the adjoint example: observed traces from the
     true model, two steepest-descent iterations from the initial model; the

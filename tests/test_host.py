@@ -163,3 +163,44 @@ stations = r.dat
 """
     r = run_cli([write_ini(tmp_path / "c.ini", body)])
     assert r.returncode != 0 and "Cannot read model file" in r.stderr and "mdl" in r.stderr
+
+
+def test_vfs_map_round_trip(seiswg):
+    v = seiswg.Vfs({"b/x.bin": b"\x00\x01\x02", "a.txt": b"hello"})
+    v["c"] = b""
+    v["a.txt"] = b"again\x00with nul"
+    assert v.keys() == ["a.txt", "b/x.bin", "c"]
+    assert v["a.txt"] == b"again\x00with nul" and v["b/x.bin"] == b"\x00\x01\x02" and v["c"] == b""
+    with pytest.raises(KeyError):
+        v["missing"]
+    v.close()
+
+
+@pytest.mark.parametrize("body,msg", [
+    (BASE.replace("mode = forward\n", ""), "[workflow] mode is required"),
+    (BASE.replace("nt = 10", "nt = ten"), "[solver] nt must be an integer"),
+    (BASE.replace("sources = sources.dat\n", ""), "[path] sources is required"),
+    (BASE.replace("model_init = model\n", ""), "No model path in config"),
+    (BASE, "Cannot read model file model/proc000000_rho.bin"),
+])
+def test_in_memory_run_failure_classes(seiswg, body, msg):
+    """seiswg_run_simulation fails as the CLI does, before any device work,
+    and looks for nothing on disk (no model files are given)."""
+    with pytest.raises(seiswg.SeiswgError) as e:
+        seiswg.run_simulation(body, "0 0\n", "0 0\n", {})
+    assert msg in str(e.value)
+
+
+def test_in_memory_run_reads_model_files_from_the_map(seiswg, tmp_path):
+    """Model files come from the map under the path the config resolves them
+    to, also when the config writes it with a leading "./": a truncated vp
+    file in the map stops the run at the same check as the CLI's."""
+    d = E.materialise("forward", str(tmp_path))
+    files = {f"model/{n}": open(os.path.join(d, "model", n), "rb").read() for n in os.listdir(os.path.join(d, "model"))}
+    files["model/proc000000_vp.bin"] = files["model/proc000000_vp.bin"][:1000]
+    ini = open(os.path.join(d, "config.ini")).read()
+    tables = [open(os.path.join(d, n)).read() for n in ("sources.dat", "stations.dat")]
+    for text in (ini, ini.replace("model_init    = model", "model_init    = ./model")):
+        with pytest.raises(seiswg.SeiswgError) as e:
+            seiswg.run_simulation(text, *tables, files)
+        assert "too short" in str(e.value) and "proc000000_vp.bin" in str(e.value)

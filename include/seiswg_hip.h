@@ -146,7 +146,13 @@ SW_API int sw_adjoint_plain(sw_ctx* ctx, int32_t nruns, const int32_t* isrc, con
  * checkpointed every ckpt_every steps and recomputed segment by segment.
  * shots_per_launch shots advance together (0 = all).  The gradients w.r.t.
  * the stored rho, mu, lambda are summed over shots into SW_F_K_RHO,
- * SW_F_K_MU, SW_F_K_LAM (read with sw_read_field); misfit[s] = J of shot s. */
+ * SW_F_K_MU, SW_F_K_LAM (read with sw_read_field); misfit[s] = J of shot s.
+ * The adjoint keeps one set of kernel accumulators per workgroup: with
+ * several shots per workgroup (sw_set_tiling) their contributions are added
+ * in shot order each step, then the sets in a fixed order at the end.
+ * Results are deterministic; with one shot per workgroup they are the same
+ * bit for bit for any checkpoint interval and shots_per_launch, and other
+ * shapes agree with them to FP32 rounding. */
 SW_API int sw_gradient(sw_ctx* ctx, int32_t nshots, const int32_t* isrc, const float* obs_ref, int32_t ckpt_every,
                        int32_t shots_per_launch, double* misfit);
 
@@ -168,7 +174,9 @@ SW_API int sw_multi_forward(sw_multi* m, const int32_t* nshots, const int32_t* c
  * aligned.  Problems with at least `merge limit` shot-cells per step
  * (nshots * nx * nz; default 3 * 2^17) fill the GPU on their own and run on their
  * own launches (see sw_multi_set_workers); the smaller ones advance together
- * in merged launches.  Results do not depend on the split. */
+ * in merged launches, which run one shot per workgroup: their results equal
+ * sw_gradient's with one shot per workgroup bit for bit (to FP32 rounding
+ * for a context whose tile puts several shots in a workgroup). */
 SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc,
                              const float* const* obs_ref, int32_t ckpt_every, double* const* misfit);
 SW_API int sw_multi_set_merge_limit(sw_multi* m, int64_t shot_cells);

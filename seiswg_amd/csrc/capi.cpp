@@ -159,6 +159,7 @@ void set_last_error(const std::string& msg) { g_err = msg; }
 // bookkeeping of a running gradient (one batch of shots)
 struct GradPlan {
   int K = 1, nck = 0, nb = 0, ncomp = 1, naux = 0;
+  int kgroup = 1;   // shots per kernel-accumulator set: the adjoint launches' shots per workgroup
   size_t state_floats = 0;
   std::vector<double> mis;
 };
@@ -1410,6 +1411,7 @@ int grad_begin(sw_ctx* c, int ckpt_every) {
 // checkpoint storage
 int grad_batch_setup(sw_ctx* c, int nb, const int32_t* isrc) {
   GradPlan& P = c->gp;
+  P.kgroup = 1;   // merged launches: one shot per workgroup
   const int nrec = c->nrec;
   std::vector<std::vector<Entry>> inj(nb), ainj(nb);
   for (int s = 0; s < nb; ++s) {
@@ -1485,10 +1487,11 @@ int grad_segment_setup(sw_ctx* c, int q, int& n0, int& n1) {
   return 0;
 }
 
-// sum the batch's per-shot kernels into the context's K fields (fixed order)
+// sum the batch's kernel accumulators (one per kgroup shots) into the
+// context's K fields (fixed order)
 int grad_reduce(sw_ctx* c) {
   const size_t npt = size_t(c->npt);
-  const int nb = c->gp.nb;
+  const int nb = c->gp.nb / c->gp.kgroup;
   HIPCHK(launch_reduce_shots(c->kshot.p + size_t(K_RHO) * npt, c->k_rho.p, nb, int64_t(npt), NK * int64_t(npt),
                              c->stream));
   HIPCHK(launch_reduce_shots(c->kshot.p + size_t(K_MU) * npt, c->kmu.p, nb, int64_t(npt), NK * int64_t(npt),
@@ -1515,6 +1518,7 @@ SW_API int sw_gradient(sw_ctx* c, int32_t nshots, const int32_t* isrc, const flo
   for (int b0 = 0; b0 < nshots; b0 += S) {
     const int nb = std::min(S, nshots - b0);
     if ((rc = grad_batch_setup(c, nb, isrc + b0))) return rc;
+    c->gp.kgroup = c->tile.group;
     // forward with checkpoints (traces recorded)
     for (int q = 0; q < P.nck; ++q) {
       if ((rc = grad_save_ckpt(c, q))) return rc;

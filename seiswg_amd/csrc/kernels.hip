@@ -1481,10 +1481,21 @@ __device__ __forceinline__ float ndp(float fm1, float f0, float fp1, float fp2, 
   return -div_rc(9.0f * (fp1 - f0) - (fp2 - fm1), d.y, d.r);
 }
 
+// sum of column t's values of the S shot groups of a workgroup, in shot order
+template <int S, int NT>
+__device__ __forceinline__ float shot_sum(const float* v, int t) {
+  float acc = v[t];
+  for (int s = 1; s < S; ++s) acc = acc + v[s * NT + t];
+  return acc;
+}
+
 template <int NT, int S>
 __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   __shared__ float lds_t[2][S * NT];
   __shared__ float lds_q[2][S * NT];
+  // several shots per workgroup: each row's K_rho, K_mu contributions of
+  // every shot, summed over the shots one iteration later
+  __shared__ float lds_k[S > 1 ? 2 : 1][S > 1 ? 2 : 1][S > 1 ? S * NT : 1];
   const TileGeom g = tile_geom<NT, S>(a, logical);
   const int t = g.t, lane = g.lane, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1497,7 +1508,12 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
   const auto raux =
       make_rsrc(a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt), NAUX_SH * npt4);
-  const auto rk = make_rsrc(a.kgrad + int64_t(g.shot) * NK * int64_t(a.npt), NK * npt4);
+  // one accumulator set per workgroup, as in adj_psv (field f summed by
+  // shot group f % S)
+  const auto rk = make_rsrc(a.kgrad + int64_t(g.shot / S) * NK * int64_t(a.npt), NK * npt4);
+  const int sub = g.base / NT;
+  const bool own_r = S == 1 || sub == 0, own_m = S == 1 || sub == 1 % S;
+  constexpr int KR_LAG = S > 1 ? 1 : 0, KS_LAG = S > 1 ? 3 : 2;   // K rows behind iteration m
   const uint32_t o_vy = 4u * a.slot_off[S_VY], o_sxy = 4u * a.slot_off[S_SXY], o_szy = 4u * a.slot_off[S_SZY];
   const uint32_t m_rho = MF_RHO * npt4, m_irho = MF_IRHO * npt4, m_mu = MF_MU * npt4;
   const uint32_t k_rho = K_RHO * npt4, k_mu = K_MU * npt4;
@@ -1534,11 +1550,13 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   // vbar, rho, 1/rho, ds and K_rho rows m; strains and K_mu rows m-2 of
   // iteration m.  D = 2, or 1 with several shots per workgroup (registers).
   constexpr int D = S > 1 ? 1 : 2;
+  static_assert(S == 1 || D == 1, "the shot sums' epilogue reads prefetch slot 0");
   float pf_sbx[D], pf_sbz[D], pf_mu[D], pf_vb[D], pf_rho[D], pf_irho[D], pf_ds[D], pf_ex[D], pf_ez[D], pf_kr[D],
       pf_km[D];
   float bx_next = taper_x(a.bx, i0 - 1, nx);
   for (int k = 0; k < D; ++k) {
     const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k), rm2 = g.row(i0 - 3 + k);
+    const uint32_t rkr = g.row(i0 - 1 + k - KR_LAG), rks = g.row(i0 - 1 + k - KS_LAG);
     pf_sbx[k] = bload(rin, jc4, o_sxy + r1);
     pf_sbz[k] = bload(rin, jc4, o_szy + r1);
     pf_mu[k] = bload(rmod, jc4, m_mu + r1);
@@ -1546,10 +1564,10 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     pf_rho[k] = bload(rmod, jc4, m_rho + r0);
     pf_irho[k] = bload(rmod, jc4, m_irho + r0);
     pf_ds[k] = bload(raux, jc4, A_DSY * npt4 + r0);
-    pf_kr[k] = bload(rk, jc4, k_rho + r0);
+    pf_kr[k] = own_r ? bload(rk, jc4, k_rho + rkr) : 0.0f;
     pf_ex[k] = bload(raux, jc4, A_EXY * npt4 + rm2);
     pf_ez[k] = bload(raux, jc4, A_EZY * npt4 + rm2);
-    pf_km[k] = bload(rk, jc4, k_mu + rm2);
+    pf_km[k] = own_m ? bload(rk, jc4, k_mu + rks) : 0.0f;
   }
 
   // one barrier per row (as in the forward step): t of row i0-1 goes to
@@ -1571,6 +1589,7 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     bx_next = taper_x(a.bx, m + 1, nx);
     {
       const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
+      const uint32_t rkr = g.row(m + D - KR_LAG), rks = g.row(m + D - KS_LAG);
       pf_sbx[f] = bload(rin, jc4, o_sxy + r3);
       pf_sbz[f] = bload(rin, jc4, o_szy + r3);
       pf_mu[f] = bload(rmod, jc4, m_mu + r3);
@@ -1578,10 +1597,10 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
       pf_rho[f] = bload(rmod, jc4, m_rho + r2);
       pf_irho[f] = bload(rmod, jc4, m_irho + r2);
       pf_ds[f] = bload(raux, jc4, A_DSY * npt4 + r2);
-      pf_kr[f] = bload(rk, jc4, k_rho + r2);
+      if (own_r) pf_kr[f] = bload(rk, jc4, k_rho + rkr);
       pf_ex[f] = bload(raux, jc4, A_EXY * npt4 + r0);
       pf_ez[f] = bload(raux, jc4, A_EZY * npt4 + r0);
-      pf_km[f] = bload(rk, jc4, k_mu + r0);
+      if (own_m) pf_km[f] = bload(rk, jc4, k_mu + rks);
     }
 
     // adjoint "velocity" at row m
@@ -1599,10 +1618,11 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     const float b = (bxm * bzb) * bzt;
     const float gv = b * vb;
     const float qv = div_rc(dt * gv, rho, irho);     // dt g / rho
+    const float cr = (-qv * ds) * irho;      // K_rho += -dt g ds / rho^2
     if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
       bstore(gv, rout, jc4, o_vy + ro);
-      bstore(kr + (-qv * ds) * irho, rk, jc4, k_rho + ro);      // K_rho += -dt g ds / rho^2
+      if constexpr (S == 1) bstore(kr + cr, rk, jc4, k_rho + ro);
     }
     q[s3] = xq(m) ? qv : 0.0f;
     const float qz = zq ? qv : 0.0f;
@@ -1620,7 +1640,20 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
       const float nsz = sbz[s0] + dzq[s0];
       bstore(nsx, rout, jc4, o_sxy + ro);
       bstore(nsz, rout, jc4, o_szy + ro);
-      bstore(km + dt * (sbx[s0] * ex + sbz[s0] * ez), rk, jc4, k_mu + ro);
+    }
+    const float cm = dt * (sbx[s0] * ex + sbz[s0] * ez);
+    if constexpr (S == 1) {
+      if (i >= i0 && g.jst) bstore(km + cm, rk, jc4, k_mu + uint32_t(i) * g.rowb);
+    } else {
+      // the previous iteration's contributions (K_rho row m-1, K_mu row m-3)
+      if (g.jst) {
+        if (own_r && m - 1 >= i0 && m - 1 < i1)
+          bstore(kr + shot_sum<S, NT>(lds_k[par ^ 1][0], t), rk, jc4, k_rho + uint32_t(m - 1) * g.rowb);
+        if (own_m && m - 3 >= i0)
+          bstore(km + shot_sum<S, NT>(lds_k[par ^ 1][1], t), rk, jc4, k_mu + uint32_t(m - 3) * g.rowb);
+      }
+      lds_k[par][0][lane] = cr;
+      lds_k[par][1][lane] = cm;
     }
   };
 
@@ -1635,6 +1668,13 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     body(Phase<3>{}, m);
     if (++m > mlast) break;
   }
+  if constexpr (S > 1) {
+    // the last iteration's K_mu contributions (row i1 - 1)
+    __syncthreads();
+    const int par = (mlast - (i0 - 1)) & 1, r = mlast - 2;
+    if (g.jst && r >= i0 && own_m)
+      bstore(pf_km[0] + shot_sum<S, NT>(lds_k[par][1], t), rk, jc4, k_mu + uint32_t(r) * g.rowb);
+  }
   if (a.advance) advance_clock(a.clock, it, -1, a.nblocks);
 }
 
@@ -1642,6 +1682,9 @@ template <int NT, int S>
 __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   __shared__ float lds_t[2][2][S * NT];   // z-masked t_xz, t_zz of row m
   __shared__ float lds_q[2][2][S * NT];   // z-masked qx, qz of row m
+  // several shots per workgroup: each row's K_rho, K_lambda, K_mu
+  // contributions of every shot, summed over the shots one iteration later
+  __shared__ float lds_k[S > 1 ? 2 : 1][S > 1 ? 3 : 1][S > 1 ? S * NT : 1];
   const TileGeom g = tile_geom<NT, S>(a, logical);
   const int t = g.t, lane = g.lane, j = g.j, nx = a.nx, nz = a.nz, i0 = g.i0, i1 = g.i1;
   const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1654,7 +1697,14 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
   const auto raux = make_rsrc(a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt),
                               NAUX_PSV * npt4);
-  const auto rk = make_rsrc(a.kgrad + int64_t(g.shot) * NK * int64_t(a.npt), NK * npt4);
+  // kernel accumulators: one set per workgroup ([shot / S][NK][npt]); with
+  // S > 1 the S shots' contributions are added in shot order by the shot
+  // group that owns the field (field f: group f % S), one read-modify-write
+  // per field and cell instead of S
+  const auto rk = make_rsrc(a.kgrad + int64_t(g.shot / S) * NK * int64_t(a.npt), NK * npt4);
+  const int sub = g.base / NT;
+  const bool own_r = S == 1 || sub == 0, own_l = S == 1 || sub == 1 % S, own_m = S == 1 || sub == 2 % S;
+  constexpr int KR_LAG = S > 1 ? 1 : 0, KS_LAG = S > 1 ? 3 : 2;   // K rows behind iteration m
   const uint32_t o_vx = 4u * a.slot_off[S_VX], o_vz = 4u * a.slot_off[S_VZ];
   const uint32_t o_sxx = 4u * a.slot_off[S_SXX], o_szz = 4u * a.slot_off[S_SZZ], o_sxz = 4u * a.slot_off[S_SXZ];
   const uint32_t m_rho = MF_RHO * npt4, m_irho = MF_IRHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4;
@@ -1700,11 +1750,13 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   // m+1; vbar, rho, 1/rho, ds and K_rho rows m; strains, K_lambda and K_mu
   // rows m-2 of iteration m.  D = 2, or 1 with several shots per workgroup.
   constexpr int D = S > 1 ? 1 : 2;
+  static_assert(S == 1 || D == 1, "the shot sums' epilogue reads prefetch slot 0");
   float pf_sxx[D], pf_szz[D], pf_sxz[D], pf_lam[D], pf_mu[D], pf_vbx[D], pf_vbz[D], pf_rho[D], pf_irho[D];
   float pf_dsx[D], pf_dsz[D], pf_exx[D], pf_ezz[D], pf_exz[D], pf_kr[D], pf_kl[D], pf_km[D];
   float bx_next = taper_x(a.bx, i0 - 1, nx);
   for (int k = 0; k < D; ++k) {
     const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k), rm2 = g.row(i0 - 3 + k);
+    const uint32_t rkr = g.row(i0 - 1 + k - KR_LAG), rks = g.row(i0 - 1 + k - KS_LAG);
     pf_sxx[k] = bload(rin, jc4, o_sxx + r1);
     pf_szz[k] = bload(rin, jc4, o_szz + r1);
     pf_sxz[k] = bload(rin, jc4, o_sxz + r1);
@@ -1716,12 +1768,12 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     pf_irho[k] = bload(rmod, jc4, m_irho + r0);
     pf_dsx[k] = bload(raux, jc4, A_DSX * npt4 + r0);
     pf_dsz[k] = bload(raux, jc4, A_DSZ * npt4 + r0);
-    pf_kr[k] = bload(rk, jc4, k_rho + r0);
+    pf_kr[k] = own_r ? bload(rk, jc4, k_rho + rkr) : 0.0f;
     pf_exx[k] = bload(raux, jc4, A_EXX * npt4 + rm2);
     pf_ezz[k] = bload(raux, jc4, A_EZZ * npt4 + rm2);
     pf_exz[k] = bload(raux, jc4, A_EXZ * npt4 + rm2);
-    pf_kl[k] = bload(rk, jc4, k_lam + rm2);
-    pf_km[k] = bload(rk, jc4, k_mu + rm2);
+    pf_kl[k] = own_l ? bload(rk, jc4, k_lam + rks) : 0.0f;
+    pf_km[k] = own_m ? bload(rk, jc4, k_mu + rks) : 0.0f;
   }
 
   // one barrier per row: t of row i0-1 goes to lds_t[0]
@@ -1745,6 +1797,7 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     bx_next = taper_x(a.bx, m + 1, nx);
     {
       const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
+      const uint32_t rkr = g.row(m + D - KR_LAG), rks = g.row(m + D - KS_LAG);
       pf_sxx[f] = bload(rin, jc4, o_sxx + r3);
       pf_szz[f] = bload(rin, jc4, o_szz + r3);
       pf_sxz[f] = bload(rin, jc4, o_sxz + r3);
@@ -1756,12 +1809,12 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
       pf_irho[f] = bload(rmod, jc4, m_irho + r2);
       pf_dsx[f] = bload(raux, jc4, A_DSX * npt4 + r2);
       pf_dsz[f] = bload(raux, jc4, A_DSZ * npt4 + r2);
-      pf_kr[f] = bload(rk, jc4, k_rho + r2);
+      if (own_r) pf_kr[f] = bload(rk, jc4, k_rho + rkr);
       pf_exx[f] = bload(raux, jc4, A_EXX * npt4 + r0);
       pf_ezz[f] = bload(raux, jc4, A_EZZ * npt4 + r0);
       pf_exz[f] = bload(raux, jc4, A_EXZ * npt4 + r0);
-      pf_kl[f] = bload(rk, jc4, k_lam + r0);
-      pf_km[f] = bload(rk, jc4, k_mu + r0);
+      if (own_l) pf_kl[f] = bload(rk, jc4, k_lam + rks);
+      if (own_m) pf_km[f] = bload(rk, jc4, k_mu + rks);
     }
 
     // adjoint "velocity" at row m
@@ -1784,11 +1837,12 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     const float b = (bxm * bzb) * bzt;
     const float gx = b * vbx, gz = b * vbz;
     const float qxv = div_rc(dt * gx, rho, irho), qzv = div_rc(dt * gz, rho, irho);   // dt g / rho
+    const float cr = (-(qxv * dsx + qzv * dsz)) * irho;   // K_rho += -dt g.ds / rho^2
     if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
       bstore(gx, rout, jc4, o_vx + ro);
       bstore(gz, rout, jc4, o_vz + ro);
-      bstore(kr + (-(qxv * dsx + qzv * dsz)) * irho, rk, jc4, k_rho + ro);   // K_rho += -dt g.ds / rho^2
+      if constexpr (S == 1) bstore(kr + cr, rk, jc4, k_rho + ro);
     }
     qx[s3] = xq(m) ? qxv : 0.0f;
     qz[s3] = xq(m) ? qzv : 0.0f;
@@ -1813,8 +1867,29 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
       bstore(nxx, rout, jc4, o_sxx + ro);
       bstore(nzz, rout, jc4, o_szz + ro);
       bstore(nxz, rout, jc4, o_sxz + ro);
-      bstore(kl + dt * (sbxx[s0] + sbzz[s0]) * (exx + ezz), rk, jc4, k_lam + ro);
-      bstore(km + dt * (2.0f * sbxx[s0] * exx + 2.0f * sbzz[s0] * ezz + sbxz[s0] * exz), rk, jc4, k_mu + ro);
+    }
+    const float cl = dt * (sbxx[s0] + sbzz[s0]) * (exx + ezz);
+    const float cm = dt * (2.0f * sbxx[s0] * exx + 2.0f * sbzz[s0] * ezz + sbxz[s0] * exz);
+    if constexpr (S == 1) {
+      if (i >= i0 && g.jst) {
+        const uint32_t ro = uint32_t(i) * g.rowb;
+        bstore(kl + cl, rk, jc4, k_lam + ro);
+        bstore(km + cm, rk, jc4, k_mu + ro);
+      }
+    } else {
+      // the previous iteration's contributions (K_rho row m-1, K_lambda and
+      // K_mu row m-3), complete in LDS since this iteration's barrier
+      if (g.jst) {
+        if (own_r && m - 1 >= i0 && m - 1 < i1)
+          bstore(kr + shot_sum<S, NT>(lds_k[par ^ 1][0], t), rk, jc4, k_rho + uint32_t(m - 1) * g.rowb);
+        if (m - 3 >= i0) {
+          if (own_l) bstore(kl + shot_sum<S, NT>(lds_k[par ^ 1][1], t), rk, jc4, k_lam + uint32_t(m - 3) * g.rowb);
+          if (own_m) bstore(km + shot_sum<S, NT>(lds_k[par ^ 1][2], t), rk, jc4, k_mu + uint32_t(m - 3) * g.rowb);
+        }
+      }
+      lds_k[par][0][lane] = cr;
+      lds_k[par][1][lane] = cl;
+      lds_k[par][2][lane] = cm;
     }
   };
 
@@ -1828,6 +1903,15 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     if (++m > mlast) break;
     body(Phase<3>{}, m);
     if (++m > mlast) break;
+  }
+  if constexpr (S > 1) {
+    // the last iteration's K_lambda / K_mu contributions (row i1 - 1)
+    __syncthreads();
+    const int par = (mlast - (i0 - 1)) & 1, r = mlast - 2;
+    if (g.jst && r >= i0) {
+      if (own_l) bstore(pf_kl[0] + shot_sum<S, NT>(lds_k[par][1], t), rk, jc4, k_lam + uint32_t(r) * g.rowb);
+      if (own_m) bstore(pf_km[0] + shot_sum<S, NT>(lds_k[par][2], t), rk, jc4, k_mu + uint32_t(r) * g.rowb);
+    }
   }
   if (a.advance) advance_clock(a.clock, it, -1, a.nblocks);
 }

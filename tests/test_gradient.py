@@ -131,16 +131,36 @@ def test_gpu_gradient_matches_oracle_adjoint(seiswg, sh, threads):
 @pytest.mark.gpu
 @pytest.mark.parametrize("sh", [True, False], ids=["sh", "psv"])
 def test_gpu_gradient_independent_of_checkpoint_interval_and_batching(seiswg, sh):
+    """One shot per workgroup: misfits and kernels bit for bit whatever the
+    checkpoint interval and the shots per launch.  Two and four shots per
+    workgroup (their kernel contributions summed per step in the workgroup,
+    then over time): the same kernels up to FP32 rounding, and bit for bit
+    across checkpoint intervals."""
     p, stf, nc = problem(sh, 50, 44, 90, seed=4)
     runs = [0, 1, 2, -1]
     obs = (np.random.default_rng(7).standard_normal((len(runs), nc, len(p.rec_idx), p.nt)) * 1e-3).astype(np.float32)
     c = gpu_problem(seiswg, p, stf)
+    c.set_tiling(128, 9, 1)
     ref = c.gradient(runs, obs, ckpt_every=90)
     for ck, spl in ((7, 0), (25, 1), (11, 3)):
         got = c.gradient(runs, obs, ckpt_every=ck, shots_per_launch=spl)
         assert np.array_equal(got[0], ref[0])
         for a, b in zip(got[1:], ref[1:]):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (ck, spl)
+    for group in (2, 4):
+        c.set_tiling(128, 9, group)
+        g90 = c.gradient(runs, obs, ckpt_every=90)
+        assert c.tiling()[2] == group
+        g7 = c.gradient(runs, obs, ckpt_every=7)
+        assert np.array_equal(g90[0], ref[0]) and np.array_equal(g7[0], ref[0])
+        for a, b, r in zip(g90[1:], g7[1:], ref[1:]):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), group
+            if np.any(r):
+                err = O.rel_l2(a, r)
+                print(f"group {group}: rel-L2 vs one shot per workgroup {err:.2e}")
+                assert err <= 1e-6, group
+            else:
+                assert not np.any(a)
 
 
 @pytest.mark.gpu
@@ -166,8 +186,8 @@ def test_gpu_gradient_two_step_launches_equal_one_step_launches(seiswg, ck):
 
 def multi_problems(seiswg):
     """Uneven problems: SH and P-SV, different grids, nt, dt and tile shapes."""
-    specs = [(True, 40, 36, 70, 0.1, (64, 7)), (False, 52, 30, 90, 0.08, (128, 16)),
-             (False, 34, 70, 55, 0.1, (64, 32)), (True, 60, 44, 90, 0.05, (128, 9))]
+    specs = [(True, 40, 36, 70, 0.1, (64, 7, 1)), (False, 52, 30, 90, 0.08, (128, 16, 1)),
+             (False, 34, 70, 55, 0.1, (64, 32, 1)), (True, 60, 44, 90, 0.05, (128, 9, 1))]
     out = []
     for q, (sh, nx, nz, nt, dt, tl) in enumerate(specs):
         p, stf, nc = problem(sh, nx, nz, nt, seed=10 + q)
@@ -200,7 +220,8 @@ def test_multi_problem_forward_equals_single_runs(seiswg):
 def test_multi_problem_gradient_equals_single_gradients(seiswg, merge_limit):
     """Merged launches (default: all four problems are small), every problem
     on its own launches (limit 0), and a split (limit 3000 shot-cells: two
-    problems merged, two on their own) give the single-problem gradients."""
+    problems merged, two on their own) give the single-problem gradients
+    (one shot per workgroup in both, as merged launches run)."""
     probs = multi_problems(seiswg)
     runs = [[0, -1], [1, 2], [0], [2, 0, 1]]
     refs = [(np.random.default_rng(q).standard_normal((len(r), nc, len(p.rec_idx), p.nt)) * 1e-3).astype(np.float32)

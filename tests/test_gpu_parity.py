@@ -453,6 +453,36 @@ def test_in_memory_adjoint_with_true_config_reproduces_kmu_fixture(seiswg, tmp_p
     assert not any(k.startswith("obs_traces/") for k in out)
 
 
+def test_in_memory_per_source_runs_and_observed_traces_from_the_map(seiswg, tmp_path):
+    """The adjoint example's true-model forward in memory with snapshots on
+    (12 per-source runs, a snapshot every 1600 of 4800 steps): 24 frames,
+    run after run and step after step, equal to the device's snapshots of the
+    same batch; then the adjoint workflow with those traces passed in the
+    input map (no true config) reproduces the reference's K_mu fixture."""
+    d = E.materialise("adjoint", str(tmp_path))
+    text = lambda n: open(os.path.join(d, n)).read()
+    true_ini = text("config_true.ini").replace("save_snapshot   = 0", "save_snapshot   = 1600")
+    assert "save_snapshot   = 1600" in true_ini
+    files = example_files(d, ["model_init", "model_true"])
+    fwd = seiswg.run_simulation(true_ini, text("sources.dat"), text("stations.dat"), files)
+    assert np.frombuffer(fwd["snapshot/nframes"], "<u4")[0] == 24
+    _, p, _ = load_example(tmp_path, "adjoint", "config_true.ini")
+    c = seiswg.Context.from_problem(p)
+    c.set_sources(p.src_idx, seiswg.ricker_table([[r[3], r[4], r[5], r[6]] for r in p.src_rows], p.nt, p.dt))
+    c.set_receivers(p.rec_idx)
+    snaps = {}
+    c.forward_batch(list(range(12)), 1600, snaps)
+    want = b"".join(snaps[(s, it, seiswg.abi.F_VY)].tobytes() for s in range(12) for it in (1600, 3200))
+    assert fwd["snapshot/vy.f32"] == want
+    c.close()
+    traces = {k: v for k, v in fwd.items() if k.startswith("obs_traces/")}
+    assert sorted(k for k in traces if "/uy_" in k) == [f"obs_traces/uy_{s:06d}.npy" for s in range(12)]
+    adj = seiswg.run_simulation(text("config.ini"), text("sources.dat"), text("stations.dat"), {**files, **traces})
+    got = np.frombuffer(adj["output/proc000000_kmu.bin"], "<f4")
+    ref = E.fixture("adjoint", "proc000000_kmu.bin")
+    assert O.max_rel_err(got, ref) <= 1e-4 and O.rel_l2(got, ref) <= 1e-5
+
+
 def test_cli_inversion_lowers_the_misfit(seiswg, tmp_path):
     """workflow = inversion on the adjoint example: observed traces from the
     true model, two steepest-descent iterations from the initial model; the

@@ -232,3 +232,81 @@ def test_fp32_restatement_error_vs_fp64_oracle(tmp_path):
     e_sn = O.rel_l2(r32[1][:, 0], r64[1][:, 0])
     print(f"FP32 vs FP64: uy traces rl2 {e_tr:.2e}, vy snapshots rl2 {e_sn:.2e}")
     assert e_tr < 1e-5 and e_sn < 1e-5
+
+
+def test_vps2lm_closed_form():
+    """vps2lm (fd2d.wgsl:188-201): mu = rho vs^2; lambda = rho (vp^2 - 2 vs^2)
+    where vp > vs, else 0 (the branch no reference fixture reaches)."""
+    nx, nz = 9, 8
+    rng = np.random.default_rng(5)
+    p = small_problem(nx, nz)
+    rho = rng.uniform(1.0, 3.0, nx * nz).astype(np.float32)
+    vs = rng.uniform(0.5, 2.0, nx * nz).astype(np.float32)
+    vp = (vs * rng.uniform(0.5, 2.5, nx * nz)).astype(np.float32)
+    vp[:5] = vs[:5]                       # vp == vs: the else branch
+    p.model.update(rho=rho, vp=vp, vs=vs)
+    assert np.any(vp < vs) and np.any(vp > vs)
+    for prec in (4, 8):
+        o = O.Oracle(p, prec)
+        r, s, v = (x.astype(np.float64) for x in (rho, vs, vp))
+        want_mu = r * s * s
+        want_lam = np.where(v > s, r * (v * v - 2 * s * s), 0.0)
+        tol = 1e-6 if prec == 4 else 1e-14
+        np.testing.assert_allclose(o.field(O.F["MU"]), want_mu, rtol=tol)
+        np.testing.assert_allclose(o.field(O.F["LAM"]), want_lam, rtol=tol, atol=tol * np.abs(want_lam).max())
+        assert np.all(o.field(O.F["LAM"])[v <= s] == 0)
+
+
+def test_spin_traces_closed_forms():
+    """Spin receivers (fd2d.wgsl:449-492): ux, uz and ry are the displacement
+    and micro-rotation fields at the receiver cells, and yi = ry - curl/2 with
+    curl = -(Dx+ uz - Dz+ ux), D+ the forward difference
+    (9 (f[i+1] - f[i]) - (f[i+2] - f[i-1])) / 8h, zero on the first and the
+    last two rows/columns.  Checked on the last sample of every receiver
+    against the fields the run ends with."""
+    nx, nz, nt = 24, 20, 30
+    rng = np.random.default_rng(8)
+    p = small_problem(nx, nz, spin=True, dt=0.05)
+    for k in ("rho", "lambda", "mu", "nu", "j", "lambda_c", "mu_c", "nu_c"):
+        p.model[k] = rng.uniform(0.5, 1.5, nx * nz).astype(np.float32)
+    p.nt = nt
+    p.src_idx = [10 * nz + 9, 4 * nz + 15]
+    cells = [(12, 10), (0, 0), (1, 1), (nx - 2, 3), (5, nz - 2), (20, 17)]
+    p.rec_idx = [i * nz + j for i, j in cells]
+    stf = rng.standard_normal((3, len(p.src_idx), nt))
+    o = O.Oracle(p, 8, stf=stf)
+    obs, _ = o.forward(-1, 0)
+    ux, uz = o.field(O.F["UX"]).reshape(nx, nz), o.field(O.F["UZ"]).reshape(nx, nz)
+    uyc = o.field(O.F["UY_C"]).reshape(nx, nz)
+    duzdx, duxdz = np.zeros((nx, nz)), np.zeros((nx, nz))
+    duzdx[1:nx - 2] = (9 * (uz[2:nx - 1] - uz[1:nx - 2]) - (uz[3:nx] - uz[0:nx - 3])) / (8 * p.dx)
+    duxdz[:, 1:nz - 2] = (9 * (ux[:, 2:nz - 1] - ux[:, 1:nz - 2]) - (ux[:, 3:nz] - ux[:, 0:nz - 3])) / (8 * p.dz)
+    curl = -(duzdx - duxdz)
+    assert np.abs(curl).max() > 0
+    for r, (i, j) in enumerate(cells):
+        assert obs[O.OBS_X][r, -1] == ux[i, j] and obs[O.OBS_Z][r, -1] == uz[i, j]
+        assert obs[O.OBS_Y][r, -1] == uyc[i, j]
+        np.testing.assert_allclose(obs[O.OBS_YI][r, -1], uyc[i, j] - 0.5 * curl[i, j], rtol=1e-12,
+                                   atol=1e-12 * np.abs(uyc).max())
+
+
+@pytest.mark.parametrize("phys", ["sh", "psv"])
+def test_taper_multiplies_the_velocities(phys):
+    """add_vy / add_vxz (fd2d.wgsl:263-271, 355-368): with zero stresses and
+    no source one step leaves v = b v0, b the taper of all four edges; the
+    P-SV case is the one no reference fixture has a taper for."""
+    nx, nz = 30, 26
+    p = small_problem(nx, nz, sh=phys == "sh", psv=phys == "psv")
+    p.abs_left = p.abs_right = p.abs_bottom = p.abs_top = True
+    p.abs_width, p.abs_alpha = 9, 0.07
+    o = O.Oracle(p, 8)
+    rng = np.random.default_rng(4)
+    names = ["VY"] if phys == "sh" else ["VX", "VZ"]
+    v0 = {n: rng.standard_normal(nx * nz) for n in names}
+    for n in names:
+        o.set_field(O.F[n], v0[n])
+    o.step(0, -1)
+    b = o.field(O.F["BOUND"])
+    assert b.min() < 0.9
+    for n in names:
+        np.testing.assert_allclose(o.field(O.F[n]), b * v0[n], rtol=1e-15, atol=0)

@@ -322,6 +322,8 @@ def main():
                     help="run only the batched-vs-plain gradient measurement (for profiler runs)")
     ap.add_argument("--uneven", action="store_true", help="also measure the uneven batch of 48 gradient problems")
     ap.add_argument("--no-small", action="store_true", help="skip the 200x200 SH and 500x500 spin forward runs")
+    ap.add_argument("--small-only", action="store_true",
+                    help="only the 200x200 SH, 500x500 spin and single-shot 2000x1000 forward runs (for profiling)")
     ap.add_argument("--uneven-nt-scale", type=float, default=0.04, help="nt of the uneven batch, as a fraction")
     ap.add_argument("--uneven-budget-gb", type=float, default=200.0,
                     help="device memory per wave of the uneven batch's multi-problem calls")
@@ -336,6 +338,16 @@ def main():
     from seiswg_amd import synth
     if a.calibrate_copy:
         seiswg.bench_copy(1 << 30, 5)
+    if a.small_only:
+        c = synth.PSV_BATCH
+        model, geom = synth.psv_batch_problem()
+        src_idx = [seiswg.grid_index(r[0], r[1], c["h"], c["h"], c["nz"]) for r in geom.src_rows()]
+        rec_idx = [seiswg.grid_index(x, z, c["h"], c["h"], c["nz"]) for x, z in zip(geom.rec_x, geom.rec_z)]
+        print(json.dumps({"single_shot_2000x1000": single_shot_bench(seiswg, synth, model, geom, src_idx, rec_idx),
+                          "sh_forward_200x200": small_config_bench(seiswg, synth, "sh", not a.no_cpu_baseline),
+                          "spin_forward_500x500": small_config_bench(seiswg, synth, "spin", not a.no_cpu_baseline)}),
+              flush=True)
+        return 0
     if a.gradient_only:
         print(json.dumps({"gradient": gradient_bench(seiswg, synth, a.gradient_nt)}), flush=True)
         return 0

@@ -59,9 +59,9 @@ SW_API const char* seiswg_vfs_key(const sw_vfs* v, int32_t i);
  * config resolves them to (e.g. "model/proc000000_rho.bin"; a key without a
  * leading "./" also matches); sources_dat / stations_dat, if not NULL, are the
  * text of the tables the config names (else those come from `files` too);
- * for the adjoint workflow, the observed traces in `files`, or, if
- * config_true_ini is not NULL, a forward run of that config first, whose
- * traces join the inputs (lib.rs:106-113).  Outputs go into `out`, keyed as
+ * for the adjoint (and inversion) workflow, the observed traces in `files`,
+ * or, if config_true_ini is not NULL, a forward run of that config first,
+ * whose traces join the inputs (lib.rs:106-113).  Outputs go into `out`, keyed as
  * the reference's Solver returns them: each trace file by its path
  * ("output/traces/uy_000000.npy" in .npy format, the raw "vy_000000.npy"
  * etc.); "snapshot/nframes" (u32: snapshot steps over all runs) and

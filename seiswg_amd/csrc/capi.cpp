@@ -1834,7 +1834,15 @@ SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* 
     std::string first_err;
     int first_rc = 0;
     auto worker = [&]() {
-      if (hipSetDevice(m->device) != hipSuccess) return;
+      const hipError_t e = hipSetDevice(m->device);
+      if (e != hipSuccess) {
+        std::lock_guard<std::mutex> lk(err_mu);
+        if (!failed.exchange(true)) {
+          first_rc = -3;
+          first_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        }
+        return;
+      }
       for (;;) {
         const size_t i = next.fetch_add(1);
         if (i >= solo.size() || failed.load()) return;

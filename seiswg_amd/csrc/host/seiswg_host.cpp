@@ -952,7 +952,7 @@ SW_API int seiswg_run_simulation(const char* config_ini, const char* sources_dat
     add_tables(P.cfg);
     // adjoint: a forward run on the true model first, whose traces join the
     // inputs (lib.rs:106-113)
-    if (P.cfg.workflow == "adjoint" && config_true_ini) {
+    if ((P.cfg.workflow == "adjoint" || P.cfg.workflow == "inversion") && config_true_ini) {
       Problem T;
       T.cfg = load_config_text(config_true_ini, fs::path());
       add_tables(T.cfg);

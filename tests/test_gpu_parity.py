@@ -512,6 +512,32 @@ def test_cli_inversion_lowers_the_misfit(seiswg, tmp_path):
     assert not np.array_equal(field(last, "vs"), field(init, "vs"))
 
 
+def test_in_memory_inversion_with_true_config(seiswg, tmp_path, monkeypatch):
+    """workflow = inversion in memory: the true-model forward of
+    config_true.ini feeds the observed traces, two iterations lower the
+    misfit, and the accepted models come back in the model-file format."""
+    d = E.materialise("adjoint", str(tmp_path))
+    empty = tmp_path / "cwd"
+    empty.mkdir()
+    monkeypatch.chdir(empty)
+    text = lambda n: open(os.path.join(d, n)).read()
+    ini = text("config.ini").replace("mode = adjoint", "mode = inversion")
+    ini += "\n[inversion]\nniter = 2\nstep = 0.02\nlinesearch = 4\ncheckpoint = 100\n"
+    files = example_files(d, ["model_init", "model_true"])
+    out = seiswg.run_simulation(ini, text("sources.dat"), text("stations.dat"), files, text("config_true.ini"))
+    assert os.listdir(empty) == []
+    log = np.array([[float(x) for x in ln.split()] for ln in out["output/misfit.txt"].decode().splitlines()])
+    print("misfit per iteration:", log[:, 1])
+    assert log.shape[0] >= 2 and np.all(np.diff(log[:, 1]) < 0)
+    last = f"output/iter_{int(log[-1, 0]):03d}"
+    npt = 201 * 201
+    for name in ("rho", "x", "z", "vs"):
+        raw = out[f"{last}/proc000000_{name}.bin"]
+        assert np.frombuffer(raw[:4], "<i4")[0] == npt and len(raw) == 4 + 4 * npt
+        same = raw == files[f"model_init/proc000000_{name}.bin"][:4 + 4 * npt]
+        assert same == (name != "vs"), name
+
+
 def test_adjoint_plain_matches_oracle(seiswg):
     nx, nz, nt = 50, 46, 300
     p, _ = random_problem(nx, nz, nt, True, False, False, seed=8, nsrc=3, nrec=6)

@@ -210,7 +210,7 @@ SW_API int sw_get_tiling(const sw_ctx* ctx, int32_t* threads, int32_t* seglen, i
  * launch.  A change to 2 takes effect at the next prepared batch. */
 SW_API int sw_set_steps_per_launch(sw_ctx* ctx, int32_t steps);
 /* Tile shape of the two-step launches (no counterpart in the reference):
- * threads per shot 64, 128, 256 or 512 (they own threads-12 z-columns), rows
+ * threads per shot 64, 128, 256, 320 or 512 (they own threads-12 z-columns), rows
  * per workgroup, shots per workgroup (0: automatic; 1; 2 or 4 with threads
  * >= 128 and at most 1024 threads per workgroup).  Without a call they get a
  * heuristic shape, or that of sw_set_tiling when it was called.

@@ -1219,7 +1219,7 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
     double best2 = 1e300;
     Tiling bt2 = c->tile2;
     int bg2 = c->group2_req;
-    for (int nt : {64, 128, 256, 512}) {
+    for (int nt : {64, 128, 256, 320, 512}) {
       for (int grp : {1, 2, 4}) {
         if (!group2_supported(nt, grp) || nshots % grp != 0) continue;
         for (int seg : segs) {
@@ -1256,7 +1256,7 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
 
 SW_API int sw_set_tiling2(sw_ctx* c, int32_t threads, int32_t seglen, int32_t shots_per_block) {
   if (!c) return fail(-1, "null ctx");
-  if (!tiling2_supported(threads)) return fail(-2, "threads must be 64, 128, 256 or 512");
+  if (!tiling2_supported(threads)) return fail(-2, "threads must be 64, 128, 256, 320 or 512");
   if (seglen < 1) return fail(-2, "seglen must be >= 1");
   if (shots_per_block != 0 && !group2_supported(threads, shots_per_block))
     return fail(-2, "shots_per_block must be 0 (automatic), 1, or 2 or 4 with threads >= 128 and at most "

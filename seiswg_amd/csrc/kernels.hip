@@ -2247,6 +2247,8 @@ hipError_t launch_two(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s
     case 256 * 8 + 1: return launch_two_ns<256, 1>(sh, a, t, s, aux);
     case 256 * 8 + 2: return launch_two_ns<256, 2>(sh, a, t, s, aux);
     case 256 * 8 + 4: return launch_two_ns<256, 4>(sh, a, t, s, aux);
+    case 320 * 8 + 1: return launch_two_ns<320, 1>(sh, a, t, s, aux);
+    case 320 * 8 + 2: return launch_two_ns<320, 2>(sh, a, t, s, aux);
     case 512 * 8 + 1: return launch_two_ns<512, 1>(sh, a, t, s, aux);
     case 512 * 8 + 2: return launch_two_ns<512, 2>(sh, a, t, s, aux);
   }
@@ -2258,7 +2260,8 @@ hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, b
 hipError_t launch_step_sh2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
   return launch_two(true, a, t, s, aux);
 }
-bool tiling2_supported(int nt) { return nt == 64 || nt == 128 || nt == 256 || nt == 512; }
+// 320 (five waves, 308-column strips) fits grids of ~600 columns in two strips
+bool tiling2_supported(int nt) { return nt == 64 || nt == 128 || nt == 256 || nt == 320 || nt == 512; }
 bool group2_supported(int nt, int group) {
   return tiling2_supported(nt) && (group == 1 || (nt >= 128 && (group == 2 || group == 4) && nt * group <= 1024));
 }
@@ -2271,6 +2274,8 @@ int step2_blocks_per_cu(int nt, int group) {
     case 256 * 8 + 1: return occupancy(k_step_psv2<256, 1, false>, 256);
     case 256 * 8 + 2: return occupancy(k_step_psv2<256, 2, false>, 512);
     case 256 * 8 + 4: return occupancy(k_step_psv2<256, 4, false>, 1024);
+    case 320 * 8 + 1: return occupancy(k_step_psv2<320, 1, false>, 320);
+    case 320 * 8 + 2: return occupancy(k_step_psv2<320, 2, false>, 640);
     case 512 * 8 + 1: return occupancy(k_step_psv2<512, 1, false>, 512);
     case 512 * 8 + 2: return occupancy(k_step_psv2<512, 2, false>, 1024);
   }

@@ -246,6 +246,35 @@ def test_two_step_shape_set_apart_from_one_step_shape(seiswg, tiling2):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (q, k)
 
 
+@pytest.mark.parametrize("tiling2", [(320, 7, 1), (320, 13, 2)])
+@pytest.mark.parametrize("phys", ["psv", "sh"])
+def test_five_wave_two_step_tiles_equal_one_step_launches(seiswg, tiling2, phys):
+    """320-thread two-step tiles (308-column strips: three strips and a
+    partly filled last one on 700 columns) against one step per launch, bit
+    for bit."""
+    nx, nz, nt = 37, 700, 9
+    sh = phys == "sh"
+    p, stf = random_problem(nx, nz, nt, sh, not sh, False, seed=33)
+    print(f"case phys={phys} tiling2={tiling2}", flush=True)
+    comps = [O.OBS_Y] if sh else [O.OBS_X, O.OBS_Z]
+    runs = [0, 1, 2, 3]
+    out = {}
+    for spl in (1, 2):
+        c = gpu_context(p, stf, seiswg)
+        c.set_steps_per_launch(spl)
+        if spl == 2:
+            c.set_tiling2(*tiling2)
+        c.forward_batch(runs)
+        if spl == 2:
+            assert c.tiling2() == tiling2
+        out[spl] = [[c.field(getattr(seiswg.abi, "F_" + n), shot=q).copy() for n in STATE[phys]] +
+                    [c.obs(q, k).copy() for k in comps] for q in range(len(runs))]
+        c.close()
+    for q in range(len(runs)):
+        for k, (a, b) in enumerate(zip(out[1][q], out[2][q])):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (q, k)
+
+
 @pytest.mark.parametrize("phys", ["psv", "sh"])
 def test_snapshots_equal_the_state_after_their_step(seiswg, phys):
     """Snapshots of a batched run (copied asynchronously while later steps

@@ -87,31 +87,45 @@ def cpu_baseline(synth, model, geom, src_idx, rec_idx, budget_s=12.0):
                   abs_width=c["abs_width"], abs_alpha=0.015, model=model, src_rows=geom.src_rows(),
                   src_idx=src_idx, rec_idx=rec_idx)
     o = O.Oracle(p, 4)
-    o.clear()
-    o.step(0, 0)     # first-touch
-    n, shot, it, t0 = 0, 0, 1, time.perf_counter()
-    while True:
-        o.step(it, shot)
-        n += 1
-        it += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
-        if it == p.nt:          # next shot of the batch, from a cleared state
-            shot, it = shot + 1, 0
-            o.clear()
     cells = c["nx"] * c["nz"]
-    return {"value": cells * n / el, "unit": "cell-updates/s", "cores": int(O.lib().or_num_threads()),
-            "kind": "port",
+
+    def timed(budget):
+        o.clear()
+        o.step(0, 0)     # first-touch
+        n, shot, it, t0 = 0, 0, 1, time.perf_counter()
+        while True:
+            o.step(it, shot)
+            n += 1
+            it += 1
+            el = time.perf_counter() - t0
+            if el >= budget:
+                break
+            if it == p.nt:          # next shot of the batch, from a cleared state
+                shot, it = shot + 1, 0
+                o.clear()
+        return n, shot, el
+
+    cores = int(O.lib().or_num_threads())
+    n, shot, el = timed(budget_s)
+    O.lib().or_set_num_threads(1)
+    try:
+        n1, _, el1 = timed(budget_s / 4)
+    finally:
+        O.lib().or_set_num_threads(cores)
+    return {"value": cells * n / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
             "sample": f"oracle FP32 restatement of the reference kernels on the 2000x1000 P-SV workload: "
-                      f"{n} shot-steps ({shot + 1} shot(s) in order, nt=4000 each) in {el:.1f} s"}
+                      f"{n} shot-steps ({shot + 1} shot(s) in order, nt=4000 each) in {el:.1f} s",
+            "single_core": {"value": cells * n1 / el1, "unit": "cell-updates/s", "cores": 1,
+                            "sample": f"the same, {n1} shot-steps in {el1:.1f} s on one thread"}}
 
 
 def gradient_bench(seiswg, synth, nt_override=0):
     """Checkpointed P-SV adjoint gradient (1000x600, 16 shots, 200 receivers,
     checkpoint every 50 steps): shots/s with all shots batched per launch vs
-    the plain one-shot-per-launch path on the same GPU, same kernels, same
-    results (bit-identical kernels are checked)."""
+    the plain one-shot-per-launch path on the same GPU, same kernels; the
+    misfits are compared bit for bit and the kernels by rel-L2 (the batched
+    adjoint sums the shots of a workgroup each step, the plain one shot by
+    shot)."""
     c = synth.PSV_GRADIENT
     init, true, geom = synth.gradient_problem()
     nt = nt_override or c["nt"]

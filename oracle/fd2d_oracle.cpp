@@ -866,6 +866,15 @@ int32_t or_num_threads(void) {
 #endif
 }
 
+// threads of the following parallel loops (timing on one core)
+void or_set_num_threads(int32_t n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
 // Misfit restated alone (solver.rs:432-467), for unit tests.
 double or_misfit(int prec, const double* syn, const double* obs, int32_t nrec, int32_t nt, float dt, double* adstf) {
   size_t n = size_t(nrec) * nt;

@@ -76,6 +76,8 @@ def lib():
         L.or_ricker_table.argtypes = [C.c_int, C.c_int32, C.c_int32, C.c_float, fp, dp]
         L.or_infer_grid.argtypes = [C.c_int32, fp, fp, ip, ip, fp, fp]
         L.or_num_threads.restype = C.c_int32
+        L.or_set_num_threads.argtypes = [C.c_int32]
+        L.or_set_num_threads.restype = None
         L.or_objective.restype = C.c_double
         L.or_objective.argtypes = [C.c_void_p, C.c_int32, ip, dp]
         L.or_gradient.restype = C.c_double

@@ -817,8 +817,10 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   const auto rout = make_rsrc(a.st[1] + so, sbytes);
   const uint32_t npt4 = 4u * uint32_t(a.npt);
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
-  const uint32_t o_vx = 4u * a.slot_off[S_VX], o_vz = 4u * a.slot_off[S_VZ];
-  const uint32_t o_sxx = 4u * a.slot_off[S_SXX], o_szz = 4u * a.slot_off[S_SZZ], o_sxz = 4u * a.slot_off[S_SXZ];
+  // P-SV alone (the only physics two-step launches run): slots vx, vz, sxx,
+  // szz, sxz in that order (the host checks), offsets as multiples of npt
+  const uint32_t o_vx = 0u, o_vz = npt4;
+  const uint32_t o_sxx = 2u * npt4, o_szz = 3u * npt4, o_sxz = 4u * npt4;
   const uint32_t m_rho = MF_RHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4, m_irho = MF_IRHO * npt4;
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
@@ -1177,7 +1179,9 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
   const auto rout = make_rsrc(a.st[1] + so, sbytes);
   const uint32_t npt4 = 4u * uint32_t(a.npt);
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
-  const uint32_t o_vy = 4u * a.slot_off[S_VY], o_sxy = 4u * a.slot_off[S_SXY], o_szy = 4u * a.slot_off[S_SZY];
+  // SH alone (the only physics two-step launches run): slots vy, sxy, szy in
+  // that order (the host checks), offsets as multiples of npt
+  const uint32_t o_vy = 0u, o_sxy = npt4, o_szy = 2u * npt4;
   const uint32_t m_rho = MF_RHO * npt4, m_mu = MF_MU * npt4, m_irho = MF_IRHO * npt4;
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
@@ -2226,9 +2230,18 @@ hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, boo
   return launch_fwd(aux ? MK_SH_AUX : MK_SH, a, t, s);
 }
 
+// the state layout the two-step kernels take as fixed: one physics, its
+// slots first to last
+bool two_step_slots(bool sh, const StepArgs& a) {
+  const int n = a.npt;
+  if (sh) return a.slot_off[S_VY] == 0 && a.slot_off[S_SXY] == n && a.slot_off[S_SZY] == 2 * n;
+  return a.slot_off[S_VX] == 0 && a.slot_off[S_VZ] == n && a.slot_off[S_SXX] == 2 * n &&
+         a.slot_off[S_SZZ] == 3 * n && a.slot_off[S_SXZ] == 4 * n;
+}
+
 template <int NT, int S>
 hipError_t launch_two_ns(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
-  if (!args_match(a, t, S)) return hipErrorInvalidValue;
+  if (!args_match(a, t, S) || !two_step_slots(sh, a)) return hipErrorInvalidValue;
   const int nblk = a.nshots / S * t.nstrips * t.nsegs;
   const dim3 grid(nblk), block(NT * S);
   if (sh && aux) hipLaunchKernelGGL((k_step_sh2<NT, S, true>), grid, block, 0, s, a);

@@ -603,6 +603,10 @@ StepArgs make_args(sw_ctx* c, StepMode mode) {
   a.aux = c->aux.p;
   a.aux_period = c->aux_period;
   a.kgrad = c->kshot.p;
+  // non-temporal state / aux stores once the state a launch reads and writes
+  // is beyond the 256 MB Infinity Cache (below it, the next launch finds the
+  // state there)
+  a.stream_out = 2.0 * c->nshots * c->nslots * double(c->npt) * 4.0 > double(256 << 20) ? 1 : 0;
   a.st[0] = c->state[0].p;
   a.st[1] = c->state[1].p;
   a.it_lo = -0x7fffffff;

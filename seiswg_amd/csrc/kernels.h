@@ -72,10 +72,11 @@ struct StepArgs {
   int nblocks;                // workgroups of this problem in the launch
   // gradient support: per-step auxiliary fields (forward recompute writes,
   // adjoint reads) laid out [step % aux_period][shot][naux][npt], and the
-  // per-shot kernels [shot][3][npt] (rho, mu, lambda)
+  // kernel accumulators (rho, mu, lambda)
   float* aux;
   int aux_period;
-  float* kgrad;
+  float* kgrad;               // kernel accumulators [shot / S][3][npt]
+  int stream_out;             // 1: state and aux stores non-temporal (working set far beyond the caches)
 };
 
 // auxiliary fields kept per recomputed forward step

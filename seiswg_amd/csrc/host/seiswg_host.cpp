@@ -361,7 +361,8 @@ std::vector<char> npy_bytes(const float* data, size_t nrows, size_t ncols) {
   header.push_back('\n');
   std::vector<char> b;
   b.reserve(10 + hl + nrows * ncols * 4);
-  b.insert(b.end(), "\x93NUMPY\x01\x00", "\x93NUMPY\x01\x00" + 8);
+  static const char magic[] = "\x93NUMPY\x01\x00";   // 8 bytes: magic, version 1.0
+  b.insert(b.end(), magic, magic + 8);
   const uint16_t h16 = uint16_t(hl);
   b.push_back(char(h16 & 0xff));
   b.push_back(char(h16 >> 8));

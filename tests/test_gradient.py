@@ -235,3 +235,46 @@ def test_multi_problem_gradient_equals_single_gradients(seiswg, merge_limit):
         assert np.array_equal(a[0], b[0])
         for x, y in zip(a[1:], b[1:]):
             assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("workers", [1, 2, 16])
+def test_multi_problem_gradient_independent_of_workers(seiswg, workers):
+    """Problems on their own launches (merge limit 0) run `workers` at a time
+    from host threads; the gradients equal the single-problem ones bit for
+    bit whatever the count."""
+    probs = multi_problems(seiswg)
+    runs = [[0, -1], [1, 2], [0], [2, 0, 1]]
+    refs = [(np.random.default_rng(q).standard_normal((len(r), nc, len(p.rec_idx), p.nt)) * 1e-3).astype(np.float32)
+            for q, ((p, stf, nc, c), r) in enumerate(zip(probs, runs))]
+    single = [c.gradient(r, o, ckpt_every=20) for (p, stf, nc, c), r, o in zip(probs, runs, refs)]
+    m = seiswg.Multi([c for *_, c in probs])
+    m.set_merge_limit(0)
+    m.set_workers(workers)
+    got = m.gradient(runs, refs, ckpt_every=20)
+    for a, b in zip(got, single):
+        assert np.array_equal(a[0], b[0])
+        for x, y in zip(a[1:], b[1:]):
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_released_memory_cache_keeps_results(seiswg):
+    """Buffers of a closed context go to the library's memory cache;
+    sw_release_cached_memory hands them back, and a new context after that
+    (fresh allocations) gives the same gradient bit for bit."""
+    import gc
+    gc.collect()        # contexts of earlier tests release their buffers now, not mid-test
+    p, stf, nc = problem(False, 50, 44, 60, seed=12)
+    runs = [0, 1]
+    obs = (np.random.default_rng(3).standard_normal((len(runs), nc, len(p.rec_idx), p.nt)) * 1e-3).astype(np.float32)
+    c = gpu_problem(seiswg, p, stf)
+    ref = c.gradient(runs, obs, ckpt_every=16)
+    c.close()
+    assert seiswg.release_cached_memory() > 0
+    assert seiswg.release_cached_memory() == 0
+    c = gpu_problem(seiswg, p, stf)
+    got = c.gradient(runs, obs, ckpt_every=16)
+    c.close()
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))

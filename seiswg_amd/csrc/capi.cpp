@@ -105,13 +105,15 @@ hipError_t cache_alloc(void** p, size_t& bytes, int dev) {
   }
   return e;
 }
-void cache_free(void* p, size_t bytes, int dev) {
+// idle: the caller has synchronised every stream that used the block, so
+// the device-wide wait (what hipFree would do) can be skipped
+void cache_free(void* p, size_t bytes, int dev, bool idle = false) {
   MemCache& c = mem_cache();
   int cur = 0;
   (void)hipGetDevice(&cur);
   if (cur != dev) (void)hipSetDevice(dev);
   if (c.on) {
-    (void)hipDeviceSynchronize();
+    if (!idle) (void)hipDeviceSynchronize();
     std::lock_guard<std::mutex> lk(c.mu);
     c.blocks.insert({{dev, bytes}, p});
   } else {
@@ -127,8 +129,8 @@ struct DBuf {
   size_t bytes = 0;
   int dev = 0;
   ~DBuf() { release(); }
-  void release() {
-    if (p) cache_free(p, bytes, dev);
+  void release(bool idle = false) {
+    if (p) cache_free(p, bytes, dev, idle);
     p = nullptr;
     n = 0;
     bytes = 0;
@@ -1867,7 +1869,10 @@ SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* 
         // its kernels are in K_RHO / K_MU / K_LAM: give the device memory of
         // the run (checkpoints, aux fields, adjoint state, per-shot kernels)
         // back before the next problem allocates its own
-        for (DBuf<float>* b : {&c->aux, &c->ckpt, &c->adj[0], &c->adj[1], &c->kshot, &c->spare}) b->release();
+        // sw_gradient has synchronised the problem's stream, the only one these
+      // buffers were used on: no device-wide wait, which would also wait for
+      // the other workers' launches
+      for (DBuf<float>* b : {&c->aux, &c->ckpt, &c->adj[0], &c->adj[1], &c->kshot, &c->spare}) b->release(true);
       }
     };
     const int nw = std::max(1, std::min<int>(m->workers, int(solo.size())));

@@ -223,7 +223,9 @@ SW_API int sw_get_tiling2(const sw_ctx* ctx, int32_t* threads, int32_t* seglen, 
  * When steps run two per launch (P-SV or SH without spin), the two-step
  * shape is tuned too and kept (as sw_set_tiling2 does; read it with
  * sw_get_tiling2); `ms` is the time per step of the configuration that runs
- * (two-step launches where they apply).  Tile shapes never change results. */
+ * (two-step launches where they apply).  Tile shapes never change forward
+ * results; a gradient's kernels depend on the adjoint's shots per workgroup
+ * at FP32 rounding (see sw_gradient). */
 SW_API int sw_autotune(sw_ctx* ctx, int32_t nshots, int32_t steps, int32_t* threads, int32_t* seglen,
                        int32_t* shots_per_block, double* ms);
 

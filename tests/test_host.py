@@ -204,3 +204,27 @@ def test_in_memory_run_reads_model_files_from_the_map(seiswg, tmp_path):
         with pytest.raises(seiswg.SeiswgError) as e:
             seiswg.run_simulation(text, *tables, files)
         assert "too short" in str(e.value) and "proc000000_vp.bin" in str(e.value)
+
+
+def test_vfs_c_abi_edges(seiswg):
+    """The map's C entry points on their edges: an absent key reads -1, an
+    index outside the map gives NULL, a negative size and a NULL map are
+    refused, and the in-memory run needs config text and an output map."""
+    import ctypes as C
+    lib = seiswg.load()
+    v = lib.seiswg_vfs_new()
+    try:
+        p = C.c_void_p()
+        assert lib.seiswg_vfs_get(v, b"none", C.byref(p)) == -1
+        assert lib.seiswg_vfs_put(v, b"k", b"abc", 3) == 0
+        assert lib.seiswg_vfs_get(v, b"k", None) == 3
+        assert lib.seiswg_vfs_count(v) == 1 and lib.seiswg_vfs_key(v, 0) == b"k"
+        assert lib.seiswg_vfs_key(v, 1) is None and lib.seiswg_vfs_key(v, -1) is None
+        assert lib.seiswg_vfs_put(v, b"k", b"abc", -1) != 0
+        assert lib.seiswg_vfs_put(None, b"k", b"abc", 3) != 0
+        assert lib.seiswg_vfs_count(None) == 0
+        assert lib.seiswg_run_simulation(None, None, None, None, None, v) != 0
+        assert b"config text" in lib.sw_last_error()
+        assert lib.seiswg_run_simulation(b"[workflow]\nmode = forward\n", None, None, None, None, None) != 0
+    finally:
+        lib.seiswg_vfs_free(v)

@@ -30,6 +30,11 @@ PEAK_HBM_GBPS = 8000.0
 METRIC = "cell-updates/s + HBM GB/s % peak; batched gradient shots/s vs plain"
 
 
+# sw_multi's default merge limit (shot-cells per step below which gradients
+# share merged launches): 0, every problem on its own launches
+DEFAULT_MERGE_LIMIT = 0
+
+
 def build_context(seiswg, synth, nt):
     c = synth.PSV_BATCH
     model, geom = synth.psv_batch_problem()
@@ -306,7 +311,7 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
         progress(f"batched: wave of {len(w)} problems, {t_batched:.1f} s so far")
     same_misfit = bool(np.array_equal(got0[0], ref0[0]))
     kernel_err = max(rel_l2(a, b) for a, b in zip(got0[1:], ref0[1:]))
-    merged = sum(1 for p, sh in zip(probs, shots) if len(sh) * p["nx"] * p["nz"] < (3 << 17))
+    merged = sum(1 for p, sh in zip(probs, shots) if len(sh) * p["nx"] * p["nz"] < DEFAULT_MERGE_LIMIT)
     return {"workload": f"{tag} of {len(probs)} SH/P-SV gradient problems (grids {spec['nx'][0]}..{spec['nx'][1]} x "
                         f"{spec['nz'][0]}..{spec['nz'][1]}, {spec['shots'][0]}-{spec['shots'][1]} shots, "
                         f"nt = {nt_scale} x 1500..4000, checkpoint every {synth.UNEVEN['ckpt']})",

@@ -172,8 +172,8 @@ SW_API int sw_multi_forward(sw_multi* m, const int32_t* nshots, const int32_t* c
  * obs_ref[q] is [nshots[q]][ncomp][nrec][nt] of problem q; misfit[q] may be
  * NULL.  A common checkpoint interval; segments of the merged problems are
  * aligned.  Problems with at least `merge limit` shot-cells per step
- * (nshots * nx * nz; default 3 * 2^17) fill the GPU on their own and run on their
- * own launches (see sw_multi_set_workers); the smaller ones advance together
+ * (nshots * nx * nz; default 0, i.e. all) run on their own launches (see
+ * sw_multi_set_workers); smaller ones, if a limit is set, advance together
  * in merged launches, which run one shot per workgroup: their results equal
  * sw_gradient's with one shot per workgroup bit for bit (to FP32 rounding
  * for a context whose tile puts several shots in a workgroup). */

@@ -1554,10 +1554,11 @@ struct sw_multi {
   int device = 0;
   hipStream_t stream = nullptr;
   // gradients: problems with at least this many shot-cells per step run on
-  // their own launches; the smaller ones share merged launches (measured:
-  // merged launches ahead up to ~0.3 M shot-cells per step, own launches
-  // from ~0.6 M on)
-  int64_t merge_limit = int64_t(3) << 17;
+  // their own launches; the smaller ones share merged launches.  Default 0:
+  // own launches for all (with three running at once they measured faster
+  // than merged launches from 0.03 M shot-cells per step on: 8 equal
+  // problems of 128x128 x 2 shots 0.207 s against 0.274 s)
+  int64_t merge_limit = 0;
   // problems on their own launches run on this many host threads at once,
   // each on its problem's stream (their launches and host work overlap)
   int workers = 3;

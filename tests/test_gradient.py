@@ -216,12 +216,13 @@ def test_multi_problem_forward_equals_single_runs(seiswg):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("merge_limit", [None, 0, 3000])
+@pytest.mark.parametrize("merge_limit", [None, 1 << 40, 3000])
 def test_multi_problem_gradient_equals_single_gradients(seiswg, merge_limit):
-    """Merged launches (default: all four problems are small), every problem
-    on its own launches (limit 0), and a split (limit 3000 shot-cells: two
-    problems merged, two on their own) give the single-problem gradients
-    (one shot per workgroup in both, as merged launches run)."""
+    """Every problem on its own launches (the default), all four in merged
+    launches (a limit above every problem), and a split (limit 3000
+    shot-cells: two problems merged, two on their own) give the
+    single-problem gradients (one shot per workgroup in both, as merged
+    launches run)."""
     probs = multi_problems(seiswg)
     runs = [[0, -1], [1, 2], [0], [2, 0, 1]]
     refs = [(np.random.default_rng(q).standard_normal((len(r), nc, len(p.rec_idx), p.nt)) * 1e-3).astype(np.float32)

@@ -428,7 +428,8 @@ class Multi:
 
     def set_merge_limit(self, shot_cells: int):
         """Gradients: problems with at least this many shot-cells per step run
-        on their own launches, smaller ones in merged launches."""
+        on their own launches, smaller ones in merged launches (default 0:
+        all on their own launches)."""
         _check(self.L.sw_multi_set_merge_limit(self.h, int(shot_cells)))
 
     def set_workers(self, workers: int):

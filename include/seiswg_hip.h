@@ -10,7 +10,10 @@
  *
  * Conventions
  *   - Every function returns 0 on success and a negative code on error;
- *     sw_last_error() then returns a thread-local message.
+ *     sw_last_error() then returns a thread-local message.  -1: bad
+ *     arguments, -2: invalid request for this context, -3: HIP error,
+ *     -4: a snapshot callback failed, -5: the wavefield diverged (NaN or
+ *     infinite values; see sw_set_finite_check).
  *   - Host pointers are borrowed for the duration of the call.  Device memory
  *     belongs to the context.
  *   - One context = one HIP device + one HIP stream; a context must not be
@@ -103,13 +106,20 @@ SW_API void sw_destroy(sw_ctx* ctx);
 SW_API int sw_set_model(sw_ctx* ctx, int32_t kind, const float* const* fields);
 
 /* Sources: grid indices k = ix*nz + iz (solver.rs:716-718) and the source
- * time functions [3][nsrc][nt] (x, y, z; solver.rs:721-739). */
+ * time functions [3][nsrc][nt] (x, y, z; solver.rs:721-739).  A batch
+ * prepared before the call is discarded: sw_prepare must be called again
+ * before sw_advance (the run entry points prepare by themselves). */
 SW_API int sw_set_sources(sw_ctx* ctx, int32_t nsrc, const int32_t* grid_index, const float* stf);
-/* Receivers: grid indices (solver.rs:745-751). */
+/* Receivers: grid indices (solver.rs:745-751).  Discards a prepared batch
+ * as sw_set_sources does. */
 SW_API int sw_set_receivers(sw_ctx* ctx, int32_t nrec, const int32_t* grid_index);
 
 /* One forward run (solver.rs:976-1046): clear wavefields and traces, run nt
- * steps with source isrc active (-1 = all sources, combine_sources). */
+ * steps with source isrc active (-1 = all sources, combine_sources).  The
+ * state of every shot is checked for NaN / infinite values at each snapshot
+ * step and after the last step (no counterpart in the reference, which keeps
+ * stepping a diverged field): a diverged run returns -5 naming the step of
+ * the first check that failed and of the last one that passed. */
 SW_API int sw_forward(sw_ctx* ctx, int32_t isrc, int32_t snap_every, sw_snapshot_fn cb, void* user);
 /* nshots forward runs advanced together, one launch per time step for all
  * shots (the reference runs them one after another, solver.rs:964-970).
@@ -126,8 +136,16 @@ SW_API int sw_write_field(sw_ctx* ctx, int32_t shot, int32_t field, const float*
 /* Low-level stepping for tests: prepare `nshots` shots (zeroed state and
  * traces, step clock at 0) without running them ... */
 SW_API int sw_prepare(sw_ctx* ctx, int32_t nshots, const int32_t* isrc);
-/* ... then advance all prepared shots by `nsteps` time steps. */
+/* ... then advance all prepared shots by `nsteps` time steps (no blow-up
+ * check: read the fields to inspect them). */
 SW_API int sw_advance(sw_ctx* ctx, int32_t nsteps);
+
+/* Blow-up detection (no counterpart in the reference): on by default.
+ * Forward runs scan the state at snapshot steps and after the last step;
+ * gradients (sw_gradient, sw_multi_gradient) check their misfits after the
+ * forward pass and their kernel fields at the end; sw_adjoint_plain its
+ * misfits and K_mu.  A failed check returns -5.  0 turns the checks off. */
+SW_API int sw_set_finite_check(sw_ctx* ctx, int32_t on);
 
 /* run_adjoint (solver.rs:1298-1529), SH, reference imaging condition:
  * nruns runs (run r fires isrc[r]); obs_ref = observed uy traces
@@ -237,6 +255,16 @@ typedef struct sw_timing {
   int64_t cell_updates;  /* nx*nz*steps*shots */
 } sw_timing;
 SW_API int sw_get_timing(const sw_ctx* ctx, sw_timing* out);
+
+/* Captured time-step blocks (no counterpart in the reference): runs of at
+ * least 16 steps replay blocks of 16 steps captured once as HIP graphs,
+ * keyed by their launch arguments (kept per context, at most 16).  *state:
+ * 1 graphs in use, 0 off (environment SEISWG_NO_GRAPHS=1 or a concurrent
+ * multi-problem run), -1 off because a capture failed (the context then
+ * launches directly; the failure is printed once on stderr and its HIP
+ * message copied to msg, up to msg_len bytes, if msg is not NULL).
+ * *captured (may be NULL): blocks held. */
+SW_API int sw_get_graph_state(const sw_ctx* ctx, int32_t* state, int32_t* captured, char* msg, int32_t msg_len);
 
 /* Device memory of freed buffers is kept for reuse by later buffers of up to
  * the same size (no counterpart in the reference): runs over many problems

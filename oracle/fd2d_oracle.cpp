@@ -27,6 +27,7 @@
 #include <algorithm>
 #ifdef _OPENMP
 #include <omp.h>
+#include <memory>
 #endif
 
 namespace {
@@ -474,60 +475,95 @@ struct Adjoint {
   uint32_t nx() const { return o.nx; }
   uint32_t nz() const { return o.nz; }
 
-  // transposes of the D operators (scatter form over the active output cells)
+  // Transposes of the D operators, scatter form over the active output cells.
+  // An x-operator only moves values along a column and a z-operator along a
+  // row, so the threads take blocks of columns (x) or rows (z): every output
+  // cell still receives its contributions in increasing i (or j) order, the
+  // order of a serial loop over i then j.
+  // [j0, j1): the columns of this thread (x-operators)
+  void my_columns(uint32_t& j0, uint32_t& j1) const {
+    const uint32_t nth = uint32_t(omp_get_num_threads()), t = uint32_t(omp_get_thread_num());
+    j0 = uint32_t(uint64_t(o.nz) * t / nth);
+    j1 = uint32_t(uint64_t(o.nz) * (t + 1) / nth);
+  }
   void dxm_t(const std::vector<T>& w, std::vector<T>& out) const {   // (Dx-)^T
-    const uint32_t nz = o.nz;
-    for (uint32_t i = 2; i + 2 < o.nx; ++i)
-      for (uint32_t j = 0; j < nz; ++j) {
-        const uint32_t k = i * nz + j;
-        const T c = w[k] / (T(8) * o.dx);
-        out[k] += T(9) * c; out[k - nz] -= T(9) * c; out[k + nz] -= c; out[k - 2 * nz] += c;
-      }
+    const uint32_t nz = o.nz, nx = o.nx;
+#pragma omp parallel
+    {
+      uint32_t j0, j1;
+      my_columns(j0, j1);
+      for (uint32_t i = 2; i + 2 < nx; ++i)
+        for (uint32_t j = j0; j < j1; ++j) {
+          const uint32_t k = i * nz + j;
+          const T c = w[k] / (T(8) * o.dx);
+          out[k] += T(9) * c; out[k - nz] -= T(9) * c; out[k + nz] -= c; out[k - 2 * nz] += c;
+        }
+    }
   }
   void dzm_t(const std::vector<T>& w, std::vector<T>& out) const {   // (Dz-)^T
     const uint32_t nz = o.nz;
-    for (uint32_t i = 0; i < o.nx; ++i)
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < int64_t(o.nx); ++ii)
       for (uint32_t j = 2; j + 2 < nz; ++j) {
-        const uint32_t k = i * nz + j;
+        const uint32_t k = uint32_t(ii) * nz + j;
         const T c = w[k] / (T(8) * o.dz);
         out[k] += T(9) * c; out[k - 1] -= T(9) * c; out[k + 1] -= c; out[k - 2] += c;
       }
   }
   void dxp_t(const std::vector<T>& w, std::vector<T>& out) const {   // (Dx+)^T
-    const uint32_t nz = o.nz;
-    for (uint32_t i = 1; i + 2 < o.nx; ++i)
-      for (uint32_t j = 0; j < nz; ++j) {
-        const uint32_t k = i * nz + j;
-        const T c = w[k] / (T(8) * o.dx);
-        out[k + nz] += T(9) * c; out[k] -= T(9) * c; out[k + 2 * nz] -= c; out[k - nz] += c;
-      }
+    const uint32_t nz = o.nz, nx = o.nx;
+#pragma omp parallel
+    {
+      uint32_t j0, j1;
+      my_columns(j0, j1);
+      for (uint32_t i = 1; i + 2 < nx; ++i)
+        for (uint32_t j = j0; j < j1; ++j) {
+          const uint32_t k = i * nz + j;
+          const T c = w[k] / (T(8) * o.dx);
+          out[k + nz] += T(9) * c; out[k] -= T(9) * c; out[k + 2 * nz] -= c; out[k - nz] += c;
+        }
+    }
   }
   void dzp_t(const std::vector<T>& w, std::vector<T>& out) const {   // (Dz+)^T
     const uint32_t nz = o.nz;
-    for (uint32_t i = 0; i < o.nx; ++i)
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < int64_t(o.nx); ++ii)
       for (uint32_t j = 1; j + 2 < nz; ++j) {
-        const uint32_t k = i * nz + j;
+        const uint32_t k = uint32_t(ii) * nz + j;
         const T c = w[k] / (T(8) * o.dz);
         out[k + 1] += T(9) * c; out[k] -= T(9) * c; out[k + 2] -= c; out[k - 1] += c;
       }
   }
+  // sources firing in run isrc, as per-cell lists in source order: head[k] is
+  // the first, next[s] the following one at the same cell (-1: none)
+  void source_lists(int32_t isrc, std::vector<int32_t>& head, std::vector<int32_t>& next) const {
+    head.assign(o.npt, -1);
+    next.assign(o.nsrc, -1);
+    for (int32_t s = int32_t(o.nsrc) - 1; s >= 0; --s) {
+      if (isrc >= 0 && s != isrc) continue;
+      const uint32_t k = o.src_id(uint32_t(s));
+      if (k >= o.npt) continue;
+      next[s] = head[k];
+      head[k] = s;
+    }
+  }
   // forward operators on a standalone field (same edge rules as fd2d.wgsl)
-  T dxm(const std::vector<T>& f, uint32_t k) const {
+  T dxm(const T* f, uint32_t k) const {
     uint32_t i = k / o.nz; const uint32_t nz = o.nz;
     if (i < 2u || i >= o.nx - 2u) return T(0);
     return (T(9) * (f[k] - f[k - nz]) - (f[k + nz] - f[k - 2 * nz])) / (T(8) * o.dx);
   }
-  T dzm(const std::vector<T>& f, uint32_t k) const {
+  T dzm(const T* f, uint32_t k) const {
     uint32_t j = k % o.nz;
     if (j < 2u || j >= o.nz - 2u) return T(0);
     return (T(9) * (f[k] - f[k - 1]) - (f[k + 1] - f[k - 2])) / (T(8) * o.dz);
   }
-  T dxp(const std::vector<T>& f, uint32_t k) const {
+  T dxp(const T* f, uint32_t k) const {
     uint32_t i = k / o.nz; const uint32_t nz = o.nz;
     if (i < 1u || i >= o.nx - 2u) return T(0);
     return (T(9) * (f[k + nz] - f[k]) - (f[k + 2 * nz] - f[k - nz])) / (T(8) * o.dx);
   }
-  T dzp(const std::vector<T>& f, uint32_t k) const {
+  T dzp(const T* f, uint32_t k) const {
     uint32_t j = k % o.nz;
     if (j < 1u || j >= o.nz - 2u) return T(0);
     return (T(9) * (f[k + 1] - f[k]) - (f[k + 2] - f[k - 1])) / (T(8) * o.dz);
@@ -556,20 +592,34 @@ struct Adjoint {
     return {OBS_X, OBS_Z};
   }
 
+  // the state fields (state_ids order) before step 0 and after every step;
+  // entry n + 1 is filled by the forward run's step n
+  struct States {
+    std::vector<std::unique_ptr<T[]>> s;
+    const T* operator[](size_t n) const { return s[n].get(); }
+  };
+
   // J for run isrc against obs [ncomp][nrec][nt]; optionally keeps all states
-  double forward(int32_t isrc, const T* obs, std::vector<std::vector<T>>* states) {
+  double forward(int32_t isrc, const T* obs, States* states) {
     o.clear_dyn(); o.clear_obs();
     const auto ids = state_ids();
-    const size_t npt = o.npt;
+    const size_t npt = o.npt, per = ids.size() * npt;
     if (states) {
-      states->assign(size_t(o.nt) + 1, std::vector<T>(ids.size() * npt, T(0)));
+      states->s.clear();
+      states->s.resize(size_t(o.nt) + 1);
+      states->s[0].reset(new T[per]());   // the cleared state
     }
     for (uint32_t n = 0; n < o.nt; ++n) {
       o.step(n, isrc);
-      if (states)
-        for (size_t q = 0; q < ids.size(); ++q)
-          std::copy(o.fields.begin() + size_t(ids[q]) * npt, o.fields.begin() + size_t(ids[q] + 1) * npt,
-                    (*states)[n + 1].begin() + q * npt);
+      if (states) {
+        states->s[n + 1].reset(new T[per]);
+        for (size_t q = 0; q < ids.size(); ++q) {
+          const T* src = o.fields.data() + size_t(ids[q]) * npt;
+          T* dst = states->s[n + 1].get() + q * npt;
+#pragma omp parallel for schedule(static)
+          for (int64_t k = 0; k < int64_t(npt); ++k) dst[k] = src[k];
+        }
+      }
     }
     const auto w = taper();
     const auto cs = comps();
@@ -586,7 +636,7 @@ struct Adjoint {
 
   // adds this run's gradient into kr, km, kl; returns J
   double gradient(int32_t isrc, const T* obs, std::vector<T>& kr, std::vector<T>& km, std::vector<T>& kl) {
-    std::vector<std::vector<T>> st;
+    States st;
     const double J = forward(isrc, obs, &st);
     const size_t npt = o.npt;
     const uint32_t nt = o.nt, nrec = o.nrec, nsrc = o.nsrc;
@@ -606,13 +656,17 @@ struct Adjoint {
       }
     const T dt = o.dt;
     auto P = [&](uint32_t f, uint32_t k) { return o.fields[size_t(f) * npt + k]; };
+    std::vector<int32_t> src_head, src_next;
+    source_lists(isrc, src_head, src_next);
+    (void)nsrc;
     if (o.sh) {
       std::vector<T> vb(npt, 0), sbx(npt, 0), sbz(npt, 0), tx(npt), tz(npt), q(npt), ex(npt), ez(npt);
       for (int64_t n = int64_t(nt) - 1; n >= 0; --n) {
-        const T* s1 = st[size_t(n) + 1].data();   // after step n: vy, sxy, szy
-        const T* s0 = st[size_t(n)].data();       // before step n
-        std::vector<T> vy1(s1, s1 + npt), sxy0(s0 + npt, s0 + 2 * npt), szy0(s0 + 2 * npt, s0 + 3 * npt);
+        const T* s1 = st[size_t(n) + 1];   // after step n: vy, sxy, szy
+        const T* s0 = st[size_t(n)];       // before step n
+        const T *vy1 = s1, *sxy0 = s0 + npt, *szy0 = s0 + 2 * npt;
         for (uint32_t r = 0; r < nrec; ++r) vb[o.rec_id(r)] += dt * A[size_t(r) * nt + size_t(n)];
+#pragma omp parallel for schedule(static)
         for (size_t k = 0; k < npt; ++k) {
           const T mu = P(F_MU, uint32_t(k));
           tx[k] = dt * mu * sbx[k];
@@ -620,12 +674,12 @@ struct Adjoint {
           km[k] += dt * (sbx[k] * dxp(vy1, uint32_t(k)) + sbz[k] * dzp(vy1, uint32_t(k)));
         }
         dxp_t(tx, vb); dzp_t(tz, vb);
+#pragma omp parallel for schedule(static)
         for (size_t k = 0; k < npt; ++k) {
           const T b = P(F_BOUND, uint32_t(k)), rho = P(F_RHO, uint32_t(k));
           const T g = b * vb[k];
           T ds = dxm(sxy0, uint32_t(k)) + dzm(szy0, uint32_t(k));
-          for (uint32_t s = 0; s < nsrc; ++s)
-            if ((isrc < 0 || int32_t(s) == isrc) && o.src_id(s) == uint32_t(k)) ds += o.stf[(size_t(STF_Y) * nsrc + s) * nt + size_t(n)];
+          for (int32_t s = src_head[k]; s >= 0; s = src_next[s]) ds += o.stf[(size_t(STF_Y) * o.nsrc + s) * nt + size_t(n)];
           kr[k] += -dt * g * ds / (rho * rho);
           q[k] = dt * g / rho;
           vb[k] = g;
@@ -636,14 +690,15 @@ struct Adjoint {
       std::vector<T> vbx(npt, 0), vbz(npt, 0), sbxx(npt, 0), sbzz(npt, 0), sbxz(npt, 0);
       std::vector<T> txx(npt), tzz(npt), txz(npt), qx(npt), qz(npt);
       for (int64_t n = int64_t(nt) - 1; n >= 0; --n) {
-        const T* s1 = st[size_t(n) + 1].data();   // vx, vz, sxx, szz, sxz after step n
-        const T* s0 = st[size_t(n)].data();
-        std::vector<T> vx1(s1, s1 + npt), vz1(s1 + npt, s1 + 2 * npt);
-        std::vector<T> sxx0(s0 + 2 * npt, s0 + 3 * npt), szz0(s0 + 3 * npt, s0 + 4 * npt), sxz0(s0 + 4 * npt, s0 + 5 * npt);
+        const T* s1 = st[size_t(n) + 1];   // vx, vz, sxx, szz, sxz after step n
+        const T* s0 = st[size_t(n)];
+        const T *vx1 = s1, *vz1 = s1 + npt;
+        const T *sxx0 = s0 + 2 * npt, *szz0 = s0 + 3 * npt, *sxz0 = s0 + 4 * npt;
         for (uint32_t r = 0; r < nrec; ++r) {
           vbx[o.rec_id(r)] += dt * A[(0 * size_t(nrec) + r) * nt + size_t(n)];
           vbz[o.rec_id(r)] += dt * A[(1 * size_t(nrec) + r) * nt + size_t(n)];
         }
+#pragma omp parallel for schedule(static)
         for (size_t k = 0; k < npt; ++k) {
           const uint32_t kk = uint32_t(k);
           const T lam = P(F_LAM, kk), mu = P(F_MU, kk);
@@ -656,16 +711,16 @@ struct Adjoint {
         }
         dxp_t(txx, vbx); dzp_t(txz, vbx);
         dzp_t(tzz, vbz); dxp_t(txz, vbz);
+#pragma omp parallel for schedule(static)
         for (size_t k = 0; k < npt; ++k) {
           const uint32_t kk = uint32_t(k);
           const T b = P(F_BOUND, kk), rho = P(F_RHO, kk);
           const T gx = b * vbx[k], gz = b * vbz[k];
           T dsx = dxm(sxx0, kk) + dzm(sxz0, kk), dsz = dxm(sxz0, kk) + dzm(szz0, kk);
-          for (uint32_t s = 0; s < nsrc; ++s)
-            if ((isrc < 0 || int32_t(s) == isrc) && o.src_id(s) == kk) {
-              dsx += o.stf[(size_t(STF_X) * nsrc + s) * nt + size_t(n)];
-              dsz += o.stf[(size_t(STF_Z) * nsrc + s) * nt + size_t(n)];
-            }
+          for (int32_t s = src_head[k]; s >= 0; s = src_next[s]) {
+            dsx += o.stf[(size_t(STF_X) * o.nsrc + s) * nt + size_t(n)];
+            dsz += o.stf[(size_t(STF_Z) * o.nsrc + s) * nt + size_t(n)];
+          }
           kr[k] += -dt * (gx * dsx + gz * dsz) / (rho * rho);
           qx[k] = dt * gx / rho;
           qz[k] = dt * gz / rho;

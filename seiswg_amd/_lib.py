@@ -51,7 +51,7 @@ EXPORTS = ["sw_last_error", "sw_create", "sw_destroy", "sw_set_model", "sw_set_s
            "sw_set_steps_per_launch", "sw_set_tiling2", "sw_get_tiling2", "sw_release_cached_memory",
            "sw_bench_copy", "sw_gradient", "sw_autotune", "sw_multi_create", "sw_multi_destroy",
            "sw_multi_forward", "sw_multi_gradient", "sw_multi_timing", "sw_multi_set_merge_limit",
-           "sw_multi_set_workers",
+           "sw_multi_set_workers", "sw_set_finite_check", "sw_get_graph_state",
            "seiswg_run", "seiswg_ricker_table", "seiswg_infer_grid", "seiswg_grid_index",
            "seiswg_vfs_new", "seiswg_vfs_free", "seiswg_vfs_put", "seiswg_vfs_get", "seiswg_vfs_count",
            "seiswg_vfs_key", "seiswg_run_simulation"]
@@ -100,6 +100,8 @@ def load():
     L.sw_autotune.argtypes = [vp, C.c_int32, C.c_int32, ip, ip, ip, dp]
     L.sw_gradient.argtypes = [vp, C.c_int32, ip, fp, C.c_int32, C.c_int32, dp]
     L.sw_bench_copy.argtypes = [C.c_int32, C.c_int64, C.c_int32, dp]
+    L.sw_set_finite_check.argtypes = [vp, C.c_int32]
+    L.sw_get_graph_state.argtypes = [vp, ip, ip, C.c_char_p, C.c_int32]
     L.seiswg_run.argtypes = [C.c_char_p]
     L.seiswg_ricker_table.argtypes = [C.c_int32, C.c_int32, C.c_float, fp, fp]
     L.seiswg_infer_grid.argtypes = [C.c_int64, fp, fp, ip, ip, fp, fp, ip]
@@ -374,13 +376,17 @@ class Context:
                                        mis.ctypes.data_as(C.POINTER(C.c_double))))
         return self.field(F_K_MU), mis[:len(isrc)]
 
-    def gradient(self, isrc_list, obs_ref, ckpt_every: int = 50, shots_per_launch: int = 0):
-        """Adjoint gradient; returns (misfit per shot, K_rho, K_mu, K_lambda)."""
+    def gradient(self, isrc_list, obs_ref, ckpt_every: int = 50, shots_per_launch: int = 0, smooth: float = 0.0):
+        """Adjoint gradient; returns (misfit per shot, K_rho, K_mu, K_lambda),
+        the kernels Gaussian-smoothed with sigma = smooth cells if smooth > 0."""
         isrc = np.ascontiguousarray(isrc_list, np.int32)
         ref = _f32(obs_ref)
         mis = np.zeros(len(isrc), np.float64)
         _check(self.L.sw_gradient(self.h, len(isrc), _iptr(isrc), _fptr(ref), ckpt_every, shots_per_launch,
                                   mis.ctypes.data_as(C.POINTER(C.c_double))))
+        if smooth > 0:
+            for f in (F_K_RHO, F_K_MU, F_K_LAM):
+                self.smooth(f, smooth)
         return mis, self.field(F_K_RHO), self.field(F_K_MU), self.field(F_K_LAM)
 
     def smooth(self, fid: int, sigma: float):
@@ -390,6 +396,19 @@ class Context:
         t = sw_timing()
         _check(self.L.sw_get_timing(self.h, C.byref(t)))
         return t
+
+    def set_finite_check(self, on: bool):
+        """Blow-up detection (on by default): forward runs, gradients and the
+        plain adjoint raise when the wavefield, misfits or kernels are not finite."""
+        _check(self.L.sw_set_finite_check(self.h, int(bool(on))))
+
+    def graph_state(self):
+        """(state, captured blocks, message): state 1 graphs in use, 0 off,
+        -1 off after a failed capture (message = the HIP error)."""
+        st, n = C.c_int32(), C.c_int32()
+        buf = C.create_string_buffer(256)
+        _check(self.L.sw_get_graph_state(self.h, C.byref(st), C.byref(n), buf, 256))
+        return st.value, n.value, buf.value.decode(errors="replace")
 
 
 class Multi:

@@ -226,14 +226,17 @@ struct sw_ctx {
   bool sh_only = false;       // the adjoint workflow dispatches the SH kernels only
   int aux_period = 1;         // checkpoint interval of the running gradient
 
-  // captured blocks of time steps (hipGraph), replayed while the launch
-  // arguments stay the same; `gen` changes whenever they may change
-  uint64_t gen = 1;
-  struct GraphEntry { uint64_t key; hipGraphExec_t exec; };
+  // captured blocks of time steps (hipGraph), keyed by everything the
+  // captured launches depend on (kernel choice, grid and the bytes of their
+  // argument blocks), so a block is replayed only with the arguments it was
+  // captured with; the graph is kept with its executable until both go
+  struct GraphEntry { uint64_t key; hipGraph_t graph; hipGraphExec_t exec; uint64_t used; };
   std::vector<GraphEntry> graphs;
-  uint64_t graph_gen = 0;
+  uint64_t graph_clock = 0;
   int graph_steps = 16;       // steps per captured block (a multiple of 4)
-  bool graphs_off = false;    // set after a failed capture
+  bool graphs_off = false;    // set after a failed capture (or by a caller)
+  bool graphs_failed = false; // a capture failed: graph_error says why
+  std::string graph_error;
 
   // plain adjoint
   DBuf<float> kmu, k_lam, k_rho, gsum, gtmp, wtab, sumx, sumz, fwd_store, adj_src, ref, taper;
@@ -266,6 +269,13 @@ struct sw_ctx {
   hipEvent_t snap_ready[2] = {nullptr, nullptr}, snap_copied[2] = {nullptr, nullptr};
   sw_timing timing{};
   int64_t launches = 0;
+
+  // blow-up detection: forward runs scan the state of every shot at their
+  // snapshot steps and after the last step; gradients and the plain adjoint
+  // check their misfits and kernel fields
+  bool finite_check = true;
+  DBuf<int> dbad;              // smallest step found not finite (INT_MAX: none)
+  int* bad_host = nullptr;     // pinned [2]: dbad as each snapshot slot's check left it
 };
 
 namespace {
@@ -547,7 +557,6 @@ std::vector<Entry> source_entries(const sw_ctx* c, int isrc) {
 // lists (solver.rs:978-981: clear dynamic fields and traces).
 int prepare_batch(sw_ctx* c, int nshots, const std::vector<std::vector<Entry>>& inj, bool record) {
   if (!c->model_set) return fail(-2, "model not set");
-  ++c->gen;
   if (nshots < 1) return fail(-2, "nshots must be >= 1");
   if (!c->tile_fixed) heuristic_tiling(c, nshots);
   if (!c->tile2_fixed && !c->tile_fixed) heuristic_tiling2(c, nshots);
@@ -629,6 +638,17 @@ StepArgs make_args(sw_ctx* c, StepMode mode) {
   return a;
 }
 
+// FNV-1a over the bytes of the captured launches' arguments (the argument
+// blocks are value-initialised, so their padding is zero)
+struct GraphKey {
+  uint64_t h = 1469598103934665603ull;
+  void add(const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t q = 0; q < n; ++q) h = (h ^ b[q]) * 1099511628211ull;
+  }
+};
+constexpr size_t kMaxGraphs = 16;   // captured blocks kept per context
+
 int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   StepArgs a = make_args(c, mode);
   const bool sh = step_sh_on(c), psv = step_psv_on(c), spin = uses_spin(c);
@@ -696,42 +716,67 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   }
   int done = 0;
   const int G = c->graph_steps;   // a multiple of 4
-  if (G >= 2 && nsteps >= 2 * G && !c->graphs_off && !getenv("SEISWG_NO_GRAPHS")) {
-    if (c->graph_gen != c->gen) {
-      for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.exec);
-      c->graphs.clear();
-      c->graph_gen = c->gen;
+  if (G >= 2 && nsteps >= G && !c->graphs_off && !getenv("SEISWG_NO_GRAPHS")) {
+    // everything the captured launches depend on: the kernels (physics,
+    // mode, tile shapes), the block structure and the argument blocks
+    GraphKey k;
+    const int32_t shape[10] = {int32_t(mode), int32_t(sh), int32_t(psv), int32_t(spin), int32_t(pairs), G,
+                               c->tile.nt_threads * 8 + c->tile.group, c->tile2.nt_threads * 8 + c->tile2.group,
+                               int32_t(c->sh_only), 0};
+    k.add(shape, sizeof shape);
+    if (pairs) {
+      k.add(&a2a, sizeof a2a);
+      k.add(&a2b, sizeof a2b);
+    } else {
+      k.add(&a, sizeof a);
+      if (sh) k.add(&a_sh, sizeof a_sh);
     }
-    const uint64_t key = uint64_t(mode) | (uint64_t(a.record) << 4) | (uint64_t(c->sh_only) << 5) |
-                         (uint64_t(pairs) << 6) | (uint64_t(c->cur) << 7) | (uint64_t(G) << 8);
     hipGraphExec_t exec = nullptr;
     for (auto& e : c->graphs)
-      if (e.key == key) exec = e.exec;
+      if (e.key == k.h) {
+        exec = e.exec;
+        e.used = ++c->graph_clock;
+      }
     if (!exec) {
       // Capture G steps.  Should the capture fail (HIP can invalidate a
       // capture because of work other host threads submit meanwhile), nothing
-      // of it ran: this context stops using graphs and launches directly.
+      // of it ran: this context stops using graphs, says so once on stderr
+      // (sw_get_graph_state reports it) and launches directly.
+      if (c->graphs.size() >= kMaxGraphs) {
+        // evict the least recently used block; it may still be running
+        HIPCHK(hipStreamSynchronize(c->stream));
+        auto lru = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                    [](const sw_ctx::GraphEntry& x, const sw_ctx::GraphEntry& y) { return x.used < y.used; });
+        (void)hipGraphExecDestroy(lru->exec);
+        (void)hipGraphDestroy(lru->graph);
+        c->graphs.erase(lru);
+      }
       hipGraph_t graph = nullptr;
       const int64_t l0 = c->launches;
-      bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
-      if (ok) {
-        int rc = 0;
+      hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+      int rc = 0;
+      if (e == hipSuccess) {
         if (pairs)
           for (int q = 0; q < G && !rc; q += 4) rc = enqueue4();
         else
           for (int q = 0; q < G && !rc; ++q) rc = enqueue();
         const hipError_t e_end = hipStreamEndCapture(c->stream, &graph);
-        ok = rc == 0 && e_end == hipSuccess && graph != nullptr &&
-             hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
-        if (graph) (void)hipGraphDestroy(graph);
+        if (e_end != hipSuccess) e = e_end;
+        else if (!graph) e = hipErrorUnknown;
+        if (e == hipSuccess && rc == 0) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       }
       c->launches = l0;   // captured (or discarded), not launched
-      if (ok) {
-        c->graphs.push_back({key, exec});
+      if (e == hipSuccess && rc == 0) {
+        c->graphs.push_back({k.h, graph, exec, ++c->graph_clock});
       } else {
+        if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         c->graphs_off = true;
+        c->graphs_failed = true;
+        c->graph_error = rc ? g_err : std::string(hipGetErrorString(e));
         (void)hipGetLastError();
+        fprintf(stderr, "seiswg: HIP graph capture failed (%s); this context launches its steps directly from now on\n",
+                c->graph_error.c_str());
       }
     }
     const int64_t per_block = pairs ? G / 2 : (mode != M_ADJ && sh && psv) ? 2 * G : G;
@@ -809,8 +854,51 @@ int run_with_snapshots(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user) {
   if (rc && c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
   return rc;
 }
+constexpr int kNoBlowup = 0x7fffffff;
+const int kNoBlowupHost = kNoBlowup;   // static storage: a source that outlives any copy
+
+int blowup_error(int step, int prev) {
+  char earlier[96] = "";
+  if (prev >= 0) snprintf(earlier, sizeof earlier, " (the check after step %d found none)", prev);
+  return fail(-5,
+              "wavefield not finite (NaN or infinite values) after step %d%s: the explicit scheme diverged (time "
+              "step above the stability limit of this model) or an input is not finite",
+              step, earlier);
+}
+
+// enqueue the scan of every shot's current state for the check tagged `step`
+int enqueue_state_check(sw_ctx* c, int step) {
+  HIPCHK(launch_nonfinite(c->state[c->cur].p, int64_t(c->nshots) * c->nslots * c->npt, step, c->dbad.p, c->stream));
+  return 0;
+}
+
+// synchronous check of n device values (gradients, kernels)
+int check_finite(sw_ctx* c, const float* p, int64_t n, const char* what) {
+  if (!c->finite_check || n <= 0) return 0;
+  HIPCHK(c->dbad.ensure(1));
+  HIPCHK(hipMemcpyAsync(c->dbad.p, &kNoBlowupHost, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_nonfinite(p, n, 0, c->dbad.p, c->stream));
+  int bad = kNoBlowup;
+  HIPCHK(hipMemcpyAsync(&bad, c->dbad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (bad != kNoBlowup)
+    return fail(-5, "%s not finite (NaN or infinite values): the forward wavefield diverged (time step above the "
+                    "stability limit of this model) or an input is not finite", what);
+  return 0;
+}
+
 int run_snapshots_body(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user) {
   const int nt = c->g.nt;
+  const bool check = c->finite_check && nt > 0;
+  if (check) {
+    HIPCHK(c->dbad.ensure(1));
+    HIPCHK(hipMemcpyAsync(c->dbad.p, &kNoBlowupHost, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (!c->bad_host) {
+      HIPCHK(hipHostMalloc((void**)&c->bad_host, 2 * sizeof(int), hipHostMallocDefault));
+      c->bad_host[0] = c->bad_host[1] = kNoBlowup;
+    }
+  }
+  int last_clean = -1;   // step of the last check that found the state finite
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   const int64_t l0 = c->launches;
   const int fields[4] = {SW_F_VY, SW_F_VX, SW_F_VZ, SW_F_VY_C};
@@ -845,6 +933,10 @@ int run_snapshots_body(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user) {
   int pending = -1, pending_it = 0, slot = 0;
   auto deliver = [&](int q, int step) -> int {
     HIPCHK(hipEventSynchronize(c->snap_copied[q]));
+    if (check) {
+      if (c->bad_host[q] != kNoBlowup) return blowup_error(c->bad_host[q], last_clean);
+      last_clean = step;
+    }
     const float* h = c->snap_host[q];
     for (int s = 0; s < c->nshots; ++s)
       for (int f = 0; f < 4; ++f) {
@@ -867,6 +959,11 @@ int run_snapshots_body(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user) {
     if (snaps && stop > 0 && stop % snap == 0) {
       // staging slot: its previous transfer must be done (it was delivered)
       HIPCHK(hipStreamWaitEvent(c->stream, c->snap_copied[slot], 0));
+      if (check) {
+        // the scan's result travels with the snapshot (read when it is delivered)
+        if ((rc = enqueue_state_check(c, stop))) return rc;
+        HIPCHK(hipMemcpyAsync(&c->bad_host[slot], c->dbad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      }
       float* d = c->snap_stage[slot].p;
       for (int s = 0; s < c->nshots; ++s)
         for (int f = 0; f < 4; ++f) {
@@ -899,6 +996,15 @@ int run_snapshots_body(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user) {
   c->timing.steps = nt;
   c->timing.step_launches = c->launches - l0;
   c->timing.cell_updates = int64_t(c->npt) * nt * c->nshots;
+  if (check) {
+    // the state after the last step (outside the timed loop)
+    int rc = enqueue_state_check(c, nt - 1);
+    if (rc) return rc;
+    int bad = kNoBlowup;
+    HIPCHK(hipMemcpyAsync(&bad, c->dbad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (bad != kNoBlowup) return blowup_error(bad, last_clean);
+  }
   return 0;
 }
 
@@ -943,13 +1049,17 @@ SW_API void sw_destroy(sw_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.exec);
+  for (auto& e : c->graphs) {
+    (void)hipGraphExecDestroy(e.exec);
+    (void)hipGraphDestroy(e.graph);
+  }
   if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
   for (int q = 0; q < 2; ++q) {
     if (c->snap_host[q]) (void)hipHostFree(c->snap_host[q]);
     if (c->snap_ready[q]) (void)hipEventDestroy(c->snap_ready[q]);
     if (c->snap_copied[q]) (void)hipEventDestroy(c->snap_copied[q]);
   }
+  if (c->bad_host) (void)hipHostFree(c->bad_host);
   if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -998,16 +1108,17 @@ SW_API int sw_set_model(sw_ctx* c, int32_t kind, const float* const* f) {
   HIPCHK(hipMemcpyAsync(c->bzt.p, c->h_bzt.data(), c->h_bzt.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->model_set = true;
-  ++c->gen;
   return 0;
 }
 
 SW_API int sw_set_sources(sw_ctx* c, int32_t nsrc, const int32_t* idx, const float* stf) {
   if (!c || nsrc < 0 || (nsrc > 0 && (!idx || !stf))) return fail(-1, "bad arguments");
   HIPCHK(hipSetDevice(c->device));
+  // a prepared batch points at the old source table and lists: it must be
+  // prepared again (sw_prepare / the next run) before it can advance
+  c->nshots = 0;
   c->nsrc = nsrc;
   c->src_k.assign(idx, idx + nsrc);
-  ++c->gen;
   const size_t n = size_t(3) * nsrc * size_t(c->g.nt);
   HIPCHK(c->stf.ensure(std::max<size_t>(n, 1)));
   if (n) {
@@ -1021,6 +1132,9 @@ SW_API int sw_set_sources(sw_ctx* c, int32_t nsrc, const int32_t* idx, const flo
 
 SW_API int sw_set_receivers(sw_ctx* c, int32_t nrec, const int32_t* idx) {
   if (!c || nrec < 0 || (nrec > 0 && !idx)) return fail(-1, "bad arguments");
+  // traces, displacement accumulators and per-tile receiver lists of a
+  // prepared batch have the old receiver count: prepare again before advancing
+  c->nshots = 0;
   c->nrec = nrec;
   c->rec_k.assign(idx, idx + nrec);
   return 0;
@@ -1156,7 +1270,6 @@ SW_API int sw_set_tiling(sw_ctx* c, int32_t threads, int32_t seglen, int32_t sho
   c->tile.seglen = seglen;
   c->group_req = shots_per_block;
   c->tile_fixed = true;
-  ++c->gen;
   c->nshots = 0;   // lists must be rebuilt
   return 0;
 }
@@ -1252,7 +1365,6 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
     best_ms = best2 / n2;
   }
   c->nshots = 0;
-  ++c->gen;
   if (threads) *threads = bt.nt_threads;
   if (seglen) *seglen = bt.seglen;
   if (shots_per_block) *shots_per_block = bt.group;
@@ -1272,7 +1384,6 @@ SW_API int sw_set_tiling2(sw_ctx* c, int32_t threads, int32_t seglen, int32_t sh
   c->tile2.group = shots_per_block > 0 ? shots_per_block : 1;
   c->group2_req = shots_per_block;
   c->tile2_fixed = true;
-  ++c->gen;
   c->nshots = 0;   // lists must be rebuilt
   return 0;
 }
@@ -1331,6 +1442,20 @@ SW_API int sw_get_timing(const sw_ctx* c, sw_timing* out) {
   return 0;
 }
 
+SW_API int sw_set_finite_check(sw_ctx* c, int32_t on) {
+  if (!c) return fail(-1, "null ctx");
+  c->finite_check = on != 0;
+  return 0;
+}
+
+SW_API int sw_get_graph_state(const sw_ctx* c, int32_t* state, int32_t* captured, char* msg, int32_t msg_len) {
+  if (!c || !state) return fail(-1, "null argument");
+  *state = c->graphs_failed ? -1 : (c->graphs_off || getenv("SEISWG_NO_GRAPHS")) ? 0 : 1;
+  if (captured) *captured = int32_t(c->graphs.size());
+  if (msg && msg_len > 0) snprintf(msg, size_t(msg_len), "%s", c->graph_error.c_str());
+  return 0;
+}
+
 SW_API int sw_smooth(sw_ctx* c, int32_t field, float sigma) {
   if (!c) return fail(-1, "null ctx");
   HIPCHK(hipSetDevice(c->device));
@@ -1338,20 +1463,24 @@ SW_API int sw_smooth(sw_ctx* c, int32_t field, float sigma) {
   if (field != SW_F_K_MU && field != SW_F_K_LAM && field != SW_F_K_RHO) return fail(-2, "only kernel fields smooth");
   if (!k) return fail(-2, "kernel field %d not computed", field);
   const int nx = c->g.nx, nz = c->g.nz, nmax = std::max(nx, nz);
-  // w[d] = exp(-d*d / (2*sigma*sigma)) in f32, the reference's expression order
+  // w[d] = exp(-d*d / (2*sigma*sigma)) in f32, the reference's expression
+  // order; nw = 1 + the last d with w[d] != 0 (the sums skip the zero terms,
+  // which leaves them unchanged)
   std::vector<float> w(nmax), sx(nx, 0.0f), sz(nz, 0.0f);
+  int nw = 0;
   for (int d = 0; d < nmax; ++d) {
     const float fd = float(d);
     w[d] = std::exp(-fd * fd / (2.0f * sigma * sigma));
+    if (w[d] != 0.0f) nw = d + 1;
   }
   for (int i = 0; i < nx; ++i) {
     float s = 0.0f;
-    for (int n = 0; n < nx; ++n) s = s + w[std::abs(i - n)];
+    for (int n = std::max(i - nw + 1, 0); n < std::min(i + nw, nx); ++n) s = s + w[std::abs(i - n)];
     sx[i] = s;
   }
   for (int j = 0; j < nz; ++j) {
     float s = 0.0f;
-    for (int n = 0; n < nz; ++n) s = s + w[std::abs(j - n)];
+    for (int n = std::max(j - nw + 1, 0); n < std::min(j + nw, nz); ++n) s = s + w[std::abs(j - n)];
     sz[j] = s;
   }
   HIPCHK(c->wtab.ensure(nmax));
@@ -1363,8 +1492,8 @@ SW_API int sw_smooth(sw_ctx* c, int32_t field, float sigma) {
   HIPCHK(hipMemcpyAsync(c->sumx.p, sx.data(), sx.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->sumz.p, sz.data(), sz.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(launch_gsum(c->sumx.p, c->sumz.p, c->gsum.p, nx, nz, c->stream));
-  HIPCHK(launch_gauss_x(c->wtab.p, k, c->gtmp.p, nx, nz, c->stream));
-  HIPCHK(launch_gauss_z(c->wtab.p, c->gtmp.p, c->gsum.p, k, nx, nz, c->stream));
+  HIPCHK(launch_gauss_x(c->wtab.p, nw, k, c->gtmp.p, nx, nz, c->stream));
+  HIPCHK(launch_gauss_z(c->wtab.p, nw, c->gtmp.p, c->gsum.p, k, nx, nz, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1471,8 +1600,12 @@ int grad_adjsrc_and_bwd_setup(sw_ctx* c, const float* obs_ref_batch) {
   }
   HIPCHK(c->aclock.ensure(2));
   c->aux_period = P.K;
-  ++c->gen;
   HIPCHK(hipStreamSynchronize(c->stream));   // obs_ref_batch and P.mis are host memory
+  if (c->finite_check)
+    for (int s = 0; s < nb; ++s)
+      if (!std::isfinite(P.mis[s]))
+        return fail(-5, "misfit of shot %d is not finite (%g): the forward wavefield diverged (time step above the "
+                        "stability limit of this model) or an input is not finite", s, P.mis[s]);
   return 0;
 }
 
@@ -1490,6 +1623,15 @@ int grad_segment_setup(sw_ctx* c, int q, int& n0, int& n1) {
   c->h_clk[0] = n0; c->h_clk[1] = 0; c->h_clk[2] = n1 - 1; c->h_clk[3] = 0;
   HIPCHK(hipMemcpyAsync(c->clock.p, c->h_clk, 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->aclock.p, c->h_clk + 2, 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+// kernel fields after a gradient
+int check_grad_finite(sw_ctx* c) {
+  int rc = 0;
+  if ((rc = check_finite(c, c->k_rho.p, c->npt, "K_rho")) || (rc = check_finite(c, c->kmu.p, c->npt, "K_mu")) ||
+      (rc = check_finite(c, c->k_lam.p, c->npt, "K_lambda")))
+    return rc;
   return 0;
 }
 
@@ -1544,7 +1686,7 @@ SW_API int sw_gradient(sw_ctx* c, int32_t nshots, const int32_t* isrc, const flo
     if (misfit)
       for (int s = 0; s < nb; ++s) misfit[b0 + s] = P.mis[s];
   }
-  return 0;
+  return check_grad_finite(c);
 }
 
 }  // extern "C"
@@ -1792,6 +1934,8 @@ int multi_gradient_merged(sw_multi* m, const int32_t* nshots, const int32_t* con
   }
   for (sw_ctx* c : m->p)
     if ((rc = grad_reduce(c))) return rc;
+  for (sw_ctx* c : m->p)
+    if ((rc = check_grad_finite(c))) return rc;
   HIPCHK(hipEventRecord(e1, m->stream));
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0;
@@ -1983,6 +2127,9 @@ SW_API int sw_adjoint_plain(sw_ctx* c, int32_t nruns, const int32_t* isrc, const
       HIPCHK(hipMemcpyAsync(&mis, c->misfit_d.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
       mis = std::sqrt(mis);
+      if (c->finite_check && !std::isfinite(mis))
+        return fail(-5, "misfit of run %d is not finite (%g): the forward wavefield diverged (time step above the "
+                        "stability limit of this model) or an input is not finite", r, mis);
     } else if (nobs) {
       HIPCHK(launch_fill(c->adj_src.p, 0.0f, int64_t(nobs), c->stream));
     }
@@ -2009,7 +2156,8 @@ SW_API int sw_adjoint_plain(sw_ctx* c, int32_t nruns, const int32_t* isrc, const
     }
   }
   HIPCHK(hipStreamSynchronize(c->stream));
-  return sw_smooth(c, SW_F_K_MU, smooth);
+  const int rc = sw_smooth(c, SW_F_K_MU, smooth);
+  return rc ? rc : check_finite(c, c->kmu.p, c->npt, "K_mu");
 }
 
 }  // extern "C"

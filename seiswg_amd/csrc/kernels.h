@@ -121,8 +121,9 @@ int step_blocks_per_cu(int kind, int nt_threads, int group);
 hipError_t launch_imaging_sh(const float* vadj, const float* vfw, float* kmu, int nx, int nz,
                              float h8x, float h8z, float ndt, hipStream_t s);
 hipError_t launch_gsum(const float* sumx, const float* sumz, float* gsum, int nx, int nz, hipStream_t s);
-hipError_t launch_gauss_x(const float* w, const float* k, float* gtmp, int nx, int nz, hipStream_t s);
-hipError_t launch_gauss_z(const float* w, const float* gtmp, const float* gsum, float* k, int nx, int nz,
+// separable Gaussian sums; w[d] == 0 for d >= nw
+hipError_t launch_gauss_x(const float* w, int nw, const float* k, float* gtmp, int nx, int nz, hipStream_t s);
+hipError_t launch_gauss_z(const float* w, int nw, const float* gtmp, const float* gsum, float* k, int nx, int nz,
                           hipStream_t s);
 hipError_t launch_misfit(const float* syn, const float* ref, const float* taper, float* adstf,
                          double* partial, int nrec, int nt, hipStream_t s, int* nblocks_out);
@@ -131,5 +132,7 @@ hipError_t launch_vps2lm(const float* rho, float* lam_vp, float* mu_vs, int64_t 
 hipError_t launch_fill(float* p, float v, int64_t n, hipStream_t s);
 hipError_t launch_recip(const float* x, float* out, int64_t n, hipStream_t s);
 hipError_t launch_copy(const float* src, float* dst, int64_t n, hipStream_t s);
+// *bad = min(*bad, tag) if any of p[0..n) is NaN or infinite
+hipError_t launch_nonfinite(const float* p, int64_t n, int tag, int* bad, hipStream_t s);
 
 }  // namespace sw

@@ -2060,8 +2060,12 @@ __global__ void k_imaging_sh(const float* __restrict__ va, const float* __restri
 
 // ----------------------------------------------------- Gaussian smoothing
 // init_gsum, gaussian_x, gaussian_z (fd2d.wgsl:517-554, 592-609).  w[d] is the
-// 1-D weight exp(-d*d/(2*sigma*sigma)) tabulated by the host; every sum runs
-// over the full axis in the reference's order.
+// 1-D weight exp(-d*d/(2*sigma*sigma)) tabulated by the host, and w[d] == 0
+// for every d >= nw.  The reference sums over the full axis; the terms with
+// |i - n| >= nw are 0 * k = +-0, and adding +-0 to a sum that starts at +0
+// (which can never become -0 in round-to-nearest) leaves it unchanged, so
+// summing over |i - n| < nw only, in the same order, gives the same bits
+// for finite fields at O(npt * nw) instead of O(npt * (nx + nz)).
 __global__ void k_gsum(const float* __restrict__ sumx, const float* __restrict__ sumz, float* __restrict__ gsum,
                        int nx, int nz) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2069,25 +2073,27 @@ __global__ void k_gsum(const float* __restrict__ sumx, const float* __restrict__
   if (j >= nz) return;
   gsum[i * nz + j] = sumx[i] * sumz[j];
 }
-__global__ void k_gauss_x(const float* __restrict__ w, const float* __restrict__ kin, float* __restrict__ gtmp,
-                          int nx, int nz) {
+__global__ void k_gauss_x(const float* __restrict__ w, int nw, const float* __restrict__ kin,
+                          float* __restrict__ gtmp, int nx, int nz) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= nz) return;
   float s = 0.0f;
-  for (int n = 0; n < nx; ++n) s = s + w[abs(i - n)] * kin[n * nz + j];
-  gtmp[i * nz + j] = s;
+  const int lo = max(i - nw + 1, 0), hi = min(i + nw, nx);
+  for (int n = lo; n < hi; ++n) s = s + w[abs(i - n)] * kin[int64_t(n) * nz + j];
+  gtmp[int64_t(i) * nz + j] = s;
 }
-__global__ void k_gauss_z(const float* __restrict__ w, const float* __restrict__ gtmp,
+__global__ void k_gauss_z(const float* __restrict__ w, int nw, const float* __restrict__ gtmp,
                           const float* __restrict__ gsum, float* __restrict__ kout, int nx, int nz) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= nz) return;
   float s = 0.0f;
-  const float* row = gtmp + i * nz;
-  for (int n = 0; n < nz; ++n) s = s + w[abs(j - n)] * row[n];
-  const float g = gsum[i * nz + j];
-  kout[i * nz + j] = (g > 1e-30f) ? s / g : 0.0f;
+  const float* row = gtmp + int64_t(i) * nz;
+  const int lo = max(j - nw + 1, 0), hi = min(j + nw, nz);
+  for (int n = lo; n < hi; ++n) s = s + w[abs(j - n)] * row[n];
+  const float g = gsum[int64_t(i) * nz + j];
+  kout[int64_t(i) * nz + j] = (g > 1e-30f) ? s / g : 0.0f;
 }
 
 // ------------------------------------------ waveform misfit + adjoint source
@@ -2146,6 +2152,21 @@ __global__ void k_recip(const float* __restrict__ x, float* __restrict__ out, in
 __global__ void k_fill(float* __restrict__ p, float v, int64_t n) {
   const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k < n) p[k] = v;
+}
+__global__ void k_fill_f64(double* __restrict__ p, double v, int64_t n) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+// Blow-up check: *bad = min(*bad, tag) when any of the n values is NaN or
+// infinite.  Each wavefront votes once, so at most one atomic per wavefront.
+__global__ __launch_bounds__(256) void k_nonfinite(const float* __restrict__ p, int64_t n, int tag, int* bad) {
+  bool nf = false;
+  const int64_t stride = int64_t(gridDim.x) * 256;
+  for (int64_t k = int64_t(blockIdx.x) * 256 + threadIdx.x; k < n; k += stride)
+    nf |= (__builtin_bit_cast(uint32_t, p[k]) & 0x7f800000u) == 0x7f800000u;
+  if (__builtin_amdgcn_ballot_w64(nf) != 0 && __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)) == 0)
+    atomicMin(bad, tag);
 }
 
 // plain streaming copy (float4), the calibration kernel for HBM counters
@@ -2353,6 +2374,9 @@ hipError_t launch_adjsrc(const float* obs_syn, const float* obs_ref, const float
     hipLaunchKernelGGL(k_adjsrc, dim3((total + 127) / 128), dim3(128), 0, s, obs_syn, obs_ref, taper, adj, part,
                        nshots, ncomp, comps, nrec, nt, dt);
     hipLaunchKernelGGL(k_misfit_shots, dim3((nshots + 63) / 64), dim3(64), 0, s, part, misfit, nshots, ncomp * nrec);
+  } else if (nshots > 0) {
+    // no receivers: no traces, a zero misfit per shot (and no adjoint source)
+    hipLaunchKernelGGL(k_fill_f64, dim3((nshots + 63) / 64), dim3(64), 0, s, misfit, 0.0, int64_t(nshots));
   }
   return hipGetLastError();
 }
@@ -2376,13 +2400,13 @@ hipError_t launch_gsum(const float* sumx, const float* sumz, float* gsum, int nx
   hipLaunchKernelGGL(k_gsum, dim3((nz + 255) / 256, nx), dim3(256), 0, s, sumx, sumz, gsum, nx, nz);
   return hipGetLastError();
 }
-hipError_t launch_gauss_x(const float* w, const float* k, float* gtmp, int nx, int nz, hipStream_t s) {
-  hipLaunchKernelGGL(k_gauss_x, dim3((nz + 255) / 256, nx), dim3(256), 0, s, w, k, gtmp, nx, nz);
+hipError_t launch_gauss_x(const float* w, int nw, const float* k, float* gtmp, int nx, int nz, hipStream_t s) {
+  hipLaunchKernelGGL(k_gauss_x, dim3((nz + 255) / 256, nx), dim3(256), 0, s, w, nw, k, gtmp, nx, nz);
   return hipGetLastError();
 }
-hipError_t launch_gauss_z(const float* w, const float* gtmp, const float* gsum, float* k, int nx, int nz,
+hipError_t launch_gauss_z(const float* w, int nw, const float* gtmp, const float* gsum, float* k, int nx, int nz,
                           hipStream_t s) {
-  hipLaunchKernelGGL(k_gauss_z, dim3((nz + 255) / 256, nx), dim3(256), 0, s, w, gtmp, gsum, k, nx, nz);
+  hipLaunchKernelGGL(k_gauss_z, dim3((nz + 255) / 256, nx), dim3(256), 0, s, w, nw, gtmp, gsum, k, nx, nz);
   return hipGetLastError();
 }
 hipError_t launch_misfit(const float* syn, const float* ref, const float* taper, float* adstf, double* partial,
@@ -2410,6 +2434,13 @@ hipError_t launch_recip(const float* x, float* out, int64_t n, hipStream_t s) {
 hipError_t launch_fill(float* p, float v, int64_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_nonfinite(const float* p, int64_t n, int tag, int* bad, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t nb = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_nonfinite, dim3(unsigned(nb)), dim3(256), 0, s, p, n, tag, bad);
   return hipGetLastError();
 }
 

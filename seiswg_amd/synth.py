@@ -115,7 +115,7 @@ def psv_batch_problem():
 
 
 PSV_GRADIENT = dict(nx=1000, nz=600, h=10.0, nshots=16, nrec=200, nt=3000, dt=1e-3, f0=8.0, t0=0.15, ang=90.0,
-          abs_width=40, seed=2, anomaly_seed=3, ckpt=50)
+          abs_width=40, seed=2, anomaly_seed=3, ckpt=50, smooth=3.0)
 
 
 def gradient_problem():
@@ -141,7 +141,8 @@ UNEVEN = dict(nproblems=48, seed=4, nx=(384, 3072), nz=(384, 1536), shots=(4, 16
 UNEVEN_SMALL = dict(UNEVEN, seed=5, nx=(128, 448), nz=(128, 448), shots=(2, 6), nrec=(40, 100), abs_width=20)
 
 
-def uneven_problems(nt_scale: float = 1.0, nproblems: int | None = None, spec: dict | None = None):
+def uneven_problems(nt_scale: float = 1.0, nproblems: int | None = None, spec: dict | None = None,
+                    only=None):
     """Uneven batch of independent SH and P-SV gradient problems (seeded):
     grid nx in [384, 3072], nz in [384, 1536], each with its own
     layered-plus-random-field model (the PSV_BATCH recipe, seed 100+q) and a true
@@ -149,8 +150,9 @@ def uneven_problems(nt_scale: float = 1.0, nproblems: int | None = None, spec: d
     receivers at 10 m; nt in [1500, 4000] (times nt_scale, at least 60);
     dt from CFL 0.5 on the model's largest vp; Ricker f0 in [5, 15] Hz,
     t0 = 1.2 / f0.  Even problems are SH, odd ones P-SV.  `spec` replaces
-    these ranges (UNEVEN_SMALL: the small-problem batch).  Returns a list of
-    dicts: kind, nx, nz, h, dt, nt, init, true, geom."""
+    these ranges (UNEVEN_SMALL: the small-problem batch).  `only` (indices)
+    builds just those problems, with the same draws as the whole batch.
+    Returns a list of dicts: kind, nx, nz, h, dt, nt, init, true, geom, index."""
     c = spec or UNEVEN
     rng = np.random.default_rng(c["seed"])
     out = []
@@ -161,6 +163,8 @@ def uneven_problems(nt_scale: float = 1.0, nproblems: int | None = None, spec: d
         nrec = int(rng.integers(c["nrec"][0], c["nrec"][1] + 1))
         nt = max(60, int(round(int(rng.integers(c["nt"][0], c["nt"][1] + 1)) * nt_scale)))
         f0 = float(rng.uniform(*c["f0"]))
+        if only is not None and q not in only:
+            continue
         init = layered_random_model(nx, nz, c["h"], 100 + q)
         true = gaussian_anomalies(init, nx, nz, c["h"], 200 + q)
         vmax = float(max(init["vp"].max(), true["vp"].max()))
@@ -168,7 +172,7 @@ def uneven_problems(nt_scale: float = 1.0, nproblems: int | None = None, spec: d
         step = max((nx - 4) // nrec, 1)
         geom = line_geometry(nx, c["h"], nshots, nrec, rec_first=2, rec_step=step, f0=f0, t0=1.2 / f0, ang=90.0)
         out.append(dict(kind="sh" if q % 2 == 0 else "psv", nx=nx, nz=nz, h=c["h"], dt=dt, nt=nt, init=init,
-                        true=true, geom=geom))
+                        true=true, geom=geom, index=q))
     return out
 
 

@@ -595,3 +595,82 @@ def test_smoothing_matches_oracle(seiswg):
     o.set_field(O.F["K_MU"], k)
     o.smooth(3.0)
     compare(c.field(seiswg.abi.F_K_MU), o.field(O.F["K_MU"]), "smoothed", tol=1e-6)
+
+
+def test_setters_discard_a_prepared_batch(seiswg):
+    """sw_set_receivers / sw_set_sources after sw_prepare: the batch's traces,
+    accumulators and lists have the old counts, so the library refuses to
+    advance (or read its traces) until the batch is prepared again."""
+    p, stf = random_problem(30, 40, 12, False, True, False, seed=1)
+    c = gpu_context(p, stf, seiswg)
+    c.prepare([0])
+    c.advance(2)
+    c.set_receivers(p.rec_idx[:3])
+    with pytest.raises(seiswg.SeiswgError, match="no prepared batch"):
+        c.advance(1)
+    with pytest.raises(seiswg.SeiswgError, match="not in last run"):
+        c.obs(0, seiswg.abi.OBS_X)
+    c.prepare([0])
+    c.advance(2)
+    c.set_sources(p.src_idx[:2], stf[:, :2])
+    with pytest.raises(seiswg.SeiswgError, match="no prepared batch"):
+        c.advance(1)
+    c.prepare([1])
+    c.advance(3)
+
+
+def _spin_problem(seiswg, j_value):
+    from seiswg_amd import synth
+    q = synth.spin_forward_problem(j_value)
+    nx, nz, h, nt, dt, g, m = q["nx"], q["nz"], q["h"], q["nt"], q["dt"], q["geom"], q["model"]
+    ctx = seiswg.Context(nx, nz, nt, h, h, dt, sh=False, psv=True, spin=True, abs_left=True, abs_right=True,
+                         abs_bottom=True, abs_top=True, abs_width=q["abs_width"], abs_alpha=0.015)
+    ctx.set_model(seiswg.abi.MODEL_LAME, [m[k] for k in ("rho", "lambda", "mu", "nu", "j", "lambda_c", "mu_c", "nu_c")])
+    rows = g.src_rows()
+    ctx.set_sources([seiswg.grid_index(r[0], r[1], h, h, nz) for r in rows],
+                    seiswg.ricker_table([[r[3], r[4], r[5], r[6]] for r in rows], nt, dt))
+    ctx.set_receivers([seiswg.grid_index(x, z, h, h, nz) for x, z in zip(g.rec_x, g.rec_z)])
+    return ctx
+
+
+def test_diverging_run_is_reported_with_its_step(seiswg):
+    """The 500x500 P-SV + spin configuration with its stated micro-inertia
+    j = 1e-2 (omega * dt ~ 358: the explicit scheme diverges): the forward
+    run returns an error naming the first snapshot step whose state is not
+    finite; with j = 1e4 the run is finite and passes; with the check off
+    the diverged run completes."""
+    print("case spin 500x500 j=1e-2 snap=500", flush=True)
+    bad = _spin_problem(seiswg, 1e-2)
+    snaps = {}
+    with pytest.raises(seiswg.SeiswgError, match=r"not finite .*after step 500\b"):
+        bad.forward_batch([-1], snap=500, snapshots=snaps)
+    bad.set_finite_check(False)
+    bad.forward_batch([-1])
+    assert not np.all(np.isfinite(bad.obs(0, seiswg.abi.OBS_X)))
+    print("case spin 500x500 j=1e4", flush=True)
+    good = _spin_problem(seiswg, 1e4)
+    good.forward_batch([-1], snap=500, snapshots=snaps)
+    assert np.all(np.isfinite(good.obs(0, seiswg.abi.OBS_X)))
+
+
+def test_graph_blocks_reused_across_prepared_batches(seiswg, monkeypatch):
+    """Runs of 16 steps or more replay captured 16-step blocks; a batch
+    prepared again with the same buffers and shapes replays the same blocks
+    (no new capture), and the results equal launches without graphs."""
+    p, stf = random_problem(60, 70, 80, False, True, False, seed=3)
+    c = gpu_context(p, stf, seiswg)
+    c.prepare([0, 1])
+    c.advance(40)
+    state, n, msg = c.graph_state()
+    assert state == 1 and n >= 1 and msg == ""
+    got = c.field(seiswg.abi.F_VX, 1)
+    c.prepare([0, 1])
+    c.advance(40)
+    assert c.graph_state()[1] == n
+    assert np.array_equal(c.field(seiswg.abi.F_VX, 1).view(np.uint32), got.view(np.uint32))
+    monkeypatch.setenv("SEISWG_NO_GRAPHS", "1")
+    d = gpu_context(p, stf, seiswg)
+    d.prepare([0, 1])
+    d.advance(40)
+    assert d.graph_state()[:2] == (0, 0)
+    assert np.array_equal(d.field(seiswg.abi.F_VX, 1).view(np.uint32), got.view(np.uint32))

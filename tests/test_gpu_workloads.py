@@ -1,0 +1,165 @@
+"""GPU parity at the real shapes of the benchmark workloads.
+
+* batched multi-shot P-SV forward: 2000x1000 grid, 64 shots, 400 receivers,
+  the two-step tile (512 threads, 500-row segments, one shot per workgroup)
+  and the one-step tile (256 threads, 500-row segments, four shots per
+  workgroup) - every state field and trace of shots 0, 31 and 63 bit for bit
+  against the FP32 oracle after six steps from random states;
+* checkpointed P-SV gradient: 1000x600, 16 shots, 200 receivers, checkpoint
+  every 50 steps, the library's default tile shapes (those the benchmark
+  measures) - kernels against the FP64 oracle adjoint (rel-L2 <= 1e-4);
+* uneven batch of gradient problems at full grid size (up to 3072x1536,
+  SH and P-SV) through sw_multi_gradient - the 3072x1536 problem against
+  the FP64 oracle adjoint, the others against their single-problem
+  gradients bit for bit.
+
+Time axes are cut (the tile shapes and batch geometry are those of the
+full runs); the reference itself cannot run these sizes (its field buffer
+exceeds wgpu's 128 MiB binding limit).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PSV_STATE = ["VX", "VZ", "SXX", "SZZ", "SXZ"]
+
+
+def _ctx(seiswg, nx, nz, nt, h, dt, model, geom, sh=False, abs_width=40):
+    c = seiswg.Context(nx, nz, nt, h, h, dt, sh=sh, psv=not sh, spin=False, abs_left=True, abs_right=True,
+                       abs_bottom=True, abs_top=False, abs_width=abs_width, abs_alpha=0.015)
+    c.set_model(seiswg.abi.MODEL_VPVS, [model["rho"], model["vp"], model["vs"]])
+    rows = geom.src_rows()
+    src_idx = [seiswg.grid_index(r[0], r[1], h, h, nz) for r in rows]
+    rec_idx = [seiswg.grid_index(x, z, h, h, nz) for x, z in zip(geom.rec_x, geom.rec_z)]
+    c.set_sources(src_idx, seiswg.ricker_table([[r[3], r[4], r[5], r[6]] for r in rows], nt, dt))
+    c.set_receivers(rec_idx)
+    return c, src_idx, rec_idx
+
+
+def _problem(nx, nz, nt, h, dt, model, geom, src_idx, rec_idx, sh=False, abs_width=40):
+    return O.Problem(nx=nx, nz=nz, nt=nt, dx=h, dz=h, dt=dt, sh=sh, psv=not sh, spin=False, abs_left=True,
+                     abs_right=True, abs_bottom=True, abs_top=False, abs_width=abs_width, abs_alpha=0.015,
+                     model=model, src_rows=geom.src_rows(), src_idx=src_idx, rec_idx=rec_idx)
+
+
+def test_batched_psv_2000x1000_64_shots_headline_tiles_match_oracle_bitwise(seiswg):
+    from seiswg_amd import synth
+    cfg = synth.PSV_BATCH
+    nx, nz, h, dt, nshots = cfg["nx"], cfg["nz"], cfg["h"], cfg["dt"], cfg["nshots"]
+    nsteps = 6                      # one pair of two-step launches, then two one-step launches
+    model, geom = synth.psv_batch_problem()
+    c, src_idx, rec_idx = _ctx(seiswg, nx, nz, nsteps + 2, h, dt, model, geom)
+    c.set_tiling(256, 500, 4)
+    c.set_tiling2(512, 500, 1)
+    c.prepare(list(range(nshots)))
+    assert c.tiling() == (256, 500, 4) and c.tiling2() == (512, 500, 1)
+    print(f"case 2000x1000 x {nshots} shots, tiles {c.tiling()} / {c.tiling2()}", flush=True)
+    check = {0: None, 31: None, 63: None}
+    rng = np.random.default_rng(123)
+    for s in range(nshots):
+        st = {}
+        for name in PSV_STATE:
+            v = rng.standard_normal(nx * nz, dtype=np.float32)
+            c.write_field(getattr(seiswg.abi, "F_" + name), v, shot=s)
+            st[name] = v
+        if s in check:
+            check[s] = st
+    c.advance(nsteps)
+    p = _problem(nx, nz, nsteps + 2, h, dt, model, geom, src_idx, rec_idx)
+    o = O.Oracle(p, 4)
+    for s, st in check.items():
+        o.clear()
+        for name, v in st.items():
+            o.set_field(O.F[name], v)
+        for it in range(nsteps):
+            o.step(it, s)
+        for name in PSV_STATE:
+            got = c.field(getattr(seiswg.abi, "F_" + name), shot=s)
+            want = np.asarray(o.field(O.F[name]), np.float64).astype(np.float32)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (s, name)
+        obs = o.obs()
+        for comp in (O.OBS_X, O.OBS_Z):
+            got = c.obs(s, comp)[:, :nsteps]
+            want = np.asarray(obs[comp][:, :nsteps], np.float64).astype(np.float32)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (s, comp)
+        print(f"shot {s}: 5 state fields and 2 trace components bit-identical", flush=True)
+
+
+def test_gradient_1000x600_16_shots_default_tiles_matches_fp64_oracle(seiswg):
+    from seiswg_amd import synth
+    cfg = synth.PSV_GRADIENT
+    nx, nz, h, dt, nshots = cfg["nx"], cfg["nz"], cfg["h"], cfg["dt"], cfg["nshots"]
+    nt, ckpt = 140, cfg["ckpt"]     # segments of 50, 50 and 40 steps
+    init, _, geom = synth.gradient_problem()
+    c, src_idx, rec_idx = _ctx(seiswg, nx, nz, nt, h, dt, init, geom)
+    rng = np.random.default_rng(4)
+    obs = (rng.standard_normal((nshots, 2, len(rec_idx), nt)) * 1e-12).astype(np.float32)
+    shots = list(range(nshots))
+    mis, kr, km, kl = c.gradient(shots, obs, ckpt_every=ckpt)
+    print(f"case 1000x600 x {nshots} shots nt={nt} ckpt={ckpt}: tiles {c.tiling()} / {c.tiling2()}", flush=True)
+    p = _problem(nx, nz, nt, h, dt, init, geom, src_idx, rec_idx)
+    o = O.Oracle(p, 8)
+    J, okr, okm, okl = o.gradient(shots, obs.astype(np.float64))
+    assert mis.sum() == pytest.approx(J, rel=1e-5)
+    for name, g, w in (("K_rho", kr, okr), ("K_mu", km, okm), ("K_lambda", kl, okl)):
+        err = O.rel_l2(g, w)
+        print(f"{name}: rel-L2 vs FP64 oracle adjoint {err:.2e}", flush=True)
+        assert err <= 1e-4, name
+
+
+def _uneven_subset(seiswg, synth):
+    """Three problems of the uneven batch (the largest SH problem and two
+    P-SV ones, at their full grid sizes and shot counts) and one P-SV
+    problem of the batch's largest grid, 3072x1536, built by the same recipe
+    with two shots; nt cut to about 3% of the batch's."""
+    pick = synth.uneven_problems(0.03, only={4, 9, 17})
+    nx, nz, hh = 3072, 1536, synth.UNEVEN["h"]
+    init = synth.layered_random_model(nx, nz, hh, 190)
+    true = synth.gaussian_anomalies(init, nx, nz, hh, 191)
+    dt = float(np.float32(synth.UNEVEN["cfl"] * hh / float(max(init["vp"].max(), true["vp"].max()))))
+    geom = synth.line_geometry(nx, hh, 2, 150, rec_first=2, rec_step=(nx - 4) // 150, f0=15.0, t0=0.08, ang=90.0)
+    pick.append(dict(kind="psv", nx=nx, nz=nz, h=hh, dt=dt, nt=100, init=init, true=true, geom=geom))
+    out = []
+    for p in pick:
+        c, src_idx, rec_idx = _ctx(seiswg, p["nx"], p["nz"], p["nt"], p["h"], p["dt"], p["init"], p["geom"],
+                                   sh=p["kind"] == "sh", abs_width=synth.UNEVEN["abs_width"])
+        out.append((p, c, src_idx, rec_idx))
+    return out
+
+
+def test_uneven_batch_full_size_problems_match_oracle_and_single_runs(seiswg):
+    from seiswg_amd import synth
+    ckpt = synth.UNEVEN["ckpt"]
+    probs = _uneven_subset(seiswg, synth)
+    runs, refs = [], []
+    rng = np.random.default_rng(6)
+    for p, c, src_idx, rec_idx in probs:
+        nc = 1 if p["kind"] == "sh" else 2
+        n = len(p["geom"].src_x)
+        runs.append(list(range(n)))
+        refs.append((rng.standard_normal((n, nc, len(rec_idx), p["nt"])) * 1e-12).astype(np.float32))
+        print(f"problem {p['kind']} {p['nx']}x{p['nz']} shots={n} nt={p['nt']}", flush=True)
+    single = [c.gradient(r, o, ckpt_every=ckpt) for (p, c, *_), r, o in zip(probs, runs, refs)]
+    m = seiswg.Multi([c for _, c, *_ in probs])
+    got = m.gradient(runs, refs, ckpt_every=ckpt)
+    for a, b in zip(got, single):
+        assert np.array_equal(a[0], b[0])
+        for x, y in zip(a[1:], b[1:]):
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    print("multi-problem gradients equal the single-problem ones bit for bit", flush=True)
+    p, c, src_idx, rec_idx = probs[-1]
+    o = O.Oracle(_problem(p["nx"], p["nz"], p["nt"], p["h"], p["dt"], p["init"], p["geom"], src_idx, rec_idx,
+                          abs_width=synth.UNEVEN["abs_width"]), 8)
+    J, okr, okm, okl = o.gradient(runs[-1], refs[-1].astype(np.float64))
+    mis, kr, km, kl = got[-1]
+    assert mis.sum() == pytest.approx(J, rel=1e-5)
+    for name, g, w in (("K_rho", kr, okr), ("K_mu", km, okm), ("K_lambda", kl, okl)):
+        err = O.rel_l2(g, w)
+        print(f"3072x1536 {name}: rel-L2 vs FP64 oracle adjoint {err:.2e}", flush=True)
+        assert err <= 1e-4, name
+    m.close()
+    for _, c, *_ in probs:
+        c.close()

@@ -239,7 +239,7 @@ struct sw_ctx {
   std::string graph_error;
 
   // plain adjoint
-  DBuf<float> kmu, k_lam, k_rho, gsum, gtmp, wtab, sumx, sumz, fwd_store, adj_src, ref, taper;
+  DBuf<float> kmu, k_lam, k_rho, gtmp, wtab, sumx, sumz, fwd_store, adj_src, ref, taper;
   DBuf<double> partial, misfit_d;
 
   // differentiable gradient: adjoint state ([shot][slot][npt] ping-pong),
@@ -854,6 +854,32 @@ int run_with_snapshots(sw_ctx* c, int snap, sw_snapshot_fn cb, void* user) {
   if (rc && c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
   return rc;
 }
+// Tables of the separable Gaussian (smooth_gradient, solver.rs:1249-1274):
+// w[d] = exp(-d*d / (2*sigma*sigma)) in f32 in the reference's expression
+// order for d < max(nx, nz), and the 1-D weight sums sx[i], sz[j] whose
+// product is init_gsum's GSUM.  Returns nw = 1 + the last d with w[d] != 0:
+// the sums skip the zero terms, which leaves them unchanged.
+int smooth_tables(int nx, int nz, float sigma, float* w, float* sx, float* sz) {
+  const int nmax = std::max(nx, nz);
+  int nw = 0;
+  for (int d = 0; d < nmax; ++d) {
+    const float fd = float(d);
+    w[d] = std::exp(-fd * fd / (2.0f * sigma * sigma));
+    if (w[d] != 0.0f) nw = d + 1;
+  }
+  for (int i = 0; i < nx; ++i) {
+    float s = 0.0f;
+    for (int n = std::max(i - nw + 1, 0); n < std::min(i + nw, nx); ++n) s = s + w[std::abs(i - n)];
+    sx[i] = s;
+  }
+  for (int j = 0; j < nz; ++j) {
+    float s = 0.0f;
+    for (int n = std::max(j - nw + 1, 0); n < std::min(j + nw, nz); ++n) s = s + w[std::abs(j - n)];
+    sz[j] = s;
+  }
+  return nw;
+}
+
 constexpr int kNoBlowup = 0x7fffffff;
 const int kNoBlowupHost = kNoBlowup;   // static storage: a source that outlives any copy
 
@@ -1463,37 +1489,17 @@ SW_API int sw_smooth(sw_ctx* c, int32_t field, float sigma) {
   if (field != SW_F_K_MU && field != SW_F_K_LAM && field != SW_F_K_RHO) return fail(-2, "only kernel fields smooth");
   if (!k) return fail(-2, "kernel field %d not computed", field);
   const int nx = c->g.nx, nz = c->g.nz, nmax = std::max(nx, nz);
-  // w[d] = exp(-d*d / (2*sigma*sigma)) in f32, the reference's expression
-  // order; nw = 1 + the last d with w[d] != 0 (the sums skip the zero terms,
-  // which leaves them unchanged)
-  std::vector<float> w(nmax), sx(nx, 0.0f), sz(nz, 0.0f);
-  int nw = 0;
-  for (int d = 0; d < nmax; ++d) {
-    const float fd = float(d);
-    w[d] = std::exp(-fd * fd / (2.0f * sigma * sigma));
-    if (w[d] != 0.0f) nw = d + 1;
-  }
-  for (int i = 0; i < nx; ++i) {
-    float s = 0.0f;
-    for (int n = std::max(i - nw + 1, 0); n < std::min(i + nw, nx); ++n) s = s + w[std::abs(i - n)];
-    sx[i] = s;
-  }
-  for (int j = 0; j < nz; ++j) {
-    float s = 0.0f;
-    for (int n = std::max(j - nw + 1, 0); n < std::min(j + nw, nz); ++n) s = s + w[std::abs(j - n)];
-    sz[j] = s;
-  }
+  std::vector<float> w(nmax), sx(nx), sz(nz);
+  const int nw = smooth_tables(nx, nz, sigma, w.data(), sx.data(), sz.data());
   HIPCHK(c->wtab.ensure(nmax));
   HIPCHK(c->sumx.ensure(nx));
   HIPCHK(c->sumz.ensure(nz));
-  HIPCHK(c->gsum.ensure(size_t(c->npt)));
   HIPCHK(c->gtmp.ensure(size_t(c->npt)));
   HIPCHK(hipMemcpyAsync(c->wtab.p, w.data(), w.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->sumx.p, sx.data(), sx.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->sumz.p, sz.data(), sz.size() * 4, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(launch_gsum(c->sumx.p, c->sumz.p, c->gsum.p, nx, nz, c->stream));
   HIPCHK(launch_gauss_x(c->wtab.p, nw, k, c->gtmp.p, nx, nz, c->stream));
-  HIPCHK(launch_gauss_z(c->wtab.p, nw, c->gtmp.p, c->gsum.p, k, nx, nz, c->stream));
+  HIPCHK(launch_gauss_z(c->wtab.p, nw, c->gtmp.p, c->sumx.p, c->sumz.p, k, nx, nz, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

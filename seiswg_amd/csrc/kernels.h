@@ -120,11 +120,10 @@ int step_blocks_per_cu(int kind, int nt_threads, int group);
 
 hipError_t launch_imaging_sh(const float* vadj, const float* vfw, float* kmu, int nx, int nz,
                              float h8x, float h8z, float ndt, hipStream_t s);
-hipError_t launch_gsum(const float* sumx, const float* sumz, float* gsum, int nx, int nz, hipStream_t s);
 // separable Gaussian sums; w[d] == 0 for d >= nw
 hipError_t launch_gauss_x(const float* w, int nw, const float* k, float* gtmp, int nx, int nz, hipStream_t s);
-hipError_t launch_gauss_z(const float* w, int nw, const float* gtmp, const float* gsum, float* k, int nx, int nz,
-                          hipStream_t s);
+hipError_t launch_gauss_z(const float* w, int nw, const float* gtmp, const float* sumx, const float* sumz, float* k,
+                          int nx, int nz, hipStream_t s);
 hipError_t launch_misfit(const float* syn, const float* ref, const float* taper, float* adstf,
                          double* partial, int nrec, int nt, hipStream_t s, int* nblocks_out);
 hipError_t launch_reduce_sum(const double* partial, int n, double* out, hipStream_t s);

@@ -2066,13 +2066,6 @@ __global__ void k_imaging_sh(const float* __restrict__ va, const float* __restri
 // (which can never become -0 in round-to-nearest) leaves it unchanged, so
 // summing over |i - n| < nw only, in the same order, gives the same bits
 // for finite fields at O(npt * nw) instead of O(npt * (nx + nz)).
-__global__ void k_gsum(const float* __restrict__ sumx, const float* __restrict__ sumz, float* __restrict__ gsum,
-                       int nx, int nz) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (j >= nz) return;
-  gsum[i * nz + j] = sumx[i] * sumz[j];
-}
 __global__ void k_gauss_x(const float* __restrict__ w, int nw, const float* __restrict__ kin,
                           float* __restrict__ gtmp, int nx, int nz) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2083,8 +2076,10 @@ __global__ void k_gauss_x(const float* __restrict__ w, int nw, const float* __re
   for (int n = lo; n < hi; ++n) s = s + w[abs(i - n)] * kin[int64_t(n) * nz + j];
   gtmp[int64_t(i) * nz + j] = s;
 }
+// GSUM (init_gsum) is the product of the 1-D weight sums, formed per cell
 __global__ void k_gauss_z(const float* __restrict__ w, int nw, const float* __restrict__ gtmp,
-                          const float* __restrict__ gsum, float* __restrict__ kout, int nx, int nz) {
+                          const float* __restrict__ sumx, const float* __restrict__ sumz, float* __restrict__ kout,
+                          int nx, int nz) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= nz) return;
@@ -2092,7 +2087,7 @@ __global__ void k_gauss_z(const float* __restrict__ w, int nw, const float* __re
   const float* row = gtmp + int64_t(i) * nz;
   const int lo = max(j - nw + 1, 0), hi = min(j + nw, nz);
   for (int n = lo; n < hi; ++n) s = s + w[abs(j - n)] * row[n];
-  const float g = gsum[int64_t(i) * nz + j];
+  const float g = sumx[i] * sumz[j];
   kout[int64_t(i) * nz + j] = (g > 1e-30f) ? s / g : 0.0f;
 }
 
@@ -2396,17 +2391,13 @@ hipError_t launch_imaging_sh(const float* va, const float* vf, float* kmu, int n
                      r8z, ndt, fd);
   return hipGetLastError();
 }
-hipError_t launch_gsum(const float* sumx, const float* sumz, float* gsum, int nx, int nz, hipStream_t s) {
-  hipLaunchKernelGGL(k_gsum, dim3((nz + 255) / 256, nx), dim3(256), 0, s, sumx, sumz, gsum, nx, nz);
-  return hipGetLastError();
-}
 hipError_t launch_gauss_x(const float* w, int nw, const float* k, float* gtmp, int nx, int nz, hipStream_t s) {
   hipLaunchKernelGGL(k_gauss_x, dim3((nz + 255) / 256, nx), dim3(256), 0, s, w, nw, k, gtmp, nx, nz);
   return hipGetLastError();
 }
-hipError_t launch_gauss_z(const float* w, int nw, const float* gtmp, const float* gsum, float* k, int nx, int nz,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(k_gauss_z, dim3((nz + 255) / 256, nx), dim3(256), 0, s, w, nw, gtmp, gsum, k, nx, nz);
+hipError_t launch_gauss_z(const float* w, int nw, const float* gtmp, const float* sumx, const float* sumz, float* k,
+                          int nx, int nz, hipStream_t s) {
+  hipLaunchKernelGGL(k_gauss_z, dim3((nz + 255) / 256, nx), dim3(256), 0, s, w, nw, gtmp, sumx, sumz, k, nx, nz);
   return hipGetLastError();
 }
 hipError_t launch_misfit(const float* syn, const float* ref, const float* taper, float* adstf, double* partial,

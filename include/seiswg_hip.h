@@ -209,6 +209,17 @@ SW_API int sw_multi_timing(const sw_multi* m, double* ms, int64_t* launches);
 /* Gaussian smoothing of a kernel field in place (smooth_gradient,
  * solver.rs:1249-1274; init_gsum, gaussian_x, gaussian_z). */
 SW_API int sw_smooth(sw_ctx* ctx, int32_t field, float sigma);
+/* sw_smooth without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  The weight table, the 1-D weight sums
+ * and the x-pass result live in one fixed device array of the library per
+ * GPU (21.2 MB, held until the process ends): grids of at most 5,242,880
+ * cells and 16384 points per axis; larger grids are rejected (-2).  The
+ * kernel field must exist on the context (after a gradient, an adjoint run
+ * or sw_write_field).  Every call on a GPU uses that GPU's array, so calls
+ * that overlap on one GPU (other threads or contexts) race; the array holds
+ * nothing a call must find cleared.  Results equal sw_smooth's bit for bit. */
+SW_API int sw_smooth_noalloc(sw_ctx* ctx, int32_t field, float sigma);
 
 /* Tile shape of the fused step kernels (no counterpart in the reference,
  * which dispatches fixed 64-wide 1-D workgroups): threads per shot (64, 128

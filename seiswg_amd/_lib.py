@@ -51,7 +51,7 @@ EXPORTS = ["sw_last_error", "sw_create", "sw_destroy", "sw_set_model", "sw_set_s
            "sw_set_steps_per_launch", "sw_set_tiling2", "sw_get_tiling2", "sw_release_cached_memory",
            "sw_bench_copy", "sw_gradient", "sw_autotune", "sw_multi_create", "sw_multi_destroy",
            "sw_multi_forward", "sw_multi_gradient", "sw_multi_timing", "sw_multi_set_merge_limit",
-           "sw_multi_set_workers", "sw_set_finite_check", "sw_get_graph_state",
+           "sw_multi_set_workers", "sw_set_finite_check", "sw_get_graph_state", "sw_smooth_noalloc",
            "seiswg_run", "seiswg_ricker_table", "seiswg_infer_grid", "seiswg_grid_index",
            "seiswg_vfs_new", "seiswg_vfs_free", "seiswg_vfs_put", "seiswg_vfs_get", "seiswg_vfs_count",
            "seiswg_vfs_key", "seiswg_run_simulation"]
@@ -83,6 +83,7 @@ def load():
     L.sw_advance.argtypes = [vp, C.c_int32]
     L.sw_adjoint_plain.argtypes = [vp, C.c_int32, ip, fp, C.c_int32, C.c_float, dp]
     L.sw_smooth.argtypes = [vp, C.c_int32, C.c_float]
+    L.sw_smooth_noalloc.argtypes = [vp, C.c_int32, C.c_float]
     L.sw_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
     L.sw_set_steps_per_launch.argtypes = [vp, C.c_int32]
     L.sw_set_tiling2.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
@@ -391,6 +392,10 @@ class Context:
 
     def smooth(self, fid: int, sigma: float):
         _check(self.L.sw_smooth(self.h, fid, C.c_float(sigma)))
+
+    def smooth_noalloc(self, fid: int, sigma: float):
+        """sw_smooth through the library's fixed device scratch (no allocation)."""
+        _check(self.L.sw_smooth_noalloc(self.h, fid, C.c_float(sigma)))
 
     def timing(self) -> sw_timing:
         t = sw_timing()

@@ -1482,6 +1482,77 @@ SW_API int sw_get_graph_state(const sw_ctx* c, int32_t* state, int32_t* captured
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Device scratch of sw_smooth_noalloc: the weight table, the two 1-D weight
+// sums and the x-pass result, for grids of at most kNoallocCells cells with
+// nx, nz <= kNoallocAxis (5,292,032 floats, 21.2 MB per GPU, held until the
+// process ends).  Host code never passes `scratch` itself to a kernel or a
+// copy: its name is a host-side stand-in, and each GPU's copy has its own
+// address, looked up once per device with that device current.
+constexpr int kNoallocAxis = 16384;
+constexpr int64_t kNoallocCells = int64_t(5) << 20;
+constexpr int64_t kNoallocFloats = 3 * int64_t(kNoallocAxis) + kNoallocCells;
+static __device__ __attribute__((used)) __align__(256) float scratch[kNoallocFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+// host tables of the call in progress (static storage: no allocation); the
+// mutex also serialises the host side of concurrent calls
+float g_noalloc_tab[3 * kNoallocAxis];
+
+// the address of `scratch` on `device` (current): looked up on the first call
+int noalloc_scratch(int device, float** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(-2, "device %d beyond the %d the library tracks", device, kMaxDevices);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(-3, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+SW_API int sw_smooth_noalloc(sw_ctx* c, int32_t field, float sigma) {
+  if (!c) return fail(-1, "null ctx");
+  if (field != SW_F_K_MU && field != SW_F_K_LAM && field != SW_F_K_RHO) return fail(-2, "only kernel fields smooth");
+  const int nx = c->g.nx, nz = c->g.nz;
+  if (nx > kNoallocAxis || nz > kNoallocAxis || int64_t(nx) * nz > kNoallocCells)
+    return fail(-2, "sw_smooth_noalloc: grid %d x %d beyond its fixed scratch (at most %lld cells, %d per axis); "
+                    "use sw_smooth", nx, nz, (long long)kNoallocCells, kNoallocAxis);
+  float* k = model_ptr(c, field);
+  if (!k) return fail(-2, "kernel field %d not computed", field);
+  HIPCHK(hipSetDevice(c->device));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  float* scr = nullptr;
+  int rc = noalloc_scratch(c->device, &scr);
+  if (rc) return rc;
+  float* d_w = scr;
+  float* d_sx = scr + kNoallocAxis;
+  float* d_sz = scr + 2 * kNoallocAxis;
+  float* d_gtmp = scr + 3 * kNoallocAxis;
+  float* h_w = g_noalloc_tab;
+  float* h_sx = g_noalloc_tab + kNoallocAxis;
+  float* h_sz = g_noalloc_tab + 2 * kNoallocAxis;
+  const int nmax = std::max(nx, nz);
+  const int nw = smooth_tables(nx, nz, sigma, h_w, h_sx, h_sz);
+  HIPCHK(hipMemcpyAsync(d_w, h_w, size_t(nmax) * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_sx, h_sx, size_t(nx) * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_sz, h_sz, size_t(nz) * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_gauss_x(d_w, nw, k, d_gtmp, nx, nz, c->stream));
+  HIPCHK(launch_gauss_z(d_w, nw, d_gtmp, d_sx, d_sz, k, nx, nz, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));   // the host tables are reused by the next call
+  return 0;
+}
+
 SW_API int sw_smooth(sw_ctx* c, int32_t field, float sigma) {
   if (!c) return fail(-1, "null ctx");
   HIPCHK(hipSetDevice(c->device));

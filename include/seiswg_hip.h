@@ -140,6 +140,17 @@ SW_API int sw_prepare(sw_ctx* ctx, int32_t nshots, const int32_t* isrc);
  * check: read the fields to inspect them). */
 SW_API int sw_advance(sw_ctx* ctx, int32_t nsteps);
 
+/* Divisions of the forward step kernels (no counterpart in the reference,
+ * whose WGSL divisions carry the shader language's 2.5-unit error bound):
+ * SW_DIV_EXACT (the default) rounds every quotient as IEEE division does,
+ * so the steps equal the FP32 restatement of the reference kernels bit for
+ * bit; SW_DIV_RECIPROCAL multiplies by the correctly rounded reciprocal
+ * (within 1.5 units), fewer instructions per cell.  The gradient's adjoint
+ * steps always use the reciprocal-and-correction form. */
+#define SW_DIV_EXACT 0
+#define SW_DIV_RECIPROCAL 1
+SW_API int sw_set_division(sw_ctx* ctx, int32_t mode);
+
 /* Blow-up detection (no counterpart in the reference): on by default.
  * Forward runs scan the state at snapshot steps and after the last step;
  * gradients (sw_gradient, sw_multi_gradient) check their misfits after the

@@ -23,6 +23,7 @@ F_LAM, F_MU, F_NU, F_J, F_LAM_C, F_MU_C, F_NU_C, F_RHO, F_BOUND = 30, 31, 32, 33
 F_K_LAM, F_K_MU, F_K_RHO = 39, 40, 41
 OBS_X, OBS_Y, OBS_Z, OBS_YI, OBS_YC = range(5)
 MODEL_VPVS, MODEL_LAME = 0, 1
+DIV_EXACT, DIV_RECIPROCAL = 0, 1
 
 
 class SeiswgError(RuntimeError):
@@ -52,6 +53,7 @@ EXPORTS = ["sw_last_error", "sw_create", "sw_destroy", "sw_set_model", "sw_set_s
            "sw_bench_copy", "sw_gradient", "sw_autotune", "sw_multi_create", "sw_multi_destroy",
            "sw_multi_forward", "sw_multi_gradient", "sw_multi_timing", "sw_multi_set_merge_limit",
            "sw_multi_set_workers", "sw_set_finite_check", "sw_get_graph_state", "sw_smooth_noalloc",
+           "sw_set_division",
            "seiswg_run", "seiswg_ricker_table", "seiswg_infer_grid", "seiswg_grid_index",
            "seiswg_vfs_new", "seiswg_vfs_free", "seiswg_vfs_put", "seiswg_vfs_get", "seiswg_vfs_count",
            "seiswg_vfs_key", "seiswg_run_simulation"]
@@ -102,6 +104,7 @@ def load():
     L.sw_gradient.argtypes = [vp, C.c_int32, ip, fp, C.c_int32, C.c_int32, dp]
     L.sw_bench_copy.argtypes = [C.c_int32, C.c_int64, C.c_int32, dp]
     L.sw_set_finite_check.argtypes = [vp, C.c_int32]
+    L.sw_set_division.argtypes = [vp, C.c_int32]
     L.sw_get_graph_state.argtypes = [vp, ip, ip, C.c_char_p, C.c_int32]
     L.seiswg_run.argtypes = [C.c_char_p]
     L.seiswg_ricker_table.argtypes = [C.c_int32, C.c_int32, C.c_float, fp, fp]
@@ -401,6 +404,11 @@ class Context:
         t = sw_timing()
         _check(self.L.sw_get_timing(self.h, C.byref(t)))
         return t
+
+    def set_division(self, reciprocal: bool):
+        """Forward-step divisions: exact (default; bit-identical to the FP32
+        restatement) or by multiplication with the rounded reciprocal."""
+        _check(self.L.sw_set_division(self.h, int(bool(reciprocal))))
 
     def set_finite_check(self, on: bool):
         """Blow-up detection (on by default): forward runs, gradients and the

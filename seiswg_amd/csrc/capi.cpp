@@ -175,6 +175,7 @@ struct sw_ctx {
   bool model_set = false;
   // every rho / J in [2^-20, 2^20]: divisions by them may use the reciprocal form
   bool rho_fast = false, j_fast = false;
+  bool recip_div = false;     // SW_DIV_RECIPROCAL: forward steps multiply by 1/y
 
   // model fields [MF_N][npt]: rho, lambda, mu, nu, j, lambda_c, mu_c, nu_c
   DBuf<float> model;
@@ -604,7 +605,7 @@ StepArgs make_args(sw_ctx* c, StepMode mode) {
   a.dt = c->g.dt; a.h8x = c->h8x; a.h8z = c->h8z;
   a.r8x = 1.0f / c->h8x; a.r8z = 1.0f / c->h8z;
   a.fast_div = ((fast_divisor(c->h8x) && fast_divisor(c->h8z)) ? FD_CONST : 0) |
-               (c->rho_fast ? FD_RHO : 0) | (c->j_fast ? FD_J : 0);
+               (c->rho_fast ? FD_RHO : 0) | (c->j_fast ? FD_J : 0) | (c->recip_div ? FD_RCP : 0);
   a.inj_off = c->inj_off.p; a.inj = c->inj.p;
   a.val_x = c->val_x; a.val_y = c->val_y; a.val_z = c->val_z; a.nt_val = c->nt_val;
   a.rec_off = c->rec_off.p; a.rec = c->rec.p;
@@ -1465,6 +1466,12 @@ SW_API int sw_bench_copy(int32_t device, int64_t nbytes, int32_t iters, double* 
 SW_API int sw_get_timing(const sw_ctx* c, sw_timing* out) {
   if (!c || !out) return fail(-1, "null argument");
   *out = c->timing;
+  return 0;
+}
+
+SW_API int sw_set_division(sw_ctx* c, int32_t mode) {
+  if (!c || (mode != SW_DIV_EXACT && mode != SW_DIV_RECIPROCAL)) return fail(-1, "division mode must be 0 or 1");
+  c->recip_div = mode == SW_DIV_RECIPROCAL;
   return 0;
 }
 

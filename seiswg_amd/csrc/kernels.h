@@ -38,7 +38,9 @@ enum ModelField : int { MF_RHO = 0, MF_LAM, MF_MU, MF_NU, MF_J, MF_LAM_C, MF_MU_
 // Exact divisions by 8h (FD_CONST), rho (FD_RHO) and J (FD_J) may use the
 // reciprocal-and-correction form when every such divisor lies in
 // [2^-20, 2^20]; otherwise the kernels divide with the IEEE sequence.
-enum FastDiv : int { FD_CONST = 1, FD_RHO = 2, FD_J = 4 };
+// FD_RCP: the forward step kernels multiply by the correctly rounded
+// reciprocal instead (x * RN(1/y), within 1.5 units of x / y).
+enum FastDiv : int { FD_CONST = 1, FD_RHO = 2, FD_J = 4, FD_RCP = 8 };
 inline bool fast_divisor(float y) { return y >= 0x1p-20f && y <= 0x1p20f; }
 
 struct StepArgs {

@@ -72,8 +72,19 @@ __device__ __forceinline__ float keep(bool active, float v) { return active ? v 
 // min(k - 1) >= bits(2^-100) << 1 - 1 (zero wraps to the maximum) and
 // max(k) <= bits(2^100) << 1, accumulated in registers, so a group costs one
 // branch instead of one per division.
+// mode (from div_mode): 1 reciprocal-and-correction with the range test,
+// 0 IEEE division, 2 x * RN(1/y) (FD_RCP)
+__device__ __forceinline__ int div_mode(int bits, int which) {
+  return (bits & FD_RCP) ? 2 : ((bits & which) == which ? 1 : 0);
+}
 template <int N>
-__device__ __forceinline__ void div_group(float (&q)[N], const float (&x)[N], const Div (&d)[N], bool fast) {
+__device__ __forceinline__ void div_group(float (&q)[N], const float (&x)[N], const Div (&d)[N], int mode) {
+  if (mode == 2) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) q[k] = x[k] * d[k].r;
+    return;
+  }
+  const bool fast = mode == 1;
   constexpr uint32_t kLo = (127u - 100u) << 24, kHi = (127u + 100u) << 24;
   uint32_t lo = 0xffffffffu, hi = 0u;
 #pragma unroll
@@ -230,8 +241,8 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
-  const bool fast_c = (a.fast_div & FD_CONST) != 0;
-  const bool fast_rho = (a.fast_div & FD_RHO) != 0;
+  const int fast_c = div_mode(a.fast_div, FD_CONST);
+  const int fast_rho = div_mode(a.fast_div, FD_RHO);
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
             tp2 = g.base + min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;
@@ -451,8 +462,8 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
-  const bool fast_c = (a.fast_div & FD_CONST) != 0;
-  const bool fast_rho = (a.fast_div & FD_RHO) != 0, fast_j = (a.fast_div & FD_J) != 0;
+  const int fast_c = div_mode(a.fast_div, FD_CONST);
+  const int fast_rho = div_mode(a.fast_div, FD_RHO), fast_rj = div_mode(a.fast_div, FD_RHO | FD_J);
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
             tp2 = g.base + min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;
@@ -608,7 +619,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     float qb[NB];
     if constexpr (SPIN) {
       const float xb[3] = {dt * dsx, dt * dsz, dt * dsyc};
-      div_group(qb, xb, {Div{rho, irho, true}, Div{rho, irho, true}, Div{jj, ij, true}}, fast_rho && fast_j);
+      div_group(qb, xb, {Div{rho, irho, true}, Div{rho, irho, true}, Div{jj, ij, true}}, fast_rj);
     } else {
       const float xb[2] = {dt * dsx, dt * dsz};
       div_group(qb, xb, {Div{rho, irho, true}, Div{rho, irho, true}}, fast_rho);
@@ -838,8 +849,8 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
-  const bool fast_c = (a.fast_div & FD_CONST) != 0;
-  const bool fast_rho = (a.fast_div & FD_RHO) != 0;
+  const int fast_c = div_mode(a.fast_div, FD_CONST);
+  const int fast_rho = div_mode(a.fast_div, FD_RHO);
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
             tp2 = g.base + min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;
@@ -1199,8 +1210,8 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
   const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
-  const bool fast_c = (a.fast_div & FD_CONST) != 0;
-  const bool fast_rho = (a.fast_div & FD_RHO) != 0;
+  const int fast_c = div_mode(a.fast_div, FD_CONST);
+  const int fast_rho = div_mode(a.fast_div, FD_RHO);
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
             tp2 = g.base + min(t + 2, NT - 1);
   const int tps = a.nstrips * a.nsegs;

@@ -339,12 +339,21 @@ def load_example(tmp_path, ex, ini_name="config.ini"):
     return d, p, ini
 
 
+@pytest.mark.parametrize("reciprocal", [False, True], ids=["exact", "reciprocal"])
 @pytest.mark.parametrize("ex,comp,fid,snap_file,trace_file",
                          [("forward", O.OBS_Y, 0, "proc000100_vy.bin", "uy_000000.npy"),
                           ("spin", O.OBS_Y, 7, "proc000200_vx.bin", "ry_000000.npy")])
-def test_examples_match_fixtures_and_oracle(seiswg, tmp_path, ex, comp, fid, snap_file, trace_file):
+def test_examples_match_fixtures_and_oracle(seiswg, tmp_path, ex, comp, fid, snap_file, trace_file, reciprocal):
+    """Exact divisions: the oracle bit for bit (compare's 1-ulp-level bar) and
+    the reference fixtures within their tolerance.  Divisions by the rounded
+    reciprocal (sw_set_division): the same fixture tolerance, and rel-L2 <=
+    5e-6 against the oracle over the whole run (1200 / 2000 steps; measured
+    1.1e-6 on the forward example's traces, where the exact path itself is
+    1.2e-6 from the reference's own outputs)."""
     d, p, ini = load_example(tmp_path, ex)
     c = seiswg.Context.from_problem(p)
+    if reciprocal:
+        c.set_division(True)
     stf = seiswg.ricker_table([[r[3], r[4], r[5], r[6]] for r in p.src_rows], p.nt, p.dt)
     c.set_sources(p.src_idx, stf)
     c.set_receivers(p.rec_idx)
@@ -361,8 +370,11 @@ def test_examples_match_fixtures_and_oracle(seiswg, tmp_path, ex, comp, fid, sna
     o = O.Oracle(p, 4)
     oobs, osnaps = o.forward(-1, ini["save_snapshot"])
     q = {0: 0, 7: 1}[fid]
-    compare(tr, oobs[comp], "trace vs oracle")
-    compare(snap, osnaps[step // ini["save_snapshot"] - 1][q], "snapshot vs oracle")
+    e_tr, e_sn = O.rel_l2(tr, oobs[comp]), O.rel_l2(snap, osnaps[step // ini["save_snapshot"] - 1][q])
+    print(f"{ex} ({'reciprocal' if reciprocal else 'exact'} division): vs oracle trace {e_tr:.2e} snapshot {e_sn:.2e}")
+    tol = 5e-6 if reciprocal else 1e-6
+    compare(tr, oobs[comp], "trace vs oracle", tol)
+    compare(snap, osnaps[step // ini["save_snapshot"] - 1][q], "snapshot vs oracle", tol)
 
 
 def run_cli(ini):

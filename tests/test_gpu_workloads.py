@@ -88,6 +88,27 @@ def test_batched_psv_2000x1000_64_shots_headline_tiles_match_oracle_bitwise(seis
         print(f"shot {s}: 5 state fields and 2 trace components bit-identical", flush=True)
 
 
+def test_reciprocal_division_2000x1000_64_shots_within_tolerance(seiswg):
+    """The headline workload over 40 steps from rest (the sources firing)
+    with forward divisions by the rounded reciprocal against exact
+    divisions: traces and state of shots 0, 31 and 63 within rel-L2 1e-6."""
+    from seiswg_amd import synth
+    cfg = synth.PSV_BATCH
+    nx, nz, h, dt, nshots, nt = cfg["nx"], cfg["nz"], cfg["h"], cfg["dt"], cfg["nshots"], 40
+    model, geom = synth.psv_batch_problem()
+    out = []
+    for rcp in (False, True):
+        c, _, _ = _ctx(seiswg, nx, nz, nt, h, dt, model, geom)
+        c.set_division(rcp)
+        c.forward_batch(list(range(nshots)))
+        out.append([c.obs(s, k) for s in (0, 31, 63) for k in (seiswg.abi.OBS_X, seiswg.abi.OBS_Z)] +
+                   [c.field(getattr(seiswg.abi, "F_" + n), shot=s) for s in (0, 31, 63) for n in PSV_STATE])
+        c.close()
+    worst = max(O.rel_l2(b, a) for a, b in zip(*out))
+    print(f"reciprocal vs exact division, 40 steps: worst rel-L2 {worst:.2e}", flush=True)
+    assert worst <= 1e-6
+
+
 def test_gradient_1000x600_16_shots_default_tiles_matches_fp64_oracle(seiswg):
     from seiswg_amd import synth
     cfg = synth.PSV_GRADIENT

@@ -72,18 +72,20 @@ struct StepArgs {
   int advance;                // forward: last block sets clock[0] = it+1; adjoint: it-1
   int it_lo, it_hi;           // the launch is a no-op unless it_lo <= it < it_hi
   int nblocks;                // workgroups of this problem in the launch
-  // gradient support: per-step auxiliary fields (forward recompute writes,
-  // adjoint reads) laid out [step % aux_period][shot][naux][npt], and the
-  // kernel accumulators (rho, mu, lambda)
+  // gradient support: the forward velocities of the recomputed segment
+  // (forward recompute writes v^{n+1} after step n, the adjoint reads v^{n+1}
+  // and v^n) laid out [n % aux_period][shot][naux][npt], with aux_period
+  // one more than the checkpoint interval, and the kernel accumulators (rho,
+  // mu, lambda)
   float* aux;
   int aux_period;
   float* kgrad;               // kernel accumulators [shot / S][3][npt]
   int stream_out;             // 1: state and aux stores non-temporal (working set far beyond the caches)
 };
 
-// auxiliary fields kept per recomputed forward step
-enum AuxSh : int { A_DSY = 0, A_EXY, A_EZY, NAUX_SH };
-enum AuxPsv : int { A_DSX = 0, A_DSZ, A_EXX, A_EZZ, A_EXZ, NAUX_PSV };
+// velocity fields kept per recomputed forward step
+enum AuxSh : int { A_VY = 0, NAUX_SH };
+enum AuxPsv : int { A_VX = 0, A_VZ, NAUX_PSV };
 enum KField : int { K_RHO = 0, K_MU, K_LAM, NK };
 
 // launches (kernels.hip)

@@ -257,8 +257,11 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   int rec_row = next_row(a.rec, rp, re);     // row of the next receiver entry
   float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
   float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
+  // forward recompute of the gradient: the new velocity v^{it+1} into its
+  // time slot (it + 1) % aux_period
   const auto raux =
-      make_rsrc(AUX ? a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt) : a.model,
+      make_rsrc(AUX ? a.aux + (int64_t((it + 1) % a.aux_period) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt)
+                    : a.model,
                 AUX ? uint32_t(NAUX_SH) * npt4 : 0u);
 
   // rings: stress row r in slot (r - (i0-3)) & 3; new-velocity row r in slot
@@ -336,7 +339,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     const float vyn = b * (vy_o + qb[0]);
     if (g.jst && m >= i0 && m < i1) {
       bstore_out(vyn, rout, jc4, o_vy + uint32_t(m) * g.rowb, a.stream_out);
-      if constexpr (AUX) bstore_out(dsy, raux, jc4, A_DSY * npt4 + uint32_t(m) * g.rowb, a.stream_out);
+      if constexpr (AUX) bstore_out(vyn, raux, jc4, A_VY * npt4 + uint32_t(m) * g.rowb, a.stream_out);
     }
     vy[s3] = vyn;              // new-velocity ring: rows m-3..m in slots s0..s3
     lds_v[par][lane] = vyn;
@@ -377,10 +380,6 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
       if (g.jst) {
         bstore_out(nsxy, rout, jc4, o_sxy + uint32_t(i) * g.rowb, a.stream_out);
         bstore_out(nszy, rout, jc4, o_szy + uint32_t(i) * g.rowb, a.stream_out);
-        if constexpr (AUX) {
-          bstore_out(dvydx, raux, jc4, A_EXY * npt4 + uint32_t(i) * g.rowb, a.stream_out);
-          bstore_out(dvydz[s0], raux, jc4, A_EZY * npt4 + uint32_t(i) * g.rowb, a.stream_out);
-        }
       }
     }
   };
@@ -479,8 +478,11 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
   float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const int nobs = a.nrec * a.nt;
+  // forward recompute of the gradient: the new velocities v^{it+1} into their
+  // time slot (it + 1) % aux_period
   const auto raux =
-      make_rsrc(AUX ? a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt) : a.model,
+      make_rsrc(AUX ? a.aux + (int64_t((it + 1) % a.aux_period) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt)
+                    : a.model,
                 AUX ? uint32_t(NAUX_PSV) * npt4 : 0u);
 
   // rings as in the SH kernel
@@ -634,8 +636,8 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
       bstore_out(vzn, rout, jc4, o_vz + ro, a.stream_out);
       if constexpr (SPIN) bstore_out(vycn, rout, jc4, o_vyc + ro, a.stream_out);
       if constexpr (AUX) {
-        bstore_out(dsx, raux, jc4, A_DSX * npt4 + ro, a.stream_out);
-        bstore_out(dsz, raux, jc4, A_DSZ * npt4 + ro, a.stream_out);
+        bstore_out(vxn, raux, jc4, A_VX * npt4 + ro, a.stream_out);
+        bstore_out(vzn, raux, jc4, A_VZ * npt4 + ro, a.stream_out);
       }
     }
     vx[s3] = vxn;
@@ -746,11 +748,6 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
         bstore_out(nsxx, rout, jc4, o_sxx + ro, a.stream_out);
         bstore_out(nszz, rout, jc4, o_szz + ro, a.stream_out);
         bstore_out(nsxz, rout, jc4, o_sxz + ro, a.stream_out);
-        if constexpr (AUX) {
-          bstore_out(dvxdx, raux, jc4, A_EXX * npt4 + ro, a.stream_out);
-          bstore_out(dvzdz[s0], raux, jc4, A_EZZ * npt4 + ro, a.stream_out);
-          bstore_out(dvxdz[s0] + dvzdx, raux, jc4, A_EXZ * npt4 + ro, a.stream_out);
-        }
       }
       if constexpr (SPIN) {
         const float dvycdx = qc[5];
@@ -872,13 +869,14 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const int nobs = a.nrec * a.nt;
   const int m0 = i0 - 4;   // first velocity row of stage 1
-  // forward recompute of the gradient: the aux fields of steps it and it+1
+  // forward recompute of the gradient: the velocities v^{it+1} (stage 1) and
+  // v^{it+2} (stage 2) into their time slots (step % aux_period)
   auto aux_rsrc = [&](int step) {
     return make_rsrc(
         AUX ? a.aux + (int64_t(step % a.aux_period) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt) : a.model,
         AUX ? uint32_t(NAUX_PSV) * npt4 : 0u);
   };
-  const auto raux1 = aux_rsrc(it), raux2 = aux_rsrc(it + 1);
+  const auto raux1 = aux_rsrc(it + 1), raux2 = aux_rsrc(it + 2);
 
   // stage 1 rings: stresses of step it (rows m-2..m+1), new velocities
   // (rows m-3..m), D+z of the new velocities (row m at slot s2, used at s0)
@@ -984,8 +982,8 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       if constexpr (AUX) {
         if (g.jst && m >= i0 && m < i1) {
           const uint32_t ro = uint32_t(m) * g.rowb;
-          bstore_out(dsx, raux1, jc4, A_DSX * npt4 + ro, a.stream_out);
-          bstore_out(dsz, raux1, jc4, A_DSZ * npt4 + ro, a.stream_out);
+          bstore_out(vx[s3], raux1, jc4, A_VX * npt4 + ro, a.stream_out);
+          bstore_out(vz[s3], raux1, jc4, A_VZ * npt4 + ro, a.stream_out);
         }
       }
     }
@@ -1033,15 +1031,6 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       t1xx[s0] = sxx[s0] + dt * (l2m * dvxdx + lam * dvzdz[s0]);
       t1zz[s0] = szz[s0] + dt * (l2m * dvzdz[s0] + lam * dvxdx);
       t1xz[s0] = sxz[s0] + dt * mu * (dvxdz[s0] + dvzdx);
-      if constexpr (AUX) {
-        const int i = m - 2;
-        if (g.jst && i >= i0 && i < i1) {
-          const uint32_t ro = uint32_t(i) * g.rowb;
-          bstore_out(dvxdx, raux1, jc4, A_EXX * npt4 + ro, a.stream_out);
-          bstore_out(dvzdz[s0], raux1, jc4, A_EZZ * npt4 + ro, a.stream_out);
-          bstore_out(dvxdz[s0] + dvzdx, raux1, jc4, A_EXZ * npt4 + ro, a.stream_out);
-        }
-      }
       lds_s2[par][0][lane] = t1xz[s0];
       lds_s2[par][1][lane] = t1zz[s0];
     }
@@ -1088,8 +1077,8 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
         bstore_out(vxn, rout, jc4, o_vx + ro, a.stream_out);
         bstore_out(vzn, rout, jc4, o_vz + ro, a.stream_out);
         if constexpr (AUX) {
-          bstore_out(dsx, raux2, jc4, A_DSX * npt4 + ro, a.stream_out);
-          bstore_out(dsz, raux2, jc4, A_DSZ * npt4 + ro, a.stream_out);
+          bstore_out(vxn, raux2, jc4, A_VX * npt4 + ro, a.stream_out);
+          bstore_out(vzn, raux2, jc4, A_VZ * npt4 + ro, a.stream_out);
         }
       }
       u2x[s0] = vxn;
@@ -1140,11 +1129,6 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
         bstore_out(nsxx, rout, jc4, o_sxx + ro, a.stream_out);
         bstore_out(nszz, rout, jc4, o_szz + ro, a.stream_out);
         bstore_out(nsxz, rout, jc4, o_sxz + ro, a.stream_out);
-        if constexpr (AUX) {
-          bstore_out(dvxdx, raux2, jc4, A_EXX * npt4 + ro, a.stream_out);
-          bstore_out(dz2z[par ^ 1], raux2, jc4, A_EZZ * npt4 + ro, a.stream_out);
-          bstore_out(dz2x[par ^ 1] + dvzdx, raux2, jc4, A_EXZ * npt4 + ro, a.stream_out);
-        }
       }
     }
     lds_v2[par][0][lane] = u2x[s0];
@@ -1231,12 +1215,13 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
   float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
   float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const int m0 = i0 - 4;   // first velocity row of stage 1
+  // forward recompute of the gradient: v^{it+1} and v^{it+2} into their time slots
   auto aux_rsrc = [&](int step) {
     return make_rsrc(
         AUX ? a.aux + (int64_t(step % a.aux_period) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt) : a.model,
         AUX ? uint32_t(NAUX_SH) * npt4 : 0u);
   };
-  const auto raux1 = aux_rsrc(it), raux2 = aux_rsrc(it + 1);
+  const auto raux1 = aux_rsrc(it + 1), raux2 = aux_rsrc(it + 2);
 
   // stage 1 rings: stresses of step it (rows m-2..m+1), new vy (rows m-3..m),
   // D+z of the new vy (row m at s2, used at s0); stage 2 rings: stresses of
@@ -1315,7 +1300,7 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
       }
       vy[s3] = b * (vy_o + qb[0]);
       if constexpr (AUX) {
-        if (g.jst && m >= i0 && m < i1) bstore_out(dsy, raux1, jc4, A_DSY * npt4 + uint32_t(m) * g.rowb, a.stream_out);
+        if (g.jst && m >= i0 && m < i1) bstore_out(vy[s3], raux1, jc4, A_VY * npt4 + uint32_t(m) * g.rowb, a.stream_out);
       }
     }
     lds_v[par][lane] = vy[s3];
@@ -1352,12 +1337,6 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
       const float dvydx = qc[1];
       t1xy[s0] = sxy[s0] + mu_dt * dvydx;
       t1zy[s0] = szy[s0] + mu_dt * dvydz[s0];
-      if constexpr (AUX) {
-        if (g.jst && i >= i0 && i < i1) {
-          bstore_out(dvydx, raux1, jc4, A_EXY * npt4 + uint32_t(i) * g.rowb, a.stream_out);
-          bstore_out(dvydz[s0], raux1, jc4, A_EZY * npt4 + uint32_t(i) * g.rowb, a.stream_out);
-        }
-      }
       lds_s2[par][lane] = t1zy[s0];
     }
     drho[s0] = rho;
@@ -1394,7 +1373,7 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
       const float vyn = b * (vy[s0] + qb[0]);
       if (g.jst && m2 >= i0 && m2 < i1) {
         bstore_out(vyn, rout, jc4, o_vy + uint32_t(m2) * g.rowb, a.stream_out);
-        if constexpr (AUX) bstore_out(dsy, raux2, jc4, A_DSY * npt4 + uint32_t(m2) * g.rowb, a.stream_out);
+        if constexpr (AUX) bstore_out(vyn, raux2, jc4, A_VY * npt4 + uint32_t(m2) * g.rowb, a.stream_out);
       }
       u2[s0] = vyn;
       if (m2 == rec_row2) {
@@ -1432,10 +1411,6 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
       if (g.jst && i2 >= i0 && i2 < i1) {
         bstore_out(nsxy, rout, jc4, o_sxy + uint32_t(i2) * g.rowb, a.stream_out);
         bstore_out(nszy, rout, jc4, o_szy + uint32_t(i2) * g.rowb, a.stream_out);
-        if constexpr (AUX) {
-          bstore_out(dvydx, raux2, jc4, A_EXY * npt4 + uint32_t(i2) * g.rowb, a.stream_out);
-          bstore_out(dz2[par ^ 1], raux2, jc4, A_EZY * npt4 + uint32_t(i2) * g.rowb, a.stream_out);
-        }
       }
     }
     lds_v2[par][lane] = u2[s0];
@@ -1489,8 +1464,11 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
 //   K_mu, K_lambda += dt sigbar' . dC/dm . strain(v')
 // with (D+)^T w = -D-(w~) and (D-)^T w = -D+(w^), where w~ / w^ are w zeroed
 // outside the rows/columns on which the forward operator is active (D+: 1 <=
-// i < n-2, D-: 2 <= i < n-2).  ds and the strains come from the aux fields
-// of the recomputed forward step.  Same marching/tiling as the forward step.
+// i < n-2, D-: 2 <= i < n-2).  Both terms come from the forward velocities
+// the recompute stored, v' = v^{n+1} and v = v^n: the strains are D+ v^{n+1}
+// (the operators of the forward's stress half-step), and since
+// v^{n+1} = B (v^n + dt ds / rho), dt g . ds / rho^2 = vbar'' . (v^{n+1} -
+// B v^n) / rho.  Same marching/tiling as the forward step.
 // The adjoint is not a reference path (its parity is against the FP64
 // adjoint, to a tolerance): its divisions use the reciprocal-and-correction
 // form without the range test, correctly rounded except for subnormal or
@@ -1506,6 +1484,10 @@ __device__ __forceinline__ float ndm(float fm2, float fm1, float f0, float fp1, 
 __device__ __forceinline__ float ndp(float fm1, float f0, float fp1, float fp2, Div d) {
   return -div_rc(9.0f * (fp1 - f0) - (fp2 - fm1), d.y, d.r);
 }
+// D+ of a forward field (the strain operator), reciprocal-and-correction division
+__device__ __forceinline__ float dpf(float fm1, float f0, float fp1, float fp2, Div d) {
+  return div_rc(9.0f * (fp1 - f0) - (fp2 - fm1), d.y, d.r);
+}
 
 // sum of column t's values of the S shot groups of a workgroup, in shot order
 template <int S, int NT>
@@ -1515,10 +1497,19 @@ __device__ __forceinline__ float shot_sum(const float* v, int t) {
   return acc;
 }
 
+// The stored forward velocities of a shot at time level n (slot n % period).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t fwd_vel(const StepArgs& a, int n, int shot, int nf) {
+  return make_rsrc(a.aux + (int64_t(n % a.aux_period) * a.nshots + shot) * nf * int64_t(a.npt),
+                   uint32_t(nf) * 4u * uint32_t(a.npt));
+}
+
 template <int NT, int S>
 __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   __shared__ float lds_t[2][S * NT];
   __shared__ float lds_q[2][S * NT];
+  // forward vy^{n+1} of rows m-3..m (slot as the register ring's), for the
+  // z-derivative two rows later
+  __shared__ float lds_f[4][S * NT];
   // several shots per workgroup: each row's K_rho, K_mu contributions of
   // every shot, summed over the shots one iteration later
   __shared__ float lds_k[S > 1 ? 2 : 1][S > 1 ? 2 : 1][S > 1 ? S * NT : 1];
@@ -1532,8 +1523,7 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   const auto rout = make_rsrc(a.st[it & 1] + so, sbytes);
   const uint32_t npt4 = 4u * uint32_t(a.npt);
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
-  const auto raux =
-      make_rsrc(a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_SH * int64_t(a.npt), NAUX_SH * npt4);
+  const auto rf1 = fwd_vel(a, it + 1, g.shot, NAUX_SH), rf0 = fwd_vel(a, it, g.shot, NAUX_SH);
   // one accumulator set per workgroup, as in adj_psv (field f summed by
   // shot group f % S)
   const auto rk = make_rsrc(a.kgrad + int64_t(g.shot / S) * NK * int64_t(a.npt), NK * npt4);
@@ -1559,8 +1549,8 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   int inj_row = next_row(a.inj, ip, ie);     // row of the next adjoint-source entry
 
   // rings (as in the forward): sigbar rows m-2..m+1 and their x-masked t;
-  // q (x-masked) rows m-3..m; -D+z(q^) rows m-2..m
-  float sbx[4], sbz[4], tx[4], q[4] = {0, 0, 0, 0}, dzq[4] = {0, 0, 0, 0};
+  // q (x-masked) rows m-3..m; -D+z(q^) rows m-2..m; forward vy^{n+1} rows m-3..m
+  float sbx[4], sbz[4], tx[4], q[4] = {0, 0, 0, 0}, dzq[4] = {0, 0, 0, 0}, f1[4] = {0, 0, 0, 0};
   float tz_cur = 0.0f;    // z-masked t_z of row m
   for (int k = 0; k < 3; ++k) {
     const int r = i0 - 3 + k;
@@ -1573,15 +1563,14 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   }
   sbx[3] = sbz[3] = tx[3] = 0.0f;
   // D-deep prefetch FIFO, slot (m - (i0-1)) % D: sigbar and mu rows m+1;
-  // vbar, rho, 1/rho, ds and K_rho rows m; strains and K_mu rows m-2 of
-  // iteration m.  D = 2, or 1 with several shots per workgroup (registers).
+  // vbar, rho, 1/rho, the forward vy^{n+1}, vy^n and K_rho rows m; K_mu rows
+  // m-2 of iteration m.  D = 2, or 1 with several shots per workgroup.
   constexpr int D = S > 1 ? 1 : 2;
   static_assert(S == 1 || D == 1, "the shot sums' epilogue reads prefetch slot 0");
-  float pf_sbx[D], pf_sbz[D], pf_mu[D], pf_vb[D], pf_rho[D], pf_irho[D], pf_ds[D], pf_ex[D], pf_ez[D], pf_kr[D],
-      pf_km[D];
+  float pf_sbx[D], pf_sbz[D], pf_mu[D], pf_vb[D], pf_rho[D], pf_irho[D], pf_f1[D], pf_f0[D], pf_kr[D], pf_km[D];
   float bx_next = taper_x(a.bx, i0 - 1, nx);
   for (int k = 0; k < D; ++k) {
-    const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k), rm2 = g.row(i0 - 3 + k);
+    const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k);
     const uint32_t rkr = g.row(i0 - 1 + k - KR_LAG), rks = g.row(i0 - 1 + k - KS_LAG);
     pf_sbx[k] = bload(rin, jc4, o_sxy + r1);
     pf_sbz[k] = bload(rin, jc4, o_szy + r1);
@@ -1589,15 +1578,15 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     pf_vb[k] = bload(rin, jc4, o_vy + r0);
     pf_rho[k] = bload(rmod, jc4, m_rho + r0);
     pf_irho[k] = bload(rmod, jc4, m_irho + r0);
-    pf_ds[k] = bload(raux, jc4, A_DSY * npt4 + r0);
+    pf_f1[k] = bload(rf1, jc4, A_VY * npt4 + r0);
+    pf_f0[k] = bload(rf0, jc4, A_VY * npt4 + r0);
     pf_kr[k] = own_r ? bload(rk, jc4, k_rho + rkr) : 0.0f;
-    pf_ex[k] = bload(raux, jc4, A_EXY * npt4 + rm2);
-    pf_ez[k] = bload(raux, jc4, A_EZY * npt4 + rm2);
     pf_km[k] = own_m ? bload(rk, jc4, k_mu + rks) : 0.0f;
   }
 
   // one barrier per row (as in the forward step): t of row i0-1 goes to
-  // lds_t[0]; iteration m writes q of row m and t of row m+1 before it
+  // lds_t[0]; iteration m writes q and the forward vy of row m and t of row
+  // m+1 before it
   lds_t[0][lane] = tz_cur;
   __syncthreads();
 
@@ -1610,11 +1599,12 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     const float mu1 = pf_mu[f];
     tx[s3] = xt(m + 1) ? dt * mu1 * sbx[s3] : 0.0f;
     const float tz_next = zt ? dt * mu1 * sbz[s3] : 0.0f;
-    const float vb_o = pf_vb[f], rho = pf_rho[f], irho = pf_irho[f], ds = pf_ds[f], ex = pf_ex[f], ez = pf_ez[f];
+    const float vb_o = pf_vb[f], rho = pf_rho[f], irho = pf_irho[f], f0m = pf_f0[f];
+    f1[s3] = settle(pf_f1[f]);     // forward vy^{n+1}, row m
     const float kr = pf_kr[f], km = pf_km[f], bxm = bx_next;
     bx_next = taper_x(a.bx, m + 1, nx);
     {
-      const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
+      const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D);
       const uint32_t rkr = g.row(m + D - KR_LAG), rks = g.row(m + D - KS_LAG);
       pf_sbx[f] = bload(rin, jc4, o_sxy + r3);
       pf_sbz[f] = bload(rin, jc4, o_szy + r3);
@@ -1622,10 +1612,9 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
       pf_vb[f] = bload(rin, jc4, o_vy + r2);
       pf_rho[f] = bload(rmod, jc4, m_rho + r2);
       pf_irho[f] = bload(rmod, jc4, m_irho + r2);
-      pf_ds[f] = bload(raux, jc4, A_DSY * npt4 + r2);
+      pf_f1[f] = bload(rf1, jc4, A_VY * npt4 + r2);
+      pf_f0[f] = bload(rf0, jc4, A_VY * npt4 + r2);
       if (own_r) pf_kr[f] = bload(rk, jc4, k_rho + rkr);
-      pf_ex[f] = bload(raux, jc4, A_EXY * npt4 + r0);
-      pf_ez[f] = bload(raux, jc4, A_EZY * npt4 + r0);
       if (own_m) pf_km[f] = bload(rk, jc4, k_mu + rks);
     }
 
@@ -1644,7 +1633,7 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     const float b = (bxm * bzb) * bzt;
     const float gv = b * vb;
     const float qv = div_rc(dt * gv, rho, irho);     // dt g / rho
-    const float cr = (-qv * ds) * irho;      // K_rho += -dt g ds / rho^2
+    const float cr = (-(vb * (f1[s3] - b * f0m))) * irho;   // K_rho += -dt g ds / rho^2
     if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
       bstore_out(gv, rout, jc4, o_vy + ro, a.stream_out);
@@ -1654,11 +1643,12 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     const float qz = zq ? qv : 0.0f;
     lds_q[par][lane] = qz;
     lds_t[par ^ 1][lane] = tz_next;
+    lds_f[s3][lane] = f1[s3];
     __syncthreads();
     dzq[s2] = ndp(lds_q[par][tm1], qz, lds_q[par][tp1], lds_q[par][tp2], Dz);
     tz_cur = tz_next;
 
-    // adjoint "stress" at row i = m-2
+    // adjoint "stress" at row i = m-2, and the strains of vy^{n+1} there
     const int i = m - 2;
     if (i >= i0 && g.jst) {
       const uint32_t ro = uint32_t(i) * g.rowb;
@@ -1667,7 +1657,9 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
       bstore_out(nsx, rout, jc4, o_sxy + ro, a.stream_out);
       bstore_out(nsz, rout, jc4, o_szy + ro, a.stream_out);
     }
-    const float cm = dt * (sbx[s0] * ex + sbz[s0] * ez);
+    const float exy = xt(i) ? dpf(f1[s0], f1[s1], f1[s2], f1[s3], Dx) : 0.0f;
+    const float ezy = zt ? dpf(lds_f[s1][tm1], f1[s1], lds_f[s1][tp1], lds_f[s1][tp2], Dz) : 0.0f;
+    const float cm = dt * (sbx[s0] * exy + sbz[s0] * ezy);
     if constexpr (S == 1) {
       if (i >= i0 && g.jst) bstore(km + cm, rk, jc4, k_mu + uint32_t(i) * g.rowb);
     } else {
@@ -1708,6 +1700,7 @@ template <int NT, int S>
 __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   __shared__ float lds_t[2][2][S * NT];   // z-masked t_xz, t_zz of row m
   __shared__ float lds_q[2][2][S * NT];   // z-masked qx, qz of row m
+  __shared__ float lds_f[4][2][S * NT];   // forward vx, vz^{n+1} of rows m-3..m (register-ring slots)
   // several shots per workgroup: each row's K_rho, K_lambda, K_mu
   // contributions of every shot, summed over the shots one iteration later
   __shared__ float lds_k[S > 1 ? 2 : 1][S > 1 ? 3 : 1][S > 1 ? S * NT : 1];
@@ -1721,8 +1714,7 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   const auto rout = make_rsrc(a.st[it & 1] + so, sbytes);
   const uint32_t npt4 = 4u * uint32_t(a.npt);
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
-  const auto raux = make_rsrc(a.aux + (int64_t(it % a.aux_period) * a.nshots + g.shot) * NAUX_PSV * int64_t(a.npt),
-                              NAUX_PSV * npt4);
+  const auto rf1 = fwd_vel(a, it + 1, g.shot, NAUX_PSV), rf0 = fwd_vel(a, it, g.shot, NAUX_PSV);
   // kernel accumulators: one set per workgroup ([shot / S][NK][npt]); with
   // S > 1 the S shots' contributions are added in shot order by the shot
   // group that owns the field (field f: group f % S), one read-modify-write
@@ -1752,6 +1744,7 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
 
   float sbxx[4], sbzz[4], sbxz[4], txx[4], txz[4];
   float qx[4] = {0, 0, 0, 0}, qz[4] = {0, 0, 0, 0}, dzqx[4] = {0, 0, 0, 0}, dzqz[4] = {0, 0, 0, 0};
+  float fx[4] = {0, 0, 0, 0}, fz[4] = {0, 0, 0, 0};   // forward vx, vz^{n+1}, rows m-3..m
   float tzxz_cur = 0.0f, tzzz_cur = 0.0f;   // z-masked t_xz, t_zz of row m
   auto tvals = [&](float lam, float mu, float sxx, float szz, float sxz, float& oxx, float& ozz, float& oxz) {
     const float l2m = lam + 2.0f * mu;
@@ -1773,15 +1766,17 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   }
   sbxx[3] = sbzz[3] = sbxz[3] = txx[3] = txz[3] = 0.0f;
   // D-deep prefetch FIFO, slot (m - (i0-1)) % D: sigbar, lambda, mu rows
-  // m+1; vbar, rho, 1/rho, ds and K_rho rows m; strains, K_lambda and K_mu
-  // rows m-2 of iteration m.  D = 2, or 1 with several shots per workgroup.
-  constexpr int D = S > 1 ? 1 : 2;
+  // m+1; vbar, rho, 1/rho, the forward velocities v^{n+1}, v^n and K_rho
+  // rows m; K_lambda and K_mu rows m-2 of iteration m.  D = 1: with D = 2
+  // the 16 fields in flight per row exceed the 128 VGPRs of four waves per
+  // SIMD.
+  constexpr int D = 1;
   static_assert(S == 1 || D == 1, "the shot sums' epilogue reads prefetch slot 0");
   float pf_sxx[D], pf_szz[D], pf_sxz[D], pf_lam[D], pf_mu[D], pf_vbx[D], pf_vbz[D], pf_rho[D], pf_irho[D];
-  float pf_dsx[D], pf_dsz[D], pf_exx[D], pf_ezz[D], pf_exz[D], pf_kr[D], pf_kl[D], pf_km[D];
+  float pf_fx1[D], pf_fz1[D], pf_fx0[D], pf_fz0[D], pf_kr[D], pf_kl[D], pf_km[D];
   float bx_next = taper_x(a.bx, i0 - 1, nx);
   for (int k = 0; k < D; ++k) {
-    const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k), rm2 = g.row(i0 - 3 + k);
+    const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k);
     const uint32_t rkr = g.row(i0 - 1 + k - KR_LAG), rks = g.row(i0 - 1 + k - KS_LAG);
     pf_sxx[k] = bload(rin, jc4, o_sxx + r1);
     pf_szz[k] = bload(rin, jc4, o_szz + r1);
@@ -1792,12 +1787,11 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     pf_vbz[k] = bload(rin, jc4, o_vz + r0);
     pf_rho[k] = bload(rmod, jc4, m_rho + r0);
     pf_irho[k] = bload(rmod, jc4, m_irho + r0);
-    pf_dsx[k] = bload(raux, jc4, A_DSX * npt4 + r0);
-    pf_dsz[k] = bload(raux, jc4, A_DSZ * npt4 + r0);
+    pf_fx1[k] = bload(rf1, jc4, A_VX * npt4 + r0);
+    pf_fz1[k] = bload(rf1, jc4, A_VZ * npt4 + r0);
+    pf_fx0[k] = bload(rf0, jc4, A_VX * npt4 + r0);
+    pf_fz0[k] = bload(rf0, jc4, A_VZ * npt4 + r0);
     pf_kr[k] = own_r ? bload(rk, jc4, k_rho + rkr) : 0.0f;
-    pf_exx[k] = bload(raux, jc4, A_EXX * npt4 + rm2);
-    pf_ezz[k] = bload(raux, jc4, A_EZZ * npt4 + rm2);
-    pf_exz[k] = bload(raux, jc4, A_EXZ * npt4 + rm2);
     pf_kl[k] = own_l ? bload(rk, jc4, k_lam + rks) : 0.0f;
     pf_km[k] = own_m ? bload(rk, jc4, k_mu + rks) : 0.0f;
   }
@@ -1816,13 +1810,14 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     txx[s3] = xt(m + 1) ? n_xx : 0.0f;
     txz[s3] = xt(m + 1) ? n_xz : 0.0f;
     const float tzxz_next = zt ? n_xz : 0.0f, tzzz_next = zt ? n_zz : 0.0f;
-    const float vbx_o = pf_vbx[f], vbz_o = pf_vbz[f], rho = pf_rho[f], irho = pf_irho[f], dsx = pf_dsx[f],
-                dsz = pf_dsz[f];
-    const float exx = pf_exx[f], ezz = pf_ezz[f], exz = pf_exz[f], kr = pf_kr[f], kl = pf_kl[f], km = pf_km[f];
+    const float vbx_o = pf_vbx[f], vbz_o = pf_vbz[f], rho = pf_rho[f], irho = pf_irho[f];
+    fx[s3] = settle(pf_fx1[f]);     // forward v^{n+1}, row m
+    fz[s3] = settle(pf_fz1[f]);
+    const float fx0 = pf_fx0[f], fz0 = pf_fz0[f], kr = pf_kr[f], kl = pf_kl[f], km = pf_km[f];
     const float bxm = bx_next;
     bx_next = taper_x(a.bx, m + 1, nx);
     {
-      const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D), r0 = g.row(m + D - 2);
+      const uint32_t r3 = g.row(m + D + 1), r2 = g.row(m + D);
       const uint32_t rkr = g.row(m + D - KR_LAG), rks = g.row(m + D - KS_LAG);
       pf_sxx[f] = bload(rin, jc4, o_sxx + r3);
       pf_szz[f] = bload(rin, jc4, o_szz + r3);
@@ -1833,12 +1828,11 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
       pf_vbz[f] = bload(rin, jc4, o_vz + r2);
       pf_rho[f] = bload(rmod, jc4, m_rho + r2);
       pf_irho[f] = bload(rmod, jc4, m_irho + r2);
-      pf_dsx[f] = bload(raux, jc4, A_DSX * npt4 + r2);
-      pf_dsz[f] = bload(raux, jc4, A_DSZ * npt4 + r2);
+      pf_fx1[f] = bload(rf1, jc4, A_VX * npt4 + r2);
+      pf_fz1[f] = bload(rf1, jc4, A_VZ * npt4 + r2);
+      pf_fx0[f] = bload(rf0, jc4, A_VX * npt4 + r2);
+      pf_fz0[f] = bload(rf0, jc4, A_VZ * npt4 + r2);
       if (own_r) pf_kr[f] = bload(rk, jc4, k_rho + rkr);
-      pf_exx[f] = bload(raux, jc4, A_EXX * npt4 + r0);
-      pf_ezz[f] = bload(raux, jc4, A_EZZ * npt4 + r0);
-      pf_exz[f] = bload(raux, jc4, A_EXZ * npt4 + r0);
       if (own_l) pf_kl[f] = bload(rk, jc4, k_lam + rks);
       if (own_m) pf_km[f] = bload(rk, jc4, k_mu + rks);
     }
@@ -1863,7 +1857,8 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     const float b = (bxm * bzb) * bzt;
     const float gx = b * vbx, gz = b * vbz;
     const float qxv = div_rc(dt * gx, rho, irho), qzv = div_rc(dt * gz, rho, irho);   // dt g / rho
-    const float cr = (-(qxv * dsx + qzv * dsz)) * irho;   // K_rho += -dt g.ds / rho^2
+    // K_rho += -dt g.ds / rho^2 = -vbar''.(v^{n+1} - B v^n) / rho
+    const float cr = (-(vbx * (fx[s3] - b * fx0) + vbz * (fz[s3] - b * fz0))) * irho;
     if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
       bstore_out(gx, rout, jc4, o_vx + ro, a.stream_out);
@@ -1877,13 +1872,15 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     lds_q[par][1][lane] = qzz;
     lds_t[par ^ 1][0][lane] = tzxz_next;
     lds_t[par ^ 1][1][lane] = tzzz_next;
+    lds_f[s3][0][lane] = fx[s3];
+    lds_f[s3][1][lane] = fz[s3];
     __syncthreads();
     dzqx[s2] = ndp(lds_q[par][0][tm1], qxz, lds_q[par][0][tp1], lds_q[par][0][tp2], Dz);
     dzqz[s2] = ndp(lds_q[par][1][tm1], qzz, lds_q[par][1][tp1], lds_q[par][1][tp2], Dz);
     tzxz_cur = tzxz_next;
     tzzz_cur = tzzz_next;
 
-    // adjoint "stress" at row i = m-2
+    // adjoint "stress" at row i = m-2, and the strains of v^{n+1} there
     const int i = m - 2;
     if (i >= i0 && g.jst) {
       const uint32_t ro = uint32_t(i) * g.rowb;
@@ -1894,6 +1891,12 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
       bstore_out(nzz, rout, jc4, o_szz + ro, a.stream_out);
       bstore_out(nxz, rout, jc4, o_sxz + ro, a.stream_out);
     }
+    const bool xs = xt(i);
+    const float exx = xs ? dpf(fx[s0], fx[s1], fx[s2], fx[s3], Dx) : 0.0f;
+    const float dvzdx = xs ? dpf(fz[s0], fz[s1], fz[s2], fz[s3], Dx) : 0.0f;
+    const float ezz = zt ? dpf(lds_f[s1][1][tm1], fz[s1], lds_f[s1][1][tp1], lds_f[s1][1][tp2], Dz) : 0.0f;
+    const float dvxdz = zt ? dpf(lds_f[s1][0][tm1], fx[s1], lds_f[s1][0][tp1], lds_f[s1][0][tp2], Dz) : 0.0f;
+    const float exz = dvxdz + dvzdx;
     const float cl = dt * (sbxx[s0] + sbzz[s0]) * (exx + ezz);
     const float cm = dt * (2.0f * sbxx[s0] * exx + 2.0f * sbzz[s0] * ezz + sbxz[s0] * exz);
     if constexpr (S == 1) {

@@ -338,6 +338,17 @@ int effective_group2(int nt, int requested, int nshots) {
   return g;
 }
 
+// multi-step tiles: LDS tiles (K = lds_steps steps per launch, halo 3K) or
+// two-step marching tiles (halo 6); columns of a strip and of its region
+bool tile2_lds(const Tiling& t) { return t.nt_threads == kLdsTileThreads; }
+int tile2_halo(const Tiling& t) { return tile2_lds(t) ? 3 * t.lds_steps : 6; }
+int tile2_strip_cols(const Tiling& t) { return tile2_lds(t) ? lds_tile_cols(t.lds_steps) : t.nt_threads - 12; }
+int tile2_region_cols(const Tiling& t) { return tile2_strip_cols(t) + 2 * tile2_halo(t); }
+
+// Batches of at most this many shot-cells take LDS tiles for their two-step
+// launches (latency-bound: a marching workgroup waits on its loads once per row)
+constexpr int64_t kLdsTileMaxShotCells = 300000;
+
 // Shape of the two-step launches (6-column halos, W = threads-12): the tile
 // width that wastes the fewest lanes on the last strip (the wider on ties;
 // for grids too small to fill the CUs, the one that gives more workgroups),
@@ -348,6 +359,16 @@ int effective_group2(int nt, int requested, int nshots) {
 void heuristic_tiling2(sw_ctx* c, int nshots) {
   Tiling& t = c->tile2;
   const int nz = c->g.nz;
+  if (int64_t(nshots) * c->npt <= kLdsTileMaxShotCells) {
+    // latency-bound batches: LDS tiles (each workgroup loads its region
+    // once), three steps per launch (two with spin: nine state fields)
+    t.nt_threads = kLdsTileThreads;
+    t.lds_steps = uses_spin(c) ? 2 : 3;
+    t.seglen = lds_tile_rows(t.lds_steps);
+    t.group = 1;
+    return;
+  }
+  t.lds_steps = 0;
   const int64_t ncu = c->ncu > 0 ? c->ncu : 256;
   int best = 64;
   double best_score = -1.0;
@@ -427,14 +448,28 @@ int check_tiling(sw_ctx* c, int nshots) {
   t.group = effective_group(t.nt_threads, c->tile_fixed ? c->group_req : t.group, nshots, uses_spin(c));
   // two-step launches: their own shape if set or chosen by the heuristic,
   // else that of the one-step launches
-  c->two_ok = c->steps_per_launch == 2 && uses_psv(c) != uses_sh(c) && !uses_spin(c);
+  // two-step launches: marching tiles for SH or P-SV alone, LDS tiles also
+  // for P-SV + spin
+  c->two_ok = c->steps_per_launch == 2 && uses_psv(c) != uses_sh(c);
   if (c->two_ok) {
     Tiling& t2 = c->tile2;
     if (!c->tile2_fixed && c->tile_fixed) {
       t2 = t;
       c->group2_req = c->group_req;
     }
-    t2.nstrips = (c->g.nz + (t2.nt_threads - 12) - 1) / (t2.nt_threads - 12);
+    if (tile2_lds(t2)) {
+      if (t2.lds_steps != 2 && t2.lds_steps != 3) t2.lds_steps = 3;
+      if (uses_spin(c)) t2.lds_steps = 2;
+      t2.seglen = lds_tile_rows(t2.lds_steps);
+    } else {
+      t2.lds_steps = 0;
+      if (uses_spin(c)) c->two_ok = false;
+    }
+  }
+  if (c->two_ok) {
+    Tiling& t2 = c->tile2;
+    const int w = tile2_strip_cols(t2);
+    t2.nstrips = (c->g.nz + w - 1) / w;
     t2.nsegs = (c->g.nx + t2.seglen - 1) / t2.seglen;
     t2.group = effective_group2(t2.nt_threads, c->group2_req > 0 ? c->group2_req : t2.group, nshots);
   }
@@ -447,12 +482,13 @@ int check_tiling(sw_ctx* c, int nshots) {
 // stable, so duplicates of one cell stay in index order.
 struct Entry { int i, j, q; };
 
-void tile_bounds(const sw_ctx* c, int tile, int& i0, int& i1, int& c0, int halo = 3) {
-  const Tiling& t = halo == 6 ? c->tile2 : c->tile;
+// tile `tile` of the one-step tiling (two2 false) or of the multi-step one
+void tile_bounds(const sw_ctx* c, int tile, int& i0, int& i1, int& c0, bool two2 = false) {
+  const Tiling& t = two2 ? c->tile2 : c->tile;
   const int strip = tile % t.nstrips, seg = tile / t.nstrips;
   i0 = seg * t.seglen;
   i1 = std::min(i0 + t.seglen, c->g.nx);
-  c0 = strip * (t.nt_threads - 2 * halo) - halo;
+  c0 = two2 ? strip * tile2_strip_cols(t) - tile2_halo(t) : strip * (t.nt_threads - 6) - 3;
 }
 
 int upload_inj(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj, DBuf<int>& off_d, DBuf<int4>& data_d) {
@@ -481,14 +517,19 @@ int upload_inj(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj, DBuf<
   return 0;
 }
 
-// Lists of one tiling: halo 3 (one step per launch) or 6 (two steps; the
-// first stage computes velocities on rows [i0-4, i1+4]).
-int build_tile_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj, int halo, DBuf<int>& off_d,
+// Lists of one tiling: the one-step tiles (halo 3) or the multi-step ones
+// (two2: two-step marching tiles, halo 6, whose first stage computes
+// velocities on rows [i0-4, i1+4]; LDS tiles, halo 3K, their whole region).
+int build_tile_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj, bool two2, DBuf<int>& off_d,
                      DBuf<int4>& data_d, DBuf<int>& roff_d, DBuf<int4>& rdata_d) {
-  const Tiling& tl = halo == 6 ? c->tile2 : c->tile;
+  const Tiling& tl = two2 ? c->tile2 : c->tile;
   const int ntile = tl.nstrips * tl.nsegs;
-  const int NT = tl.nt_threads;
-  const int before = halo == 6 ? 4 : 1, after = halo == 6 ? 5 : 2;
+  const int halo = two2 ? tile2_halo(tl) : 3;
+  const int NT = two2 ? tile2_region_cols(tl) : tl.nt_threads;
+  // rows whose new velocities a tile computes: one-step marching rows
+  // [i0-1, i1+2), two-step marching [i0-4, i1+5), LDS tiles their region
+  const bool lds = two2 && tile2_lds(tl);
+  const int before = lds ? halo : two2 ? 4 : 1, after = lds ? halo : two2 ? 5 : 2;
   std::vector<int> off(size_t(c->nshots) * ntile + 1, 0);
   std::vector<int4> data;
   for (int s = 0; s < c->nshots; ++s) {
@@ -496,7 +537,7 @@ int build_tile_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj,
     std::stable_sort(pts.begin(), pts.end(), [](const Entry& a, const Entry& b) { return a.i < b.i; });
     for (int t = 0; t < ntile; ++t) {
       int i0, i1, c0;
-      tile_bounds(c, t, i0, i1, c0, halo);
+      tile_bounds(c, t, i0, i1, c0, two2);
       off[size_t(s) * ntile + t] = int(data.size());
       for (const Entry& e : pts)
         if (e.i >= i0 - before && e.i < i1 + after && e.j >= c0 && e.j < c0 + NT)
@@ -516,7 +557,7 @@ int build_tile_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj,
   std::vector<int4> rdata;
   for (int t = 0; t < ntile; ++t) {
     int i0, i1, c0;
-    tile_bounds(c, t, i0, i1, c0, halo);
+    tile_bounds(c, t, i0, i1, c0, two2);
     roff[t] = int(rdata.size());
     for (const Entry& e : recs)
       if (e.i >= i0 && e.i < i1 && e.j >= c0 + halo && e.j < c0 + NT - halo && e.j < c->g.nz)
@@ -538,9 +579,9 @@ int build_tile_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj,
 }
 
 int build_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj) {
-  int rc = build_tile_lists(c, shot_inj, 3, c->inj_off, c->inj, c->rec_off, c->rec);
+  int rc = build_tile_lists(c, shot_inj, false, c->inj_off, c->inj, c->rec_off, c->rec);
   if (rc || !c->two_ok) return rc;
-  return build_tile_lists(c, shot_inj, 6, c->inj2_off, c->inj2, c->rec2_off, c->rec2);
+  return build_tile_lists(c, shot_inj, true, c->inj2_off, c->inj2, c->rec2_off, c->rec2);
 }
 
 std::vector<Entry> source_entries(const sw_ctx* c, int isrc) {
@@ -656,12 +697,16 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   const bool aux = mode == M_FWD_AUX;
   StepArgs a_sh = a;
   a_sh.advance = psv ? 0 : 1;   // with both physics the P-SV kernel advances the clock
-  // Four steps as two two-step launches: state[cur] -> spare -> state[cur],
-  // so the buffer of every step stays that of its parity.
-  const bool pairs = (mode == M_FWD || mode == M_FWD_AUX) && c->two_ok && c->steps_per_launch == 2 && nsteps >= 4;
+  // Multi-step launches keep the buffer of every step that of its parity:
+  // two-step launches go in pairs, state[cur] -> spare -> state[cur] (four
+  // steps); three-step LDS launches go state[p] -> state[p ^ 1].  (LDS tiles
+  // have no variant that stores the gradient's velocities.)
+  const int ldsk = tile2_lds(c->tile2) ? c->tile2.lds_steps : 0;
+  const bool multi = (mode == M_FWD || (mode == M_FWD_AUX && !ldsk)) && c->two_ok && c->steps_per_launch == 2;
+  const bool triples = multi && ldsk == 3 && nsteps >= 3;
+  const bool pairs = multi && ldsk != 3 && nsteps >= 4;
   StepArgs a2a{}, a2b{};
-  if (pairs) {
-    HIPCHK(c->spare.ensure(size_t(c->nslots) * size_t(c->nshots) * size_t(c->npt)));
+  if (pairs || triples) {
     a2a = a;
     a2a.nstrips = c->tile2.nstrips;
     a2a.nsegs = c->tile2.nsegs;
@@ -670,16 +715,32 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     a2a.rec_off = c->rec2_off.p; a2a.rec = c->rec2.p;
     a2a.nblocks = c->nshots / c->tile2.group * c->tile2.nstrips * c->tile2.nsegs;
     a2b = a2a;
+  }
+  if (pairs) {
+    HIPCHK(c->spare.ensure(size_t(c->nslots) * size_t(c->nshots) * size_t(c->npt)));
     a2a.st[0] = c->state[c->cur].p; a2a.st[1] = c->spare.p;
     a2b.st[0] = c->spare.p; a2b.st[1] = c->state[c->cur].p;
   }
+  const int cur0 = c->cur;
+  if (triples) {
+    // a2a from the current parity, a2b from the other one
+    a2a.st[0] = c->state[c->cur].p; a2a.st[1] = c->state[c->cur ^ 1].p;
+    a2b.st[0] = c->state[c->cur ^ 1].p; a2b.st[1] = c->state[c->cur].p;
+  }
+  // three steps from the buffer of parity cur (flip: the other one)
+  auto enqueue3 = [&](bool flip) -> int {
+    const StepArgs& x = flip ? a2b : a2a;
+    HIPCHK(sh ? launch_step_sh2(x, c->tile2, c->stream, false) : launch_step_psv2(x, c->tile2, c->stream, false, spin));
+    c->launches += 1;
+    return 0;
+  };
   auto enqueue4 = [&]() -> int {
     if (sh) {
       HIPCHK(launch_step_sh2(a2a, c->tile2, c->stream, aux));
       HIPCHK(launch_step_sh2(a2b, c->tile2, c->stream, aux));
     } else {
-      HIPCHK(launch_step_psv2(a2a, c->tile2, c->stream, aux));
-      HIPCHK(launch_step_psv2(a2b, c->tile2, c->stream, aux));
+      HIPCHK(launch_step_psv2(a2a, c->tile2, c->stream, aux, spin));
+      HIPCHK(launch_step_psv2(a2b, c->tile2, c->stream, aux, spin));
     }
     c->launches += 2;
     return 0;
@@ -716,16 +777,20 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     return 0;
   }
   int done = 0;
-  const int G = c->graph_steps;   // a multiple of 4
+  // steps per captured block: a whole number of launch units that returns
+  // the state to the buffer it started in, since a block is replayed back to
+  // back (pairs: 4 steps; three-step launches: an even number of them, 6
+  // steps; one step otherwise)
+  const int G = triples ? c->graph_steps / 6 * 6 : c->graph_steps;   // graph_steps: a multiple of 4
   if (G >= 2 && nsteps >= G && !c->graphs_off && !getenv("SEISWG_NO_GRAPHS")) {
     // everything the captured launches depend on: the kernels (physics,
     // mode, tile shapes), the block structure and the argument blocks
     GraphKey k;
     const int32_t shape[10] = {int32_t(mode), int32_t(sh), int32_t(psv), int32_t(spin), int32_t(pairs), G,
                                c->tile.nt_threads * 8 + c->tile.group, c->tile2.nt_threads * 8 + c->tile2.group,
-                               int32_t(c->sh_only), 0};
+                               int32_t(c->sh_only), int32_t(triples) + 2 * c->tile2.lds_steps};
     k.add(shape, sizeof shape);
-    if (pairs) {
+    if (pairs || triples) {
       k.add(&a2a, sizeof a2a);
       k.add(&a2b, sizeof a2b);
     } else {
@@ -759,6 +824,8 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
       if (e == hipSuccess) {
         if (pairs)
           for (int q = 0; q < G && !rc; q += 4) rc = enqueue4();
+        else if (triples)
+          for (int q = 0; q < G && !rc; q += 3) rc = enqueue3(((q / 3) & 1) != 0);
         else
           for (int q = 0; q < G && !rc; ++q) rc = enqueue();
         const hipError_t e_end = hipStreamEndCapture(c->stream, &graph);
@@ -780,7 +847,7 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
                 c->graph_error.c_str());
       }
     }
-    const int64_t per_block = pairs ? G / 2 : (mode != M_ADJ && sh && psv) ? 2 * G : G;
+    const int64_t per_block = pairs ? G / 2 : triples ? G / 3 : (mode != M_ADJ && sh && psv) ? 2 * G : G;
     for (; exec && done + G <= nsteps; done += G) {
       HIPCHK(hipGraphLaunch(exec, c->stream));
       c->launches += per_block;
@@ -792,6 +859,13 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
       int rc = enqueue4();
       if (rc) return rc;
       count_step(4);
+    }
+  if (triples)
+    for (int q = 0; done + 3 <= nsteps; done += 3, ++q) {
+      // from the parity the (possibly replayed) blocks left
+      int rc = enqueue3(c->cur != cur0);
+      if (rc) return rc;
+      count_step(3);
     }
   for (; done < nsteps; ++done) {
     int rc = enqueue();
@@ -1309,7 +1383,8 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
   std::vector<int32_t> isrc(nshots);
   for (int s = 0; s < nshots; ++s) isrc[s] = c->nsrc > 0 ? s % c->nsrc : -1;
   const int spl = c->steps_per_launch;
-  const bool two = spl == 2 && uses_psv(c) != uses_sh(c) && !uses_spin(c);
+  // two-step shapes: marching tiles for SH or P-SV alone, LDS tiles also with spin
+  const bool two = spl == 2 && uses_psv(c) != uses_sh(c);
   const int n2 = std::max(4, (steps + 3) / 4 * 4);   // whole pairs of two-step launches
   if (c->g.nt < steps + 2 || (two && c->g.nt < n2 + 4))
     return fail(-2, "nt=%d too short to time %d steps", c->g.nt, two ? n2 : steps);
@@ -1365,13 +1440,18 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
     double best2 = 1e300;
     Tiling bt2 = c->tile2;
     int bg2 = c->group2_req;
-    for (int nt : {64, 128, 256, 320, 512}) {
+    for (int nt : {64, 128, 256, 320, 512, kLdsTileThreads}) {
+      if (uses_spin(c) && nt != kLdsTileThreads) continue;
       for (int grp : {1, 2, 4}) {
         if (!group2_supported(nt, grp) || nshots % grp != 0) continue;
         for (int seg : segs) {
           if (seg > 2 * c->g.nx) continue;
+          // LDS tiles: the kernel fixes its tile; steps per launch 2 or 3 (2 with spin)
+          const int lk = nt != kLdsTileThreads ? 0 : seg == segs[0] ? 2 : seg == segs[1] ? 3 : -1;
+          if (lk < 0 || (lk == 3 && uses_spin(c))) continue;
           c->tile2.nt_threads = nt;
-          c->tile2.seglen = seg;
+          c->tile2.seglen = lk ? lds_tile_rows(lk) : seg;
+          c->tile2.lds_steps = lk;
           c->tile2.group = grp;
           c->group2_req = grp;
           c->tile2_fixed = true;
@@ -1401,13 +1481,24 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
 
 SW_API int sw_set_tiling2(sw_ctx* c, int32_t threads, int32_t seglen, int32_t shots_per_block) {
   if (!c) return fail(-1, "null ctx");
-  if (!tiling2_supported(threads)) return fail(-2, "threads must be 64, 128, 256, 320 or 512");
+  if (!tiling2_supported(threads))
+    return fail(-2, "threads must be 64, 128, 256, 320 or 512 (marching tiles) or %d (LDS tiles)", kLdsTileThreads);
+  // LDS tiles: `seglen` is the steps per launch (2 or 3; 0: 3, or 2 with
+  // spin); the kernel fixes its tile
+  int lds_steps = 0;
+  if (threads == kLdsTileThreads) {
+    lds_steps = seglen == 0 ? (uses_spin(c) ? 2 : 3) : seglen;
+    if (lds_steps != 2 && lds_steps != 3) return fail(-2, "LDS tiles run 2 or 3 steps per launch");
+    if (uses_spin(c) && lds_steps != 2) return fail(-2, "LDS tiles with spin run 2 steps per launch");
+    seglen = lds_tile_rows(lds_steps);
+  }
   if (seglen < 1) return fail(-2, "seglen must be >= 1");
   if (shots_per_block != 0 && !group2_supported(threads, shots_per_block))
     return fail(-2, "shots_per_block must be 0 (automatic), 1, or 2 or 4 with threads >= 128 and at most "
                     "1024 threads per workgroup");
   c->tile2.nt_threads = threads;
   c->tile2.seglen = seglen;
+  c->tile2.lds_steps = lds_steps;
   c->tile2.group = shots_per_block > 0 ? shots_per_block : 1;
   c->group2_req = shots_per_block;
   c->tile2_fixed = true;
@@ -1417,7 +1508,7 @@ SW_API int sw_set_tiling2(sw_ctx* c, int32_t threads, int32_t seglen, int32_t sh
 SW_API int sw_get_tiling2(const sw_ctx* c, int32_t* threads, int32_t* seglen, int32_t* shots_per_block) {
   if (!c || !threads || !seglen) return fail(-1, "null argument");
   *threads = c->two_ok ? c->tile2.nt_threads : 0;
-  *seglen = c->two_ok ? c->tile2.seglen : 0;
+  *seglen = !c->two_ok ? 0 : tile2_lds(c->tile2) ? c->tile2.lds_steps : c->tile2.seglen;
   if (shots_per_block) *shots_per_block = c->two_ok ? c->tile2.group : 0;
   return 0;
 }

@@ -29,6 +29,7 @@ struct Tiling {
   int seglen;       // rows (x) marched by one workgroup
   int nstrips, nsegs;
   int group = 1;    // shots per workgroup of the forward steps (1, 2 or 4)
+  int lds_steps = 0;   // multi-step launches: 0 marching tiles; 2 or 3 LDS tiles, steps per launch
 };
 
 // Rows of the model buffer [MF_N][npt].
@@ -91,10 +92,16 @@ enum KField : int { K_RHO = 0, K_MU, K_LAM, NK };
 // launches (kernels.hip)
 hipError_t launch_step_sh(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false);
 hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStream_t s, bool aux = false);
-// two P-SV steps (no spin) per launch: reads st[0], writes st[1] (a buffer
-// distinct from st[0]); tiles of nt_threads-12 columns; aux: also the aux
-// fields of both steps (forward recompute of the gradient)
-hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false);
+// two P-SV steps per launch: reads st[0], writes st[1] (a buffer distinct
+// from st[0]); marching tiles of nt_threads-12 columns (no spin); aux: also
+// the velocities of both steps (forward recompute of the gradient).  With
+// nt_threads == kLdsTileThreads: t.lds_steps (2 or 3) steps on LDS tiles of
+// lds_tile_rows(K) x lds_tile_cols(K) cells (small grids; spin too, K = 2
+// only; no aux)
+constexpr int kLdsTileThreads = 1024;
+hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false, bool spin = false);
+int lds_tile_rows(int k);
+int lds_tile_cols(int k);
 // the same for SH
 hipError_t launch_step_sh2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false);
 hipError_t launch_adj_sh(const StepArgs& a, const Tiling& t, hipStream_t s);

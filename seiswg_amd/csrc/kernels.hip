@@ -131,11 +131,14 @@ __device__ __forceinline__ void bstore_out(float v, __amdgpu_buffer_rsrc_t r, ui
   else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, int(voff), int(soff), 0);
 }
 
-// Advance the step clock once every workgroup of this launch has read it.
+// Advance the step clock once every workgroup of this launch has read it:
+// every workgroup read clock[0] at its start, before it counts itself here,
+// so the last one to count may overwrite it.  Device-scope atomics order
+// the count among workgroups; the next launch sees the new clock because a
+// kernel boundary makes this launch's writes visible (no fence needed).
 __device__ __forceinline__ void advance_clock(int* clock, int it, int dir, int nblk) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
     const int done = atomicAdd(&clock[1], 1);
     if (done == nblk - 1) {
       atomicExch(&clock[1], 0);
@@ -1451,6 +1454,301 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
   if (a.advance) advance_clock(a.clock, it, 2, a.nblocks);
 }
 
+// ------------------------ small grids: K steps per launch on LDS tiles
+// Grids too small to give every CU several marching workgroups (up to a few
+// hundred thousand shot-cells) are latency-bound: a marching workgroup waits
+// on its row loads about once per row.  Here a workgroup of 1024 threads
+// loads a 32 x 64 region of every state field into LDS at once - its tile
+// of (32 - 2H) x (64 - 2H) cells plus an H = 3K-cell halo on each side, the
+// reach of K steps - and advances the region K steps in LDS (velocity
+// half-step | barrier | receivers, stress half-step | barrier, K times),
+// then writes the tile back.  Halo cells are recomputed, never exchanged: a
+// step shrinks the region's valid part by 3 cells on each side (the velocity
+// reads stresses 2 before / 1 after, the stress reads new velocities 1
+// before / 2 after), so after K steps the tile is exact; cells outside the
+// grid stay 0 and are never computed (the FD edge rules mask them).  Every
+// cell's arithmetic is that of step_sh / step_psv operation for operation,
+// so results equal one-step launches bit for bit.  Thread t owns the cells
+// (t / 64, t % 64) and (t / 64 + 16, t % 64) of the region: a wavefront is
+// one region row.  Reads st[0], writes st[1]: with K = 2 the host runs
+// launches in pairs through a spare buffer (as the marching two-step
+// launches), with K = 3 st[1] is the buffer of the other parity.
+constexpr int TL_R = 32, TL_C = 64;
+enum TilePhys : int { TP_SH = 0, TP_PSV, TP_SPIN };
+
+template <int PH, int K>
+__device__ __forceinline__ void tile2(const StepArgs& a, int logical) {
+  constexpr int TL_H = 3 * K, TL_TR = TL_R - 2 * TL_H, TL_TC = TL_C - 2 * TL_H;
+  constexpr bool SH = PH == TP_SH, SPIN = PH == TP_SPIN;
+  constexpr int NF = SH ? 3 : SPIN ? 9 : 5;
+  // LDS fields: SH vy, sxy, szy; P-SV vx, vz, sxx, szz, sxz; spin + vy_c,
+  // syx_c, syy_c, syz_c (the state slot order)
+  enum { L_VY = 0, L_SXY = 1, L_SZY = 2 };
+  enum { L_VX = 0, L_VZ, L_SXX, L_SZZ, L_SXZ, L_VYC, L_SYX, L_SYY, L_SYZ };
+  // 2 cells of padding around the region: every neighbour read is a constant
+  // offset from the cell (padding cells only feed cells outside the valid part)
+  __shared__ float L[NF][TL_R + 4][TL_C + 4];
+  const int tps = a.nstrips * a.nsegs;
+  const int shot = logical / tps, tile = logical - shot * tps;
+  const int strip = tile % a.nstrips, seg = tile / a.nstrips;
+  const int nx = a.nx, nz = a.nz;
+  const int r0 = seg * TL_TR - TL_H, c0 = strip * TL_TC - TL_H;   // grid row / column of region cell (0, 0)
+  const int tid = threadIdx.x, cc = tid & (TL_C - 1), rr0 = tid >> 6;
+  const int j = c0 + cc;
+  const bool jin = j >= 0 && j < nz;
+  const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const float* __restrict__ s_in = a.st[0] + int64_t(shot) * a.shot_floats;
+  float* __restrict__ s_out = a.st[1] + int64_t(shot) * a.shot_floats;
+  const int npt = a.npt;
+  const float dt = a.dt;
+  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const int fast_c = div_mode(a.fast_div, FD_CONST);
+  const int fast_rho = div_mode(a.fast_div, FD_RHO), fast_rj = div_mode(a.fast_div, FD_RHO | FD_J);
+  const int jc = min(max(j, 0), nz - 1);
+  const float bzb = a.bzb[jc], bzt = a.bzt[jc];
+  const bool zm_ok = j >= 2 && j < nz - 2, zp_ok = j >= 1 && j < nz - 2;
+  int slot[NF];
+  if constexpr (SH) {
+    slot[L_VY] = a.slot_off[S_VY]; slot[L_SXY] = a.slot_off[S_SXY]; slot[L_SZY] = a.slot_off[S_SZY];
+  } else {
+    slot[L_VX] = a.slot_off[S_VX]; slot[L_VZ] = a.slot_off[S_VZ]; slot[L_SXX] = a.slot_off[S_SXX];
+    slot[L_SZZ] = a.slot_off[S_SZZ]; slot[L_SXZ] = a.slot_off[S_SXZ];
+    if constexpr (SPIN) {
+      slot[L_VYC] = a.slot_off[S_VYC]; slot[L_SYX] = a.slot_off[S_SYXC];
+      slot[L_SYY] = a.slot_off[S_SYYC]; slot[L_SYZ] = a.slot_off[S_SYZC];
+    }
+  }
+  // the padding (and cells outside the grid) hold zeros: they feed the
+  // outermost cells only, with finite values, so the divisions keep their
+  // fast path there
+  {
+    float* flat = &L[0][0][0];
+    for (int q = tid; q < NF * (TL_R + 4) * (TL_C + 4); q += kLdsTileThreads) flat[q] = 0.0f;
+  }
+  __syncthreads();
+  // the two cells of this thread: region rows rr0 and rr0 + 16
+  constexpr int NC = 2;
+  int rr[NC], ii[NC];
+  bool in[NC];
+  float rho[NC], irho[NC], lam[NC], mu[NC], nu[NC], jj[NC], ij[NC], muc[NC], nuc[NC], bb[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    rr[q] = rr0 + 16 * q;
+    ii[q] = r0 + rr[q];
+    in[q] = jin && ii[q] >= 0 && ii[q] < nx;
+    const int k = in[q] ? ii[q] * nz + j : 0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) L[f][rr[q] + 2][cc + 2] = in[q] ? s_in[slot[f] + k] : 0.0f;
+    rho[q] = a.model[MF_RHO * npt + k];
+    irho[q] = a.model[MF_IRHO * npt + k];
+    mu[q] = a.model[MF_MU * npt + k];
+    lam[q] = SH ? 0.0f : a.model[MF_LAM * npt + k];
+    nu[q] = SPIN ? a.model[MF_NU * npt + k] : 0.0f;
+    jj[q] = SPIN ? a.model[MF_J * npt + k] : 1.0f;
+    ij[q] = SPIN ? a.model[MF_IJ * npt + k] : 1.0f;
+    muc[q] = SPIN ? a.model[MF_MU_C * npt + k] : 0.0f;
+    nuc[q] = SPIN ? a.model[MF_NU_C * npt + k] : 0.0f;
+    bb[q] = (taper_x(a.bx, ii[q], nx) * bzb) * bzt;
+  }
+  // lists of this tile: injections anywhere in the region, receivers in the tile
+  const int ip0 = __builtin_amdgcn_readfirstlane(a.inj_off[shot * tps + tile]);
+  const int ie = __builtin_amdgcn_readfirstlane(a.inj_off[shot * tps + tile + 1]);
+  int rp0 = 0, re = 0;
+  if (a.record) {
+    rp0 = __builtin_amdgcn_readfirstlane(a.rec_off[tile]);
+    re = __builtin_amdgcn_readfirstlane(a.rec_off[tile + 1]);
+  }
+  float* uacc = a.uacc + int64_t(shot) * a.nrec * N_UACC;
+  float* obs = a.obs + int64_t(shot) * 5 * a.nrec * a.nt;
+  const int nobs = a.nrec * a.nt;
+  // neighbour reads clamp to the region: values past its edge only reach
+  // cells outside the valid part, which are discarded
+  auto at = [&](int f, int r, int c) -> float { return L[f][r + 2][c + 2]; };
+  const int cm2 = cc - 2, cm1 = cc - 1, cp1 = cc + 1, cp2 = cc + 2;
+  __syncthreads();
+
+  for (int s = 0; s < K; ++s) {
+    const int its = it + s;
+    // ---- velocity half-step at every cell of the region (in place)
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      if (!in[q]) continue;
+      const int r = rr[q], i = ii[q];
+      const bool xm_ok = i >= 2 && i < nx - 2;
+      if constexpr (SH) {
+        float qa[2];
+        const float xa[2] = {xm_ok ? nminus(at(L_SXY, r - 2, cc), at(L_SXY, r - 1, cc), at(L_SXY, r, cc),
+                                            at(L_SXY, r + 1, cc))
+                                   : 0.0f,
+                             zm_ok ? nminus(at(L_SZY, r, cm2), at(L_SZY, r, cm1), at(L_SZY, r, cc), at(L_SZY, r, cp1))
+                                   : 0.0f};
+        div_group(qa, xa, {Dx, Dz}, fast_c);
+        float dsy = qa[0] + qa[1];
+        for (int p = ip0; p < ie; ++p) {
+          const int4 e = a.inj[p];
+          if (e.x == i && e.y == j) dsy = dsy + a.val_y[int64_t(e.z) * a.nt_val + its];
+        }
+        float qb[1];
+        const float xb[1] = {dt * dsy};
+        div_group(qb, xb, {Div{rho[q], irho[q], true}}, fast_rho);
+        L[L_VY][r + 2][cc + 2] = bb[q] * (L[L_VY][r + 2][cc + 2] + qb[0]);
+      } else {
+        constexpr int NA = SPIN ? 8 : 4;
+        float qa[NA], xa[NA];
+        xa[0] = keep(xm_ok, nminus(at(L_SXX, r - 2, cc), at(L_SXX, r - 1, cc), at(L_SXX, r, cc), at(L_SXX, r + 1, cc)));
+        xa[1] = keep(zm_ok, nminus(at(L_SXZ, r, cm2), at(L_SXZ, r, cm1), at(L_SXZ, r, cc), at(L_SXZ, r, cp1)));
+        xa[2] = keep(xm_ok, nminus(at(L_SXZ, r - 2, cc), at(L_SXZ, r - 1, cc), at(L_SXZ, r, cc), at(L_SXZ, r + 1, cc)));
+        xa[3] = keep(zm_ok, nminus(at(L_SZZ, r, cm2), at(L_SZZ, r, cm1), at(L_SZZ, r, cc), at(L_SZZ, r, cp1)));
+        if constexpr (SPIN) {
+          xa[4] = keep(xm_ok, nminus(at(L_SYX, r - 2, cc), at(L_SYX, r - 1, cc), at(L_SYX, r, cc), at(L_SYX, r + 1, cc)));
+          xa[5] = keep(zm_ok, nminus(at(L_SYZ, r, cm2), at(L_SYZ, r, cm1), at(L_SYZ, r, cc), at(L_SYZ, r, cp1)));
+          xa[6] = keep(zm_ok, nminus(at(L_SYY, r, cm2), at(L_SYY, r, cm1), at(L_SYY, r, cc), at(L_SYY, r, cp1)));
+          xa[7] = keep(xm_ok, nminus(at(L_SYY, r - 2, cc), at(L_SYY, r - 1, cc), at(L_SYY, r, cc), at(L_SYY, r + 1, cc)));
+          div_group(qa, xa, {Dx, Dz, Dx, Dz, Dx, Dz, Dz, Dx}, fast_c);
+        } else {
+          div_group(qa, xa, {Dx, Dz, Dx, Dz}, fast_c);
+        }
+        float dsx = qa[0] + qa[1];
+        float dsz = qa[2] + qa[3];
+        float dsyc = 0.0f;
+        if constexpr (SPIN) {
+          dsyc = qa[4] + qa[5];
+          dsx = dsx + qa[6];
+          dsz = dsz + (-qa[7]);
+          dsyc = dsyc + 2.0f * L[L_SYY][r + 2][cc + 2];
+        }
+        for (int p = ip0; p < ie; ++p) {
+          const int4 e = a.inj[p];
+          if (e.x == i && e.y == j) {
+            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + its];
+            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + its];
+          }
+        }
+        constexpr int NB = SPIN ? 3 : 2;
+        float qb[NB];
+        if constexpr (SPIN) {
+          const float xb[3] = {dt * dsx, dt * dsz, dt * dsyc};
+          div_group(qb, xb, {Div{rho[q], irho[q], true}, Div{rho[q], irho[q], true}, Div{jj[q], ij[q], true}}, fast_rj);
+          L[L_VYC][r + 2][cc + 2] = bb[q] * (L[L_VYC][r + 2][cc + 2] + qb[2]);
+        } else {
+          const float xb[2] = {dt * dsx, dt * dsz};
+          div_group(qb, xb, {Div{rho[q], irho[q], true}, Div{rho[q], irho[q], true}}, fast_rho);
+        }
+        L[L_VX][r + 2][cc + 2] = bb[q] * (L[L_VX][r + 2][cc + 2] + qb[0]);
+        L[L_VZ][r + 2][cc + 2] = bb[q] * (L[L_VZ][r + 2][cc + 2] + qb[1]);
+      }
+    }
+    __syncthreads();
+
+    // ---- receivers of step its: u = sum v dt at the cells the traces need
+    // (each receiver belongs to one tile; its owner thread reads the new
+    // velocities of the neighbouring cells from LDS)
+    for (int p = rp0; p < re; ++p) {
+      const int4 e = a.rec[p];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        if (!(in[q] && e.x == ii[q] && e.y == j)) continue;
+        float* u = uacc + e.z * N_UACC;
+        const int kt = e.z * a.nt + its;
+        const int r = rr[q];
+        if constexpr (SH) {
+          const float uy = u[U_UY] + L[L_VY][r + 2][cc + 2] * dt;
+          u[U_UY] = uy;
+          obs[1 * nobs + kt] = uy;
+        } else if constexpr (!SPIN) {
+          const float ux = u[U_UX] + L[L_VX][r + 2][cc + 2] * dt, uz = u[U_UZ] + L[L_VZ][r + 2][cc + 2] * dt;
+          u[U_UX] = ux;
+          u[U_UZ] = uz;
+          obs[0 * nobs + kt] = ux;
+          obs[2 * nobs + kt] = uz;
+        } else {
+          u[U_UZ_M1] = u[U_UZ_M1] + at(L_VZ, r - 1, cc) * dt;
+          u[U_UX] = u[U_UX] + L[L_VX][r + 2][cc + 2] * dt;
+          u[U_UZ] = u[U_UZ] + L[L_VZ][r + 2][cc + 2] * dt;
+          u[U_UYC] = u[U_UYC] + L[L_VYC][r + 2][cc + 2] * dt;
+          u[U_UX_M1] = u[U_UX_M1] + at(L_VX, r, cm1) * dt;
+          u[U_UX_P1] = u[U_UX_P1] + at(L_VX, r, cp1) * dt;
+          u[U_UX_P2] = u[U_UX_P2] + at(L_VX, r, cp2) * dt;
+          u[U_UZ_P1] = u[U_UZ_P1] + at(L_VZ, r + 1, cc) * dt;
+          const float uz_p2 = u[U_UZ_P2] + at(L_VZ, r + 2, cc) * dt;
+          u[U_UZ_P2] = uz_p2;
+          const float ux = u[U_UX], uz = u[U_UZ], uyc = u[U_UYC];
+          const bool xp = e.x >= 1 && e.x < nx - 2;
+          const bool zp = e.y >= 1 && e.y < nz - 2;
+          const float duzdx = xp ? dplus(u[U_UZ_M1], uz, u[U_UZ_P1], uz_p2, Dx) : 0.0f;
+          const float duxdz = zp ? dplus(u[U_UX_M1], ux, u[U_UX_P1], u[U_UX_P2], Dz) : 0.0f;
+          const float curl = -(duzdx - duxdz);
+          const float iso = uyc - 0.5f * curl;
+          obs[0 * nobs + kt] = ux;
+          obs[2 * nobs + kt] = uz;
+          obs[1 * nobs + kt] = uyc;
+          obs[3 * nobs + kt] = iso;
+          obs[4 * nobs + kt] = curl;
+        }
+      }
+    }
+
+    // ---- stress half-step at every cell of the region (in place)
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      if (!in[q]) continue;
+      const int r = rr[q], i = ii[q];
+      const bool xp_ok = i >= 1 && i < nx - 2;
+      if constexpr (SH) {
+        float qc[2];
+        const float vy0 = L[L_VY][r + 2][cc + 2];
+        const float xc[2] = {zp_ok ? nplus(at(L_VY, r, cm1), vy0, at(L_VY, r, cp1), at(L_VY, r, cp2)) : 0.0f,
+                             xp_ok ? nplus(at(L_VY, r - 1, cc), vy0, at(L_VY, r + 1, cc), at(L_VY, r + 2, cc)) : 0.0f};
+        div_group(qc, xc, {Dz, Dx}, fast_c);
+        const float mu_dt = mu[q] * dt;
+        L[L_SXY][r + 2][cc + 2] = L[L_SXY][r + 2][cc + 2] + mu_dt * qc[1];
+        L[L_SZY][r + 2][cc + 2] = L[L_SZY][r + 2][cc + 2] + mu_dt * qc[0];
+      } else {
+        constexpr int NCD = SPIN ? 6 : 4;
+        float qc[NCD], xc[NCD];
+        const float vx0 = L[L_VX][r + 2][cc + 2], vz0 = L[L_VZ][r + 2][cc + 2];
+        xc[0] = keep(zp_ok, nplus(at(L_VX, r, cm1), vx0, at(L_VX, r, cp1), at(L_VX, r, cp2)));
+        xc[1] = keep(zp_ok, nplus(at(L_VZ, r, cm1), vz0, at(L_VZ, r, cp1), at(L_VZ, r, cp2)));
+        xc[2] = keep(xp_ok, nplus(at(L_VX, r - 1, cc), vx0, at(L_VX, r + 1, cc), at(L_VX, r + 2, cc)));
+        xc[3] = keep(xp_ok, nplus(at(L_VZ, r - 1, cc), vz0, at(L_VZ, r + 1, cc), at(L_VZ, r + 2, cc)));
+        float vyc0 = 0.0f;
+        if constexpr (SPIN) {
+          vyc0 = L[L_VYC][r + 2][cc + 2];
+          xc[4] = keep(zp_ok, nplus(at(L_VYC, r, cm1), vyc0, at(L_VYC, r, cp1), at(L_VYC, r, cp2)));
+          xc[5] = keep(xp_ok, nplus(at(L_VYC, r - 1, cc), vyc0, at(L_VYC, r + 1, cc), at(L_VYC, r + 2, cc)));
+          div_group(qc, xc, {Dz, Dz, Dx, Dx, Dz, Dx}, fast_c);
+        } else {
+          div_group(qc, xc, {Dz, Dz, Dx, Dx}, fast_c);
+        }
+        const float dvxdz = qc[0], dvzdz = qc[1], dvxdx = qc[2], dvzdx = qc[3];
+        const float l2m = lam[q] + 2.0f * mu[q];
+        L[L_SXX][r + 2][cc + 2] = L[L_SXX][r + 2][cc + 2] + dt * (l2m * dvxdx + lam[q] * dvzdz);
+        L[L_SZZ][r + 2][cc + 2] = L[L_SZZ][r + 2][cc + 2] + dt * (l2m * dvzdz + lam[q] * dvxdx);
+        L[L_SXZ][r + 2][cc + 2] = L[L_SXZ][r + 2][cc + 2] + dt * mu[q] * (dvxdz + dvzdx);
+        if constexpr (SPIN) {
+          const float dvycdz = qc[4], dvycdx = qc[5];
+          L[L_SYY][r + 2][cc + 2] = L[L_SYY][r + 2][cc + 2] + 2.0f * dt * nu[q] * (-vyc0 - 0.5f * (dvzdx - dvxdz));
+          const float c2 = dt * (muc[q] + nuc[q]);
+          L[L_SYX][r + 2][cc + 2] = L[L_SYX][r + 2][cc + 2] + c2 * dvycdx;
+          L[L_SYZ][r + 2][cc + 2] = L[L_SYZ][r + 2][cc + 2] + c2 * dvycdz;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- the tile back to memory
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    const int r = rr[q];
+    if (!in[q] || r < TL_H || r >= TL_R - TL_H || cc < TL_H || cc >= TL_C - TL_H) continue;
+    const int k = ii[q] * nz + j;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) s_out[slot[f] + k] = L[f][r + 2][cc + 2];
+  }
+  if (a.advance) advance_clock(a.clock, it, K, a.nblocks);
+}
+
 // ------------------------------------------------------ adjoint time steps
 // Exact transpose of one forward step (SH or P-SV), run in reverse time.
 // With the forward step
@@ -1967,6 +2265,10 @@ template <int NT, int S, bool AUX>
 __global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(4))) void k_step_sh2(const StepArgs a) {
   step_sh2<NT, S, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
+template <int PH, int K>
+__global__ __launch_bounds__(1024) void k_tile2(const StepArgs a) {
+  tile2<PH, K>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
 template <int NT, int S>
 __global__ __launch_bounds__(NT * S) void k_step_spin(const StepArgs a) {
   step_psv<NT, S, true, false>(a, xcd_remap(blockIdx.x, gridDim.x));
@@ -2308,18 +2610,47 @@ hipError_t launch_two(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s
   }
   return hipErrorInvalidValue;
 }
-hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
+// K steps on LDS tiles (small grids): the tile geometry is fixed by the
+// kernel (lds_tile_rows(K)-row segments, lds_tile_cols(K)-column strips), one
+// shot per workgroup
+hipError_t launch_tile2(int phys, const StepArgs& a, const Tiling& t, hipStream_t s) {
+  const int K = t.lds_steps;
+  if (t.nt_threads != kLdsTileThreads || (K != 2 && K != 3) || t.seglen != lds_tile_rows(K) || t.group != 1 ||
+      !args_match(a, t, 1) || a.nstrips != (a.nz + lds_tile_cols(K) - 1) / lds_tile_cols(K) ||
+      a.nsegs != (a.nx + lds_tile_rows(K) - 1) / lds_tile_rows(K))
+    return hipErrorInvalidValue;
+  const dim3 grid(a.nshots * t.nstrips * t.nsegs), block(kLdsTileThreads);
+  switch (phys * 4 + K) {
+    case TP_SH * 4 + 2: hipLaunchKernelGGL((k_tile2<TP_SH, 2>), grid, block, 0, s, a); break;
+    case TP_SH * 4 + 3: hipLaunchKernelGGL((k_tile2<TP_SH, 3>), grid, block, 0, s, a); break;
+    case TP_PSV * 4 + 2: hipLaunchKernelGGL((k_tile2<TP_PSV, 2>), grid, block, 0, s, a); break;
+    case TP_PSV * 4 + 3: hipLaunchKernelGGL((k_tile2<TP_PSV, 3>), grid, block, 0, s, a); break;
+    case TP_SPIN * 4 + 2: hipLaunchKernelGGL((k_tile2<TP_SPIN, 2>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux, bool spin) {
+  if (t.nt_threads == kLdsTileThreads) return aux ? hipErrorInvalidValue : launch_tile2(spin ? TP_SPIN : TP_PSV, a, t, s);
+  if (spin) return hipErrorInvalidValue;
   return launch_two(false, a, t, s, aux);
 }
 hipError_t launch_step_sh2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
+  if (t.nt_threads == kLdsTileThreads) return aux ? hipErrorInvalidValue : launch_tile2(TP_SH, a, t, s);
   return launch_two(true, a, t, s, aux);
 }
+int lds_tile_rows(int k) { return TL_R - 6 * k; }
+int lds_tile_cols(int k) { return TL_C - 6 * k; }
 // 320 (five waves, 308-column strips) fits grids of ~600 columns in two strips
-bool tiling2_supported(int nt) { return nt == 64 || nt == 128 || nt == 256 || nt == 320 || nt == 512; }
+bool tiling2_supported(int nt) {
+  return nt == 64 || nt == 128 || nt == 256 || nt == 320 || nt == 512 || nt == kLdsTileThreads;
+}
 bool group2_supported(int nt, int group) {
+  if (nt == kLdsTileThreads) return group == 1;
   return tiling2_supported(nt) && (group == 1 || (nt >= 128 && (group == 2 || group == 4) && nt * group <= 1024));
 }
 int step2_blocks_per_cu(int nt, int group) {
+  if (nt == kLdsTileThreads) return occupancy(k_tile2<TP_PSV, 2>, kLdsTileThreads);
   switch (nt * 8 + group) {
     case 64 * 8 + 1: return occupancy(k_step_psv2<64, 1, false>, 64);
     case 128 * 8 + 1: return occupancy(k_step_psv2<128, 1, false>, 128);

@@ -322,7 +322,8 @@ def test_autotuned_shapes_keep_results(seiswg, phys):
         c.forward_batch(runs)
         if tune:
             assert c.tiling() == (threads, seglen, group)
-            assert (c.tiling2()[0] > 0) == (not spin)
+            # two-step shapes: marching or LDS tiles, LDS tiles only with spin
+            assert c.tiling2()[0] > 0 and (not spin or c.tiling2()[0] == 1024)
         out.append([c.obs(q, k).copy() for q in range(len(runs)) for k in comps] +
                    [c.field(getattr(seiswg.abi, "F_" + n), shot=q).copy() for q in range(len(runs)) for n in fields])
         c.close()
@@ -686,3 +687,49 @@ def test_graph_blocks_reused_across_prepared_batches(seiswg, monkeypatch):
     d.advance(40)
     assert d.graph_state()[:2] == (0, 0)
     assert np.array_equal(d.field(seiswg.abi.F_VX, 1).view(np.uint32), got.view(np.uint32))
+
+
+@pytest.mark.parametrize("phys,k", [("sh", 2), ("sh", 3), ("psv", 2), ("psv", 3), ("spin", 2)])
+@pytest.mark.parametrize("nx,nz", [(45, 131), (30, 40), (61, 150)])
+def test_lds_tile_multi_step_launches_match_oracle_bitwise(seiswg, phys, k, nx, nz):
+    """K steps per launch on LDS tiles (small grids; sw_set_tiling2 with 1024
+    threads and K = 2 or 3): three shots from random states, sources and
+    receivers on edges and duplicated, 7 steps (pairs of two-step launches
+    or three-step launches, then single steps) - every state field and trace
+    bit for bit against the FP32 oracle."""
+    sh, psv, spin = phys == "sh", phys in ("psv", "spin"), phys == "spin"
+    nsteps = 7
+    p, stf = random_problem(nx, nz, nsteps + 1, sh, psv, spin, seed=31)
+    print(f"case phys={phys} K={k} grid={nx}x{nz}", flush=True)
+    c = gpu_context(p, stf, seiswg)
+    c.set_tiling2(1024, k, 0)
+    runs = [0, -1, 2]
+    c.prepare(runs)
+    assert c.tiling2()[:2] == (1024, k)
+    fields = (STATE["sh"] if sh else []) + (STATE["psv"] if psv else []) + (STATE["spin"] if spin else [])
+    rng = np.random.default_rng(9)
+    init = []
+    for s in range(len(runs)):
+        st = {n: rng.standard_normal(nx * nz).astype(np.float32) for n in fields}
+        for n, v in st.items():
+            c.write_field(getattr(seiswg.abi, "F_" + n), v, shot=s)
+        init.append(st)
+    c.advance(nsteps)
+    comps = ([O.OBS_Y] if sh or spin else []) + ([O.OBS_X, O.OBS_Z] if psv else []) + \
+            ([O.OBS_YI, O.OBS_YC] if spin else [])
+    o = O.Oracle(p, 4, stf=stf.astype(np.float64))
+    for s, isrc in enumerate(runs):
+        o.clear()
+        for n, v in init[s].items():
+            o.set_field(O.F[n], v)
+        for it in range(nsteps):
+            o.step(it, isrc)
+        for n in fields:
+            got = c.field(getattr(seiswg.abi, "F_" + n), shot=s)
+            want = np.asarray(o.field(O.F[n]), np.float64).astype(np.float32)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (s, n)
+        obs = o.obs()
+        for k in comps:
+            got = c.obs(s, k)[:, :nsteps]
+            want = np.asarray(obs[k][:, :nsteps], np.float64).astype(np.float32)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (s, k)

@@ -207,13 +207,14 @@ def small_config_bench(seiswg, synth, which, with_cpu=True):
     ctx.forward_batch([-1])
     wall = time.perf_counter() - t0
     tm = ctx.timing()
-    tiling = list(ctx.tiling())
+    tiling, tiling2 = list(ctx.tiling()), list(ctx.tiling2())
     ctx.close()
     cells = nx * nz * nt
     out = {"workload": ("plain SH forward, 200x200, nt=2000, 25 receivers, taper on all edges" if not spin else
                         "P-SV + micropolar spin forward, 500x500, nt=3000, j=1e4, 7 receivers"),
            "cell_updates_per_s": cells / wall, "s_total": wall, "us_per_step": wall / nt * 1e6,
-           "kernel_loop_ms": tm.loop_ms, "launches": int(tm.step_launches), "tiling": tiling}
+           "kernel_loop_ms": tm.loop_ms, "launches": int(tm.step_launches), "tiling": tiling,
+           "tiling_multi_step": tiling2}
     if with_cpu:
         from oracle import oracle as O
         p = O.Problem(nx=nx, nz=nz, nt=nt, dx=h, dz=h, dt=dt, sh=not spin, psv=spin, spin=spin, abs_left=True,
@@ -229,23 +230,27 @@ def small_config_bench(seiswg, synth, which, with_cpu=True):
     return out
 
 
-def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
+def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None, only=None, tune=False):
     """Uneven batch of 48 independent SH / P-SV gradient problems (grids
     384..3072 x 384..1536, 4-16 shots, 100-300 receivers, nt scaled from
     1500..4000 by nt_scale, checkpoint every 50; `spec` = synth.UNEVEN_SMALL: grids
-    128..448, 2-6 shots): total shots/s through sw_multi_gradient (problems
-    below its merge limit share merged launches, the others run on their own
-    launches; in waves that fit budget_gb of device memory) vs one problem per
-    launch (sw_gradient per problem, its shots batched)."""
+    128..448, 2-6 shots; `only`: those problem indices): total shots/s through
+    sw_multi_gradient (problems below its merge limit share merged launches,
+    the others run on their own launches; in waves that fit budget_gb of
+    device memory) vs one problem per launch (sw_gradient per problem, its
+    shots batched).  tune: tile shapes autotuned once per size class (kind
+    and shot-cells to the nearest power of two) on the class's first problem
+    and set on every problem of the class, for both paths alike."""
+    import math
     import numpy as np
     abi = seiswg.abi
     spec = spec or synth.UNEVEN
-    tag = "uneven batch" if spec is synth.UNEVEN else "small uneven batch"
+    tag = ("uneven batch" if spec is synth.UNEVEN else "small uneven batch") + (" subset" if only else "")
 
     def progress(msg):
         print(f"[{tag}] {msg}", file=sys.stderr, flush=True)
 
-    probs = synth.uneven_problems(nt_scale, spec=spec)
+    probs = synth.uneven_problems(nt_scale, spec=spec, only=set(only) if only else None)
     progress(f"{len(probs)} problems generated")
 
     def make_ctx(p, model):
@@ -272,6 +277,33 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
         c.close()
     progress("observed data computed")
 
+    shapes, classes, t_tune = {}, {}, 0.0
+    if tune:
+        for q, p in enumerate(probs):
+            key = (p["kind"], int(round(math.log2(len(shots[q]) * p["nx"] * p["nz"]))))
+            classes.setdefault(key, []).append(q)
+        t0 = time.perf_counter()
+        for key, qs in classes.items():
+            p = probs[qs[0]]
+            c = make_ctx(p, p["init"])
+            c.autotune(len(shots[qs[0]]), steps=8)
+            c.prepare(shots[qs[0]])
+            shape = (c.tiling(), c.tiling2())
+            c.close()
+            for q in qs:
+                shapes[q] = shape
+        t_tune = time.perf_counter() - t0
+        progress(f"{len(classes)} size classes tuned in {t_tune:.1f} s")
+
+    def tuned_ctx(q):
+        c = make_ctx(probs[q], probs[q]["init"])
+        if q in shapes:
+            t1, t2 = shapes[q]
+            c.set_tiling(*t1)
+            if t2[0]:
+                c.set_tiling2(*t2)
+        return c
+
     def mem_bytes(p, n):
         nslot = 3 if p["kind"] == "sh" else 5
         npt = p["nx"] * p["nz"]
@@ -281,7 +313,7 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
     total_shots = sum(len(x) for x in shots)
     t_plain, ref0 = 0.0, None
     for q, (p, o, sh) in enumerate(zip(probs, obs, shots)):
-        c = make_ctx(p, p["init"])
+        c = tuned_ctx(q)
         t0 = time.perf_counter()
         r = c.gradient(sh, o, ckpt_every=synth.UNEVEN["ckpt"])
         t_plain += time.perf_counter() - t0
@@ -301,7 +333,7 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
     waves.append(cur)
     t_batched, got0 = 0.0, None
     for w in waves:
-        ctxs = [make_ctx(probs[q], probs[q]["init"]) for q in w]
+        ctxs = [tuned_ctx(q) for q in w]
         m = seiswg.Multi(ctxs)
         t0 = time.perf_counter()
         res = m.gradient([shots[q] for q in w], [obs[q] for q in w], ckpt_every=synth.UNEVEN["ckpt"])
@@ -315,14 +347,24 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None):
     same_misfit = bool(np.array_equal(got0[0], ref0[0]))
     kernel_err = max(rel_l2(a, b) for a, b in zip(got0[1:], ref0[1:]))
     merged = sum(1 for p, sh in zip(probs, shots) if len(sh) * p["nx"] * p["nz"] < DEFAULT_MERGE_LIMIT)
-    return {"workload": f"{tag} of {len(probs)} SH/P-SV gradient problems (grids {spec['nx'][0]}..{spec['nx'][1]} x "
-                        f"{spec['nz'][0]}..{spec['nz'][1]}, {spec['shots'][0]}-{spec['shots'][1]} shots, "
-                        f"nt = {nt_scale} x 1500..4000, checkpoint every {synth.UNEVEN['ckpt']})",
-            "problems": len(probs), "shots": total_shots, "waves": len(waves),
-            "problems_merged": merged, "problems_own_launches": len(probs) - merged,
-            "shots_per_s_batched": total_shots / t_batched, "shots_per_s_plain": total_shots / t_plain,
-            "speedup": t_plain / t_batched, "s_batched": t_batched, "s_plain": t_plain,
-            "problem0_misfits_bit_identical": same_misfit, "problem0_kernels_rel_l2_vs_plain": kernel_err}
+    nts = [p["nt"] for p in probs]
+    out = {"workload": f"{tag} of {len(probs)} SH/P-SV gradient problems (grids {spec['nx'][0]}..{spec['nx'][1]} x "
+                       f"{spec['nz'][0]}..{spec['nz'][1]}, {spec['shots'][0]}-{spec['shots'][1]} shots, "
+                       f"nt = {nt_scale} x 1500..4000, checkpoint every {synth.UNEVEN['ckpt']})",
+           "problems": len(probs), "shots": total_shots, "waves": len(waves), "nt_range": [min(nts), max(nts)],
+           "cell_updates_forward": int(sum(len(sh) * p["nx"] * p["nz"] * p["nt"] for p, sh in zip(probs, shots))),
+           "problems_merged": merged, "problems_own_launches": len(probs) - merged,
+           "shots_per_s_batched": total_shots / t_batched, "shots_per_s_plain": total_shots / t_plain,
+           "speedup": t_plain / t_batched, "s_batched": t_batched, "s_plain": t_plain,
+           "problem0_misfits_bit_identical": same_misfit, "problem0_kernels_rel_l2_vs_plain": kernel_err}
+    if only:
+        out["problem_indices"] = [p["index"] for p in probs]
+        out["grids"] = [[p["kind"], p["nx"], p["nz"], len(sh), p["nt"]] for p, sh in zip(probs, shots)]
+    if tune:
+        out["autotune"] = {"size_classes": len(classes), "seconds": t_tune,
+                           "shapes": {f"{k[0]}~2^{k[1]} shot-cells": [list(shapes[qs[0]][0]), list(shapes[qs[0]][1])]
+                                      for k, qs in classes.items()}}
+    return out
 
 
 def main():
@@ -347,6 +389,10 @@ def main():
     ap.add_argument("--gradient-only", action="store_true",
                     help="run only the batched-vs-plain gradient measurement (for profiler runs)")
     ap.add_argument("--uneven", action="store_true", help="also measure the uneven batch of 48 gradient problems")
+    ap.add_argument("--no-uneven-subset", action="store_true",
+                    help="skip the seeded 8-problem subset of the uneven batch at its full nt")
+    ap.add_argument("--uneven-subset-only", action="store_true",
+                    help="run only the seeded 8-problem subset of the uneven batch at its full nt")
     ap.add_argument("--no-small", action="store_true", help="skip the 200x200 SH and 500x500 spin forward runs")
     ap.add_argument("--small-only", action="store_true",
                     help="only the 200x200 SH, 500x500 spin and single-shot 2000x1000 forward runs (for profiling)")
@@ -376,6 +422,10 @@ def main():
         return 0
     if a.gradient_only:
         print(json.dumps({"gradient": gradient_bench(seiswg, synth, a.gradient_nt)}), flush=True)
+        return 0
+    if a.uneven_subset_only:
+        print(json.dumps({"uneven_subset_full_nt": uneven_bench(seiswg, synth, 1.0, a.uneven_budget_gb,
+                                                                only=synth.uneven_subset(), tune=True)}), flush=True)
         return 0
     nt = max(synth.PSV_BATCH["nt"], a.warmup + a.steps)
     ctx, model, geom, src_idx, rec_idx = build_context(seiswg, synth, nt)
@@ -476,6 +526,9 @@ def main():
         out["single_shot_2000x1000"] = single_shot_bench(seiswg, synth, model, geom, src_idx, rec_idx)
         out["sh_forward_200x200"] = small_config_bench(seiswg, synth, "sh", not a.no_cpu_baseline)
         out["spin_forward_500x500"] = small_config_bench(seiswg, synth, "spin", not a.no_cpu_baseline)
+    if not a.no_uneven_subset:
+        out["uneven_subset_full_nt"] = uneven_bench(seiswg, synth, 1.0, a.uneven_budget_gb,
+                                                    only=synth.uneven_subset(), tune=True)
     if a.uneven:
         out["uneven_batch"] = uneven_bench(seiswg, synth, a.uneven_nt_scale, a.uneven_budget_gb)
         out["small_uneven_batch"] = uneven_bench(seiswg, synth, 1.0, spec=synth.UNEVEN_SMALL)

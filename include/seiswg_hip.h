@@ -241,21 +241,30 @@ SW_API int sw_smooth_noalloc(sw_ctx* ctx, int32_t field, float sigma);
  * sw_get_tiling reports the shape of the last prepared batch. */
 SW_API int sw_set_tiling(sw_ctx* ctx, int32_t threads, int32_t seglen, int32_t shots_per_block);
 SW_API int sw_get_tiling(const sw_ctx* ctx, int32_t* threads, int32_t* seglen, int32_t* shots_per_block);
-/* Time steps per launch of the P-SV forward step without spin (no
- * counterpart in the reference, which dispatches every half-step
- * separately): 2 (the default) runs steps in pairs of launches that each
- * advance two steps, reading the state once and writing it once per launch;
- * 1 runs one step per launch.  Results are the same bit for bit.  SH, spin
- * and the gradient's recompute/adjoint steps always run one step per
- * launch.  A change to 2 takes effect at the next prepared batch. */
+/* Time steps per launch (no counterpart in the reference, which dispatches
+ * every half-step separately): 2 (the default) runs the steps of SH or P-SV
+ * alone, with or without spin, in launches that each advance several steps,
+ * reading the state once and writing it once per launch (the shape set by
+ * sw_set_tiling2); 1 runs one step per launch.  Results are the same bit for
+ * bit.  SH and P-SV together, and the gradient's adjoint, always run one
+ * step per launch; its recompute runs two-step marching launches (one step
+ * per launch where the batch has LDS tiles).  A change takes effect at the
+ * next prepared batch. */
 SW_API int sw_set_steps_per_launch(sw_ctx* ctx, int32_t steps);
-/* Tile shape of the two-step launches (no counterpart in the reference):
- * threads per shot 64, 128, 256, 320 or 512 (they own threads-12 z-columns), rows
- * per workgroup, shots per workgroup (0: automatic; 1; 2 or 4 with threads
- * >= 128 and at most 1024 threads per workgroup).  Without a call they get a
- * heuristic shape, or that of sw_set_tiling when it was called.
+/* Tile shape of the multi-step launches (no counterpart in the reference):
+ *  - marching tiles, two steps per launch: threads per shot 64, 128, 256,
+ *    320 or 512 (they own threads-12 z-columns), `seglen` rows per
+ *    workgroup, shots per workgroup (0: automatic; 1; 2 or 4 with threads
+ *    >= 128 and at most 1024 threads per workgroup); not with spin;
+ *  - LDS tiles (small grids, latency-bound): threads = 1024 and `seglen` =
+ *    K, the steps per launch (2..4; with spin 2..3; 0: 4, with spin 3).
+ *    Each workgroup loads a 32 x 64-cell region once, advances it K steps
+ *    in LDS and stores its (32-6K) x (64-6K)-cell interior; one shot per
+ *    workgroup (shots_per_block 0 or 1).
+ * Without a call they get a heuristic shape (LDS tiles for batches of at
+ * most 300000 shot-cells), or that of sw_set_tiling when it was called.
  * sw_get_tiling2 reports the shape of the last prepared batch (zeros when it
- * runs one step per launch). */
+ * runs one step per launch; LDS tiles: 1024, K, 1). */
 SW_API int sw_set_tiling2(sw_ctx* ctx, int32_t threads, int32_t seglen, int32_t shots_per_block);
 SW_API int sw_get_tiling2(const sw_ctx* ctx, int32_t* threads, int32_t* seglen, int32_t* shots_per_block);
 /* Autotuner: times `steps` forward steps of `nshots` shots for a set of

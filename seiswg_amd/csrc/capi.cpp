@@ -345,6 +345,11 @@ int tile2_halo(const Tiling& t) { return tile2_lds(t) ? 3 * t.lds_steps : 6; }
 int tile2_strip_cols(const Tiling& t) { return tile2_lds(t) ? lds_tile_cols(t.lds_steps) : t.nt_threads - 12; }
 int tile2_region_cols(const Tiling& t) { return tile2_strip_cols(t) + 2 * tile2_halo(t); }
 
+// Steps per LDS-tile launch: up to 4 (the region's 32 rows keep 8 after
+// halos of 12), up to 3 with spin (its nine fields and eight moduli)
+int lds_max_steps(const sw_ctx* c) { return uses_spin(c) ? 3 : 4; }
+int lds_default_steps(const sw_ctx* c) { return uses_spin(c) ? 2 : 4; }
+
 // Batches of at most this many shot-cells take LDS tiles for their two-step
 // launches (latency-bound: a marching workgroup waits on its loads once per row)
 constexpr int64_t kLdsTileMaxShotCells = 300000;
@@ -361,9 +366,23 @@ void heuristic_tiling2(sw_ctx* c, int nshots) {
   const int nz = c->g.nz;
   if (int64_t(nshots) * c->npt <= kLdsTileMaxShotCells) {
     // latency-bound batches: LDS tiles (each workgroup loads its region
-    // once), three steps per launch (two with spin: nine state fields)
+    // once).  A launch of K steps takes about a + b K per round of resident
+    // workgroups (a: launch, region load and store; b: one step in LDS;
+    // measured a / b = 3.5 for SH, 6.7 with spin): the K with the fewest
+    // rounds (4 + K) / K per step.
     t.nt_threads = kLdsTileThreads;
-    t.lds_steps = uses_spin(c) ? 2 : 3;
+    const int64_t ncu = c->ncu > 0 ? c->ncu : 256;
+    double best_cost = 1e300;
+    for (int k = 2; k <= lds_max_steps(c); ++k) {
+      const int64_t wg = int64_t(nshots) * ((nz + lds_tile_cols(k) - 1) / lds_tile_cols(k)) *
+                         ((c->g.nx + lds_tile_rows(k) - 1) / lds_tile_rows(k));
+      const int64_t slots = ncu * lds_tile_blocks_per_cu(uses_sh(c), uses_spin(c), k);
+      const double cost = double((wg + slots - 1) / slots) * (4.0 + k) / k;
+      if (cost < best_cost) {
+        best_cost = cost;
+        t.lds_steps = k;
+      }
+    }
     t.seglen = lds_tile_rows(t.lds_steps);
     t.group = 1;
     return;
@@ -458,8 +477,7 @@ int check_tiling(sw_ctx* c, int nshots) {
       c->group2_req = c->group_req;
     }
     if (tile2_lds(t2)) {
-      if (t2.lds_steps != 2 && t2.lds_steps != 3) t2.lds_steps = 3;
-      if (uses_spin(c)) t2.lds_steps = 2;
+      if (t2.lds_steps < 2 || t2.lds_steps > lds_max_steps(c)) t2.lds_steps = lds_default_steps(c);
       t2.seglen = lds_tile_rows(t2.lds_steps);
     } else {
       t2.lds_steps = 0;
@@ -697,16 +715,20 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   const bool aux = mode == M_FWD_AUX;
   StepArgs a_sh = a;
   a_sh.advance = psv ? 0 : 1;   // with both physics the P-SV kernel advances the clock
-  // Multi-step launches keep the buffer of every step that of its parity:
-  // two-step launches go in pairs, state[cur] -> spare -> state[cur] (four
-  // steps); three-step LDS launches go state[p] -> state[p ^ 1].  (LDS tiles
-  // have no variant that stores the gradient's velocities.)
+  // Multi-step launches (K steps each: two for marching tiles, lds_steps for
+  // LDS tiles) keep the buffer of every step that of its parity.  Odd K:
+  // launches alternate state[p] -> state[p ^ 1] and back, one launch per
+  // unit; even K: launches go in pairs, state[cur] -> spare -> state[cur]
+  // (2K steps).  LDS tiles have no variant that stores the gradient's
+  // velocities.
   const int ldsk = tile2_lds(c->tile2) ? c->tile2.lds_steps : 0;
-  const bool multi = (mode == M_FWD || (mode == M_FWD_AUX && !ldsk)) && c->two_ok && c->steps_per_launch == 2;
-  const bool triples = multi && ldsk == 3 && nsteps >= 3;
-  const bool pairs = multi && ldsk != 3 && nsteps >= 4;
+  const bool multi_ok = (mode == M_FWD || (mode == M_FWD_AUX && !ldsk)) && c->two_ok && c->steps_per_launch == 2;
+  const int K = ldsk ? ldsk : 2;                 // steps per multi-step launch
+  const bool odd = (K & 1) != 0;
+  const int U = odd ? K : 2 * K;                 // steps per launch unit
+  const bool multi = multi_ok && nsteps >= U;
   StepArgs a2a{}, a2b{};
-  if (pairs || triples) {
+  if (multi) {
     a2a = a;
     a2a.nstrips = c->tile2.nstrips;
     a2a.nsegs = c->tile2.nsegs;
@@ -715,35 +737,28 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     a2a.rec_off = c->rec2_off.p; a2a.rec = c->rec2.p;
     a2a.nblocks = c->nshots / c->tile2.group * c->tile2.nstrips * c->tile2.nsegs;
     a2b = a2a;
-  }
-  if (pairs) {
-    HIPCHK(c->spare.ensure(size_t(c->nslots) * size_t(c->nshots) * size_t(c->npt)));
-    a2a.st[0] = c->state[c->cur].p; a2a.st[1] = c->spare.p;
-    a2b.st[0] = c->spare.p; a2b.st[1] = c->state[c->cur].p;
+    if (odd) {
+      // a2a from the current parity, a2b from the other one
+      a2a.st[0] = c->state[c->cur].p; a2a.st[1] = c->state[c->cur ^ 1].p;
+      a2b.st[0] = c->state[c->cur ^ 1].p; a2b.st[1] = c->state[c->cur].p;
+    } else {
+      HIPCHK(c->spare.ensure(size_t(c->nslots) * size_t(c->nshots) * size_t(c->npt)));
+      a2a.st[0] = c->state[c->cur].p; a2a.st[1] = c->spare.p;
+      a2b.st[0] = c->spare.p; a2b.st[1] = c->state[c->cur].p;
+    }
   }
   const int cur0 = c->cur;
-  if (triples) {
-    // a2a from the current parity, a2b from the other one
-    a2a.st[0] = c->state[c->cur].p; a2a.st[1] = c->state[c->cur ^ 1].p;
-    a2b.st[0] = c->state[c->cur ^ 1].p; a2b.st[1] = c->state[c->cur].p;
-  }
-  // three steps from the buffer of parity cur (flip: the other one)
-  auto enqueue3 = [&](bool flip) -> int {
-    const StepArgs& x = flip ? a2b : a2a;
-    HIPCHK(sh ? launch_step_sh2(x, c->tile2, c->stream, false) : launch_step_psv2(x, c->tile2, c->stream, false, spin));
+  auto launch2 = [&](const StepArgs& x) -> int {
+    HIPCHK(sh ? launch_step_sh2(x, c->tile2, c->stream, aux) : launch_step_psv2(x, c->tile2, c->stream, aux, spin));
     c->launches += 1;
     return 0;
   };
-  auto enqueue4 = [&]() -> int {
-    if (sh) {
-      HIPCHK(launch_step_sh2(a2a, c->tile2, c->stream, aux));
-      HIPCHK(launch_step_sh2(a2b, c->tile2, c->stream, aux));
-    } else {
-      HIPCHK(launch_step_psv2(a2a, c->tile2, c->stream, aux, spin));
-      HIPCHK(launch_step_psv2(a2b, c->tile2, c->stream, aux, spin));
-    }
-    c->launches += 2;
-    return 0;
+  // one launch unit (U steps); odd K: from the buffer of parity cur, or with
+  // flip from the other one
+  auto enqueue_unit = [&](bool flip) -> int {
+    if (odd) return launch2(flip ? a2b : a2a);
+    const int rc = launch2(a2a);
+    return rc ? rc : launch2(a2b);
   };
   // one time step: the fused kernel(s) of the enabled physics
   auto enqueue = [&]() -> int {
@@ -779,18 +794,17 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   int done = 0;
   // steps per captured block: a whole number of launch units that returns
   // the state to the buffer it started in, since a block is replayed back to
-  // back (pairs: 4 steps; three-step launches: an even number of them, 6
-  // steps; one step otherwise)
-  const int G = triples ? c->graph_steps / 6 * 6 : c->graph_steps;   // graph_steps: a multiple of 4
+  // back (multi-step launches: 2K steps; one step otherwise)
+  const int G = multi ? c->graph_steps / (2 * K) * (2 * K) : c->graph_steps;   // graph_steps: a multiple of 4
   if (G >= 2 && nsteps >= G && !c->graphs_off && !getenv("SEISWG_NO_GRAPHS")) {
     // everything the captured launches depend on: the kernels (physics,
     // mode, tile shapes), the block structure and the argument blocks
     GraphKey k;
-    const int32_t shape[10] = {int32_t(mode), int32_t(sh), int32_t(psv), int32_t(spin), int32_t(pairs), G,
+    const int32_t shape[10] = {int32_t(mode), int32_t(sh), int32_t(psv), int32_t(spin), int32_t(multi), G,
                                c->tile.nt_threads * 8 + c->tile.group, c->tile2.nt_threads * 8 + c->tile2.group,
-                               int32_t(c->sh_only), int32_t(triples) + 2 * c->tile2.lds_steps};
+                               int32_t(c->sh_only), K};
     k.add(shape, sizeof shape);
-    if (pairs || triples) {
+    if (multi) {
       k.add(&a2a, sizeof a2a);
       k.add(&a2b, sizeof a2b);
     } else {
@@ -822,10 +836,8 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
       hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
       int rc = 0;
       if (e == hipSuccess) {
-        if (pairs)
-          for (int q = 0; q < G && !rc; q += 4) rc = enqueue4();
-        else if (triples)
-          for (int q = 0; q < G && !rc; q += 3) rc = enqueue3(((q / 3) & 1) != 0);
+        if (multi)
+          for (int q = 0; q < G && !rc; q += U) rc = enqueue_unit(((q / U) & 1) != 0);
         else
           for (int q = 0; q < G && !rc; ++q) rc = enqueue();
         const hipError_t e_end = hipStreamEndCapture(c->stream, &graph);
@@ -847,25 +859,19 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
                 c->graph_error.c_str());
       }
     }
-    const int64_t per_block = pairs ? G / 2 : triples ? G / 3 : (mode != M_ADJ && sh && psv) ? 2 * G : G;
+    const int64_t per_block = multi ? G / K : (mode != M_ADJ && sh && psv) ? 2 * G : G;
     for (; exec && done + G <= nsteps; done += G) {
       HIPCHK(hipGraphLaunch(exec, c->stream));
       c->launches += per_block;
       count_step(G);
     }
   }
-  if (pairs)
-    for (; done + 4 <= nsteps; done += 4) {
-      int rc = enqueue4();
-      if (rc) return rc;
-      count_step(4);
-    }
-  if (triples)
-    for (int q = 0; done + 3 <= nsteps; done += 3, ++q) {
+  if (multi)
+    for (; done + U <= nsteps; done += U) {
       // from the parity the (possibly replayed) blocks left
-      int rc = enqueue3(c->cur != cur0);
+      int rc = enqueue_unit(c->cur != cur0);
       if (rc) return rc;
-      count_step(3);
+      count_step(U);
     }
   for (; done < nsteps; ++done) {
     int rc = enqueue();
@@ -1385,7 +1391,11 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
   const int spl = c->steps_per_launch;
   // two-step shapes: marching tiles for SH or P-SV alone, LDS tiles also with spin
   const bool two = spl == 2 && uses_psv(c) != uses_sh(c);
-  const int n2 = std::max(4, (steps + 3) / 4 * 4);   // whole pairs of two-step launches
+  // whole launch units of every multi-step shape (pairs of two-step
+  // launches; LDS tiles: 2K steps, so 24 steps for K = 2, 3, 4) when nt
+  // leaves room, else whole pairs
+  int n2 = std::max(4, (steps + 3) / 4 * 4);
+  if (two && c->g.nt >= (steps + 23) / 24 * 24 + 4) n2 = std::max(24, (steps + 23) / 24 * 24);
   if (c->g.nt < steps + 2 || (two && c->g.nt < n2 + 4))
     return fail(-2, "nt=%d too short to time %d steps", c->g.nt, two ? n2 : steps);
   // times `n` steps (after a warm-up) of the prepared batch, in ms
@@ -1446,9 +1456,10 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
         if (!group2_supported(nt, grp) || nshots % grp != 0) continue;
         for (int seg : segs) {
           if (seg > 2 * c->g.nx) continue;
-          // LDS tiles: the kernel fixes its tile; steps per launch 2 or 3 (2 with spin)
-          const int lk = nt != kLdsTileThreads ? 0 : seg == segs[0] ? 2 : seg == segs[1] ? 3 : -1;
-          if (lk < 0 || (lk == 3 && uses_spin(c))) continue;
+          // LDS tiles: the kernel fixes its tile; steps per launch 2..4 (the
+          // first segment lengths stand for K = 2, 3, 4)
+          const int lk = nt != kLdsTileThreads ? 0 : seg == segs[0] ? 2 : seg == segs[1] ? 3 : seg == segs[2] ? 4 : -1;
+          if (lk < 0 || lk > lds_max_steps(c)) continue;
           c->tile2.nt_threads = nt;
           c->tile2.seglen = lk ? lds_tile_rows(lk) : seg;
           c->tile2.lds_steps = lk;
@@ -1483,13 +1494,13 @@ SW_API int sw_set_tiling2(sw_ctx* c, int32_t threads, int32_t seglen, int32_t sh
   if (!c) return fail(-1, "null ctx");
   if (!tiling2_supported(threads))
     return fail(-2, "threads must be 64, 128, 256, 320 or 512 (marching tiles) or %d (LDS tiles)", kLdsTileThreads);
-  // LDS tiles: `seglen` is the steps per launch (2 or 3; 0: 3, or 2 with
-  // spin); the kernel fixes its tile
+  // LDS tiles: `seglen` is the steps per launch (2..4, with spin 2..3; 0:
+  // the default); the kernel fixes its tile
   int lds_steps = 0;
   if (threads == kLdsTileThreads) {
-    lds_steps = seglen == 0 ? (uses_spin(c) ? 2 : 3) : seglen;
-    if (lds_steps != 2 && lds_steps != 3) return fail(-2, "LDS tiles run 2 or 3 steps per launch");
-    if (uses_spin(c) && lds_steps != 2) return fail(-2, "LDS tiles with spin run 2 steps per launch");
+    lds_steps = seglen == 0 ? lds_default_steps(c) : seglen;
+    if (lds_steps < 2 || lds_steps > lds_max_steps(c))
+      return fail(-2, "LDS tiles run 2 to %d steps per launch%s", lds_max_steps(c), uses_spin(c) ? " with spin" : "");
     seglen = lds_tile_rows(lds_steps);
   }
   if (seglen < 1) return fail(-2, "seglen must be >= 1");

@@ -29,7 +29,7 @@ struct Tiling {
   int seglen;       // rows (x) marched by one workgroup
   int nstrips, nsegs;
   int group = 1;    // shots per workgroup of the forward steps (1, 2 or 4)
-  int lds_steps = 0;   // multi-step launches: 0 marching tiles; 2 or 3 LDS tiles, steps per launch
+  int lds_steps = 0;   // multi-step launches: 0 marching tiles; 2..4 LDS tiles, steps per launch
 };
 
 // Rows of the model buffer [MF_N][npt].
@@ -95,9 +95,9 @@ hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStr
 // two P-SV steps per launch: reads st[0], writes st[1] (a buffer distinct
 // from st[0]); marching tiles of nt_threads-12 columns (no spin); aux: also
 // the velocities of both steps (forward recompute of the gradient).  With
-// nt_threads == kLdsTileThreads: t.lds_steps (2 or 3) steps on LDS tiles of
-// lds_tile_rows(K) x lds_tile_cols(K) cells (small grids; spin too, K = 2
-// only; no aux)
+// nt_threads == kLdsTileThreads: t.lds_steps (K = 2..4) steps on LDS tiles
+// of lds_tile_rows(K) x lds_tile_cols(K) cells (small grids; spin too, K = 2
+// or 3; no aux)
 constexpr int kLdsTileThreads = 1024;
 hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false, bool spin = false);
 int lds_tile_rows(int k);
@@ -125,6 +125,8 @@ bool group_supported(int nt_threads, int group);
 bool tiling2_supported(int nt_threads);
 bool group2_supported(int nt_threads, int group);
 int step2_blocks_per_cu(int nt_threads, int group);
+// resident LDS-tile workgroups per CU (K steps per launch)
+int lds_tile_blocks_per_cu(bool sh, bool spin, int k);
 // resident workgroups per CU of the step kernel of a MultiKind at a tile width
 // and shots per workgroup (adjoint kinds: group 1 only)
 int step_blocks_per_cu(int kind, int nt_threads, int group);

@@ -2615,17 +2615,20 @@ hipError_t launch_two(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s
 // shot per workgroup
 hipError_t launch_tile2(int phys, const StepArgs& a, const Tiling& t, hipStream_t s) {
   const int K = t.lds_steps;
-  if (t.nt_threads != kLdsTileThreads || (K != 2 && K != 3) || t.seglen != lds_tile_rows(K) || t.group != 1 ||
+  if (t.nt_threads != kLdsTileThreads || K < 2 || K > 4 || t.seglen != lds_tile_rows(K) || t.group != 1 ||
       !args_match(a, t, 1) || a.nstrips != (a.nz + lds_tile_cols(K) - 1) / lds_tile_cols(K) ||
       a.nsegs != (a.nx + lds_tile_rows(K) - 1) / lds_tile_rows(K))
     return hipErrorInvalidValue;
   const dim3 grid(a.nshots * t.nstrips * t.nsegs), block(kLdsTileThreads);
-  switch (phys * 4 + K) {
-    case TP_SH * 4 + 2: hipLaunchKernelGGL((k_tile2<TP_SH, 2>), grid, block, 0, s, a); break;
-    case TP_SH * 4 + 3: hipLaunchKernelGGL((k_tile2<TP_SH, 3>), grid, block, 0, s, a); break;
-    case TP_PSV * 4 + 2: hipLaunchKernelGGL((k_tile2<TP_PSV, 2>), grid, block, 0, s, a); break;
-    case TP_PSV * 4 + 3: hipLaunchKernelGGL((k_tile2<TP_PSV, 3>), grid, block, 0, s, a); break;
-    case TP_SPIN * 4 + 2: hipLaunchKernelGGL((k_tile2<TP_SPIN, 2>), grid, block, 0, s, a); break;
+  switch (phys * 8 + K) {
+    case TP_SH * 8 + 2: hipLaunchKernelGGL((k_tile2<TP_SH, 2>), grid, block, 0, s, a); break;
+    case TP_SH * 8 + 3: hipLaunchKernelGGL((k_tile2<TP_SH, 3>), grid, block, 0, s, a); break;
+    case TP_PSV * 8 + 2: hipLaunchKernelGGL((k_tile2<TP_PSV, 2>), grid, block, 0, s, a); break;
+    case TP_PSV * 8 + 3: hipLaunchKernelGGL((k_tile2<TP_PSV, 3>), grid, block, 0, s, a); break;
+    case TP_SH * 8 + 4: hipLaunchKernelGGL((k_tile2<TP_SH, 4>), grid, block, 0, s, a); break;
+    case TP_PSV * 8 + 4: hipLaunchKernelGGL((k_tile2<TP_PSV, 4>), grid, block, 0, s, a); break;
+    case TP_SPIN * 8 + 2: hipLaunchKernelGGL((k_tile2<TP_SPIN, 2>), grid, block, 0, s, a); break;
+    case TP_SPIN * 8 + 3: hipLaunchKernelGGL((k_tile2<TP_SPIN, 3>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -2648,6 +2651,21 @@ bool tiling2_supported(int nt) {
 bool group2_supported(int nt, int group) {
   if (nt == kLdsTileThreads) return group == 1;
   return tiling2_supported(nt) && (group == 1 || (nt >= 128 && (group == 2 || group == 4) && nt * group <= 1024));
+}
+int lds_tile_blocks_per_cu(bool sh, bool spin, int k) {
+  int n = 0;
+  switch ((sh ? TP_SH : spin ? TP_SPIN : TP_PSV) * 8 + k) {
+    case TP_SH * 8 + 2: n = occupancy(k_tile2<TP_SH, 2>, kLdsTileThreads); break;
+    case TP_SH * 8 + 3: n = occupancy(k_tile2<TP_SH, 3>, kLdsTileThreads); break;
+    case TP_SH * 8 + 4: n = occupancy(k_tile2<TP_SH, 4>, kLdsTileThreads); break;
+    case TP_PSV * 8 + 2: n = occupancy(k_tile2<TP_PSV, 2>, kLdsTileThreads); break;
+    case TP_PSV * 8 + 3: n = occupancy(k_tile2<TP_PSV, 3>, kLdsTileThreads); break;
+    case TP_PSV * 8 + 4: n = occupancy(k_tile2<TP_PSV, 4>, kLdsTileThreads); break;
+    case TP_SPIN * 8 + 2: n = occupancy(k_tile2<TP_SPIN, 2>, kLdsTileThreads); break;
+    case TP_SPIN * 8 + 3: n = occupancy(k_tile2<TP_SPIN, 3>, kLdsTileThreads); break;
+    default: break;
+  }
+  return std::max(n, 1);
 }
 int step2_blocks_per_cu(int nt, int group) {
   if (nt == kLdsTileThreads) return occupancy(k_tile2<TP_PSV, 2>, kLdsTileThreads);

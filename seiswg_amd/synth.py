@@ -136,6 +136,12 @@ UNEVEN = dict(nproblems=48, seed=4, nx=(384, 3072), nz=(384, 1536), shots=(4, 16
           f0=(5.0, 15.0), h=10.0, cfl=0.5, abs_width=40, ckpt=50)
 
 
+def uneven_subset(k: int = 8, seed: int = 7):
+    """Seeded subset of the uneven batch's problem indices (k of 48, sorted):
+    the problems measured at their full nt (1500-4000)."""
+    return sorted(np.random.default_rng(seed).choice(UNEVEN["nproblems"], k, replace=False).tolist())
+
+
 # a batch of small uneven problems (same recipe, grids 128-448): the sizes at
 # which merging problems into shared launches pays
 UNEVEN_SMALL = dict(UNEVEN, seed=5, nx=(128, 448), nz=(128, 448), shots=(2, 6), nrec=(40, 100), abs_width=20)

@@ -689,16 +689,19 @@ def test_graph_blocks_reused_across_prepared_batches(seiswg, monkeypatch):
     assert np.array_equal(d.field(seiswg.abi.F_VX, 1).view(np.uint32), got.view(np.uint32))
 
 
-@pytest.mark.parametrize("phys,k", [("sh", 2), ("sh", 3), ("psv", 2), ("psv", 3), ("spin", 2)])
+@pytest.mark.parametrize("phys,k", [("sh", 2), ("sh", 3), ("sh", 4), ("psv", 2), ("psv", 3), ("psv", 4),
+                                    ("spin", 2), ("spin", 3)])
 @pytest.mark.parametrize("nx,nz", [(45, 131), (30, 40), (61, 150)])
 def test_lds_tile_multi_step_launches_match_oracle_bitwise(seiswg, phys, k, nx, nz):
     """K steps per launch on LDS tiles (small grids; sw_set_tiling2 with 1024
-    threads and K = 2 or 3): three shots from random states, sources and
-    receivers on edges and duplicated, 7 steps (pairs of two-step launches
-    or three-step launches, then single steps) - every state field and trace
-    bit for bit against the FP32 oracle."""
+    threads and K = 2..4): three shots from random states, sources and
+    receivers on edges and duplicated, 11 steps launched directly (launch
+    units - pairs of even-K launches, single odd-K launches - then single
+    steps), then 29 through the captured blocks (16 steps for K = 2 and 4, 12
+    for K = 3, replayed back to back) - every state field and trace bit for
+    bit against the FP32 oracle."""
     sh, psv, spin = phys == "sh", phys in ("psv", "spin"), phys == "spin"
-    nsteps = 7
+    nsteps = 40
     p, stf = random_problem(nx, nz, nsteps + 1, sh, psv, spin, seed=31)
     print(f"case phys={phys} K={k} grid={nx}x{nz}", flush=True)
     c = gpu_context(p, stf, seiswg)
@@ -714,7 +717,10 @@ def test_lds_tile_multi_step_launches_match_oracle_bitwise(seiswg, phys, k, nx, 
         for n, v in st.items():
             c.write_field(getattr(seiswg.abi, "F_" + n), v, shot=s)
         init.append(st)
-    c.advance(nsteps)
+    c.advance(11)
+    c.advance(nsteps - 11)
+    gst, ncap, _ = c.graph_state()
+    assert gst == 1 and ncap >= 1, (gst, ncap)
     comps = ([O.OBS_Y] if sh or spin else []) + ([O.OBS_X, O.OBS_Z] if psv else []) + \
             ([O.OBS_YI, O.OBS_YC] if spin else [])
     o = O.Oracle(p, 4, stf=stf.astype(np.float64))

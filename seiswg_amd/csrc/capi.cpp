@@ -172,6 +172,8 @@ struct sw_ctx {
   hipStream_t stream = nullptr;
   int64_t npt = 0;
   float h8x = 0, h8z = 0;
+  // divisions by 8 dx, 8 dz in the step kernels (two-operation form, see const_div)
+  struct ConstDiv { bool exact = false; float zh = 0, zl = 0; uint32_t lo_bits = 0; } cdx, cdz;
   bool model_set = false;
   // every rho / J in [2^-20, 2^20]: divisions by them may use the reciprocal form
   bool rho_fast = false, j_fast = false;
@@ -318,6 +320,52 @@ void build_taper(sw_ctx* c) {
 // Default tile shape: the widest strip and longest segment that still give
 // the GPU's 256 CUs several workgroups each (>= 1024 workgroups).  Long
 // segments amortise each workgroup's start-up (list lookups, first rows).
+// Division by the constant y = 8h in the step kernels: fma(x, zh, x * zl)
+// with 1/y split as zh + zl, zh = RN(1/y) and zl = RN(1/y - zh).  zl must
+// not be negative: x = -0 would give (-0) + (+0) = +0 instead of -0, so
+// where RN(1/y) lies above 1/y the split takes zh = RD(1/y) instead.  The
+// kernels use the form only for a divisor it is exact for: checked here on
+// every significand x in [1, 2) against the IEEE quotient (about 30 ms per
+// divisor, cached; both splits fail for about 2 % of divisors, which then
+// take the IEEE division), which carries over to x 2^k while x zl, x zh and
+// x / y stay normal, so numerators of magnitude below the threshold T
+// (>= 2^-100, one binade above where those products leave the normal range)
+// take the IEEE division.
+sw_ctx::ConstDiv const_div(float y) {
+  static std::mutex mu;
+  static std::map<uint32_t, sw_ctx::ConstDiv> cache;
+  uint32_t key;
+  std::memcpy(&key, &y, 4);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  sw_ctx::ConstDiv d;
+  float zh = 1.0f / y;
+  if (double(zh) > 1.0 / double(y)) zh = std::nextafter(zh, 0.0f);
+  d.zh = zh;
+  d.zl = float(1.0 / double(y) - double(zh));
+  d.exact = fast_divisor(y) && d.zl >= 0.0f;
+  for (uint32_t m = 0; d.exact && m < (1u << 23); ++m) {
+    const uint32_t b = 0x3f800000u | m;
+    float x;
+    std::memcpy(&x, &b, 4);
+    if (std::fma(x, zh, x * d.zl) != x / y) d.exact = false;
+  }
+  double t = 0x1p-100;
+  if (d.zl != 0.0f) t = std::max(t, 0x1p-125 / std::fabs(double(d.zl)));
+  t = std::max(t, 0x1p-125 * double(y));
+  t = std::ldexp(1.0, int(std::ceil(std::log2(t))));
+  const float tf = float(t);
+  uint32_t tb;
+  std::memcpy(&tb, &tf, 4);
+  d.lo_bits = (tb << 1) - 1u;
+  std::lock_guard<std::mutex> lk(mu);
+  cache[key] = d;
+  return d;
+}
+
 bool all_fast_divisors(const float* v, size_t n) {
   for (size_t k = 0; k < n; ++k)
     if (!fast_divisor(v[k])) return false;
@@ -663,7 +711,10 @@ StepArgs make_args(sw_ctx* c, StepMode mode) {
   a.nstrips = c->tile.nstrips; a.nsegs = c->tile.nsegs; a.seglen = c->tile.seglen;
   a.dt = c->g.dt; a.h8x = c->h8x; a.h8z = c->h8z;
   a.r8x = 1.0f / c->h8x; a.r8z = 1.0f / c->h8z;
-  a.fast_div = ((fast_divisor(c->h8x) && fast_divisor(c->h8z)) ? FD_CONST : 0) |
+  a.z8x = c->cdx.zh; a.z8z = c->cdz.zh;
+  a.l8x = c->cdx.zl; a.l8z = c->cdz.zl;
+  a.c_lo = std::max(c->cdx.lo_bits, c->cdz.lo_bits);
+  a.fast_div = ((c->cdx.exact && c->cdz.exact) ? FD_CONST : 0) |
                (c->rho_fast ? FD_RHO : 0) | (c->j_fast ? FD_J : 0) | (c->recip_div ? FD_RCP : 0);
   a.inj_off = c->inj_off.p; a.inj = c->inj.p;
   a.val_x = c->val_x; a.val_y = c->val_y; a.val_z = c->val_z; a.nt_val = c->nt_val;
@@ -1139,6 +1190,8 @@ SW_API int sw_create(const sw_grid* g, int32_t device, sw_ctx** out) {
   c->npt = int64_t(g->nx) * g->nz;
   c->h8x = 8.0f * g->dx;
   c->h8z = 8.0f * g->dz;
+  c->cdx = const_div(c->h8x);
+  c->cdz = const_div(c->h8z);
   if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) c->ncu = 0;
   compute_slots(c);
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -36,11 +36,13 @@ struct Tiling {
 // MF_IRHO / MF_IJ hold the correctly rounded reciprocals 1/rho, 1/J.
 enum ModelField : int { MF_RHO = 0, MF_LAM, MF_MU, MF_NU, MF_J, MF_LAM_C, MF_MU_C, MF_NU_C, MF_IRHO, MF_IJ, MF_N };
 
-// Exact divisions by 8h (FD_CONST), rho (FD_RHO) and J (FD_J) may use the
+// Exact divisions by rho (FD_RHO) and J (FD_J) may use the
 // reciprocal-and-correction form when every such divisor lies in
-// [2^-20, 2^20]; otherwise the kernels divide with the IEEE sequence.
-// FD_RCP: the forward step kernels multiply by the correctly rounded
-// reciprocal instead (x * RN(1/y), within 1.5 units of x / y).
+// [2^-20, 2^20]; divisions by 8h (FD_CONST) take the two-operation form
+// fma(x, zh, x * zl) with 1/(8h) split as zh + zl, once the host has shown
+// it exact for that divisor (bmr_exact).  Otherwise the kernels divide with
+// the IEEE sequence.  FD_RCP: the forward step kernels multiply by the
+// correctly rounded reciprocal instead (x * RN(1/y), within 1.5 units of x / y).
 enum FastDiv : int { FD_CONST = 1, FD_RHO = 2, FD_J = 4, FD_RCP = 8 };
 inline bool fast_divisor(float y) { return y >= 0x1p-20f && y <= 0x1p20f; }
 
@@ -58,6 +60,9 @@ struct StepArgs {
   int nstrips, nsegs, seglen;
   float dt, h8x, h8z;         // h8 = 8*h (fd2d.wgsl:156)
   float r8x, r8z;             // correctly rounded 1/h8
+  float z8x, z8z, l8x, l8z;   // 1/h8 split as zh + zl for FD_CONST (zl >= 0)
+  unsigned c_lo;              // FD_CONST: numerators x of magnitude below T take the
+                              // IEEE division; c_lo = (bits(T) << 1) - 1
   int fast_div;               // FastDiv bits
   // body-force injections: CSR over (shot, tile), entries (i, j, q) sorted by i
   const int* inj_off; const int4* inj;

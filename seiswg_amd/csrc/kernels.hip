@@ -65,18 +65,56 @@ __device__ __forceinline__ float nplus(float fm1, float f0, float fp1, float fp2
 // as a select of a value computed on every lane rather than a branch around it
 __device__ __forceinline__ float keep(bool active, float v) { return active ? v : 0.0f; }
 
-// A group of divisions q[k] = x[k] / d[k].y sharing one range test: the
-// reciprocal form for all of them, and the IEEE division for the lanes where
-// any numerator is outside {0} U [2^-100, 2^100] or a divisor is outside the
-// fast range.  The test runs on the integer k = bits(x) << 1 (sign dropped):
-// min(k - 1) >= bits(2^-100) << 1 - 1 (zero wraps to the maximum) and
-// max(k) <= bits(2^100) << 1, accumulated in registers, so a group costs one
-// branch instead of one per division.
-// mode (from div_mode): 1 reciprocal-and-correction with the range test,
-// 0 IEEE division, 2 x * RN(1/y) (FD_RCP)
+// Division by a constant y = 8h (Brisebarre, Muller and Raina, "Accelerating
+// correctly rounded floating-point division when the divisor is known in
+// advance", IEEE Trans. Computers 53(8), 2004): with 1/y split as zh + zl
+// (zh = RN(1/y), or RD(1/y) where that keeps zl = RN(1/y - zh) from being
+// negative, so that x = -0 gives -0), q = RN(x zh + RN(x zl)) - one
+// multiplication and one FMA.  For most divisors this equals RN(x / y) for
+// every x; the host checks the divisor at hand on every significand of one
+// binade (const_div), which covers every x whose products x zh, x zl and
+// quotient stay normal (scaling by a power of two commutes with rounding
+// there), and passes in `lo` the smallest numerator magnitude T for which
+// that holds.  r = RN(1/y) serves the reciprocal-and-correction form
+// (dplus) and FD_RCP.
+struct CDiv {
+  float y, r, zh, zl;
+  uint32_t lo;     // (bits(T) << 1) - 1
+  bool fast;       // y in [2^-20, 2^20] and the two-operation form exact for it
+  __device__ operator Div() const { return Div{y, r, fast}; }
+};
+
+// Range test of a group of numerators: true when every x is zero or
+// T <= |x| <= 2^100.  (bits(x) << 1) - 1 drops the sign and wraps zero to
+// the maximum, so the minimum over the group is >= (bits(T) << 1) - 1
+// exactly when no nonzero |x| lies below T (a shift-add per numerator and a
+// three-input minimum per two); the maximum of |x| (absolute values are
+// free operand modifiers, a three-input maximum per two) bounds the other
+// end and catches infinities.  NaN numerators give NaN on either path.
+template <int N>
+__device__ __forceinline__ bool div_in_range(const float (&x)[N], uint32_t lo_bits) {
+  uint32_t lo = 0xffffffffu;
+  float hi = 0.0f;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    lo = min(lo, (__builtin_bit_cast(uint32_t, x[k]) << 1) - 1u);
+    hi = __builtin_fmaxf(hi, __builtin_fabsf(x[k]));
+  }
+  return lo >= lo_bits && hi <= 0x1p100f;
+}
+// the threshold of the reciprocal-and-correction form: 2^-100
+constexpr uint32_t kMarksteinLo = ((127u - 100u) << 24) - 1u;
+
+// A group of divisions q[k] = x[k] / d[k].y sharing one range test: the fast
+// form for all of them, and the IEEE division for the lanes where a
+// numerator is outside the range or a divisor outside the fast set, so a
+// group costs one branch instead of one per division.
+// mode (from div_mode): 1 fast form with the range test, 0 IEEE division,
+// 2 x * RN(1/y) (FD_RCP)
 __device__ __forceinline__ int div_mode(int bits, int which) {
   return (bits & FD_RCP) ? 2 : ((bits & which) == which ? 1 : 0);
 }
+// divisors rho and J: reciprocal-and-correction
 template <int N>
 __device__ __forceinline__ void div_group(float (&q)[N], const float (&x)[N], const Div (&d)[N], int mode) {
   if (mode == 2) {
@@ -84,19 +122,28 @@ __device__ __forceinline__ void div_group(float (&q)[N], const float (&x)[N], co
     for (int k = 0; k < N; ++k) q[k] = x[k] * d[k].r;
     return;
   }
-  const bool fast = mode == 1;
-  constexpr uint32_t kLo = (127u - 100u) << 24, kHi = (127u + 100u) << 24;
-  uint32_t lo = 0xffffffffu, hi = 0u;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const float q0 = x[k] * d[k].r;
     const float e = __builtin_fmaf(q0, d[k].y, -x[k]);
     q[k] = __builtin_fmaf(-e, d[k].r, q0);
-    const uint32_t b = __builtin_bit_cast(uint32_t, x[k]) << 1;
-    lo = min(lo, b - 1u);
-    hi = max(hi, b);
   }
-  if (__builtin_expect(!(fast && lo >= kLo - 1u && hi <= kHi), 0)) {
+  if (__builtin_expect(!(mode == 1 && div_in_range(x, kMarksteinLo)), 0)) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) q[k] = x[k] / d[k].y;
+  }
+}
+// divisors 8 dx, 8 dz: the two-operation form
+template <int N>
+__device__ __forceinline__ void div_group(float (&q)[N], const float (&x)[N], const CDiv (&d)[N], int mode) {
+  if (mode == 2) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) q[k] = x[k] * d[k].r;
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) q[k] = __builtin_fmaf(x[k], d[k].zh, x[k] * d[k].zl);
+  if (__builtin_expect(!(mode == 1 && div_in_range(x, d[0].lo)), 0)) {
 #pragma unroll
     for (int k = 0; k < N; ++k) q[k] = x[k] / d[k].y;
   }
@@ -243,7 +290,8 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
-  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const CDiv Dx{a.h8x, a.r8x, a.z8x, a.l8x, a.c_lo, (a.fast_div & FD_CONST) != 0},
+             Dz{a.h8z, a.r8z, a.z8z, a.l8z, a.c_lo, (a.fast_div & FD_CONST) != 0};
   const int fast_c = div_mode(a.fast_div, FD_CONST);
   const int fast_rho = div_mode(a.fast_div, FD_RHO);
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
@@ -463,7 +511,8 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
-  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const CDiv Dx{a.h8x, a.r8x, a.z8x, a.l8x, a.c_lo, (a.fast_div & FD_CONST) != 0},
+             Dz{a.h8z, a.r8z, a.z8z, a.l8z, a.c_lo, (a.fast_div & FD_CONST) != 0};
   const int fast_c = div_mode(a.fast_div, FD_CONST);
   const int fast_rho = div_mode(a.fast_div, FD_RHO), fast_rj = div_mode(a.fast_div, FD_RHO | FD_J);
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
@@ -848,7 +897,8 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
-  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const CDiv Dx{a.h8x, a.r8x, a.z8x, a.l8x, a.c_lo, (a.fast_div & FD_CONST) != 0},
+             Dz{a.h8z, a.r8z, a.z8z, a.l8z, a.c_lo, (a.fast_div & FD_CONST) != 0};
   const int fast_c = div_mode(a.fast_div, FD_CONST);
   const int fast_rho = div_mode(a.fast_div, FD_RHO);
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
@@ -1196,7 +1246,8 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
   const int jc = int(jc4 >> 2);
   const float bzb = a.bzb[jc], bzt = a.bzt[jc];
   const float dt = a.dt;
-  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const CDiv Dx{a.h8x, a.r8x, a.z8x, a.l8x, a.c_lo, (a.fast_div & FD_CONST) != 0},
+             Dz{a.h8z, a.r8z, a.z8z, a.l8z, a.c_lo, (a.fast_div & FD_CONST) != 0};
   const int fast_c = div_mode(a.fast_div, FD_CONST);
   const int fast_rho = div_mode(a.fast_div, FD_RHO);
   const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
@@ -1501,7 +1552,8 @@ __device__ __forceinline__ void tile2(const StepArgs& a, int logical) {
   float* __restrict__ s_out = a.st[1] + int64_t(shot) * a.shot_floats;
   const int npt = a.npt;
   const float dt = a.dt;
-  const Div Dx{a.h8x, a.r8x, (a.fast_div & FD_CONST) != 0}, Dz{a.h8z, a.r8z, (a.fast_div & FD_CONST) != 0};
+  const CDiv Dx{a.h8x, a.r8x, a.z8x, a.l8x, a.c_lo, (a.fast_div & FD_CONST) != 0},
+             Dz{a.h8z, a.r8z, a.z8z, a.l8z, a.c_lo, (a.fast_div & FD_CONST) != 0};
   const int fast_c = div_mode(a.fast_div, FD_CONST);
   const int fast_rho = div_mode(a.fast_div, FD_RHO), fast_rj = div_mode(a.fast_div, FD_RHO | FD_J);
   const int jc = min(max(j, 0), nz - 1);

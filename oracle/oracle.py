@@ -323,6 +323,66 @@ class Oracle:
         self.L.or_smooth(self.h, sigma)
 
 
+def smooth_array(o: Oracle, v, sigma: float) -> np.ndarray:
+    """Gaussian smoothing (solver.rs:1249-1274) of an arbitrary field through
+    the oracle's K_mu slot; the slot's previous contents are restored."""
+    saved = o.field(F["K_MU"])
+    o.set_field(F["K_MU"], v)
+    o.smooth(sigma)
+    out = o.field(F["K_MU"])
+    o.set_field(F["K_MU"], saved)
+    return out
+
+
+def velocity_gradient(model: dict, k_mu, k_lam, psv: bool):
+    """Gradient of the misfit with respect to (vs, vp) from the one with
+    respect to (mu, lambda), through vps2lm (fd2d.wgsl:197-210):
+    mu = rho vs^2 and lambda = rho (vp^2 - 2 vs^2) where vp > vs, else 0, so
+        dJ/dvs = 2 rho vs K_mu - 4 rho vs K_lambda [vp > vs]
+        dJ/dvp = 2 rho vp K_lambda [vp > vs]
+    (SH: K_lambda = 0).  The chain rule of the inversion workflow."""
+    rho, vp, vs = (np.asarray(model[k], np.float64) for k in ("rho", "vp", "vs"))
+    k_mu = np.asarray(k_mu, np.float64)
+    k_lam = np.asarray(k_lam, np.float64) if psv else np.zeros_like(k_mu)
+    on = vp > vs
+    gvs = 2.0 * rho * vs * k_mu + np.where(on, -4.0 * rho * vs * k_lam, 0.0)
+    gvp = np.where(on, 2.0 * rho * vp * k_lam, 0.0)
+    return gvs, gvp
+
+
+def steepest_descent_iteration(p: Problem, runs, obs, step: float, tries: int, sigma: float):
+    """One iteration of the inversion workflow (mode = inversion; the
+    reference lists it as planned, README.md:61, without a specification, so
+    the rules are this build's own, restated here in FP64):
+    gradient = the exact adjoint one (or_gradient) with K_mu and K_lambda
+    Gaussian-smoothed (sigma cells), taken to (vs, vp) by velocity_gradient;
+    trial t (t = 0 .. tries-1) uses s = step / 2^t and moves
+    vs <- max(vs - s max(vs) / max|g_vs| g_vs, 0) and likewise vp (P-SV only,
+    rho fixed); the first trial whose misfit is below the current one is
+    accepted.  Returns (J0, vs1, vp1, J1, accepted trial) or (J0, None, None,
+    None, None) when no trial lowers the misfit."""
+    o = Oracle(p, 8)
+    J0, _, km, kl = o.gradient(runs, obs)
+    km = smooth_array(o, km, sigma)
+    kl = smooth_array(o, kl, sigma) if p.psv else np.zeros_like(km)
+    gvs, gvp = velocity_gradient(p.model, km, kl, p.psv)
+    vs = np.asarray(p.model["vs"], np.float64)
+    vp = np.asarray(p.model["vp"], np.float64)
+    gs, gp = np.abs(gvs).max(), np.abs(gvp).max()
+    for t in range(tries):
+        s = step / 2.0 ** t
+        a_s = s * vs.max() / gs if gs > 0 else 0.0
+        a_p = s * vp.max() / gp if (p.psv and gp > 0) else 0.0
+        vs1 = np.maximum(vs - a_s * gvs, 0.0)
+        vp1 = np.maximum(vp - a_p * gvp, 0.0) if p.psv else vp
+        m1 = dict(p.model, vs=vs1.astype(np.float32), vp=vp1.astype(np.float32))
+        p1 = Problem(**{**p.__dict__, "model": m1})
+        J1 = Oracle(p1, 8).objective(runs, obs)
+        if J1 < J0:
+            return J0, vs1, vp1, J1, t
+    return J0, None, None, None, None
+
+
 def max_rel_err(got, ref) -> float:
     """Peak-normalised max error (tests/integration.rs:153-170)."""
     got = np.asarray(got, np.float64).ravel()

@@ -80,15 +80,6 @@ def write_model_dir(d, nx, nz, h, model):
             np.asarray(v, "<f4").tofile(f)
 
 
-def write_npy(path, a):
-    a = np.ascontiguousarray(a, "<f4")
-    hdr = "{'descr': '<f4', 'fortran_order': False, 'shape': (%d,), }" % a.size
-    hdr = hdr + " " * (63 - len(hdr) % 64) + "\n"
-    with open(path, "wb") as f:
-        f.write(b"\x93NUMPY\x01\x00" + len(hdr).to_bytes(2, "little") + hdr.encode())
-        a.tofile(f)
-
-
 @pytest.mark.gpu
 def test_cli_inversion_iteration_matches_fp64_restatement(seiswg, tmp_path):
     """A P-SV inversion on a 64 x 48 grid (three sources, eight receivers,

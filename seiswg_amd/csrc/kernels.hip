@@ -2367,32 +2367,69 @@ hipError_t launch_multi_nt(int kind, const StepArgs* args, const int2* work, int
 // backwards, A^n = sum_{k>=n} 2 w_k^2 (d^k - obs^k) (double accumulation),
 // writes dt * A^n into adj[comp][shot*nrec + r][n]; per-shot misfit partials
 // are reduced in a fixed order by a second kernel.
-__global__ void k_adjsrc(const float* __restrict__ obs_syn, const float* __restrict__ obs_ref,
-                         const float* __restrict__ taper, float* __restrict__ adj, double* __restrict__ part,
-                         int nshots, int ncomp, const int* __restrict__ comps, int nrec, int nt, float dt) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+// Wavefront-level sums (64 lanes, fixed order: lane pairs at distance 32, 16,
+// ..., 1, so results do not depend on timing).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;   // lane 0 holds the sum
+}
+// Inclusive suffix sum over the lanes: lane l gets v_l + v_{l+1} + ... + v_63.
+__device__ __forceinline__ double wave_suffix_sum(double v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double y = __shfl_down(v, off, 64);
+    if (lane + off < 64) v += y;
+  }
+  return v;
+}
+
+// Misfit and adjoint source of the gradient (objective J = sum (w (syn -
+// obs))^2, w the cosine taper of compute_waveform_misfit, solver.rs:441-455):
+// one wavefront per trace (shot, component, receiver).  The adjoint source
+// at step n is dt * sum_{m >= n} 2 w_m^2 (syn_m - obs_m) (the adjoint of the
+// displacement accumulation u^{n+1} = u^n + dt v): the trace is walked in
+// 64-sample tiles from its end, each tile's suffix sums formed by a
+// wavefront scan and carried into the tile before it, so loads and stores
+// are coalesced; the squared misfit is summed per lane and reduced across
+// the wavefront at the end (part[trace]).
+__global__ __launch_bounds__(256) void k_adjsrc(const float* __restrict__ obs_syn, const float* __restrict__ obs_ref,
+                                                const float* __restrict__ taper, float* __restrict__ adj,
+                                                double* __restrict__ part, int nshots, int ncomp,
+                                                const int* __restrict__ comps, int nrec, int nt, float dt) {
+  const int lane = int(threadIdx.x) & 63;
+  const int g = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * 4 + int(threadIdx.x >> 6));
   const int total = nshots * ncomp * nrec;
   if (g >= total) return;
-  const int r = g % nrec, c = (g / nrec) % ncomp, s = g / (nrec * ncomp);
-  const float* syn = obs_syn + ((int64_t(s) * 5 + comps[c]) * nrec + r) * nt;
-  const float* ref = obs_ref + ((int64_t(s) * ncomp + c) * nrec + r) * nt;
-  float* out = adj + (int64_t(c) * nshots * nrec + int64_t(s) * nrec + r) * nt;
-  double acc = 0.0, mis = 0.0;
-  for (int n = nt - 1; n >= 0; --n) {
-    const double w = taper[n];
-    const double d = double(syn[n]) - double(ref[n]);
-    acc += 2.0 * w * w * d;
-    mis += (w * d) * (w * d);
-    out[n] = float(double(dt) * acc);
+  const int r = g % nrec, c = (g / nrec) % ncomp, sh = g / (nrec * ncomp);
+  const float* syn = obs_syn + ((int64_t(sh) * 5 + comps[c]) * nrec + r) * nt;
+  const float* ref = obs_ref + ((int64_t(sh) * ncomp + c) * nrec + r) * nt;
+  float* out = adj + (int64_t(c) * nshots * nrec + int64_t(sh) * nrec + r) * nt;
+  double carry = 0.0, mis = 0.0;
+  for (int t0 = ((nt - 1) / 64) * 64; t0 >= 0; t0 -= 64) {
+    const int n = t0 + lane;
+    double v = 0.0;
+    if (n < nt) {
+      const double w = taper[n];
+      const double d = double(syn[n]) - double(ref[n]);
+      v = 2.0 * w * w * d;
+      mis += (w * d) * (w * d);
+    }
+    const double suf = wave_suffix_sum(v, lane);
+    if (n < nt) out[n] = float(double(dt) * (carry + suf));
+    carry += __shfl(suf, 0, 64);
   }
-  part[g] = mis;
+  mis = wave_sum(mis);
+  if (lane == 0) part[g] = mis;
 }
-__global__ void k_misfit_shots(const double* __restrict__ part, double* __restrict__ mis, int nshots, int per) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nshots) return;
+// per-shot misfit: one wavefront per shot sums its traces' partials
+__global__ __launch_bounds__(64) void k_misfit_shots(const double* __restrict__ part, double* __restrict__ mis,
+                                                     int nshots, int per) {
+  const int sh = blockIdx.x, lane = threadIdx.x;
   double acc = 0.0;
-  for (int q = 0; q < per; ++q) acc += part[int64_t(s) * per + q];
-  mis[s] = acc;
+  for (int q = lane; q < per; q += 64) acc += part[int64_t(sh) * per + q];
+  acc = wave_sum(acc);
+  if (lane == 0) mis[sh] = acc;
 }
 __global__ void k_reduce_shots(const float* __restrict__ kshot, float* __restrict__ kout, int nshots, int64_t n,
                                int64_t shot_stride) {
@@ -2465,7 +2502,7 @@ __global__ void k_gauss_z(const float* __restrict__ w, int nw, const float* __re
 __global__ __launch_bounds__(256) void k_misfit(const float* __restrict__ syn, const float* __restrict__ ref,
                                                 const float* __restrict__ taper, float* __restrict__ adstf,
                                                 double* __restrict__ partial, int nrec, int nt) {
-  __shared__ double red[256];
+  __shared__ double red[4];
   const int64_t n = int64_t(nrec) * nt;
   const int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x;
   double acc = 0.0;
@@ -2476,25 +2513,20 @@ __global__ __launch_bounds__(256) void k_misfit(const float* __restrict__ syn, c
     acc = double(diff) * double(diff);
     adstf[int64_t(r) * nt + (nt - 1 - it)] = diff * tp * 2.0f;
   }
-  red[threadIdx.x] = acc;
+  // wavefront sums, then the block's four in order
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (int(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 __global__ __launch_bounds__(256) void k_reduce_sum(const double* __restrict__ partial, int n, double* out) {
-  __shared__ double red[256];
+  __shared__ double red[4];
   double acc = 0.0;
   for (int q = threadIdx.x; q < n; q += 256) acc += partial[q];
-  red[threadIdx.x] = acc;
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (int(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = red[0];
+  if (threadIdx.x == 0) *out = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 // vps2lm (fd2d.wgsl:188-201)
@@ -2781,9 +2813,10 @@ hipError_t launch_adjsrc(const float* obs_syn, const float* obs_ref, const float
   if (total > 0) {
     // partials live after the per-shot misfits in the same buffer
     double* part = misfit + nshots;
-    hipLaunchKernelGGL(k_adjsrc, dim3((total + 127) / 128), dim3(128), 0, s, obs_syn, obs_ref, taper, adj, part,
+    // one wavefront per trace, four per workgroup; one wavefront per shot
+    hipLaunchKernelGGL(k_adjsrc, dim3((total + 3) / 4), dim3(256), 0, s, obs_syn, obs_ref, taper, adj, part,
                        nshots, ncomp, comps, nrec, nt, dt);
-    hipLaunchKernelGGL(k_misfit_shots, dim3((nshots + 63) / 64), dim3(64), 0, s, part, misfit, nshots, ncomp * nrec);
+    hipLaunchKernelGGL(k_misfit_shots, dim3(nshots), dim3(64), 0, s, part, misfit, nshots, ncomp * nrec);
   } else if (nshots > 0) {
     // no receivers: no traces, a zero misfit per shot (and no adjoint source)
     hipLaunchKernelGGL(k_fill_f64, dim3((nshots + 63) / 64), dim3(64), 0, s, misfit, 0.0, int64_t(nshots));

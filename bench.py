@@ -378,7 +378,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="measure with the heuristic tile shapes instead of autotuned ones")
     ap.add_argument("--steps-per-launch", type=int, default=0,
-                    help="P-SV forward steps per launch (1 or 2); 0 = library default (2)")
+                    help="P-SV forward steps per launch (1, 2 or 3); 0 = library default (2)")
     ap.add_argument("--reciprocal-division", action="store_true",
                     help="forward-step divisions as multiplications by the rounded reciprocal (not bit-exact)")
     ap.add_argument("--no-reciprocal-line", action="store_true",

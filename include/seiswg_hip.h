@@ -245,11 +245,13 @@ SW_API int sw_get_tiling(const sw_ctx* ctx, int32_t* threads, int32_t* seglen, i
  * every half-step separately): 2 (the default) runs the steps of SH or P-SV
  * alone, with or without spin, in launches that each advance several steps,
  * reading the state once and writing it once per launch (the shape set by
- * sw_set_tiling2); 1 runs one step per launch.  Results are the same bit for
- * bit.  SH and P-SV together, and the gradient's adjoint, always run one
- * step per launch; its recompute runs two-step marching launches (one step
- * per launch where the batch has LDS tiles).  A change takes effect at the
- * next prepared batch. */
+ * sw_set_tiling2); 3 runs P-SV alone (no spin) in marching launches of three
+ * steps each (tiles of 256, 384 or 512 threads, one shot per workgroup; SH
+ * keeps two); 1 runs one step per launch.  Results are the same bit for bit.
+ * SH and P-SV together, and the gradient's adjoint, always run one step per
+ * launch; its recompute runs two-step marching launches (one step per launch
+ * where the batch has LDS tiles or three-step tiles).  A change takes effect
+ * at the next prepared batch. */
 SW_API int sw_set_steps_per_launch(sw_ctx* ctx, int32_t steps);
 /* Tile shape of the multi-step launches (no counterpart in the reference):
  *  - marching tiles, two steps per launch: threads per shot 64, 128, 256,

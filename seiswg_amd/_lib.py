@@ -306,8 +306,9 @@ class Context:
         _check(self.L.sw_set_tiling(self.h, threads, seglen, shots_per_block))
 
     def set_steps_per_launch(self, steps: int):
-        """P-SV forward steps per launch (2: pairs of two-step launches, the
-        default; 1: one step per launch).  Same results either way."""
+        """Forward steps per launch (2: launches of two steps, the default;
+        3: P-SV alone in launches of three steps; 1: one step per launch).
+        Same results bit for bit."""
         _check(self.L.sw_set_steps_per_launch(self.h, steps))
 
     def set_tiling2(self, threads: int, seglen: int, shots_per_block: int = 0):

@@ -389,8 +389,14 @@ int effective_group2(int nt, int requested, int nshots) {
 // multi-step tiles: LDS tiles (K = lds_steps steps per launch, halo 3K) or
 // two-step marching tiles (halo 6); columns of a strip and of its region
 bool tile2_lds(const Tiling& t) { return t.nt_threads == kLdsTileThreads; }
-int tile2_halo(const Tiling& t) { return tile2_lds(t) ? 3 * t.lds_steps : 6; }
-int tile2_strip_cols(const Tiling& t) { return tile2_lds(t) ? lds_tile_cols(t.lds_steps) : t.nt_threads - 12; }
+int tile2_halo(const Tiling& t) { return tile2_lds(t) ? 3 * t.lds_steps : 3 * t.msteps; }
+int tile2_strip_cols(const Tiling& t) {
+  return tile2_lds(t) ? lds_tile_cols(t.lds_steps) : t.nt_threads - 2 * tile2_halo(t);
+}
+// three steps per marching launch: requested (sw_set_steps_per_launch) and P-SV alone
+bool three_steps(const sw_ctx* c) {
+  return c->steps_per_launch == 3 && uses_psv(c) && !uses_sh(c) && !uses_spin(c);
+}
 int tile2_region_cols(const Tiling& t) { return tile2_strip_cols(t) + 2 * tile2_halo(t); }
 
 // Steps per LDS-tile launch: up to 4 (the region's 32 rows keep 8 after
@@ -437,6 +443,30 @@ void heuristic_tiling2(sw_ctx* c, int nshots) {
   }
   t.lds_steps = 0;
   const int64_t ncu = c->ncu > 0 ? c->ncu : 256;
+  if (three_steps(c)) {
+    // three steps per launch (9-column halos, one shot per workgroup): the
+    // width that wastes the fewest lanes on the last strip, segments that
+    // fill every CU's resident slots once
+    t.msteps = 3;
+    int best3 = 384;
+    double best_eff = -1.0;
+    for (int nt : {512, 384, 256}) {
+      const int64_t strips = (nz + nt - 18 - 1) / (nt - 18);
+      const double eff = double(nz) / (double(strips) * nt);
+      if (eff > best_eff + 1e-9) {
+        best_eff = eff;
+        best3 = nt;
+      }
+    }
+    const int64_t resident = std::max(step3_blocks_per_cu(best3), 1);
+    const int64_t base = int64_t(nshots) * ((nz + best3 - 18 - 1) / (best3 - 18));
+    const int64_t nsegs = std::max<int64_t>(1, (ncu * resident) / base);
+    t.nt_threads = best3;
+    t.seglen = std::min(int(std::max<int64_t>((c->g.nx + nsegs - 1) / nsegs, 16)), c->g.nx);
+    t.group = 1;
+    return;
+  }
+  t.msteps = 2;
   int best = 64;
   double best_score = -1.0;
   for (int nt : {512, 256, 128, 64}) {
@@ -517,12 +547,16 @@ int check_tiling(sw_ctx* c, int nshots) {
   // else that of the one-step launches
   // two-step launches: marching tiles for SH or P-SV alone, LDS tiles also
   // for P-SV + spin
-  c->two_ok = c->steps_per_launch == 2 && uses_psv(c) != uses_sh(c);
+  c->two_ok = c->steps_per_launch >= 2 && uses_psv(c) != uses_sh(c);
   if (c->two_ok) {
     Tiling& t2 = c->tile2;
     if (!c->tile2_fixed && c->tile_fixed) {
-      t2 = t;
-      c->group2_req = c->group_req;
+      if (three_steps(c)) {
+        heuristic_tiling2(c, nshots);   // the one-step widths are not three-step ones
+      } else {
+        t2 = t;
+        c->group2_req = c->group_req;
+      }
     }
     if (tile2_lds(t2)) {
       if (t2.lds_steps < 2 || t2.lds_steps > lds_max_steps(c)) t2.lds_steps = lds_default_steps(c);
@@ -530,6 +564,16 @@ int check_tiling(sw_ctx* c, int nshots) {
     } else {
       t2.lds_steps = 0;
       if (uses_spin(c)) c->two_ok = false;
+      t2.msteps = three_steps(c) ? 3 : 2;
+      if (t2.msteps == 3) {
+        if (!tiling3_supported(t2.nt_threads))
+          return fail(-2, "three steps per launch take tiles of 256, 384 or 512 threads (got %d)", t2.nt_threads);
+        t2.group = 1;
+        c->group2_req = 1;
+      } else if (!tiling2_supported(t2.nt_threads)) {
+        return fail(-2, "two steps per launch take tiles of 64, 128, 256, 320 or 512 threads (got %d)",
+                    t2.nt_threads);
+      }
     }
   }
   if (c->two_ok) {
@@ -595,7 +639,7 @@ int build_tile_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj,
   // rows whose new velocities a tile computes: one-step marching rows
   // [i0-1, i1+2), two-step marching [i0-4, i1+5), LDS tiles their region
   const bool lds = two2 && tile2_lds(tl);
-  const int before = lds ? halo : two2 ? 4 : 1, after = lds ? halo : two2 ? 5 : 2;
+  const int before = lds ? halo : two2 ? 3 * tl.msteps - 2 : 1, after = lds ? halo : two2 ? 3 * tl.msteps - 1 : 2;
   std::vector<int> off(size_t(c->nshots) * ntile + 1, 0);
   std::vector<int4> data;
   for (int s = 0; s < c->nshots; ++s) {
@@ -773,8 +817,9 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   // (2K steps).  LDS tiles have no variant that stores the gradient's
   // velocities.
   const int ldsk = tile2_lds(c->tile2) ? c->tile2.lds_steps : 0;
-  const bool multi_ok = (mode == M_FWD || (mode == M_FWD_AUX && !ldsk)) && c->two_ok && c->steps_per_launch == 2;
-  const int K = ldsk ? ldsk : 2;                 // steps per multi-step launch
+  const bool multi_ok = (mode == M_FWD || (mode == M_FWD_AUX && !ldsk && c->tile2.msteps == 2)) && c->two_ok &&
+                        c->steps_per_launch >= 2;
+  const int K = ldsk ? ldsk : c->tile2.msteps;   // steps per multi-step launch
   const bool odd = (K & 1) != 0;
   const int U = odd ? K : 2 * K;                 // steps per launch unit
   const bool multi = multi_ok && nsteps >= U;
@@ -1414,7 +1459,7 @@ SW_API int sw_write_field(sw_ctx* c, int32_t shot, int32_t field, const float* i
 }
 
 SW_API int sw_set_steps_per_launch(sw_ctx* c, int32_t steps) {
-  if (!c || (steps != 1 && steps != 2)) return fail(-1, "steps per launch must be 1 or 2");
+  if (!c || steps < 1 || steps > 3) return fail(-1, "steps per launch must be 1, 2 or 3");
   c->steps_per_launch = steps;
   if (steps == 1) c->two_ok = false;
   return 0;
@@ -1443,7 +1488,8 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
   for (int s = 0; s < nshots; ++s) isrc[s] = c->nsrc > 0 ? s % c->nsrc : -1;
   const int spl = c->steps_per_launch;
   // two-step shapes: marching tiles for SH or P-SV alone, LDS tiles also with spin
-  const bool two = spl == 2 && uses_psv(c) != uses_sh(c);
+  const bool two = spl >= 2 && uses_psv(c) != uses_sh(c);
+  const bool three = three_steps(c);
   // whole launch units of every multi-step shape (pairs of two-step
   // launches; LDS tiles: 2K steps, so 24 steps for K = 2, 3, 4) when nt
   // leaves room, else whole pairs
@@ -1503,10 +1549,13 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
     double best2 = 1e300;
     Tiling bt2 = c->tile2;
     int bg2 = c->group2_req;
-    for (int nt : {64, 128, 256, 320, 512, kLdsTileThreads}) {
+    for (int nt : {64, 128, 256, 320, 384, 512, kLdsTileThreads}) {
       if (uses_spin(c) && nt != kLdsTileThreads) continue;
+      // three steps per launch: marching widths 256, 384 and 512, one shot
+      // per workgroup (LDS tiles as for two)
+      if (three ? (nt != kLdsTileThreads && !tiling3_supported(nt)) : (nt == 384)) continue;
       for (int grp : {1, 2, 4}) {
-        if (!group2_supported(nt, grp) || nshots % grp != 0) continue;
+        if (three && nt != kLdsTileThreads ? grp != 1 : (!group2_supported(nt, grp) || nshots % grp != 0)) continue;
         for (int seg : segs) {
           if (seg > 2 * c->g.nx) continue;
           // LDS tiles: the kernel fixes its tile; steps per launch 2..4 (the
@@ -1545,8 +1594,9 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
 
 SW_API int sw_set_tiling2(sw_ctx* c, int32_t threads, int32_t seglen, int32_t shots_per_block) {
   if (!c) return fail(-1, "null ctx");
-  if (!tiling2_supported(threads))
-    return fail(-2, "threads must be 64, 128, 256, 320 or 512 (marching tiles) or %d (LDS tiles)", kLdsTileThreads);
+  if (!tiling2_supported(threads) && !tiling3_supported(threads))
+    return fail(-2, "threads must be 64, 128, 256, 320, 384 or 512 (marching tiles) or %d (LDS tiles)",
+                kLdsTileThreads);
   // LDS tiles: `seglen` is the steps per launch (2..4, with spin 2..3; 0:
   // the default); the kernel fixes its tile
   int lds_steps = 0;
@@ -1557,7 +1607,7 @@ SW_API int sw_set_tiling2(sw_ctx* c, int32_t threads, int32_t seglen, int32_t sh
     seglen = lds_tile_rows(lds_steps);
   }
   if (seglen < 1) return fail(-2, "seglen must be >= 1");
-  if (shots_per_block != 0 && !group2_supported(threads, shots_per_block))
+  if (shots_per_block > 1 && !group2_supported(threads, shots_per_block))
     return fail(-2, "shots_per_block must be 0 (automatic), 1, or 2 or 4 with threads >= 128 and at most "
                     "1024 threads per workgroup");
   c->tile2.nt_threads = threads;

@@ -30,6 +30,7 @@ struct Tiling {
   int nstrips, nsegs;
   int group = 1;    // shots per workgroup of the forward steps (1, 2 or 4)
   int lds_steps = 0;   // multi-step launches: 0 marching tiles; 2..4 LDS tiles, steps per launch
+  int msteps = 2;      // marching multi-step tiles: steps per launch (2, or 3 for P-SV alone)
 };
 
 // Rows of the model buffer [MF_N][npt].
@@ -130,6 +131,8 @@ bool group_supported(int nt_threads, int group);
 bool tiling2_supported(int nt_threads);
 bool group2_supported(int nt_threads, int group);
 int step2_blocks_per_cu(int nt_threads, int group);
+bool tiling3_supported(int nt_threads);
+int step3_blocks_per_cu(int nt_threads);
 // resident LDS-tile workgroups per CU (K steps per launch)
 int lds_tile_blocks_per_cu(bool sh, bool spin, int k);
 // resident workgroups per CU of the step kernel of a MultiKind at a tile width

@@ -1218,6 +1218,440 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   if (a.advance) advance_clock(a.clock, it, 2, a.nblocks);
 }
 
+// ----------------------------------------- three P-SV steps per launch
+// Steps it, it+1 and it+2 of P-SV without spin in one march: the two stages
+// of step_psv2 and a third trailing the second by three rows, so neither the
+// state of step it+1 nor that of step it+2 goes to memory: a workgroup reads
+// the state of step it once and writes that of step it+3 once (a third of
+// the state traffic of three one-step launches).  The price is a 9-column
+// halo (W = NT-18) and 7 + 8 recomputed rows per segment.  Iteration m runs
+//   stage 1 (step it):   velocity row m | barrier | stress row m-2
+//   stage 2 (step it+1): velocity row m-3, D+z of velocity row m-4, stress row m-5
+//   stage 3 (step it+2): velocity row m-6, D+z of velocity row m-7, stress row m-8
+// Stage-2 and stage-3 z-neighbours go through LDS written after the barrier
+// of one iteration and read after the barrier of the next; model values and
+// taper factors ride along in register rings (stage 3 takes what stage 2
+// used three rows earlier).  Every cell's arithmetic is that of step_psv,
+// operation for operation, so the result equals three one-step launches bit
+// for bit.  Reads st[0], writes st[1] (the buffer of the other parity).  No
+// gradient variant: the recompute of the gradient runs two-step launches.
+template <int NT, int S>
+__device__ __forceinline__ void step_psv3(const StepArgs& a, int logical) {
+  __shared__ float lds_s[2][2][S * NT];    // stage 1: sxz, szz of step it
+  __shared__ float lds_v[2][2][S * NT];    // stage 1: vx, vz of step it+1
+  __shared__ float lds_s2[2][2][S * NT];   // stage 2: sxz, szz of step it+1
+  __shared__ float lds_v2[2][2][S * NT];   // stage 2: vx, vz of step it+2
+  __shared__ float lds_s3[2][2][S * NT];   // stage 3: sxz, szz of step it+2
+  __shared__ float lds_v3[2][2][S * NT];   // stage 3: vx, vz of step it+3
+  const TileGeom g = tile_geom<NT, S, 9>(a, logical);
+  const int t = g.t, lane = g.lane, j = g.j, nx = a.nx, i0 = g.i0, i1 = g.i1;
+  const int it = __hip_atomic_load(&a.clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int64_t so = int64_t(g.shot) * a.shot_floats;
+  const uint32_t sbytes = 4u * uint32_t(a.shot_floats);
+  const auto rin = make_rsrc(a.st[0] + so, sbytes);
+  const auto rout = make_rsrc(a.st[1] + so, sbytes);
+  const uint32_t npt4 = 4u * uint32_t(a.npt);
+  const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
+  // P-SV alone: slots vx, vz, sxx, szz, sxz in that order (the host checks)
+  const uint32_t o_vx = 0u, o_vz = npt4;
+  const uint32_t o_sxx = 2u * npt4, o_szz = 3u * npt4, o_sxz = 4u * npt4;
+  const uint32_t m_rho = MF_RHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4, m_irho = MF_IRHO * npt4;
+  const uint32_t jc4 = g.jc4;
+  const int jc = int(jc4 >> 2);
+  const float bzb = a.bzb[jc], bzt = a.bzt[jc];
+  const float dt = a.dt;
+  const CDiv Dx{a.h8x, a.r8x, a.z8x, a.l8x, a.c_lo, (a.fast_div & FD_CONST) != 0},
+             Dz{a.h8z, a.r8z, a.z8z, a.l8z, a.c_lo, (a.fast_div & FD_CONST) != 0};
+  const int fast_c = div_mode(a.fast_div, FD_CONST);
+  const int fast_rho = div_mode(a.fast_div, FD_RHO);
+  const int tm2 = g.base + max(t - 2, 0), tm1 = g.base + max(t - 1, 0), tp1 = g.base + min(t + 1, NT - 1),
+            tp2 = g.base + min(t + 2, NT - 1);
+  const int tps = a.nstrips * a.nsegs;
+  // one injection list per tile, walked by a cursor per stage
+  int ip = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile]);
+  const int ie = __builtin_amdgcn_readfirstlane(a.inj_off[g.shot * tps + g.tile + 1]);
+  int ip2 = ip, ip3 = ip;
+  int inj_row = next_row(a.inj, ip, ie);
+  int inj_row2 = inj_row, inj_row3 = inj_row;
+  int rp = 0, re = 0;
+  if (a.record) {
+    rp = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile]);
+    re = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile + 1]);
+  }
+  int rp2 = rp, rp3 = rp;
+  int rec_row = next_row(a.rec, rp, re);
+  int rec_row2 = rec_row, rec_row3 = rec_row;
+  float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
+  float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
+  const int nobs = a.nrec * a.nt;
+  const int m0 = i0 - 7;   // first velocity row of stage 1
+
+  // stage 1 rings: stresses of step it (rows m-2..m+1), new velocities
+  // (rows m-3..m), D+z of the new velocities (row m at slot s2, used at s0)
+  float sxx[4], sxz[4], szz[4];
+  float vx[4] = {0, 0, 0, 0}, vz[4] = {0, 0, 0, 0};
+  float dvxdz[4] = {0, 0, 0, 0}, dvzdz[4] = {0, 0, 0, 0};
+  // stage 2 rings: stresses of step it+1 (row m-2 at s0), velocities of step
+  // it+2 (row m-3 at s0), D+z of those velocities (row m-4 at [par], m-5 at
+  // [par^1]), model values / taper factor stage 1 used (written at s0, read at s1)
+  float t1xx[4] = {0, 0, 0, 0}, t1xz[4] = {0, 0, 0, 0}, t1zz[4] = {0, 0, 0, 0};
+  float u2x[4] = {0, 0, 0, 0}, u2z[4] = {0, 0, 0, 0};
+  float dz2x[2] = {0, 0}, dz2z[2] = {0, 0};
+  float drho[4] = {1, 1, 1, 1}, dirho[4] = {1, 1, 1, 1}, dlam[4] = {0, 0, 0, 0}, dmu[4] = {0, 0, 0, 0};
+  float dbx[4] = {0, 0, 0, 0};
+  // stage 3 rings: stresses of step it+2 (row m-5 at s0), velocities of step
+  // it+3 (row m-6 at s0), D+z of those (row m-7 at [par], m-8 at [par^1]),
+  // the model values / taper factor stage 2 used (written at s0, read at s1)
+  float t2xx[4] = {0, 0, 0, 0}, t2xz[4] = {0, 0, 0, 0}, t2zz[4] = {0, 0, 0, 0};
+  float u3x[4] = {0, 0, 0, 0}, u3z[4] = {0, 0, 0, 0};
+  float dz3x[2] = {0, 0}, dz3z[2] = {0, 0};
+  float erho[4] = {1, 1, 1, 1}, eirho[4] = {1, 1, 1, 1}, elam[4] = {0, 0, 0, 0}, emu[4] = {0, 0, 0, 0};
+  float ebx[4] = {0, 0, 0, 0};
+  for (int q = 0; q < 3; ++q) {
+    const uint32_t r = g.row(m0 - 2 + q);
+    sxx[q] = settle(bload(rin, jc4, o_sxx + r));
+    sxz[q] = settle(bload(rin, jc4, o_sxz + r));
+    szz[q] = settle(bload(rin, jc4, o_szz + r));
+  }
+  sxx[3] = sxz[3] = szz[3] = 0.0f;
+  // 1-deep prefetch: stress rows m+1, velocity / density rows m, moduli rows
+  // m-2 of iteration m
+  float pf_sxx, pf_sxz, pf_szz, pf_vx, pf_vz, pf_rho, pf_irho, pf_lam, pf_mu;
+  float bx_next = taper_x(a.bx, m0, nx);
+  // clamped row offsets, slot (row - m0) & 3
+  uint32_t rofs[4];
+  for (int k = -1; k <= 1; ++k) rofs[k & 3] = g.row(m0 + k);
+  {
+    const uint32_t r1 = g.row(m0 + 1), r0 = g.row(m0), rm2 = g.row(m0 - 2);
+    pf_sxx = bload(rin, jc4, o_sxx + r1);
+    pf_sxz = bload(rin, jc4, o_sxz + r1);
+    pf_szz = bload(rin, jc4, o_szz + r1);
+    pf_vx = bload(rin, jc4, o_vx + r0);
+    pf_vz = bload(rin, jc4, o_vz + r0);
+    pf_rho = bload(rmod, jc4, m_rho + r0);
+    pf_irho = bload(rmod, jc4, m_irho + r0);
+    pf_lam = bload(rmod, jc4, m_lam + rm2);
+    pf_mu = bload(rmod, jc4, m_mu + rm2);
+  }
+  lds_s[0][0][lane] = sxz[2];
+  lds_s[0][1][lane] = szz[2];
+  __syncthreads();
+
+  const bool zm_ok = g.zm_ok, zp_ok = g.zp_ok;
+  auto body = [&](auto phase, int m) __attribute__((always_inline)) {
+    constexpr int R = decltype(phase)::value;
+    constexpr int s0 = R & 3, s1 = (R + 1) & 3, s2 = (R + 2) & 3, s3 = (R + 3) & 3, par = R & 1;
+    sxx[s3] = settle(pf_sxx); sxz[s3] = settle(pf_sxz); szz[s3] = settle(pf_szz);     // row m+1
+    const float vx_o = pf_vx, vz_o = pf_vz, rho = pf_rho, irho = pf_irho, lam = pf_lam, mu = pf_mu;
+    const float bxm = bx_next;
+    bx_next = taper_x(a.bx, m + 1, nx);
+    {
+      constexpr int k1 = (R + 2) & 3, k0 = (R + 1) & 3, km2 = (R + 3) & 3;
+      rofs[k1] = g.row(m + 2);
+      const uint32_t r3 = rofs[k1], r2 = rofs[k0], r0 = rofs[km2];
+      pf_sxx = bload(rin, jc4, o_sxx + r3);
+      pf_sxz = bload(rin, jc4, o_sxz + r3);
+      pf_szz = bload(rin, jc4, o_szz + r3);
+      pf_vx = bload(rin, jc4, o_vx + r2);
+      pf_vz = bload(rin, jc4, o_vz + r2);
+      pf_rho = bload(rmod, jc4, m_rho + r2);
+      pf_irho = bload(rmod, jc4, m_irho + r2);
+      pf_lam = bload(rmod, jc4, m_lam + r0);
+      pf_mu = bload(rmod, jc4, m_mu + r0);
+    }
+
+    // ---- stage 1: velocity of step it+1 at row m (not stored)
+    {
+      const bool xm_ok = m >= 2 && m < nx - 2;
+      float qa[4];
+      {
+        const float n0 = nminus(sxx[s0], sxx[s1], sxx[s2], sxx[s3]);
+        const float n1 = nminus(lds_s[par][0][tm2], lds_s[par][0][tm1], sxz[s2], lds_s[par][0][tp1]);
+        const float n2 = nminus(sxz[s0], sxz[s1], sxz[s2], sxz[s3]);
+        const float n3 = nminus(lds_s[par][1][tm2], lds_s[par][1][tm1], szz[s2], lds_s[par][1][tp1]);
+        const float xa[4] = {keep(xm_ok, n0), keep(zm_ok, n1), keep(xm_ok, n2), keep(zm_ok, n3)};
+        div_group(qa, xa, {Dx, Dz, Dx, Dz}, fast_c);
+      }
+      float dsx = qa[0] + qa[1];
+      float dsz = qa[2] + qa[3];
+      if (m == inj_row) {
+        for (; ip < ie; ++ip) {
+          const int4 e = a.inj[ip];
+          if (e.x != m) break;
+          if (e.y == j) {
+            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it];
+            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it];
+          }
+        }
+        inj_row = next_row(a.inj, ip, ie);
+      }
+      const float b = (bxm * bzb) * bzt;
+      float qb[2];
+      const float xb[2] = {dt * dsx, dt * dsz};
+      div_group(qb, xb, {Div{rho, irho, true}, Div{rho, irho, true}}, fast_rho);
+      vx[s3] = b * (vx_o + qb[0]);
+      vz[s3] = b * (vz_o + qb[1]);
+    }
+    lds_v[par][0][lane] = vx[s3];
+    lds_v[par][1][lane] = vz[s3];
+    lds_s[par ^ 1][0][lane] = sxz[s3];
+    lds_s[par ^ 1][1][lane] = szz[s3];
+    __syncthreads();
+
+    // ---- stage 1: D+ of the new velocities (z at row m, x at row m-2)
+    {
+      const int i = m - 2;
+      const bool xp_ok = i >= 1 && i < nx - 2;
+      float qc[4];
+      {
+        const float n0 = nplus(lds_v[par][0][tm1], vx[s3], lds_v[par][0][tp1], lds_v[par][0][tp2]);
+        const float n1 = nplus(lds_v[par][1][tm1], vz[s3], lds_v[par][1][tp1], lds_v[par][1][tp2]);
+        const float n2 = nplus(vx[s0], vx[s1], vx[s2], vx[s3]);
+        const float n3 = nplus(vz[s0], vz[s1], vz[s2], vz[s3]);
+        const float xc[4] = {keep(zp_ok, n0), keep(zp_ok, n1), keep(xp_ok, n2), keep(xp_ok, n3)};
+        div_group(qc, xc, {Dz, Dz, Dx, Dx}, fast_c);
+      }
+      dvxdz[s2] = qc[0];
+      dvzdz[s2] = qc[1];
+      // receivers of step it (row m)
+      if (m == rec_row) {
+        for (; rp < re; ++rp) {
+          const int4 e = a.rec[rp];
+          if (e.x != m) break;
+          if (e.y == j) {
+            float* u = uacc + e.z * N_UACC;
+            const float ux = u[U_UX] + vx[s3] * dt, uz = u[U_UZ] + vz[s3] * dt;
+            u[U_UX] = ux;
+            u[U_UZ] = uz;
+            obs[0 * nobs + e.z * a.nt + it] = ux;
+            obs[2 * nobs + e.z * a.nt + it] = uz;
+          }
+        }
+        rec_row = next_row(a.rec, rp, re);
+      }
+      // stress of step it+1 at row m-2: into the stage-2 ring and LDS
+      const float dvxdx = qc[2];
+      const float dvzdx = qc[3];
+      const float l2m = lam + 2.0f * mu;
+      t1xx[s0] = sxx[s0] + dt * (l2m * dvxdx + lam * dvzdz[s0]);
+      t1zz[s0] = szz[s0] + dt * (l2m * dvzdz[s0] + lam * dvxdx);
+      t1xz[s0] = sxz[s0] + dt * mu * (dvxdz[s0] + dvzdx);
+      lds_s2[par][0][lane] = t1xz[s0];
+      lds_s2[par][1][lane] = t1zz[s0];
+    }
+    // the model values stage 2 used three rows ago go on to stage 3
+    erho[s0] = drho[s1];
+    eirho[s0] = dirho[s1];
+    elam[s0] = dlam[s1];
+    emu[s0] = dmu[s1];
+    ebx[s0] = dbx[s1];
+    drho[s0] = rho;
+    dirho[s0] = irho;
+    dlam[s0] = lam;
+    dmu[s0] = mu;
+    dbx[s0] = bxm;
+
+    // ---- stage 2: velocity of step it+2 at row m-3 (not stored)
+    {
+      const int m2 = m - 3;
+      const bool xm_ok = m2 >= 2 && m2 < nx - 2;
+      float qa[4];
+      {
+        const float n0 = nminus(t1xx[s1], t1xx[s2], t1xx[s3], t1xx[s0]);
+        const float n1 = nminus(lds_s2[par ^ 1][0][tm2], lds_s2[par ^ 1][0][tm1], t1xz[s3], lds_s2[par ^ 1][0][tp1]);
+        const float n2 = nminus(t1xz[s1], t1xz[s2], t1xz[s3], t1xz[s0]);
+        const float n3 = nminus(lds_s2[par ^ 1][1][tm2], lds_s2[par ^ 1][1][tm1], t1zz[s3], lds_s2[par ^ 1][1][tp1]);
+        const float xa[4] = {keep(xm_ok, n0), keep(zm_ok, n1), keep(xm_ok, n2), keep(zm_ok, n3)};
+        div_group(qa, xa, {Dx, Dz, Dx, Dz}, fast_c);
+      }
+      float dsx = qa[0] + qa[1];
+      float dsz = qa[2] + qa[3];
+      if (m2 == inj_row2) {
+        for (; ip2 < ie; ++ip2) {
+          const int4 e = a.inj[ip2];
+          if (e.x != m2) break;
+          if (e.y == j) {
+            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it + 1];
+            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it + 1];
+          }
+        }
+        inj_row2 = next_row(a.inj, ip2, ie);
+      }
+      const float b = (dbx[s1] * bzb) * bzt;
+      float qb[2];
+      const float xb[2] = {dt * dsx, dt * dsz};
+      div_group(qb, xb, {Div{drho[s1], dirho[s1], true}, Div{drho[s1], dirho[s1], true}}, fast_rho);
+      const float vxn = b * (vx[s0] + qb[0]);
+      const float vzn = b * (vz[s0] + qb[1]);
+      u2x[s0] = vxn;
+      u2z[s0] = vzn;
+      // receivers of step it+1 (row m-3)
+      if (m2 == rec_row2) {
+        for (; rp2 < re; ++rp2) {
+          const int4 e = a.rec[rp2];
+          if (e.x != m2) break;
+          if (e.y == j) {
+            float* u = uacc + e.z * N_UACC;
+            const float ux = u[U_UX] + vxn * dt, uz = u[U_UZ] + vzn * dt;
+            u[U_UX] = ux;
+            u[U_UZ] = uz;
+            obs[0 * nobs + e.z * a.nt + it + 1] = ux;
+            obs[2 * nobs + e.z * a.nt + it + 1] = uz;
+          }
+        }
+        rec_row2 = next_row(a.rec, rp2, re);
+      }
+    }
+
+    // ---- stage 2: D+z of velocity row m-4, D+x at row m-5, stress of step
+    // it+2 at row m-5: into the stage-3 ring and LDS
+    {
+      const int i2 = m - 5;
+      const bool xp_ok = i2 >= 1 && i2 < nx - 2;
+      float qc[4];
+      {
+        const float n0 = nplus(lds_v2[par ^ 1][0][tm1], u2x[s3], lds_v2[par ^ 1][0][tp1], lds_v2[par ^ 1][0][tp2]);
+        const float n1 = nplus(lds_v2[par ^ 1][1][tm1], u2z[s3], lds_v2[par ^ 1][1][tp1], lds_v2[par ^ 1][1][tp2]);
+        const float n2 = nplus(u2x[s1], u2x[s2], u2x[s3], u2x[s0]);
+        const float n3 = nplus(u2z[s1], u2z[s2], u2z[s3], u2z[s0]);
+        const float xc[4] = {keep(zp_ok, n0), keep(zp_ok, n1), keep(xp_ok, n2), keep(xp_ok, n3)};
+        div_group(qc, xc, {Dz, Dz, Dx, Dx}, fast_c);
+      }
+      dz2x[par] = qc[0];
+      dz2z[par] = qc[1];
+      const float lam2 = dlam[s1], mu2 = dmu[s1];
+      const float dvxdx = qc[2];
+      const float dvzdx = qc[3];
+      const float l2m = lam2 + 2.0f * mu2;
+      t2xx[s0] = t1xx[s1] + dt * (l2m * dvxdx + lam2 * dz2z[par ^ 1]);
+      t2zz[s0] = t1zz[s1] + dt * (l2m * dz2z[par ^ 1] + lam2 * dvxdx);
+      t2xz[s0] = t1xz[s1] + dt * mu2 * (dz2x[par ^ 1] + dvzdx);
+      lds_s3[par][0][lane] = t2xz[s0];
+      lds_s3[par][1][lane] = t2zz[s0];
+    }
+    lds_v2[par][0][lane] = u2x[s0];
+    lds_v2[par][1][lane] = u2z[s0];
+
+    // ---- stage 3: velocity of step it+3 at row m-6
+    {
+      const int m3 = m - 6;
+      const bool xm_ok = m3 >= 2 && m3 < nx - 2;
+      float qa[4];
+      {
+        const float n0 = nminus(t2xx[s1], t2xx[s2], t2xx[s3], t2xx[s0]);
+        const float n1 = nminus(lds_s3[par ^ 1][0][tm2], lds_s3[par ^ 1][0][tm1], t2xz[s3], lds_s3[par ^ 1][0][tp1]);
+        const float n2 = nminus(t2xz[s1], t2xz[s2], t2xz[s3], t2xz[s0]);
+        const float n3 = nminus(lds_s3[par ^ 1][1][tm2], lds_s3[par ^ 1][1][tm1], t2zz[s3], lds_s3[par ^ 1][1][tp1]);
+        const float xa[4] = {keep(xm_ok, n0), keep(zm_ok, n1), keep(xm_ok, n2), keep(zm_ok, n3)};
+        div_group(qa, xa, {Dx, Dz, Dx, Dz}, fast_c);
+      }
+      float dsx = qa[0] + qa[1];
+      float dsz = qa[2] + qa[3];
+      if (m3 == inj_row3) {
+        for (; ip3 < ie; ++ip3) {
+          const int4 e = a.inj[ip3];
+          if (e.x != m3) break;
+          if (e.y == j) {
+            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it + 2];
+            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it + 2];
+          }
+        }
+        inj_row3 = next_row(a.inj, ip3, ie);
+      }
+      const float b = (ebx[s1] * bzb) * bzt;
+      float qb[2];
+      const float xb[2] = {dt * dsx, dt * dsz};
+      div_group(qb, xb, {Div{erho[s1], eirho[s1], true}, Div{erho[s1], eirho[s1], true}}, fast_rho);
+      const float vxn = b * (u2x[s1] + qb[0]);
+      const float vzn = b * (u2z[s1] + qb[1]);
+      if (g.jst && m3 >= i0 && m3 < i1) {
+        const uint32_t ro = uint32_t(m3) * g.rowb;
+        bstore_out(vxn, rout, jc4, o_vx + ro, a.stream_out);
+        bstore_out(vzn, rout, jc4, o_vz + ro, a.stream_out);
+      }
+      u3x[s0] = vxn;
+      u3z[s0] = vzn;
+      // receivers of step it+2 (row m-6)
+      if (m3 == rec_row3) {
+        for (; rp3 < re; ++rp3) {
+          const int4 e = a.rec[rp3];
+          if (e.x != m3) break;
+          if (e.y == j) {
+            float* u = uacc + e.z * N_UACC;
+            const float ux = u[U_UX] + vxn * dt, uz = u[U_UZ] + vzn * dt;
+            u[U_UX] = ux;
+            u[U_UZ] = uz;
+            obs[0 * nobs + e.z * a.nt + it + 2] = ux;
+            obs[2 * nobs + e.z * a.nt + it + 2] = uz;
+          }
+        }
+        rec_row3 = next_row(a.rec, rp3, re);
+      }
+    }
+
+    // ---- stage 3: D+z of velocity row m-7 (LDS of the previous iteration),
+    // D+x at row m-8, stress of step it+3 at row m-8
+    {
+      const int i3 = m - 8;
+      const bool xp_ok = i3 >= 1 && i3 < nx - 2;
+      float qc[4];
+      {
+        const float n0 = nplus(lds_v3[par ^ 1][0][tm1], u3x[s3], lds_v3[par ^ 1][0][tp1], lds_v3[par ^ 1][0][tp2]);
+        const float n1 = nplus(lds_v3[par ^ 1][1][tm1], u3z[s3], lds_v3[par ^ 1][1][tp1], lds_v3[par ^ 1][1][tp2]);
+        const float n2 = nplus(u3x[s1], u3x[s2], u3x[s3], u3x[s0]);
+        const float n3 = nplus(u3z[s1], u3z[s2], u3z[s3], u3z[s0]);
+        const float xc[4] = {keep(zp_ok, n0), keep(zp_ok, n1), keep(xp_ok, n2), keep(xp_ok, n3)};
+        div_group(qc, xc, {Dz, Dz, Dx, Dx}, fast_c);
+      }
+      dz3x[par] = qc[0];
+      dz3z[par] = qc[1];
+      const float lam3 = elam[s1], mu3 = emu[s1];
+      const float dvxdx = qc[2];
+      const float dvzdx = qc[3];
+      const float l2m = lam3 + 2.0f * mu3;
+      const float nsxx = t2xx[s1] + dt * (l2m * dvxdx + lam3 * dz3z[par ^ 1]);
+      const float nszz = t2zz[s1] + dt * (l2m * dz3z[par ^ 1] + lam3 * dvxdx);
+      const float nsxz = t2xz[s1] + dt * mu3 * (dz3x[par ^ 1] + dvzdx);
+      if (g.jst && i3 >= i0 && i3 < i1) {
+        const uint32_t ro = uint32_t(i3) * g.rowb;
+        bstore_out(nsxx, rout, jc4, o_sxx + ro, a.stream_out);
+        bstore_out(nszz, rout, jc4, o_szz + ro, a.stream_out);
+        bstore_out(nsxz, rout, jc4, o_sxz + ro, a.stream_out);
+      }
+    }
+    lds_v3[par][0][lane] = u3x[s0];
+    lds_v3[par][1][lane] = u3z[s0];
+  };
+
+  // four peeled rows, then 4 phases per loop trip (every ring repeats with
+  // period 4)
+  const int mlast = i1 + 7;
+  int m = m0;
+  body(Phase<0>{}, m);
+  if (++m <= mlast) {
+    body(Phase<1>{}, m);
+    if (++m <= mlast) {
+      body(Phase<2>{}, m);
+      if (++m <= mlast) {
+        body(Phase<3>{}, m);
+        if (++m <= mlast) {
+          for (;;) {
+            body(Phase<4>{}, m);
+            if (++m > mlast) break;
+            body(Phase<5>{}, m);
+            if (++m > mlast) break;
+            body(Phase<6>{}, m);
+            if (++m > mlast) break;
+            body(Phase<7>{}, m);
+            if (++m > mlast) break;
+          }
+        }
+      }
+    }
+  }
+  if (a.advance) advance_clock(a.clock, it, 3, a.nblocks);
+}
+
 // ------------------------------------------------- two SH steps per launch
 // Steps it and it+1 of SH in one march, pipelined as step_psv2 (stage 2
 // trails stage 1 by three rows; the state of step it+1 stays in registers
@@ -2313,6 +2747,11 @@ __global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(4))) voi
     const StepArgs a) {
   step_psv2<NT, S, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
+template <int NT, int S>
+__global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(3))) void k_step_psv3(
+    const StepArgs a) {
+  step_psv3<NT, S>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
 template <int NT, int S, bool AUX>
 __global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(4))) void k_step_sh2(const StepArgs a) {
   step_sh2<NT, S, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
@@ -2717,9 +3156,32 @@ hipError_t launch_tile2(int phys, const StepArgs& a, const Tiling& t, hipStream_
   }
   return hipGetLastError();
 }
+// three steps per launch (marching tiles, P-SV alone, one shot per workgroup)
+hipError_t launch_three(const StepArgs& a, const Tiling& t, hipStream_t s) {
+  if (t.group != 1 || !args_match(a, t, 1) || !two_step_slots(false, a)) return hipErrorInvalidValue;
+  const dim3 grid(a.nshots * t.nstrips * t.nsegs);
+  switch (t.nt_threads) {
+    case 256: hipLaunchKernelGGL((k_step_psv3<256, 1>), grid, dim3(256), 0, s, a); break;
+    case 384: hipLaunchKernelGGL((k_step_psv3<384, 1>), grid, dim3(384), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((k_step_psv3<512, 1>), grid, dim3(512), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+bool tiling3_supported(int nt) { return nt == 256 || nt == 384 || nt == 512; }
+int step3_blocks_per_cu(int nt) {
+  switch (nt) {
+    case 256: return occupancy(k_step_psv3<256, 1>, 256);
+    case 384: return occupancy(k_step_psv3<384, 1>, 384);
+    case 512: return occupancy(k_step_psv3<512, 1>, 512);
+  }
+  return 0;
+}
+
 hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux, bool spin) {
   if (t.nt_threads == kLdsTileThreads) return aux ? hipErrorInvalidValue : launch_tile2(spin ? TP_SPIN : TP_PSV, a, t, s);
   if (spin) return hipErrorInvalidValue;
+  if (t.msteps == 3) return aux ? hipErrorInvalidValue : launch_three(a, t, s);
   return launch_two(false, a, t, s, aux);
 }
 hipError_t launch_step_sh2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {

@@ -275,6 +275,42 @@ def test_five_wave_two_step_tiles_equal_one_step_launches(seiswg, tiling2, phys)
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (q, k)
 
 
+@pytest.mark.parametrize("tiling2", [(256, 5, 1), (384, 7, 1), (512, 13, 1), (384, 200, 1), (256, 1, 1)])
+@pytest.mark.parametrize("nt", [13, 38])
+def test_three_step_launches_equal_one_step_launches(seiswg, tiling2, nt):
+    """P-SV forward with three steps per launch (sw_set_steps_per_launch(3):
+    9-column halos, 7 + 8 recomputed rows per segment) against one step per
+    launch, state and traces bit for bit: 700 columns (strips of 238, 366 and
+    494 columns, the last one partly filled), segments of 1 to 200 rows,
+    sources and receivers on edges and on duplicated cells, step counts that
+    leave one or two single steps after the three-step launches, and with 38
+    steps captured graph blocks."""
+    nx, nz = 61, 700
+    p, stf = random_problem(nx, nz, nt, False, True, False, seed=37)
+    print(f"case tiling2={tiling2} nt={nt}", flush=True)
+    runs = [0, 1, 2, 3]
+    out, launches = {}, {}
+    for spl in (1, 3):
+        c = gpu_context(p, stf, seiswg)
+        c.set_steps_per_launch(spl)
+        if spl == 3:
+            c.set_tiling2(*tiling2)
+        c.forward_batch(runs)
+        if spl == 3:
+            assert c.tiling2() == tiling2
+        launches[spl] = c.timing().step_launches
+        out[spl] = [[c.field(getattr(seiswg.abi, "F_" + n), shot=q).copy() for n in STATE["psv"]] +
+                    [c.obs(q, O.OBS_X).copy(), c.obs(q, O.OBS_Z).copy()] for q in range(len(runs))]
+        c.close()
+    assert launches[1] == nt and launches[3] == nt // 3 + nt % 3, launches
+    for q in range(len(runs)):
+        for k, (a, b) in enumerate(zip(out[1][q], out[3][q])):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (q, k)
+    o = O.Oracle(p, 4, stf=stf.astype(np.float64))
+    obs, _ = o.forward(2, 0)
+    compare(out[3][2][5], obs[O.OBS_X], "three-step shot 2 trace")
+
+
 @pytest.mark.parametrize("phys", ["psv", "sh"])
 def test_snapshots_equal_the_state_after_their_step(seiswg, phys):
     """Snapshots of a batched run (copied asynchronously while later steps

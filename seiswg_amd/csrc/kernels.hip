@@ -954,7 +954,9 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   sxx[3] = sxz[3] = szz[3] = 0.0f;
   // D-deep prefetch FIFO, slot (m - m0) % D: stress rows m+1, velocity /
   // density rows m, moduli rows m-2 of iteration m
-  constexpr int D = 1;   // an iteration of two stages outlasts a load's latency
+  // 2-deep for the forward (111 VGPRs + 9: still four waves per SIMD), 1-deep
+  // for the recompute, whose velocity stores leave no registers for a second row
+  constexpr int D = AUX ? 1 : 2;
   float pf_sxx[D], pf_sxz[D], pf_szz[D], pf_vx[D], pf_vz[D], pf_rho[D], pf_irho[D], pf_lam[D], pf_mu[D];
   float bx_next = taper_x(a.bx, m0, nx);
   // clamped row offsets, slot (row - m0) & 3: each iteration computes the
@@ -1190,7 +1192,7 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
 
   // four peeled rows, then 4 phases per loop trip (with a 1-deep FIFO every
   // ring repeats with period 4; half the code of an 8-phase trip)
-  static_assert(D == 1, "the 4-phase loop assumes a 1-deep prefetch");
+  static_assert(D == 1 || D == 2, "the 4-phase loop needs prefetch slots that repeat with a period dividing 4");
   const int mlast = i1 + 4;
   int m = m0;
   body(Phase<0>{}, m);

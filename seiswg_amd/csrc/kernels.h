@@ -88,6 +88,7 @@ struct StepArgs {
   int aux_period;
   float* kgrad;               // kernel accumulators [shot / S][3][npt]
   int stream_out;             // 1: state and aux stores non-temporal (working set far beyond the caches)
+  int tile_major;             // 1: consecutive workgroups take the same tile of consecutive shot groups
 };
 
 // velocity fields kept per recomputed forward step

@@ -247,8 +247,17 @@ __device__ __forceinline__ TileGeom tile_geom(const StepArgs& a, int logical) {
   constexpr int W = NT - 2 * H;
   TileGeom g;
   const int tps = a.nstrips * a.nsegs;
-  const int grp = logical / tps;
-  g.tile = logical - grp * tps;
+  int grp;
+  if (a.tile_major) {
+    // the shot groups of one tile run side by side (on one XCD after
+    // xcd_remap), so they share its model rows in that XCD's L2
+    const int ngrp = a.nshots / S;
+    g.tile = logical / ngrp;
+    grp = logical - g.tile * ngrp;
+  } else {
+    grp = logical / tps;
+    g.tile = logical - grp * tps;
+  }
   const int sub = S == 1 ? 0 : __builtin_amdgcn_readfirstlane(int(threadIdx.x) / NT);   // NT is a multiple of 64
   g.shot = grp * S + sub;
   g.base = sub * NT;

@@ -488,7 +488,12 @@ void heuristic_tiling2(sw_ctx* c, int nshots) {
   const int64_t resident = std::max(step2_blocks_per_cu(best, group), 1);
   const int64_t strips = (nz + best - 12 - 1) / (best - 12);
   const int64_t base = int64_t(nshots / group) * strips;
-  const int64_t nsegs = std::max<int64_t>(1, (ncu * resident) / base);
+  int64_t nsegs = std::max<int64_t>(1, (ncu * resident) / base);
+  // two rounds of resident workgroups where segments stay long (at least 128
+  // rows): 2000x1000 x 64 shots, 250-row segments (two rounds) 1.382 ms per
+  // launch against 1.433 with 500 rows (one round); 1.5 or 2.5 rounds (334 or
+  // 200 rows) 1.480 and 1.444
+  if ((c->g.nx + 2 * nsegs - 1) / (2 * nsegs) >= 128) nsegs *= 2;
   const int min_seg = base * ((c->g.nx + 7) / 8) < ncu ? 4 : 8;
   int seg = int(std::max<int64_t>((c->g.nx + nsegs - 1) / nsegs, min_seg));
   t.nt_threads = best;
@@ -1564,7 +1569,20 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
       if (three ? (nt != kLdsTileThreads && !tiling3_supported(nt)) : (nt == 384)) continue;
       for (int grp : {1, 2, 4}) {
         if (three && nt != kLdsTileThreads ? grp != 1 : (!group2_supported(nt, grp) || nshots % grp != 0)) continue;
-        for (int seg : segs) {
+        // besides the fixed lengths: segments that make 1-4 whole rounds of
+        // resident workgroups (equal lengths; partial rounds leave CUs idle)
+        std::vector<int> cand(std::begin(segs), std::end(segs));
+        if (nt != kLdsTileThreads) {
+          const int halo = three ? 9 : 6;
+          const int64_t res = three ? step3_blocks_per_cu(nt) : step2_blocks_per_cu(nt, grp);
+          const int64_t base = int64_t(nshots / grp) * ((c->g.nz + nt - 2 * halo - 1) / (nt - 2 * halo));
+          const int64_t ncu = c->ncu > 0 ? c->ncu : 256;
+          for (int k = 1; k <= 4 && res > 0; ++k) {
+            const int64_t ns = k * ncu * res / base;
+            if (ns >= 1 && (k * ncu * res) % base == 0) cand.push_back(int((c->g.nx + ns - 1) / ns));
+          }
+        }
+        for (int seg : cand) {
           if (seg > 2 * c->g.nx) continue;
           // LDS tiles: the kernel fixes its tile; steps per launch 2..4 (the
           // first segment lengths stand for K = 2, 3, 4)

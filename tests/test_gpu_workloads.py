@@ -1,10 +1,12 @@
 """GPU parity at the real shapes of the benchmark workloads.
 
 * batched multi-shot P-SV forward: 2000x1000 grid, 64 shots, 400 receivers,
-  the two-step tile (512 threads, 500-row segments, one shot per workgroup)
-  and the one-step tile (256 threads, 500-row segments, four shots per
-  workgroup) - every state field and trace of shots 0, 31 and 63 bit for bit
-  against the FP32 oracle after six steps from random states;
+  the library's default shapes (two-step tile: 512 threads, 250-row
+  segments, one shot per workgroup) and the one-round two-step tile
+  (500-row segments) with the one-step tile (256 threads, 500-row segments,
+  four shots per workgroup) - every state field and trace of shots 0, 31
+  and 63 bit for bit against the FP32 oracle after six steps from random
+  states;
 * checkpointed P-SV gradient: 1000x600, 16 shots, 200 receivers, checkpoint
   every 50 steps, the library's default tile shapes (those the benchmark
   measures) - kernels against the FP64 oracle adjoint (rel-L2 <= 1e-4);
@@ -45,17 +47,25 @@ def _problem(nx, nz, nt, h, dt, model, geom, src_idx, rec_idx, sh=False, abs_wid
                      model=model, src_rows=geom.src_rows(), src_idx=src_idx, rec_idx=rec_idx)
 
 
-def test_batched_psv_2000x1000_64_shots_headline_tiles_match_oracle_bitwise(seiswg):
+@pytest.mark.parametrize("tiling2", [None, (512, 500, 1)], ids=["default", "one-round"])
+def test_batched_psv_2000x1000_64_shots_headline_tiles_match_oracle_bitwise(seiswg, tiling2):
+    """The library's default two-step shape for this workload (512 threads,
+    250-row segments: two rounds of resident workgroups) and the one-round
+    shape (500-row segments)."""
     from seiswg_amd import synth
     cfg = synth.PSV_BATCH
     nx, nz, h, dt, nshots = cfg["nx"], cfg["nz"], cfg["h"], cfg["dt"], cfg["nshots"]
     nsteps = 6                      # one pair of two-step launches, then two one-step launches
     model, geom = synth.psv_batch_problem()
     c, src_idx, rec_idx = _ctx(seiswg, nx, nz, nsteps + 2, h, dt, model, geom)
-    c.set_tiling(256, 500, 4)
-    c.set_tiling2(512, 500, 1)
-    c.prepare(list(range(nshots)))
-    assert c.tiling() == (256, 500, 4) and c.tiling2() == (512, 500, 1)
+    if tiling2 is None:
+        c.prepare(list(range(nshots)))
+        assert c.tiling2() == (512, 250, 1)
+    else:
+        c.set_tiling(256, 500, 4)
+        c.set_tiling2(*tiling2)
+        c.prepare(list(range(nshots)))
+        assert c.tiling() == (256, 500, 4) and c.tiling2() == tiling2
     print(f"case 2000x1000 x {nshots} shots, tiles {c.tiling()} / {c.tiling2()}", flush=True)
     check = {0: None, 31: None, 63: None}
     rng = np.random.default_rng(123)

@@ -13,7 +13,11 @@
 * uneven batch of gradient problems at full grid size (up to 3072x1536,
   SH and P-SV) through sw_multi_gradient - the 3072x1536 problem against
   the FP64 oracle adjoint, the others against their single-problem
-  gradients bit for bit.
+  gradients bit for bit;
+* the two small forward configurations in full (200x200 SH, 2000 steps;
+  500x500 P-SV + spin, 3000 steps, with the stable micro-inertia j = 1e4) on
+  their default LDS-tile launches - every trace component bit for bit
+  against the FP32 oracle.
 
 Time axes are cut (the tile shapes and batch geometry are those of the
 full runs); the reference itself cannot run these sizes (its field buffer
@@ -194,3 +198,37 @@ def test_uneven_batch_full_size_problems_match_oracle_and_single_runs(seiswg):
     m.close()
     for _, c, *_ in probs:
         c.close()
+
+
+@pytest.mark.parametrize("which", ["sh", "spin"])
+def test_small_forward_configurations_full_runs_match_oracle_bitwise(seiswg, which):
+    from seiswg_amd import synth
+    abi = seiswg.abi
+    spin = which == "spin"
+    q = synth.spin_forward_problem() if spin else synth.sh_forward_problem()
+    nx, nz, h, nt, dt, g, m = q["nx"], q["nz"], q["h"], q["nt"], q["dt"], q["geom"], q["model"]
+    print(f"case {which} {nx}x{nz} nt={nt}", flush=True)
+    c = seiswg.Context(nx, nz, nt, h, h, dt, sh=not spin, psv=spin, spin=spin, abs_left=True, abs_right=True,
+                       abs_bottom=True, abs_top=True, abs_width=q["abs_width"], abs_alpha=0.015)
+    if spin:
+        c.set_model(abi.MODEL_LAME, [m[k] for k in ("rho", "lambda", "mu", "nu", "j", "lambda_c", "mu_c", "nu_c")])
+    else:
+        c.set_model(abi.MODEL_VPVS, [m["rho"], m["vp"], m["vs"]])
+    rows = g.src_rows()
+    src_idx = [seiswg.grid_index(r[0], r[1], h, h, nz) for r in rows]
+    rec_idx = [seiswg.grid_index(x, z, h, h, nz) for x, z in zip(g.rec_x, g.rec_z)]
+    c.set_sources(src_idx, seiswg.ricker_table([[r[3], r[4], r[5], r[6]] for r in rows], nt, dt))
+    c.set_receivers(rec_idx)
+    c.forward_batch([-1])
+    assert c.tiling2()[0] == 1024, c.tiling2()          # LDS tiles, K steps per launch
+    p = O.Problem(nx=nx, nz=nz, nt=nt, dx=h, dz=h, dt=dt, sh=not spin, psv=spin, spin=spin, abs_left=True,
+                  abs_right=True, abs_bottom=True, abs_top=True, abs_width=q["abs_width"], abs_alpha=0.015,
+                  model=m, src_rows=rows, src_idx=src_idx, rec_idx=rec_idx)
+    obs, _ = O.Oracle(p, 4).forward(-1, 0)
+    comps = [O.OBS_X, O.OBS_Z, O.OBS_Y, O.OBS_YI] if spin else [O.OBS_Y]
+    for comp in comps:
+        got = c.obs(0, comp)
+        want = np.asarray(obs[comp], np.float64).astype(np.float32)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), comp
+        assert np.any(got != 0), comp
+    print(f"{which}: {len(comps)} trace components bit-identical over {nt} steps", flush=True)

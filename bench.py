@@ -393,6 +393,9 @@ def main():
                     help="skip the seeded 8-problem subset of the uneven batch at its full nt")
     ap.add_argument("--uneven-subset-only", action="store_true",
                     help="run only the seeded 8-problem subset of the uneven batch at its full nt")
+    ap.add_argument("--uneven-full-only", action="store_true",
+                    help="run only the whole uneven batch (48 gradient problems) at its full nt, tiles autotuned "
+                         "per size class (about 3 minutes)")
     ap.add_argument("--no-small", action="store_true", help="skip the 200x200 SH and 500x500 spin forward runs")
     ap.add_argument("--small-only", action="store_true",
                     help="only the 200x200 SH, 500x500 spin and single-shot 2000x1000 forward runs (for profiling)")
@@ -422,6 +425,10 @@ def main():
         return 0
     if a.gradient_only:
         print(json.dumps({"gradient": gradient_bench(seiswg, synth, a.gradient_nt)}), flush=True)
+        return 0
+    if a.uneven_full_only:
+        print(json.dumps({"uneven_batch_full_nt": uneven_bench(seiswg, synth, 1.0, a.uneven_budget_gb, tune=True)}),
+              flush=True)
         return 0
     if a.uneven_subset_only:
         print(json.dumps({"uneven_subset_full_nt": uneven_bench(seiswg, synth, 1.0, a.uneven_budget_gb,

@@ -96,17 +96,23 @@ def test_steps_from_random_state_match_oracle(seiswg, phys, tiling):
     assert all(exact)
 
 
+@pytest.mark.parametrize("spacing", [(1.7, 2.3), (5.02, 5.6)], ids=["admitted", "rejected"])
+@pytest.mark.parametrize("nsteps", [3, 6])
 @pytest.mark.parametrize("phys", ["sh", "psv", "spin"])
-def test_steps_through_subnormal_range_match_oracle_bitwise(seiswg, phys):
-    """State fields spanning zero, the subnormal range and normal magnitudes:
-    divisions whose quotient is subnormal take the IEEE path in the kernels,
-    the others the reciprocal-and-correction path; both must equal the
-    oracle's IEEE divisions bit for bit."""
+def test_steps_through_subnormal_range_match_oracle_bitwise(seiswg, phys, nsteps, spacing):
+    """State fields spanning zero (both signs), the subnormal range and
+    normal magnitudes: numerators outside the fast forms' ranges take the
+    IEEE division in the kernels, the others the fast forms; both must equal
+    the oracle's IEEE divisions bit for bit.  Spacings (5.02, 5.6) give
+    divisors 8h that the host's check rejects for the two-operation form
+    (the kernels then divide by them the IEEE way); six steps include a
+    two-step launch."""
     sh, psv, spin = phys == "sh", phys in ("psv", "spin"), phys == "spin"
-    nx, nz, nsteps = 40, 75, 3
-    p, stf = random_problem(nx, nz, 6, sh, psv, spin, seed=21)
+    nx, nz = 40, 75
+    p, stf = random_problem(nx, nz, 8, sh, psv, spin, seed=21)
+    p.dx, p.dz = float(np.float32(spacing[0])), float(np.float32(spacing[1]))
     stf = (stf * 1e-40).astype(np.float32)
-    print(f"case phys={phys}", flush=True)
+    print(f"case phys={phys} nsteps={nsteps} spacing={spacing}", flush=True)
     c = gpu_context(p, stf, seiswg)
     c.set_tiling(64, 9)
     o = O.Oracle(p, 4, stf=stf.astype(np.float64))

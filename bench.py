@@ -230,7 +230,7 @@ def small_config_bench(seiswg, synth, which, with_cpu=True):
     return out
 
 
-def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None, only=None, tune=False):
+def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None, only=None, tune=False, workers=0):
     """Uneven batch of 48 independent SH / P-SV gradient problems (grids
     384..3072 x 384..1536, 4-16 shots, 100-300 receivers, nt scaled from
     1500..4000 by nt_scale, checkpoint every 50; `spec` = synth.UNEVEN_SMALL: grids
@@ -335,6 +335,8 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None, only=
     for w in waves:
         ctxs = [tuned_ctx(q) for q in w]
         m = seiswg.Multi(ctxs)
+        if workers:
+            m.set_workers(workers)
         t0 = time.perf_counter()
         res = m.gradient([shots[q] for q in w], [obs[q] for q in w], ckpt_every=synth.UNEVEN["ckpt"])
         t_batched += time.perf_counter() - t0

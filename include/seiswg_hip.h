@@ -209,9 +209,11 @@ SW_API int sw_multi_forward(sw_multi* m, const int32_t* nshots, const int32_t* c
 SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* const* isrc,
                              const float* const* obs_ref, int32_t ckpt_every, double* const* misfit);
 SW_API int sw_multi_set_merge_limit(sw_multi* m, int64_t shot_cells);
-/* Problems on their own launches run `workers` at a time (default 3), each
- * from its own host thread on its context's stream, largest first; their
- * graph capture is off while they run.  1 runs them one after the other. */
+/* Problems on their own launches run concurrently, largest first, each from
+ * its own host thread on its context's stream, with their graph capture off:
+ * `workers` (default 3) problems that fill the GPU (4 M shot-cells per step
+ * or more) at a time, a smaller problem counting for its share of one, on up
+ * to 4 * workers threads.  1 runs them one after the other. */
 SW_API int sw_multi_set_workers(sw_multi* m, int32_t workers);
 /* Elapsed time of the last multi call (HIP events for merged forward runs,
  * host clock for gradients) and merged launches issued. */

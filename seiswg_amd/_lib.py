@@ -466,7 +466,8 @@ class Multi:
         _check(self.L.sw_multi_set_merge_limit(self.h, int(shot_cells)))
 
     def set_workers(self, workers: int):
-        """Problems on their own launches run this many at a time."""
+        """Problems on their own launches run this many GPU-filling problems'
+        worth at a time (smaller problems count for their share)."""
         _check(self.L.sw_multi_set_workers(self.h, workers))
 
     def gradient(self, isrc_lists, obs_refs, ckpt_every: int = 50):

@@ -2472,10 +2472,14 @@ SW_API int sw_adjoint_plain(sw_ctx* c, int32_t nruns, const int32_t* isrc, const
     int rc = prepare_batch(c, 1, inj, true);
     if (rc) return rc;
     HIPCHK(launch_fill(c->fwd_store.p, 0.0f, int64_t(nsa) * int64_t(npt), c->stream));
-    for (int it = 0; it < nt; ++it) {
-      if ((rc = launch_steps(c, 1))) return rc;
-      if ((it + 1) % adj_interval == 0 && sh) {
-        const int isa = nsa - (it + 1) / adj_interval;
+    // blocks of adj_interval steps (multi-step launches, bit-identical to
+    // one step per launch), vy kept after each
+    for (int it = 0; it < nt;) {
+      const int n = std::min(adj_interval - it % adj_interval, nt - it);
+      if ((rc = launch_steps(c, n))) return rc;
+      it += n;
+      if (it % adj_interval == 0 && sh) {
+        const int isa = nsa - it / adj_interval;
         HIPCHK(hipMemcpyAsync(c->fwd_store.p + size_t(isa) * npt, slot_ptr(c, S_VY, 0), npt * 4,
                               hipMemcpyDeviceToDevice, c->stream));
       }
@@ -2509,10 +2513,14 @@ SW_API int sw_adjoint_plain(sw_ctx* c, int32_t nruns, const int32_t* isrc, const
     std::vector<std::vector<Entry>> ainj(1, rin);
     c->val_x = c->adj_src.p; c->val_y = c->adj_src.p; c->val_z = c->adj_src.p; c->nt_val = nt;
     if ((rc = prepare_batch(c, 1, ainj, false))) return rc;
-    for (int it = 0; it < nt; ++it) {
-      if ((rc = launch_steps(c, 1))) return rc;
-      if (it % adj_interval == 0 && sh) {
-        const int isae = it / adj_interval;
+    // imaging after steps 0, I, 2I, ...: a block of one step, then blocks of I
+    for (int it = 0; it < nt;) {
+      const int n = std::min(it == 0 ? 1 : adj_interval, nt - it);
+      if ((rc = launch_steps(c, n))) return rc;
+      const int last = it + n - 1;   // the step after which the state now is
+      it += n;
+      if (last % adj_interval == 0 && sh) {
+        const int isae = last / adj_interval;
         if (isae < nsa)
           HIPCHK(launch_imaging_sh(slot_ptr(c, S_VY, 0), c->fwd_store.p + size_t(isae) * npt, c->kmu.p, c->g.nx,
                                    c->g.nz, c->h8x, c->h8z, ndt, c->stream));

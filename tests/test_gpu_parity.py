@@ -781,3 +781,53 @@ def test_lds_tile_multi_step_launches_match_oracle_bitwise(seiswg, phys, k, nx, 
             got = c.obs(s, k)[:, :nsteps]
             want = np.asarray(obs[k][:, :nsteps], np.float64).astype(np.float32)
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (s, k)
+
+
+@pytest.mark.parametrize("shape", [(4, 200), (200, 4), (7, 7), (4, 500), (600, 5), (4, 4)])
+@pytest.mark.parametrize("phys", ["sh", "psv", "spin"])
+def test_degenerate_grid_shapes_match_oracle(seiswg, phys, shape):
+    """Grids narrower than a stencil or a halo in one or both directions (the
+    FD operators are zero near every edge, so such grids mostly scale and
+    taper), on the library's default launches (LDS tiles for these sizes):
+    state and traces of two shots bit for bit against the FP32 oracle.
+    Grids below 4x4 are refused with an error naming the limit."""
+    nx, nz = shape
+    sh, psv, spin = phys == "sh", phys in ("psv", "spin"), phys == "spin"
+    nt = 21
+    rng = np.random.default_rng(nx * 1000 + nz)
+    npt = nx * nz
+    if spin:
+        model = {n: rng.uniform(0.5, 1.5, npt).astype(np.float32)
+                 for n in ["rho", "lambda", "mu", "nu", "j", "lambda_c", "mu_c", "nu_c"]}
+    else:
+        model = {"rho": rng.uniform(1.0, 2.0, npt).astype(np.float32),
+                 "vp": rng.uniform(1.5, 2.5, npt).astype(np.float32),
+                 "vs": rng.uniform(0.5, 1.2, npt).astype(np.float32)}
+    cells = [(nx // 2, nz // 2), (0, 0), (nx - 1, nz - 1)]
+    src_idx = [i * nz + j for i, j in cells]
+    rec_idx = [i * nz + j for i, j in cells + [(nx - 1, 0), (0, nz - 1)]]
+    p = O.Problem(nx=nx, nz=nz, nt=nt, dx=float(np.float32(1.7)), dz=float(np.float32(2.3)),
+                  dt=float(np.float32(0.05)), sh=sh, psv=psv, spin=spin, abs_left=True, abs_right=True,
+                  abs_bottom=True, abs_top=True, abs_width=6, abs_alpha=float(np.float32(0.1)), model=model,
+                  src_rows=[], src_idx=src_idx, rec_idx=rec_idx)
+    stf = rng.standard_normal((3, len(src_idx), nt)).astype(np.float32)
+    print(f"case phys={phys} shape={shape}", flush=True)
+    c = gpu_context(p, stf, seiswg)
+    runs = [0, 2]
+    c.forward_batch(runs)
+    comps = ([O.OBS_Y] if sh or spin else []) + ([O.OBS_X, O.OBS_Z] if psv else []) + ([O.OBS_YI] if spin else [])
+    fields = (STATE["sh"] if sh else []) + (STATE["psv"] if psv else []) + (STATE["spin"] if spin else [])
+    o = O.Oracle(p, 4, stf=stf.astype(np.float64))
+    for q, r in enumerate(runs):
+        obs, _ = o.forward(r, 0)
+        for comp in comps:
+            got = c.obs(q, comp)
+            want = np.asarray(obs[comp], np.float64).astype(np.float32)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (r, comp)
+        for name in fields:
+            got = c.field(getattr(seiswg.abi, "F_" + name), shot=q)
+            want = np.asarray(o.field(O.F[name]), np.float64).astype(np.float32)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (r, name)
+
+    with pytest.raises(seiswg.SeiswgError, match="at least 4x4"):
+        seiswg.Context(3, 200, 5, 1.0, 1.0, 0.1, sh=sh, psv=psv, spin=spin)

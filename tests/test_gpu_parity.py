@@ -783,13 +783,14 @@ def test_lds_tile_multi_step_launches_match_oracle_bitwise(seiswg, phys, k, nx, 
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (s, k)
 
 
-@pytest.mark.parametrize("shape", [(4, 200), (200, 4), (7, 7), (4, 500), (600, 5), (4, 4)])
+@pytest.mark.parametrize("shape", [(4, 200), (200, 4), (7, 7), (4, 500), (600, 5), (4, 4), (5, 70000), (70000, 5)])
 @pytest.mark.parametrize("phys", ["sh", "psv", "spin"])
 def test_degenerate_grid_shapes_match_oracle(seiswg, phys, shape):
     """Grids narrower than a stencil or a halo in one or both directions (the
     FD operators are zero near every edge, so such grids mostly scale and
-    taper), on the library's default launches (LDS tiles for these sizes):
-    state and traces of two shots bit for bit against the FP32 oracle.
+    taper), on the library's default launches (LDS tiles up to 300 000
+    shot-cells, marching tiles for the 350 000-cell grids): state and traces
+    of two shots bit for bit against the FP32 oracle.
     Grids below 4x4 are refused with an error naming the limit."""
     nx, nz = shape
     sh, psv, spin = phys == "sh", phys in ("psv", "spin"), phys == "spin"

@@ -296,12 +296,16 @@ def uneven_bench(seiswg, synth, nt_scale=0.04, budget_gb=200.0, spec=None, only=
         progress(f"{len(classes)} size classes tuned in {t_tune:.1f} s")
 
     def tuned_ctx(q):
+        # the class's multi-step shape where the forward runs multi-step
+        # launches (the one-step shape, which then serves the adjoint, stays
+        # the library's choice per problem), else its one-step shape
         c = make_ctx(probs[q], probs[q]["init"])
         if q in shapes:
             t1, t2 = shapes[q]
-            c.set_tiling(*t1)
             if t2[0]:
                 c.set_tiling2(*t2)
+            else:
+                c.set_tiling(*t1)
         return c
 
     def mem_bytes(p, n):

@@ -1524,13 +1524,26 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
     return 0;
   };
   const int segs[] = {8, 16, 32, 48, 64, 128, 256, 400, 500, 512};
-  // one-step launches (also used for the remainder steps, the gradient's
-  // adjoint and the physics without two-step kernels)
+  // one-step launches: timed and tuned only where they run the forward
+  // steps (physics without multi-step kernels, one step per launch).  Where
+  // multi-step launches run the forward, the one-step shape serves remainder
+  // steps and the gradient's adjoint, whose fastest shape the forward's
+  // timing does not find (16-shot 1000x600 gradient: 0.21 s with the
+  // heuristic shape, 0.24 s with the one-step forward's autotuned one), so
+  // it stays the heuristic's, chosen again for each prepared batch.
   c->steps_per_launch = 1;
   double best = 1e300;
   Tiling bt = c->tile;
   int bg = c->group_req;
+  if (two) {
+    c->steps_per_launch = spl;
+    c->tile_fixed = false;
+    c->group_req = 0;
+    heuristic_tiling(c, nshots);
+    bt = c->tile;
+  }
   for (int nt : {64, 128, 256}) {
+    if (two) break;
     for (int grp : {1, 2, 4}) {
       if (!group_supported(nt, grp) || nshots % grp != 0) continue;
       for (int seg : segs) {
@@ -1554,10 +1567,12 @@ SW_API int sw_autotune(sw_ctx* c, int32_t nshots, int32_t steps, int32_t* thread
     }
   }
   c->steps_per_launch = spl;
-  c->tile = bt;
-  c->group_req = bg;
-  c->tile_fixed = true;
   double best_ms = best / steps;
+  if (!two) {
+    c->tile = bt;
+    c->group_req = bg;
+    c->tile_fixed = true;
+  }
   if (two) {
     // two-step launches: their own widths (up to 512), groups and segments
     double best2 = 1e300;

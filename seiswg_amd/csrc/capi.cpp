@@ -1922,8 +1922,8 @@ int grad_adjsrc_and_bwd_setup(sw_ctx* c, const float* obs_ref_batch) {
                        c->g.dt, c->stream));
   P.mis.assign(nb, 0.0);
   HIPCHK(hipMemcpyAsync(P.mis.data(), c->gmis.p, size_t(nb) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  // forward velocities of a recomputed segment: time levels n0..n0+K, slot n % (K+1)
-  HIPCHK(c->aux.ensure(std::max<size_t>(size_t(P.K + 1) * nb * P.naux * npt, 1)));
+  // forward velocities of a recomputed segment: time levels n0+1..n0+K, slot n % K
+  HIPCHK(c->aux.ensure(std::max<size_t>(size_t(P.K) * nb * P.naux * npt, 1)));
   HIPCHK(c->kshot.ensure(size_t(nb) * NK * npt));
   HIPCHK(launch_fill(c->kshot.p, 0.0f, int64_t(nb) * NK * int64_t(npt), c->stream));
   for (int p = 0; p < 2; ++p) {
@@ -1931,7 +1931,7 @@ int grad_adjsrc_and_bwd_setup(sw_ctx* c, const float* obs_ref_batch) {
     HIPCHK(launch_fill(c->adj[p].p, 0.0f, int64_t(P.state_floats), c->stream));
   }
   HIPCHK(c->aclock.ensure(2));
-  c->aux_period = P.K + 1;
+  c->aux_period = P.K;
   HIPCHK(hipStreamSynchronize(c->stream));   // obs_ref_batch and P.mis are host memory
   if (c->finite_check)
     for (int s = 0; s < nb; ++s)
@@ -1952,14 +1952,6 @@ int grad_segment_setup(sw_ctx* c, int q, int& n0, int& n1) {
   else
     HIPCHK(hipMemcpyAsync(c->state[c->cur].p, c->ckpt.p + size_t(q) * P.state_floats, P.state_floats * 4,
                           hipMemcpyDeviceToDevice, c->stream));
-  // the velocities v^{n0} of the restored state go to their time slot (the
-  // recompute writes v^{n0+1} .. v^{n1}): the velocity slots lead each
-  // shot's state, [shot][slot][npt] -> [shot][naux][npt]
-  const size_t npt = size_t(c->npt), vel_bytes = size_t(P.naux) * npt * 4;
-  const int vslot = uses_sh(c) ? S_VY : S_VX;
-  HIPCHK(hipMemcpy2DAsync(c->aux.p + size_t(n0 % c->aux_period) * P.nb * P.naux * npt, vel_bytes,
-                          c->state[c->cur].p + size_t(c->slot_index[vslot]) * npt, size_t(c->nslots) * npt * 4,
-                          vel_bytes, size_t(P.nb), hipMemcpyDeviceToDevice, c->stream));
   c->h_clk[0] = n0; c->h_clk[1] = 0; c->h_clk[2] = n1 - 1; c->h_clk[3] = 0;
   HIPCHK(hipMemcpyAsync(c->clock.p, c->h_clk, 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->aclock.p, c->h_clk + 2, 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -2251,11 +2243,8 @@ int multi_gradient_merged(sw_multi* m, const int32_t* nshots, const int32_t* con
       c->cur = done & 1;
     }
   }
-  for (int q = 0; q < n; ++q) {
-    const size_t per = size_t(m->p[q]->gp.ncomp) * m->p[q]->nrec * m->p[q]->g.nt;
-    (void)per;
+  for (int q = 0; q < n; ++q)
     if ((rc = grad_adjsrc_and_bwd_setup(m->p[q], obs_ref[q]))) return rc;
-  }
   // backward, segment by segment from the last
   for (int s = max_nck - 1; s >= 0; --s) {
     int nl = 0;
@@ -2389,11 +2378,11 @@ SW_API int sw_multi_gradient(sw_multi* m, const int32_t* nshots, const int32_t* 
         }
         // its kernels are in K_RHO / K_MU / K_LAM: give the device memory of
         // the run (checkpoints, aux fields, adjoint state, per-shot kernels)
-        // back before the next problem allocates its own
-        // sw_gradient has synchronised the problem's stream, the only one these
-      // buffers were used on: no device-wide wait, which would also wait for
-      // the other workers' launches
-      for (DBuf<float>* b : {&c->aux, &c->ckpt, &c->adj[0], &c->adj[1], &c->kshot, &c->spare}) b->release(true);
+        // back before the next problem allocates its own.  sw_gradient has
+        // synchronised the problem's stream, the only one these buffers were
+        // used on: no device-wide wait, which would also wait for the other
+        // workers' launches
+        for (DBuf<float>* b : {&c->aux, &c->ckpt, &c->adj[0], &c->adj[1], &c->kshot, &c->spare}) b->release(true);
         done();
       }
     };

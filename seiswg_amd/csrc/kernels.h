@@ -80,10 +80,9 @@ struct StepArgs {
   int it_lo, it_hi;           // the launch is a no-op unless it_lo <= it < it_hi
   int nblocks;                // workgroups of this problem in the launch
   // gradient support: the forward velocities of the recomputed segment
-  // (forward recompute writes v^{n+1} after step n, the adjoint reads v^{n+1}
-  // and v^n) laid out [n % aux_period][shot][naux][npt], with aux_period
-  // one more than the checkpoint interval, and the kernel accumulators (rho,
-  // mu, lambda)
+  // (forward recompute writes v^{n+1} after step n, the adjoint of step n
+  // reads it) laid out [n % aux_period][shot][naux][npt], with aux_period
+  // the checkpoint interval, and the kernel accumulators (rho, mu, lambda)
   float* aux;
   int aux_period;
   float* kgrad;               // kernel accumulators [shot / S][3][npt]

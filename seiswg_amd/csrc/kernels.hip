@@ -2260,10 +2260,15 @@ __device__ __forceinline__ void tile2(const StepArgs& a, int logical) {
 // with (D+)^T w = -D-(w~) and (D-)^T w = -D+(w^), where w~ / w^ are w zeroed
 // outside the rows/columns on which the forward operator is active (D+: 1 <=
 // i < n-2, D-: 2 <= i < n-2).  Both terms come from the forward velocities
-// the recompute stored, v' = v^{n+1} and v = v^n: the strains are D+ v^{n+1}
-// (the operators of the forward's stress half-step), and since
-// v^{n+1} = B (v^n + dt ds / rho), dt g . ds / rho^2 = vbar'' . (v^{n+1} -
-// B v^n) / rho.  Same marching/tiling as the forward step.
+// v' = v^{n+1} the recompute stored: the strains are D+ v^{n+1} (the
+// operators of the forward's stress half-step), and since v^{n+1} = B (v^n +
+// dt ds / rho), dt g . ds / rho^2 = (vbar'' . v^{n+1} - g . v^n) / rho.
+// Summed over the steps, g of step n+1 is the vbar' of step n, so the sum
+// telescopes (g after the last adjoint step and v^0 are zero) into
+//   K_rho += -v^{n+1} . (vbar'' - vbar') / rho = -v^{n+1} . (dt A^n + (D+)^T t) / rho
+// per step: one time level of stored velocities, and the difference is the
+// update itself, formed before it is added (no cancellation).  Same
+// marching/tiling as the forward step.
 // The adjoint is not a reference path (its parity is against the FP64
 // adjoint, to a tolerance): its divisions use the reciprocal-and-correction
 // form without the range test, correctly rounded except for subnormal or
@@ -2318,7 +2323,7 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   const auto rout = make_rsrc(a.st[it & 1] + so, sbytes);
   const uint32_t npt4 = 4u * uint32_t(a.npt);
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
-  const auto rf1 = fwd_vel(a, it + 1, g.shot, NAUX_SH), rf0 = fwd_vel(a, it, g.shot, NAUX_SH);
+  const auto rf1 = fwd_vel(a, it + 1, g.shot, NAUX_SH);
   // one accumulator set per workgroup, as in adj_psv (field f summed by
   // shot group f % S)
   const auto rk = make_rsrc(a.kgrad + int64_t(g.shot / S) * NK * int64_t(a.npt), NK * npt4);
@@ -2358,11 +2363,11 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   }
   sbx[3] = sbz[3] = tx[3] = 0.0f;
   // D-deep prefetch FIFO, slot (m - (i0-1)) % D: sigbar and mu rows m+1;
-  // vbar, rho, 1/rho, the forward vy^{n+1}, vy^n and K_rho rows m; K_mu rows
-  // m-2 of iteration m.  D = 2, or 1 with several shots per workgroup.
+  // vbar, rho, 1/rho, the forward vy^{n+1} and K_rho rows m; K_mu rows m-2
+  // of iteration m.  D = 2, or 1 with several shots per workgroup.
   constexpr int D = S > 1 ? 1 : 2;
   static_assert(S == 1 || D == 1, "the shot sums' epilogue reads prefetch slot 0");
-  float pf_sbx[D], pf_sbz[D], pf_mu[D], pf_vb[D], pf_rho[D], pf_irho[D], pf_f1[D], pf_f0[D], pf_kr[D], pf_km[D];
+  float pf_sbx[D], pf_sbz[D], pf_mu[D], pf_vb[D], pf_rho[D], pf_irho[D], pf_f1[D], pf_kr[D], pf_km[D];
   float bx_next = taper_x(a.bx, i0 - 1, nx);
   for (int k = 0; k < D; ++k) {
     const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k);
@@ -2374,7 +2379,6 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     pf_rho[k] = bload(rmod, jc4, m_rho + r0);
     pf_irho[k] = bload(rmod, jc4, m_irho + r0);
     pf_f1[k] = bload(rf1, jc4, A_VY * npt4 + r0);
-    pf_f0[k] = bload(rf0, jc4, A_VY * npt4 + r0);
     pf_kr[k] = own_r ? bload(rk, jc4, k_rho + rkr) : 0.0f;
     pf_km[k] = own_m ? bload(rk, jc4, k_mu + rks) : 0.0f;
   }
@@ -2394,7 +2398,7 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     const float mu1 = pf_mu[f];
     tx[s3] = xt(m + 1) ? dt * mu1 * sbx[s3] : 0.0f;
     const float tz_next = zt ? dt * mu1 * sbz[s3] : 0.0f;
-    const float vb_o = pf_vb[f], rho = pf_rho[f], irho = pf_irho[f], f0m = pf_f0[f];
+    const float vb_o = pf_vb[f], rho = pf_rho[f], irho = pf_irho[f];
     f1[s3] = settle(pf_f1[f]);     // forward vy^{n+1}, row m
     const float kr = pf_kr[f], km = pf_km[f], bxm = bx_next;
     bx_next = taper_x(a.bx, m + 1, nx);
@@ -2408,27 +2412,31 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
       pf_rho[f] = bload(rmod, jc4, m_rho + r2);
       pf_irho[f] = bload(rmod, jc4, m_irho + r2);
       pf_f1[f] = bload(rf1, jc4, A_VY * npt4 + r2);
-      pf_f0[f] = bload(rf0, jc4, A_VY * npt4 + r2);
       if (own_r) pf_kr[f] = bload(rk, jc4, k_rho + rkr);
       if (own_m) pf_km[f] = bload(rk, jc4, k_mu + rks);
     }
 
-    // adjoint "velocity" at row m
-    float vb = vb_o;
+    // adjoint "velocity" at row m: vbar'' = vbar' + inj + div
+    float vb = vb_o, inj = 0.0f;
     if (m >= inj_row) {
       for (; ip < ie; ++ip) {
         const int4 e = a.inj[ip];
         if (e.x > m) break;
-        if (e.x == m && e.y == j) vb = vb + a.val_y[int64_t(e.z) * a.nt_val + it];
+        if (e.x == m && e.y == j) {
+          const float v = a.val_y[int64_t(e.z) * a.nt_val + it];
+          vb = vb + v;
+          inj = inj + v;
+        }
       }
       inj_row = next_row(a.inj, ip, ie);
     }
-    vb = vb + (ndm(tx[s0], tx[s1], tx[s2], tx[s3], Dx) +
-               ndm(lds_t[par][tm2], lds_t[par][tm1], tz_cur, lds_t[par][tp1], Dz));
+    const float div = ndm(tx[s0], tx[s1], tx[s2], tx[s3], Dx) +
+                      ndm(lds_t[par][tm2], lds_t[par][tm1], tz_cur, lds_t[par][tp1], Dz);
+    vb = vb + div;
     const float b = (bxm * bzb) * bzt;
     const float gv = b * vb;
     const float qv = div_rc(dt * gv, rho, irho);     // dt g / rho
-    const float cr = (-(vb * (f1[s3] - b * f0m))) * irho;   // K_rho += -dt g ds / rho^2
+    const float cr = (-(f1[s3] * (inj + div))) * irho;   // K_rho += -v^{n+1} (vbar'' - vbar') / rho
     if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
       bstore_out(gv, rout, jc4, o_vy + ro, a.stream_out);
@@ -2509,7 +2517,7 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   const auto rout = make_rsrc(a.st[it & 1] + so, sbytes);
   const uint32_t npt4 = 4u * uint32_t(a.npt);
   const auto rmod = make_rsrc(a.model, uint32_t(MF_N) * npt4);
-  const auto rf1 = fwd_vel(a, it + 1, g.shot, NAUX_PSV), rf0 = fwd_vel(a, it, g.shot, NAUX_PSV);
+  const auto rf1 = fwd_vel(a, it + 1, g.shot, NAUX_PSV);
   // kernel accumulators: one set per workgroup ([shot / S][NK][npt]); with
   // S > 1 the S shots' contributions are added in shot order by the shot
   // group that owns the field (field f: group f % S), one read-modify-write
@@ -2561,14 +2569,13 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   }
   sbxx[3] = sbzz[3] = sbxz[3] = txx[3] = txz[3] = 0.0f;
   // D-deep prefetch FIFO, slot (m - (i0-1)) % D: sigbar, lambda, mu rows
-  // m+1; vbar, rho, 1/rho, the forward velocities v^{n+1}, v^n and K_rho
-  // rows m; K_lambda and K_mu rows m-2 of iteration m.  D = 1: with D = 2
-  // the 16 fields in flight per row exceed the 128 VGPRs of four waves per
-  // SIMD.
+  // m+1; vbar, rho, 1/rho, the forward velocities v^{n+1} and K_rho rows m;
+  // K_lambda and K_mu rows m-2 of iteration m.  D = 1: with D = 2 the 14
+  // fields in flight per row exceed the 128 VGPRs of four waves per SIMD.
   constexpr int D = 1;
   static_assert(S == 1 || D == 1, "the shot sums' epilogue reads prefetch slot 0");
   float pf_sxx[D], pf_szz[D], pf_sxz[D], pf_lam[D], pf_mu[D], pf_vbx[D], pf_vbz[D], pf_rho[D], pf_irho[D];
-  float pf_fx1[D], pf_fz1[D], pf_fx0[D], pf_fz0[D], pf_kr[D], pf_kl[D], pf_km[D];
+  float pf_fx1[D], pf_fz1[D], pf_kr[D], pf_kl[D], pf_km[D];
   float bx_next = taper_x(a.bx, i0 - 1, nx);
   for (int k = 0; k < D; ++k) {
     const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k);
@@ -2584,8 +2591,6 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     pf_irho[k] = bload(rmod, jc4, m_irho + r0);
     pf_fx1[k] = bload(rf1, jc4, A_VX * npt4 + r0);
     pf_fz1[k] = bload(rf1, jc4, A_VZ * npt4 + r0);
-    pf_fx0[k] = bload(rf0, jc4, A_VX * npt4 + r0);
-    pf_fz0[k] = bload(rf0, jc4, A_VZ * npt4 + r0);
     pf_kr[k] = own_r ? bload(rk, jc4, k_rho + rkr) : 0.0f;
     pf_kl[k] = own_l ? bload(rk, jc4, k_lam + rks) : 0.0f;
     pf_km[k] = own_m ? bload(rk, jc4, k_mu + rks) : 0.0f;
@@ -2608,7 +2613,7 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     const float vbx_o = pf_vbx[f], vbz_o = pf_vbz[f], rho = pf_rho[f], irho = pf_irho[f];
     fx[s3] = settle(pf_fx1[f]);     // forward v^{n+1}, row m
     fz[s3] = settle(pf_fz1[f]);
-    const float fx0 = pf_fx0[f], fz0 = pf_fz0[f], kr = pf_kr[f], kl = pf_kl[f], km = pf_km[f];
+    const float kr = pf_kr[f], kl = pf_kl[f], km = pf_km[f];
     const float bxm = bx_next;
     bx_next = taper_x(a.bx, m + 1, nx);
     {
@@ -2625,35 +2630,38 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
       pf_irho[f] = bload(rmod, jc4, m_irho + r2);
       pf_fx1[f] = bload(rf1, jc4, A_VX * npt4 + r2);
       pf_fz1[f] = bload(rf1, jc4, A_VZ * npt4 + r2);
-      pf_fx0[f] = bload(rf0, jc4, A_VX * npt4 + r2);
-      pf_fz0[f] = bload(rf0, jc4, A_VZ * npt4 + r2);
       if (own_r) pf_kr[f] = bload(rk, jc4, k_rho + rkr);
       if (own_l) pf_kl[f] = bload(rk, jc4, k_lam + rks);
       if (own_m) pf_km[f] = bload(rk, jc4, k_mu + rks);
     }
 
-    // adjoint "velocity" at row m
-    float vbx = vbx_o, vbz = vbz_o;
+    // adjoint "velocity" at row m: vbar'' = vbar' + inj + div
+    float vbx = vbx_o, vbz = vbz_o, injx = 0.0f, injz = 0.0f;
     if (m >= inj_row) {
       for (; ip < ie; ++ip) {
         const int4 e = a.inj[ip];
         if (e.x > m) break;
         if (e.x == m && e.y == j) {
-          vbx = vbx + a.val_x[int64_t(e.z) * a.nt_val + it];
-          vbz = vbz + a.val_z[int64_t(e.z) * a.nt_val + it];
+          const float vx = a.val_x[int64_t(e.z) * a.nt_val + it], vz = a.val_z[int64_t(e.z) * a.nt_val + it];
+          vbx = vbx + vx;
+          vbz = vbz + vz;
+          injx = injx + vx;
+          injz = injz + vz;
         }
       }
       inj_row = next_row(a.inj, ip, ie);
     }
-    vbx = vbx + (ndm(txx[s0], txx[s1], txx[s2], txx[s3], Dx) +
-                 ndm(lds_t[par][0][tm2], lds_t[par][0][tm1], tzxz_cur, lds_t[par][0][tp1], Dz));
-    vbz = vbz + (ndm(lds_t[par][1][tm2], lds_t[par][1][tm1], tzzz_cur, lds_t[par][1][tp1], Dz) +
-                 ndm(txz[s0], txz[s1], txz[s2], txz[s3], Dx));
+    const float divx = ndm(txx[s0], txx[s1], txx[s2], txx[s3], Dx) +
+                       ndm(lds_t[par][0][tm2], lds_t[par][0][tm1], tzxz_cur, lds_t[par][0][tp1], Dz);
+    const float divz = ndm(lds_t[par][1][tm2], lds_t[par][1][tm1], tzzz_cur, lds_t[par][1][tp1], Dz) +
+                       ndm(txz[s0], txz[s1], txz[s2], txz[s3], Dx);
+    vbx = vbx + divx;
+    vbz = vbz + divz;
     const float b = (bxm * bzb) * bzt;
     const float gx = b * vbx, gz = b * vbz;
     const float qxv = div_rc(dt * gx, rho, irho), qzv = div_rc(dt * gz, rho, irho);   // dt g / rho
-    // K_rho += -dt g.ds / rho^2 = -vbar''.(v^{n+1} - B v^n) / rho
-    const float cr = (-(vbx * (fx[s3] - b * fx0) + vbz * (fz[s3] - b * fz0))) * irho;
+    // K_rho += -v^{n+1}.(vbar'' - vbar') / rho (the telescoped -dt g.ds / rho^2)
+    const float cr = (-(fx[s3] * (injx + divx) + fz[s3] * (injz + divz))) * irho;
     if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
       bstore_out(gx, rout, jc4, o_vx + ro, a.stream_out);

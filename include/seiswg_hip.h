@@ -260,6 +260,9 @@ SW_API int sw_set_steps_per_launch(sw_ctx* ctx, int32_t steps);
  *    320 or 512 (they own threads-12 z-columns), `seglen` rows per
  *    workgroup, shots per workgroup (0: automatic; 1; 2 or 4 with threads
  *    >= 128 and at most 1024 threads per workgroup); not with spin;
+ *  - marching tiles, three steps per launch (sw_set_steps_per_launch 3):
+ *    threads per shot 256, 384 or 512 (they own threads-18 z-columns), one
+ *    shot per workgroup;
  *  - LDS tiles (small grids, latency-bound): threads = 1024 and `seglen` =
  *    K, the steps per launch (2..4; with spin 2..3; 0: 4, with spin 3).
  *    Each workgroup loads a 32 x 64-cell region once, advances it K steps

@@ -234,7 +234,8 @@ struct sw_ctx {
   // captured launches depend on (kernel choice, grid and the bytes of their
   // argument blocks), so a block is replayed only with the arguments it was
   // captured with; the graph is kept with its executable until both go
-  struct GraphEntry { uint64_t key; hipGraph_t graph; hipGraphExec_t exec; uint64_t used; };
+  // a captured block: the hash and the full bytes of what its launches depend on
+  struct GraphEntry { uint64_t key; std::string bytes; hipGraph_t graph; hipGraphExec_t exec; uint64_t used; };
   std::vector<GraphEntry> graphs;
   uint64_t graph_clock = 0;
   int graph_steps = 16;       // steps per captured block (a multiple of 4)
@@ -807,13 +808,16 @@ StepArgs make_args(sw_ctx* c, StepMode mode) {
   return a;
 }
 
-// FNV-1a over the bytes of the captured launches' arguments (the argument
-// blocks are value-initialised, so their padding is zero)
+// The bytes of the captured launches' arguments (the argument blocks are
+// value-initialised, so their padding is zero) and their FNV-1a hash: a
+// cached block is reused only when both match
 struct GraphKey {
   uint64_t h = 1469598103934665603ull;
+  std::string bytes;
   void add(const void* p, size_t n) {
     const unsigned char* b = static_cast<const unsigned char*>(p);
     for (size_t q = 0; q < n; ++q) h = (h ^ b[q]) * 1099511628211ull;
+    bytes.append(static_cast<const char*>(p), n);
   }
 };
 constexpr size_t kMaxGraphs = 16;   // captured blocks kept per context
@@ -923,7 +927,7 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     }
     hipGraphExec_t exec = nullptr;
     for (auto& e : c->graphs)
-      if (e.key == k.h) {
+      if (e.key == k.h && e.bytes == k.bytes) {
         exec = e.exec;
         e.used = ++c->graph_clock;
       }
@@ -957,7 +961,7 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
       }
       c->launches = l0;   // captured (or discarded), not launched
       if (e == hipSuccess && rc == 0) {
-        c->graphs.push_back({k.h, graph, exec, ++c->graph_clock});
+        c->graphs.push_back({k.h, std::move(k.bytes), graph, exec, ++c->graph_clock});
       } else {
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;

@@ -2270,23 +2270,18 @@ __device__ __forceinline__ void tile2(const StepArgs& a, int logical) {
 // update itself, formed before it is added (no cancellation).  Same
 // marching/tiling as the forward step.
 // The adjoint is not a reference path (its parity is against the FP64
-// adjoint, to a tolerance): its divisions use the reciprocal-and-correction
-// form without the range test, correctly rounded except for subnormal or
-// tiny quotients, where they are within one unit.
-__device__ __forceinline__ float div_rc(float x, float y, float r) {
-  const float q0 = x * r;
-  const float e = __builtin_fmaf(q0, y, -x);
-  return __builtin_fmaf(-e, r, q0);
-}
+// adjoint, to a tolerance): its divisions by 8h and by rho are
+// multiplications by the correctly rounded reciprocals (within 1.5 units in
+// the last place of the quotient), so it reads 1/rho and not rho.
 __device__ __forceinline__ float ndm(float fm2, float fm1, float f0, float fp1, Div d) {
-  return -div_rc(9.0f * (f0 - fm1) - (fp1 - fm2), d.y, d.r);
+  return -((9.0f * (f0 - fm1) - (fp1 - fm2)) * d.r);
 }
 __device__ __forceinline__ float ndp(float fm1, float f0, float fp1, float fp2, Div d) {
-  return -div_rc(9.0f * (fp1 - f0) - (fp2 - fm1), d.y, d.r);
+  return -((9.0f * (fp1 - f0) - (fp2 - fm1)) * d.r);
 }
-// D+ of a forward field (the strain operator), reciprocal-and-correction division
+// D+ of a forward field (the strain operator)
 __device__ __forceinline__ float dpf(float fm1, float f0, float fp1, float fp2, Div d) {
-  return div_rc(9.0f * (fp1 - f0) - (fp2 - fm1), d.y, d.r);
+  return (9.0f * (fp1 - f0) - (fp2 - fm1)) * d.r;
 }
 
 // sum of column t's values of the S shot groups of a workgroup, in shot order
@@ -2331,7 +2326,7 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   const bool own_r = S == 1 || sub == 0, own_m = S == 1 || sub == 1 % S;
   constexpr int KR_LAG = S > 1 ? 1 : 0, KS_LAG = S > 1 ? 3 : 2;   // K rows behind iteration m
   const uint32_t o_vy = 4u * a.slot_off[S_VY], o_sxy = 4u * a.slot_off[S_SXY], o_szy = 4u * a.slot_off[S_SZY];
-  const uint32_t m_rho = MF_RHO * npt4, m_irho = MF_IRHO * npt4, m_mu = MF_MU * npt4;
+  const uint32_t m_irho = MF_IRHO * npt4, m_mu = MF_MU * npt4;
   const uint32_t k_rho = K_RHO * npt4, k_mu = K_MU * npt4;
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
@@ -2363,11 +2358,11 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
   }
   sbx[3] = sbz[3] = tx[3] = 0.0f;
   // D-deep prefetch FIFO, slot (m - (i0-1)) % D: sigbar and mu rows m+1;
-  // vbar, rho, 1/rho, the forward vy^{n+1} and K_rho rows m; K_mu rows m-2
+  // vbar, 1/rho, the forward vy^{n+1} and K_rho rows m; K_mu rows m-2
   // of iteration m.  D = 2, or 1 with several shots per workgroup.
   constexpr int D = S > 1 ? 1 : 2;
   static_assert(S == 1 || D == 1, "the shot sums' epilogue reads prefetch slot 0");
-  float pf_sbx[D], pf_sbz[D], pf_mu[D], pf_vb[D], pf_rho[D], pf_irho[D], pf_f1[D], pf_kr[D], pf_km[D];
+  float pf_sbx[D], pf_sbz[D], pf_mu[D], pf_vb[D], pf_irho[D], pf_f1[D], pf_kr[D], pf_km[D];
   float bx_next = taper_x(a.bx, i0 - 1, nx);
   for (int k = 0; k < D; ++k) {
     const uint32_t r1 = g.row(i0 + k), r0 = g.row(i0 - 1 + k);
@@ -2376,7 +2371,6 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     pf_sbz[k] = bload(rin, jc4, o_szy + r1);
     pf_mu[k] = bload(rmod, jc4, m_mu + r1);
     pf_vb[k] = bload(rin, jc4, o_vy + r0);
-    pf_rho[k] = bload(rmod, jc4, m_rho + r0);
     pf_irho[k] = bload(rmod, jc4, m_irho + r0);
     pf_f1[k] = bload(rf1, jc4, A_VY * npt4 + r0);
     pf_kr[k] = own_r ? bload(rk, jc4, k_rho + rkr) : 0.0f;
@@ -2398,7 +2392,7 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     const float mu1 = pf_mu[f];
     tx[s3] = xt(m + 1) ? dt * mu1 * sbx[s3] : 0.0f;
     const float tz_next = zt ? dt * mu1 * sbz[s3] : 0.0f;
-    const float vb_o = pf_vb[f], rho = pf_rho[f], irho = pf_irho[f];
+    const float vb_o = pf_vb[f], irho = pf_irho[f];
     f1[s3] = settle(pf_f1[f]);     // forward vy^{n+1}, row m
     const float kr = pf_kr[f], km = pf_km[f], bxm = bx_next;
     bx_next = taper_x(a.bx, m + 1, nx);
@@ -2409,7 +2403,6 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
       pf_sbz[f] = bload(rin, jc4, o_szy + r3);
       pf_mu[f] = bload(rmod, jc4, m_mu + r3);
       pf_vb[f] = bload(rin, jc4, o_vy + r2);
-      pf_rho[f] = bload(rmod, jc4, m_rho + r2);
       pf_irho[f] = bload(rmod, jc4, m_irho + r2);
       pf_f1[f] = bload(rf1, jc4, A_VY * npt4 + r2);
       if (own_r) pf_kr[f] = bload(rk, jc4, k_rho + rkr);
@@ -2435,7 +2428,7 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     vb = vb + div;
     const float b = (bxm * bzb) * bzt;
     const float gv = b * vb;
-    const float qv = div_rc(dt * gv, rho, irho);     // dt g / rho
+    const float qv = (dt * gv) * irho;     // dt g / rho
     const float cr = (-(f1[s3] * (inj + div))) * irho;   // K_rho += -v^{n+1} (vbar'' - vbar') / rho
     if (g.jst && m >= i0 && m < i1) {
       const uint32_t ro = uint32_t(m) * g.rowb;
@@ -2528,7 +2521,7 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   constexpr int KR_LAG = S > 1 ? 1 : 0, KS_LAG = S > 1 ? 3 : 2;   // K rows behind iteration m
   const uint32_t o_vx = 4u * a.slot_off[S_VX], o_vz = 4u * a.slot_off[S_VZ];
   const uint32_t o_sxx = 4u * a.slot_off[S_SXX], o_szz = 4u * a.slot_off[S_SZZ], o_sxz = 4u * a.slot_off[S_SXZ];
-  const uint32_t m_rho = MF_RHO * npt4, m_irho = MF_IRHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4;
+  const uint32_t m_irho = MF_IRHO * npt4, m_lam = MF_LAM * npt4, m_mu = MF_MU * npt4;
   const uint32_t k_rho = K_RHO * npt4, k_lam = K_LAM * npt4, k_mu = K_MU * npt4;
   const uint32_t jc4 = g.jc4;
   const int jc = int(jc4 >> 2);
@@ -2569,12 +2562,12 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   }
   sbxx[3] = sbzz[3] = sbxz[3] = txx[3] = txz[3] = 0.0f;
   // D-deep prefetch FIFO, slot (m - (i0-1)) % D: sigbar, lambda, mu rows
-  // m+1; vbar, rho, 1/rho, the forward velocities v^{n+1} and K_rho rows m;
-  // K_lambda and K_mu rows m-2 of iteration m.  D = 1: with D = 2 the 14
+  // m+1; vbar, 1/rho, the forward velocities v^{n+1} and K_rho rows m;
+  // K_lambda and K_mu rows m-2 of iteration m.  D = 1: with D = 2 the 13
   // fields in flight per row exceed the 128 VGPRs of four waves per SIMD.
   constexpr int D = 1;
   static_assert(S == 1 || D == 1, "the shot sums' epilogue reads prefetch slot 0");
-  float pf_sxx[D], pf_szz[D], pf_sxz[D], pf_lam[D], pf_mu[D], pf_vbx[D], pf_vbz[D], pf_rho[D], pf_irho[D];
+  float pf_sxx[D], pf_szz[D], pf_sxz[D], pf_lam[D], pf_mu[D], pf_vbx[D], pf_vbz[D], pf_irho[D];
   float pf_fx1[D], pf_fz1[D], pf_kr[D], pf_kl[D], pf_km[D];
   float bx_next = taper_x(a.bx, i0 - 1, nx);
   for (int k = 0; k < D; ++k) {
@@ -2587,7 +2580,6 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     pf_mu[k] = bload(rmod, jc4, m_mu + r1);
     pf_vbx[k] = bload(rin, jc4, o_vx + r0);
     pf_vbz[k] = bload(rin, jc4, o_vz + r0);
-    pf_rho[k] = bload(rmod, jc4, m_rho + r0);
     pf_irho[k] = bload(rmod, jc4, m_irho + r0);
     pf_fx1[k] = bload(rf1, jc4, A_VX * npt4 + r0);
     pf_fz1[k] = bload(rf1, jc4, A_VZ * npt4 + r0);
@@ -2610,7 +2602,7 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     txx[s3] = xt(m + 1) ? n_xx : 0.0f;
     txz[s3] = xt(m + 1) ? n_xz : 0.0f;
     const float tzxz_next = zt ? n_xz : 0.0f, tzzz_next = zt ? n_zz : 0.0f;
-    const float vbx_o = pf_vbx[f], vbz_o = pf_vbz[f], rho = pf_rho[f], irho = pf_irho[f];
+    const float vbx_o = pf_vbx[f], vbz_o = pf_vbz[f], irho = pf_irho[f];
     fx[s3] = settle(pf_fx1[f]);     // forward v^{n+1}, row m
     fz[s3] = settle(pf_fz1[f]);
     const float kr = pf_kr[f], kl = pf_kl[f], km = pf_km[f];
@@ -2626,7 +2618,6 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
       pf_mu[f] = bload(rmod, jc4, m_mu + r3);
       pf_vbx[f] = bload(rin, jc4, o_vx + r2);
       pf_vbz[f] = bload(rin, jc4, o_vz + r2);
-      pf_rho[f] = bload(rmod, jc4, m_rho + r2);
       pf_irho[f] = bload(rmod, jc4, m_irho + r2);
       pf_fx1[f] = bload(rf1, jc4, A_VX * npt4 + r2);
       pf_fz1[f] = bload(rf1, jc4, A_VZ * npt4 + r2);
@@ -2659,7 +2650,7 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     vbz = vbz + divz;
     const float b = (bxm * bzb) * bzt;
     const float gx = b * vbx, gz = b * vbz;
-    const float qxv = div_rc(dt * gx, rho, irho), qzv = div_rc(dt * gz, rho, irho);   // dt g / rho
+    const float qxv = (dt * gx) * irho, qzv = (dt * gz) * irho;   // dt g / rho
     // K_rho += -v^{n+1}.(vbar'' - vbar') / rho (the telescoped -dt g.ds / rho^2)
     const float cr = (-(fx[s3] * (injx + divx) + fz[s3] * (injz + divz))) * irho;
     if (g.jst && m >= i0 && m < i1) {

@@ -219,6 +219,7 @@ struct sw_ctx {
   int group2_req = 0;
   DBuf<int> inj2_off, rec2_off;
   DBuf<int4> inj2, rec2;
+  int nrec2_ent = 0;          // receiver entries of the multi-step tile lists
   DBuf<float> spare;
   int64_t steps_done = 0;
   // injection values of the current batch (forward: stf; adjoint: adjoint source)
@@ -678,9 +679,10 @@ int build_tile_lists(sw_ctx* c, const std::vector<std::vector<Entry>>& shot_inj,
     roff[t] = int(rdata.size());
     for (const Entry& e : recs)
       if (e.i >= i0 && e.i < i1 && e.j >= c0 + halo && e.j < c0 + NT - halo && e.j < c->g.nz)
-        rdata.push_back(make_int4(e.i, e.j, e.q, 0));
+        rdata.push_back(make_int4(e.i, e.j, e.q, t));   // t: the owning tile (k_record2)
   }
   roff[ntile] = int(rdata.size());
+  if (two2) c->nrec2_ent = int(rdata.size());
   HIPCHK(off_d.ensure(off.size()));
   HIPCHK(hipMemcpyAsync(off_d.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHK(data_d.ensure(std::max<size_t>(data.size(), 1)));
@@ -821,6 +823,12 @@ struct GraphKey {
   }
 };
 constexpr size_t kMaxGraphs = 16;   // captured blocks kept per context
+// Two-step launches of at least this many shot-cells leave their traces to a
+// receiver kernel after them: recording in the march costs about 4.5 % of a
+// launch (64 shots of 2000x1000: 62 us of 1.38 ms), the extra launch a few
+// microseconds, so the march records them itself below ~8 M shot-cells (one
+// shot of 2000x1000: 23.4 against 25.5 us per step with the extra launch).
+constexpr int64_t kRecordKernelShotCells = int64_t(8) << 20;
 
 int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
   StepArgs a = make_args(c, mode);
@@ -862,8 +870,20 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     }
   }
   const int cur0 = c->cur;
+  // traces of two-step marching launches: from a receiver kernel after each
+  // launch of at least kRecordKernelShotCells shot-cells, from the march
+  // itself below that (LDS tiles and three-step launches record their own)
+  const bool marching2 = multi && a2a.record && !ldsk && c->tile2.msteps == 2 && c->nrec2_ent > 0;
+  // SEISWG_RECORD_KERNEL=1 / 0: the receiver kernel for every / no such launch
+  const char* rk = getenv("SEISWG_RECORD_KERNEL");
+  const bool rec_kernel = marching2 && (rk && (rk[0] == '0' || rk[0] == '1')
+                                            ? rk[0] == '1'
+                                            : int64_t(c->nshots) * c->npt >= kRecordKernelShotCells);
+  const bool rec_loop = marching2 && !rec_kernel;
   auto launch2 = [&](const StepArgs& x) -> int {
-    HIPCHK(sh ? launch_step_sh2(x, c->tile2, c->stream, aux) : launch_step_psv2(x, c->tile2, c->stream, aux, spin));
+    HIPCHK(sh ? launch_step_sh2(x, c->tile2, c->stream, aux, rec_loop)
+              : launch_step_psv2(x, c->tile2, c->stream, aux, spin, rec_loop));
+    if (rec_kernel) HIPCHK(launch_record2(sh, x, c->nrec2_ent, c->stream));
     c->launches += 1;
     return 0;
   };
@@ -914,9 +934,9 @@ int launch_steps(sw_ctx* c, int nsteps, StepMode mode = M_FWD) {
     // everything the captured launches depend on: the kernels (physics,
     // mode, tile shapes), the block structure and the argument blocks
     GraphKey k;
-    const int32_t shape[10] = {int32_t(mode), int32_t(sh), int32_t(psv), int32_t(spin), int32_t(multi), G,
+    const int32_t shape[13] = {int32_t(mode), int32_t(sh), int32_t(psv), int32_t(spin), int32_t(multi), G,
                                c->tile.nt_threads * 8 + c->tile.group, c->tile2.nt_threads * 8 + c->tile2.group,
-                               int32_t(c->sh_only), K};
+                               int32_t(c->sh_only), K, int32_t(rec_kernel), int32_t(rec_loop), c->nrec2_ent};
     k.add(shape, sizeof shape);
     if (multi) {
       k.add(&a2a, sizeof a2a);

@@ -105,11 +105,17 @@ hipError_t launch_step_psv(const StepArgs& a, const Tiling& t, bool spin, hipStr
 // of lds_tile_rows(K) x lds_tile_cols(K) cells (small grids; spin too, K = 2
 // or 3; no aux)
 constexpr int kLdsTileThreads = 1024;
-hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false, bool spin = false);
+// rec: a marching launch records the receivers itself (else the caller runs
+// launch_record2 after it when traces are recorded)
+hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false, bool spin = false,
+                            bool rec = false);
 int lds_tile_rows(int k);
 int lds_tile_cols(int k);
 // the same for SH
-hipError_t launch_step_sh2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false);
+hipError_t launch_step_sh2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux = false, bool rec = false);
+// traces of a two-step marching launch (SH or P-SV alone), after it: nent
+// receiver entries in a.rec (fourth field: the owning two-step tile), every shot
+hipError_t launch_record2(bool sh, const StepArgs& a, int nent, hipStream_t s);
 hipError_t launch_adj_sh(const StepArgs& a, const Tiling& t, hipStream_t s);
 hipError_t launch_adj_psv(const StepArgs& a, const Tiling& t, hipStream_t s);
 

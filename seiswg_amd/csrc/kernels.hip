@@ -881,8 +881,12 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
 // iteration and read after the barrier of the next.  Every cell's arithmetic
 // is that of step_psv, operation for operation, so the result equals two
 // one-step launches bit for bit.  The host passes the buffer to read in st[0]
-// and the one to write in st[1].
-template <int NT, int S, bool AUX>
+// and the one to write in st[1].  Receivers are recorded in the march (REC)
+// for launches of a few million shot-cells; larger launches leave them to
+// k_record2 after the launch: the per-row receiver checks, cursors and
+// displacement loads inside the march cost 4.5 % of the launch time on the
+// 64-shot 2000x1000 workload, against a few microseconds for the extra launch.
+template <int NT, int S, bool AUX, bool REC>
 __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   __shared__ float lds_s[2][2][S * NT];    // stage 1: sxz, szz of step it
   __shared__ float lds_v[2][2][S * NT];    // stage 1: vx, vz of step it+1
@@ -919,17 +923,18 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
   int ip2 = ip;
   int inj_row = next_row(a.inj, ip, ie);
   int inj_row2 = inj_row;
-  int rp = 0, re = 0;
-  if (a.record) {
+  // receivers recorded in the march (REC; otherwise k_record2 after the launch)
+  [[maybe_unused]] int rp = 0, re = 0;
+  if (REC && a.record) {
     rp = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile]);
     re = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile + 1]);
   }
-  int rp2 = rp;
-  int rec_row = next_row(a.rec, rp, re);
-  int rec_row2 = rec_row;
-  float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
-  float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
-  const int nobs = a.nrec * a.nt;
+  [[maybe_unused]] int rp2 = rp;
+  [[maybe_unused]] int rec_row = REC ? next_row(a.rec, rp, re) : 0;
+  [[maybe_unused]] int rec_row2 = rec_row;
+  [[maybe_unused]] float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
+  [[maybe_unused]] float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
+  [[maybe_unused]] const int nobs = a.nrec * a.nt;
   const int m0 = i0 - 4;   // first velocity row of stage 1
   // forward recompute of the gradient: the velocities v^{it+1} (stage 1) and
   // v^{it+2} (stage 2) into their time slots (step % aux_period)
@@ -1073,20 +1078,22 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       dvxdz[s2] = qc[0];
       dvzdz[s2] = qc[1];
       // receivers of step it (row m)
-      if (m == rec_row) {
-        for (; rp < re; ++rp) {
-          const int4 e = a.rec[rp];
-          if (e.x != m) break;
-          if (e.y == j) {
-            float* u = uacc + e.z * N_UACC;
-            const float ux = u[U_UX] + vx[s3] * dt, uz = u[U_UZ] + vz[s3] * dt;
-            u[U_UX] = ux;
-            u[U_UZ] = uz;
-            obs[0 * nobs + e.z * a.nt + it] = ux;
-            obs[2 * nobs + e.z * a.nt + it] = uz;
+      if constexpr (REC) {
+        if (m == rec_row) {
+          for (; rp < re; ++rp) {
+            const int4 e = a.rec[rp];
+            if (e.x != m) break;
+            if (e.y == j) {
+              float* u = uacc + e.z * N_UACC;
+              const float ux = u[U_UX] + vx[s3] * dt, uz = u[U_UZ] + vz[s3] * dt;
+              u[U_UX] = ux;
+              u[U_UZ] = uz;
+              obs[0 * nobs + e.z * a.nt + it] = ux;
+              obs[2 * nobs + e.z * a.nt + it] = uz;
+            }
           }
+          rec_row = next_row(a.rec, rp, re);
         }
-        rec_row = next_row(a.rec, rp, re);
       }
       // stress of step it+1 at row m-2: into the stage-2 ring and LDS
       const float dvxdx = qc[2];
@@ -1148,20 +1155,22 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       u2x[s0] = vxn;
       u2z[s0] = vzn;
       // receivers of step it+1 (row m-3)
-      if (m2 == rec_row2) {
-        for (; rp2 < re; ++rp2) {
-          const int4 e = a.rec[rp2];
-          if (e.x != m2) break;
-          if (e.y == j) {
-            float* u = uacc + e.z * N_UACC;
-            const float ux = u[U_UX] + vxn * dt, uz = u[U_UZ] + vzn * dt;
-            u[U_UX] = ux;
-            u[U_UZ] = uz;
-            obs[0 * nobs + e.z * a.nt + it + 1] = ux;
-            obs[2 * nobs + e.z * a.nt + it + 1] = uz;
+      if constexpr (REC) {
+        if (m2 == rec_row2) {
+          for (; rp2 < re; ++rp2) {
+            const int4 e = a.rec[rp2];
+            if (e.x != m2) break;
+            if (e.y == j) {
+              float* u = uacc + e.z * N_UACC;
+              const float ux = u[U_UX] + vxn * dt, uz = u[U_UZ] + vzn * dt;
+              u[U_UX] = ux;
+              u[U_UZ] = uz;
+              obs[0 * nobs + e.z * a.nt + it + 1] = ux;
+              obs[2 * nobs + e.z * a.nt + it + 1] = uz;
+            }
           }
+          rec_row2 = next_row(a.rec, rp2, re);
         }
-        rec_row2 = next_row(a.rec, rp2, re);
       }
     }
 
@@ -1668,7 +1677,7 @@ __device__ __forceinline__ void step_psv3(const StepArgs& a, int logical) {
 // trails stage 1 by three rows; the state of step it+1 stays in registers
 // and LDS); every cell's arithmetic is that of step_sh.  The host passes the
 // buffer to read in st[0] and the one to write in st[1].
-template <int NT, int S, bool AUX>
+template <int NT, int S, bool AUX, bool REC>
 __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
   __shared__ float lds_s[2][S * NT];    // stage 1: szy of step it
   __shared__ float lds_v[2][S * NT];    // stage 1: vy of step it+1
@@ -1703,16 +1712,17 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
   int ip2 = ip;
   int inj_row = next_row(a.inj, ip, ie);
   int inj_row2 = inj_row;
-  int rp = 0, re = 0;
-  if (a.record) {
+  // receivers recorded in the march (REC; otherwise k_record2 after the launch)
+  [[maybe_unused]] int rp = 0, re = 0;
+  if (REC && a.record) {
     rp = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile]);
     re = __builtin_amdgcn_readfirstlane(a.rec_off[g.tile + 1]);
   }
-  int rp2 = rp;
-  int rec_row = next_row(a.rec, rp, re);
-  int rec_row2 = rec_row;
-  float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
-  float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
+  [[maybe_unused]] int rp2 = rp;
+  [[maybe_unused]] int rec_row = REC ? next_row(a.rec, rp, re) : 0;
+  [[maybe_unused]] int rec_row2 = rec_row;
+  [[maybe_unused]] float* uacc = a.uacc + int64_t(g.shot) * a.nrec * N_UACC;
+  [[maybe_unused]] float* obs = a.obs + int64_t(g.shot) * 5 * a.nrec * a.nt;
   const int m0 = i0 - 4;   // first velocity row of stage 1
   // forward recompute of the gradient: v^{it+1} and v^{it+2} into their time slots
   auto aux_rsrc = [&](int step) {
@@ -1806,8 +1816,8 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
     lds_s[par ^ 1][lane] = szy[s3];
     __syncthreads();
 
-    // ---- stage 1: D+ of the new vy (z at row m, x at row m-2), receivers of
-    // step it, stress of step it+1 at row m-2 into the stage-2 ring and LDS
+    // ---- stage 1: D+ of the new vy (z at row m, x at row m-2), stress of
+    // step it+1 at row m-2 into the stage-2 ring and LDS
     {
       const int i = m - 2;
       const bool xp_ok = i >= 1 && i < nx - 2;
@@ -1819,18 +1829,20 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
         div_group(qc, xc, {Dz, Dx}, fast_c);
       }
       dvydz[s2] = qc[0];
-      if (m == rec_row) {
-        for (; rp < re; ++rp) {
-          const int4 e = a.rec[rp];
-          if (e.x != m) break;
-          if (e.y == j) {
-            float* u = uacc + e.z * N_UACC;
-            const float uy = u[U_UY] + vy[s3] * dt;
-            u[U_UY] = uy;
-            obs[(1 * a.nrec + e.z) * a.nt + it] = uy;
+      if constexpr (REC) {
+        if (m == rec_row) {
+          for (; rp < re; ++rp) {
+            const int4 e = a.rec[rp];
+            if (e.x != m) break;
+            if (e.y == j) {
+              float* u = uacc + e.z * N_UACC;
+              const float uy = u[U_UY] + vy[s3] * dt;
+              u[U_UY] = uy;
+              obs[(1 * a.nrec + e.z) * a.nt + it] = uy;
+            }
           }
+          rec_row = next_row(a.rec, rp, re);
         }
-        rec_row = next_row(a.rec, rp, re);
       }
       const float mu_dt = mu * dt;
       const float dvydx = qc[1];
@@ -1875,18 +1887,20 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
         if constexpr (AUX) bstore_out(vyn, raux2, jc4, A_VY * npt4 + uint32_t(m2) * g.rowb, a.stream_out);
       }
       u2[s0] = vyn;
-      if (m2 == rec_row2) {
-        for (; rp2 < re; ++rp2) {
-          const int4 e = a.rec[rp2];
-          if (e.x != m2) break;
-          if (e.y == j) {
-            float* u = uacc + e.z * N_UACC;
-            const float uy = u[U_UY] + vyn * dt;
-            u[U_UY] = uy;
-            obs[(1 * a.nrec + e.z) * a.nt + it + 1] = uy;
+      if constexpr (REC) {
+        if (m2 == rec_row2) {
+          for (; rp2 < re; ++rp2) {
+            const int4 e = a.rec[rp2];
+            if (e.x != m2) break;
+            if (e.y == j) {
+              float* u = uacc + e.z * N_UACC;
+              const float uy = u[U_UY] + vyn * dt;
+              u[U_UY] = uy;
+              obs[(1 * a.nrec + e.z) * a.nt + it + 1] = uy;
+            }
           }
+          rec_row2 = next_row(a.rec, rp2, re);
         }
-        rec_row2 = next_row(a.rec, rp2, re);
       }
     }
 
@@ -2739,6 +2753,111 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
   if (a.advance) advance_clock(a.clock, it, -1, a.nblocks);
 }
 
+// ------------------------------------------- receivers of two-step launches
+// Traces of a two-step marching launch (steps it and it+1; it = clock - 2,
+// the launch has advanced the clock), one thread per (shot, receiver entry of
+// the launch's tile lists, whose fourth field is the owning tile).  The
+// launch keeps the velocities of step it+1 on chip, so their receiver-cell
+// values are recomputed here from the launch's input state st[0] with the
+// operations of its first stage (same operands, same order: the same bits);
+// those of step it+2 are read from its output st[1].  Then, as one-step
+// launches do: u += v dt at step it, trace(it) = u, again with the step it+2
+// velocities, trace(it+1) = u.
+template <bool SH>
+__global__ __launch_bounds__(256) void k_record2(const StepArgs a, int nent) {
+  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(nent) * a.nshots) return;
+  const int shot = int(idx / nent);
+  const int4 e = a.rec[idx - int64_t(shot) * nent];
+  const int i = e.x, j = e.y, r = e.z, tile = e.w;
+  const int it = a.clock[0] - 2;
+  const int nx = a.nx, nz = a.nz;
+  const int64_t npt = a.npt, k = int64_t(i) * nz + j;
+  const float* s_in = a.st[0] + int64_t(shot) * a.shot_floats;
+  const float* s_out = a.st[1] + int64_t(shot) * a.shot_floats;
+  // a state value at a row / column clamped to the grid, as the march loads it
+  auto at = [&](const float* f, int ii, int jj) {
+    return f[int64_t(min(max(ii, 0), nx - 1)) * nz + min(max(jj, 0), nz - 1)];
+  };
+  const CDiv Dx{a.h8x, a.r8x, a.z8x, a.l8x, a.c_lo, (a.fast_div & FD_CONST) != 0},
+             Dz{a.h8z, a.r8z, a.z8z, a.l8z, a.c_lo, (a.fast_div & FD_CONST) != 0};
+  const int fast_c = div_mode(a.fast_div, FD_CONST);
+  const int fast_rho = div_mode(a.fast_div, FD_RHO);
+  const bool xm_ok = i >= 2 && i < nx - 2, zm_ok = j >= 2 && j < nz - 2;
+  const float dt = a.dt;
+  const float b = (a.bx[min(max(i, 0), nx - 1)] * a.bzb[j]) * a.bzt[j];
+  const float rho = a.model[MF_RHO * npt + k], irho = a.model[MF_IRHO * npt + k];
+  const int tps = a.nstrips * a.nsegs;
+  const int q0 = a.inj_off[shot * tps + tile], q1 = a.inj_off[shot * tps + tile + 1];
+  float* u = a.uacc + (int64_t(shot) * a.nrec + r) * N_UACC;
+  float* obs = a.obs + int64_t(shot) * 5 * a.nrec * a.nt;
+  if constexpr (SH) {
+    // slots vy, sxy, szy (as the march reads them)
+    const float *vy = s_in, *sxy = s_in + npt, *szy = s_in + 2 * npt;
+    float qa[2];
+    {
+      const float n0 = nminus(at(sxy, i - 2, j), at(sxy, i - 1, j), at(sxy, i, j), at(sxy, i + 1, j));
+      const float n1 = nminus(at(szy, i, j - 2), at(szy, i, j - 1), at(szy, i, j), at(szy, i, j + 1));
+      const float xa[2] = {keep(xm_ok, n0), keep(zm_ok, n1)};
+      div_group(qa, xa, {Dx, Dz}, fast_c);
+    }
+    float dsy = qa[0] + qa[1];
+    for (int q = q0; q < q1; ++q) {
+      const int4 f = a.inj[q];
+      if (f.x > i) break;
+      if (f.x == i && f.y == j) dsy = dsy + a.val_y[int64_t(f.z) * a.nt_val + it];
+    }
+    float qb[1];
+    {
+      const float xb[1] = {dt * dsy};
+      div_group(qb, xb, {Div{rho, irho, true}}, fast_rho);
+    }
+    const float v1 = b * (vy[k] + qb[0]);
+    const float uy = u[U_UY] + v1 * dt;
+    obs[(1 * a.nrec + r) * a.nt + it] = uy;
+    const float uy2 = uy + s_out[k] * dt;
+    obs[(1 * a.nrec + r) * a.nt + it + 1] = uy2;
+    u[U_UY] = uy2;
+  } else {
+    // slots vx, vz, sxx, szz, sxz
+    const float *vx = s_in, *vz = s_in + npt, *sxx = s_in + 2 * npt, *szz = s_in + 3 * npt, *sxz = s_in + 4 * npt;
+    float qa[4];
+    {
+      const float n0 = nminus(at(sxx, i - 2, j), at(sxx, i - 1, j), at(sxx, i, j), at(sxx, i + 1, j));
+      const float n1 = nminus(at(sxz, i, j - 2), at(sxz, i, j - 1), at(sxz, i, j), at(sxz, i, j + 1));
+      const float n2 = nminus(at(sxz, i - 2, j), at(sxz, i - 1, j), at(sxz, i, j), at(sxz, i + 1, j));
+      const float n3 = nminus(at(szz, i, j - 2), at(szz, i, j - 1), at(szz, i, j), at(szz, i, j + 1));
+      const float xa[4] = {keep(xm_ok, n0), keep(zm_ok, n1), keep(xm_ok, n2), keep(zm_ok, n3)};
+      div_group(qa, xa, {Dx, Dz, Dx, Dz}, fast_c);
+    }
+    float dsx = qa[0] + qa[1];
+    float dsz = qa[2] + qa[3];
+    for (int q = q0; q < q1; ++q) {
+      const int4 f = a.inj[q];
+      if (f.x > i) break;
+      if (f.x == i && f.y == j) {
+        dsx = dsx + a.val_x[int64_t(f.z) * a.nt_val + it];
+        dsz = dsz + a.val_z[int64_t(f.z) * a.nt_val + it];
+      }
+    }
+    float qb[2];
+    {
+      const float xb[2] = {dt * dsx, dt * dsz};
+      div_group(qb, xb, {Div{rho, irho, true}, Div{rho, irho, true}}, fast_rho);
+    }
+    const float v1x = b * (vx[k] + qb[0]), v1z = b * (vz[k] + qb[1]);
+    const int nobs = a.nrec * a.nt;
+    const float ux = u[U_UX] + v1x * dt, uz = u[U_UZ] + v1z * dt;
+    obs[0 * nobs + r * a.nt + it] = ux;
+    obs[2 * nobs + r * a.nt + it] = uz;
+    const float ux2 = ux + s_out[k] * dt, uz2 = uz + s_out[npt + k] * dt;
+    obs[0 * nobs + r * a.nt + it + 1] = ux2;
+    obs[2 * nobs + r * a.nt + it + 1] = uz2;
+    u[U_UX] = ux2;
+    u[U_UZ] = uz2;
+  }
+}
+
 // --------------------------------------------------------- entry points
 template <int NT, int S, bool AUX>
 __global__ __launch_bounds__(NT * S) void k_step_sh(const StepArgs a) {
@@ -2752,19 +2871,19 @@ __global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(S >= 4 ?
     const StepArgs a) {
   step_psv<NT, S, false, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
-template <int NT, int S, bool AUX>
+template <int NT, int S, bool AUX, bool REC = false>
 __global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(4))) void k_step_psv2(
     const StepArgs a) {
-  step_psv2<NT, S, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
+  step_psv2<NT, S, AUX, REC>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 template <int NT, int S>
 __global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(3))) void k_step_psv3(
     const StepArgs a) {
   step_psv3<NT, S>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
-template <int NT, int S, bool AUX>
+template <int NT, int S, bool AUX, bool REC = false>
 __global__ __launch_bounds__(NT * S) __attribute__((amdgpu_waves_per_eu(4))) void k_step_sh2(const StepArgs a) {
-  step_sh2<NT, S, AUX>(a, xcd_remap(blockIdx.x, gridDim.x));
+  step_sh2<NT, S, AUX, REC>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 template <int PH, int K>
 __global__ __launch_bounds__(1024) void k_tile2(const StepArgs a) {
@@ -3116,30 +3235,32 @@ bool two_step_slots(bool sh, const StepArgs& a) {
 }
 
 template <int NT, int S>
-hipError_t launch_two_ns(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
+hipError_t launch_two_ns(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s, bool aux, bool rec) {
   if (!args_match(a, t, S) || !two_step_slots(sh, a)) return hipErrorInvalidValue;
   const int nblk = a.nshots / S * t.nstrips * t.nsegs;
   const dim3 grid(nblk), block(NT * S);
   if (sh && aux) hipLaunchKernelGGL((k_step_sh2<NT, S, true>), grid, block, 0, s, a);
+  else if (sh && rec) hipLaunchKernelGGL((k_step_sh2<NT, S, false, true>), grid, block, 0, s, a);
   else if (sh) hipLaunchKernelGGL((k_step_sh2<NT, S, false>), grid, block, 0, s, a);
   else if (aux) hipLaunchKernelGGL((k_step_psv2<NT, S, true>), grid, block, 0, s, a);
+  else if (rec) hipLaunchKernelGGL((k_step_psv2<NT, S, false, true>), grid, block, 0, s, a);
   else hipLaunchKernelGGL((k_step_psv2<NT, S, false>), grid, block, 0, s, a);
   return hipGetLastError();
 }
-hipError_t launch_two(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
+hipError_t launch_two(bool sh, const StepArgs& a, const Tiling& t, hipStream_t s, bool aux, bool rec) {
   if (t.group < 1 || a.nshots % t.group != 0) return hipErrorInvalidValue;
   switch (t.nt_threads * 8 + t.group) {
-    case 64 * 8 + 1: return launch_two_ns<64, 1>(sh, a, t, s, aux);
-    case 128 * 8 + 1: return launch_two_ns<128, 1>(sh, a, t, s, aux);
-    case 128 * 8 + 2: return launch_two_ns<128, 2>(sh, a, t, s, aux);
-    case 128 * 8 + 4: return launch_two_ns<128, 4>(sh, a, t, s, aux);
-    case 256 * 8 + 1: return launch_two_ns<256, 1>(sh, a, t, s, aux);
-    case 256 * 8 + 2: return launch_two_ns<256, 2>(sh, a, t, s, aux);
-    case 256 * 8 + 4: return launch_two_ns<256, 4>(sh, a, t, s, aux);
-    case 320 * 8 + 1: return launch_two_ns<320, 1>(sh, a, t, s, aux);
-    case 320 * 8 + 2: return launch_two_ns<320, 2>(sh, a, t, s, aux);
-    case 512 * 8 + 1: return launch_two_ns<512, 1>(sh, a, t, s, aux);
-    case 512 * 8 + 2: return launch_two_ns<512, 2>(sh, a, t, s, aux);
+    case 64 * 8 + 1: return launch_two_ns<64, 1>(sh, a, t, s, aux, rec);
+    case 128 * 8 + 1: return launch_two_ns<128, 1>(sh, a, t, s, aux, rec);
+    case 128 * 8 + 2: return launch_two_ns<128, 2>(sh, a, t, s, aux, rec);
+    case 128 * 8 + 4: return launch_two_ns<128, 4>(sh, a, t, s, aux, rec);
+    case 256 * 8 + 1: return launch_two_ns<256, 1>(sh, a, t, s, aux, rec);
+    case 256 * 8 + 2: return launch_two_ns<256, 2>(sh, a, t, s, aux, rec);
+    case 256 * 8 + 4: return launch_two_ns<256, 4>(sh, a, t, s, aux, rec);
+    case 320 * 8 + 1: return launch_two_ns<320, 1>(sh, a, t, s, aux, rec);
+    case 320 * 8 + 2: return launch_two_ns<320, 2>(sh, a, t, s, aux, rec);
+    case 512 * 8 + 1: return launch_two_ns<512, 1>(sh, a, t, s, aux, rec);
+    case 512 * 8 + 2: return launch_two_ns<512, 2>(sh, a, t, s, aux, rec);
   }
   return hipErrorInvalidValue;
 }
@@ -3188,15 +3309,23 @@ int step3_blocks_per_cu(int nt) {
   return 0;
 }
 
-hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux, bool spin) {
+hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux, bool spin, bool rec) {
   if (t.nt_threads == kLdsTileThreads) return aux ? hipErrorInvalidValue : launch_tile2(spin ? TP_SPIN : TP_PSV, a, t, s);
   if (spin) return hipErrorInvalidValue;
   if (t.msteps == 3) return aux ? hipErrorInvalidValue : launch_three(a, t, s);
-  return launch_two(false, a, t, s, aux);
+  return launch_two(false, a, t, s, aux, rec);
 }
-hipError_t launch_step_sh2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux) {
+hipError_t launch_record2(bool sh, const StepArgs& a, int nent, hipStream_t s) {
+  const int64_t n = int64_t(nent) * a.nshots;
+  if (n == 0) return hipSuccess;
+  const dim3 grid(unsigned((n + 255) / 256));
+  if (sh) hipLaunchKernelGGL(k_record2<true>, grid, dim3(256), 0, s, a, nent);
+  else hipLaunchKernelGGL(k_record2<false>, grid, dim3(256), 0, s, a, nent);
+  return hipGetLastError();
+}
+hipError_t launch_step_sh2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux, bool rec) {
   if (t.nt_threads == kLdsTileThreads) return aux ? hipErrorInvalidValue : launch_tile2(TP_SH, a, t, s);
-  return launch_two(true, a, t, s, aux);
+  return launch_two(true, a, t, s, aux, rec);
 }
 int lds_tile_rows(int k) { return TL_R - 6 * k; }
 int lds_tile_cols(int k) { return TL_C - 6 * k; }

@@ -226,6 +226,56 @@ def test_two_step_launches_equal_one_step_launches(seiswg, threads, seglen, grou
     compare(out[2][0][1][0], obs[comps[0]], "two-step shot 1 trace")
 
 
+@pytest.mark.parametrize("threads,seglen,group", [(64, 5, 1), (256, 9, 2), (128, 16, 4), (320, 7, 1), (512, 11, 2)])
+@pytest.mark.parametrize("phys", ["psv", "sh"])
+def test_receiver_kernel_equals_recording_in_the_march(seiswg, monkeypatch, threads, seglen, group, phys):
+    """Traces of two-step launches recorded by the receiver kernel after each
+    launch (what launches of many shot-cells do) and in the march itself
+    (small launches), forced both ways, against one step per launch: traces
+    and state bit for bit, with receivers on source cells (the receiver kernel
+    repeats the launch's first-stage update there, injection included), on
+    edges and duplicated, several shots per workgroup, 37 steps (captured
+    graphs, a single step at the end)."""
+    nx, nz, nt = 61, 150, 37
+    sh = phys == "sh"
+    p, stf = random_problem(nx, nz, nt, sh, not sh, False, seed=31, nrec=7)
+    # receivers also on the source cells, one of them (the duplicated source
+    # cell) twice
+    p.rec_idx = list(p.rec_idx) + list(p.src_idx) + [p.src_idx[0]]
+    print(f"case phys={phys} threads={threads} seglen={seglen} group={group}", flush=True)
+    comps = [O.OBS_Y] if sh else [O.OBS_X, O.OBS_Z]
+    fields = STATE[phys]
+    runs = [0, 1, 2, 3]
+    out = {}
+    for mode, spl, env in (("one-step", 1, None), ("march", 2, "0"), ("kernel", 2, "1")):
+        if env is None:
+            monkeypatch.delenv("SEISWG_RECORD_KERNEL", raising=False)
+        else:
+            monkeypatch.setenv("SEISWG_RECORD_KERNEL", env)
+        c = gpu_context(p, stf, seiswg)
+        c.set_tiling(threads if threads <= 256 else 256, seglen, group)
+        c.set_tiling2(threads, seglen, group)
+        c.set_steps_per_launch(spl)
+        c.forward_batch(runs)
+        assert spl == 1 or c.tiling2()[0] == threads
+        out[mode] = ([[c.obs(q, k).copy() for k in comps] for q in range(len(runs))],
+                     [[c.field(getattr(seiswg.abi, "F_" + n), shot=q).copy() for n in fields]
+                      for q in range(len(runs))])
+        c.close()
+    monkeypatch.delenv("SEISWG_RECORD_KERNEL", raising=False)
+    for mode in ("march", "kernel"):
+        for q in range(len(runs)):
+            for k, a, b in zip(comps, out["one-step"][0][q], out[mode][0][q]):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (mode, q, k)
+            for n, a, b in zip(fields, out["one-step"][1][q], out[mode][1][q]):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (mode, q, n)
+    # the traces on source cells carry the injection
+    assert np.any(out["kernel"][0][0][0][7:] != 0)
+    o = O.Oracle(p, 4, stf=stf.astype(np.float64))
+    obs, _ = o.forward(2, 0)
+    compare(out["kernel"][0][2][0], obs[comps[0]], "receiver-kernel shot 2 trace")
+
+
 @pytest.mark.parametrize("tiling2", [(512, 7, 1), (512, 13, 2), (256, 20, 4), (128, 5, 2), (64, 9, 1)])
 def test_two_step_shape_set_apart_from_one_step_shape(seiswg, tiling2):
     """Two-step launches with a tile shape of their own (sw_set_tiling2),

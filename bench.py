@@ -381,6 +381,8 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="tile width (64/128/256); 0 = library default")
     ap.add_argument("--seglen", type=int, default=0)
     ap.add_argument("--group", type=int, default=0, help="shots per workgroup (1, 2, 4); 0 = library default")
+    ap.add_argument("--tiling2", type=str, default="",
+                    help="shape of the two-step launches as threads,seglen,shots_per_block (no autotuning)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="measure with the heuristic tile shapes instead of autotuned ones")
     ap.add_argument("--steps-per-launch", type=int, default=0,
@@ -449,9 +451,12 @@ def main():
         ctx.set_division(True)
     ctx.prepare(list(range(nshots)))
     autotune = None
-    if a.threads or a.seglen or a.group:
-        t, s, g = ctx.tiling()
-        ctx.set_tiling(a.threads or t, a.seglen or s, a.group or g)
+    if a.threads or a.seglen or a.group or a.tiling2:
+        if a.threads or a.seglen or a.group:
+            t, s, g = ctx.tiling()
+            ctx.set_tiling(a.threads or t, a.seglen or s, a.group or g)
+        if a.tiling2:
+            ctx.set_tiling2(*[int(v) for v in a.tiling2.split(",")])
         ctx.prepare(list(range(nshots)))
     elif not a.no_autotune:
         t0 = time.perf_counter()
@@ -483,10 +488,12 @@ def main():
     traffic = None
     tf = os.path.join(ROOT, "profiles", "psv_2000x1000_traffic.json")
     if os.path.exists(tf):
+        # measured HBM bytes per launch of the shape that ran (null when that
+        # shape has not been profiled)
         prof = json.load(open(tf))
-        if steps_per_launch < 1.5:
-            prof = prof.get("one_step_kernel", {})
-        traffic = prof.get("hbm_bytes_per_launch")
+        shape = ",".join(str(v) for v in (ctx.tiling() if steps_per_launch < 1.5 else ctx.tiling2()))
+        key = "one_step" if steps_per_launch < 1.5 else "two_step"
+        traffic = prof.get(key, {}).get(shape, {}).get("hbm_bytes_per_launch")
     out = {
         "metric": METRIC, "value": cu / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",

@@ -2764,8 +2764,8 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
 // launches do: u += v dt at step it, trace(it) = u, again with the step it+2
 // velocities, trace(it+1) = u.
 template <bool SH>
-__global__ __launch_bounds__(256) void k_record2(const StepArgs a, int nent) {
-  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+__global__ __launch_bounds__(64) void k_record2(const StepArgs a, int nent) {
+  const int64_t idx = int64_t(blockIdx.x) * 64 + threadIdx.x;
   if (idx >= int64_t(nent) * a.nshots) return;
   const int shot = int(idx / nent);
   const int4 e = a.rec[idx - int64_t(shot) * nent];
@@ -2794,6 +2794,10 @@ __global__ __launch_bounds__(256) void k_record2(const StepArgs a, int nent) {
   if constexpr (SH) {
     // slots vy, sxy, szy (as the march reads them)
     const float *vy = s_in, *sxy = s_in + npt, *szy = s_in + 2 * npt;
+    // every load before the first store (the trace and displacement stores
+    // may alias the state for the compiler, which would then order the later
+    // loads behind them)
+    const float v2 = s_out[k], u0 = u[U_UY];
     float qa[2];
     {
       const float n0 = nminus(at(sxy, i - 2, j), at(sxy, i - 1, j), at(sxy, i, j), at(sxy, i + 1, j));
@@ -2813,14 +2817,15 @@ __global__ __launch_bounds__(256) void k_record2(const StepArgs a, int nent) {
       div_group(qb, xb, {Div{rho, irho, true}}, fast_rho);
     }
     const float v1 = b * (vy[k] + qb[0]);
-    const float uy = u[U_UY] + v1 * dt;
+    const float uy = u0 + v1 * dt;
     obs[(1 * a.nrec + r) * a.nt + it] = uy;
-    const float uy2 = uy + s_out[k] * dt;
+    const float uy2 = uy + v2 * dt;
     obs[(1 * a.nrec + r) * a.nt + it + 1] = uy2;
     u[U_UY] = uy2;
   } else {
     // slots vx, vz, sxx, szz, sxz
     const float *vx = s_in, *vz = s_in + npt, *sxx = s_in + 2 * npt, *szz = s_in + 3 * npt, *sxz = s_in + 4 * npt;
+    const float v2x = s_out[k], v2z = s_out[npt + k], u0x = u[U_UX], u0z = u[U_UZ];   // loads before stores
     float qa[4];
     {
       const float n0 = nminus(at(sxx, i - 2, j), at(sxx, i - 1, j), at(sxx, i, j), at(sxx, i + 1, j));
@@ -2847,10 +2852,10 @@ __global__ __launch_bounds__(256) void k_record2(const StepArgs a, int nent) {
     }
     const float v1x = b * (vx[k] + qb[0]), v1z = b * (vz[k] + qb[1]);
     const int nobs = a.nrec * a.nt;
-    const float ux = u[U_UX] + v1x * dt, uz = u[U_UZ] + v1z * dt;
+    const float ux = u0x + v1x * dt, uz = u0z + v1z * dt;
     obs[0 * nobs + r * a.nt + it] = ux;
     obs[2 * nobs + r * a.nt + it] = uz;
-    const float ux2 = ux + s_out[k] * dt, uz2 = uz + s_out[npt + k] * dt;
+    const float ux2 = ux + v2x * dt, uz2 = uz + v2z * dt;
     obs[0 * nobs + r * a.nt + it + 1] = ux2;
     obs[2 * nobs + r * a.nt + it + 1] = uz2;
     u[U_UX] = ux2;
@@ -3318,9 +3323,11 @@ hipError_t launch_step_psv2(const StepArgs& a, const Tiling& t, hipStream_t s, b
 hipError_t launch_record2(bool sh, const StepArgs& a, int nent, hipStream_t s) {
   const int64_t n = int64_t(nent) * a.nshots;
   if (n == 0) return hipSuccess;
-  const dim3 grid(unsigned((n + 255) / 256));
-  if (sh) hipLaunchKernelGGL(k_record2<true>, grid, dim3(256), 0, s, a, nent);
-  else hipLaunchKernelGGL(k_record2<false>, grid, dim3(256), 0, s, a, nent);
+  // one wavefront per workgroup: the receivers spread over every CU (the
+  // kernel waits on scattered loads, not on arithmetic)
+  const dim3 grid(unsigned((n + 63) / 64));
+  if (sh) hipLaunchKernelGGL(k_record2<true>, grid, dim3(64), 0, s, a, nent);
+  else hipLaunchKernelGGL(k_record2<false>, grid, dim3(64), 0, s, a, nent);
   return hipGetLastError();
 }
 hipError_t launch_step_sh2(const StepArgs& a, const Tiling& t, hipStream_t s, bool aux, bool rec) {

@@ -209,12 +209,24 @@ __device__ __forceinline__ float settle(float x) {
   return y;
 }
 
-// Row of CSR entry q of a per-tile list (INT_MAX past the end), read into a
-// scalar register where it is loaded: the wait for the load stays on the
-// rare path that reads the list, instead of reaching the per-row compare
-// (and, through it, the prefetched rows) of every later iteration.
+// Uniform reads of data a launch never writes (per-tile event lists, source
+// and adjoint-source values) through the scalar cache (constant address
+// space).  Their waits count scalar loads only: a vector load here would make
+// the row that reads an event wait for every row prefetched before it (in
+// the adjoint, whose sources sit at the receivers, that cost 4 % of the
+// gradient).  A divergent index falls back to a vector load.
+__device__ __forceinline__ int4 list_entry(const int4* list, int q) {
+  const auto* c = (const __attribute__((address_space(4))) int*)list;
+  return make_int4(c[4 * q], c[4 * q + 1], c[4 * q + 2], c[4 * q + 3]);
+}
+__device__ __forceinline__ float uniform_value(const float* p, int64_t k) {
+  return ((const __attribute__((address_space(4))) float*)p)[k];
+}
+
+// Row of CSR entry q of a per-tile list (INT_MAX past the end), a scalar: the
+// per-row compare against it needs no vector wait.
 __device__ __forceinline__ int next_row(const int4* list, int q, int end) {
-  return __builtin_amdgcn_readfirstlane(q < end ? list[q].x : 0x7fffffff);
+  return __builtin_amdgcn_readfirstlane(q < end ? list_entry(list, q).x : 0x7fffffff);
 }
 
 // Uniform read through the scalar cache (constant address space): the x
@@ -384,9 +396,9 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     float dsy = qa[0] + qa[1];
     if (m == inj_row) {
       for (; ip < ie; ++ip) {
-        const int4 e = a.inj[ip];
+        const int4 e = list_entry(a.inj, ip);
         if (e.x != m) break;
-        if (e.y == j) dsy = dsy + a.val_y[int64_t(e.z) * a.nt_val + it];
+        if (e.y == j) dsy = dsy + uniform_value(a.val_y, int64_t(e.z) * a.nt_val + it);
       }
       inj_row = next_row(a.inj, ip, ie);
     }
@@ -419,7 +431,7 @@ __device__ __forceinline__ void step_sh(const StepArgs& a, int logical) {
     // receivers on row m: uy += vy*dt, trace = uy
     if (m == rec_row) {
       for (; rp < re; ++rp) {
-        const int4 e = a.rec[rp];
+        const int4 e = list_entry(a.rec, rp);
         if (e.x != m) break;
         if (e.y == j) {
           float* u = uacc + e.z * N_UACC;
@@ -667,11 +679,11 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     }
     if (m == inj_row) {
       for (; ip < ie; ++ip) {
-        const int4 e = a.inj[ip];
+        const int4 e = list_entry(a.inj, ip);
         if (e.x != m) break;
         if (e.y == j) {
-          dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it];
-          dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it];
+          dsx = dsx + uniform_value(a.val_x, int64_t(e.z) * a.nt_val + it);
+          dsz = dsz + uniform_value(a.val_z, int64_t(e.z) * a.nt_val + it);
         }
       }
       inj_row = next_row(a.inj, ip, ie);
@@ -739,7 +751,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
     if constexpr (!SPIN) {
       if (m == rec_row) {
         for (; rp < re; ++rp) {
-          const int4 e = a.rec[rp];
+          const int4 e = list_entry(a.rec, rp);
           if (e.x != m) break;
           if (e.y == j) {
             float* u = uacc + e.z * N_UACC;
@@ -756,7 +768,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
       // a receiver on row ir needs uz on rows ir-1..ir+2 and ux on columns j-1..j+2
       if (m >= rec_row - 1) {
         for (int q = rp; q < re; ++q) {
-          const int4 e = a.rec[q];
+          const int4 e = list_entry(a.rec, q);
           if (e.x - 1 > m) break;
           if (e.y == j) {
             float* u = uacc + e.z * N_UACC;
@@ -791,7 +803,7 @@ __device__ __forceinline__ void step_psv(const StepArgs& a, int logical) {
             }
           }
         }
-        while (rp < re && a.rec[rp].x + 2 <= m) ++rp;
+        while (rp < re && list_entry(a.rec, rp).x + 2 <= m) ++rp;
         rec_row = next_row(a.rec, rp, re);
       }
     }
@@ -1033,11 +1045,11 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       float dsz = qa[2] + qa[3];
       if (m == inj_row) {
         for (; ip < ie; ++ip) {
-          const int4 e = a.inj[ip];
+          const int4 e = list_entry(a.inj, ip);
           if (e.x != m) break;
           if (e.y == j) {
-            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it];
-            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it];
+            dsx = dsx + uniform_value(a.val_x, int64_t(e.z) * a.nt_val + it);
+            dsz = dsz + uniform_value(a.val_z, int64_t(e.z) * a.nt_val + it);
           }
         }
         inj_row = next_row(a.inj, ip, ie);
@@ -1081,7 +1093,7 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       if constexpr (REC) {
         if (m == rec_row) {
           for (; rp < re; ++rp) {
-            const int4 e = a.rec[rp];
+            const int4 e = list_entry(a.rec, rp);
             if (e.x != m) break;
             if (e.y == j) {
               float* u = uacc + e.z * N_UACC;
@@ -1128,11 +1140,11 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       float dsz = qa[2] + qa[3];
       if (m2 == inj_row2) {
         for (; ip2 < ie; ++ip2) {
-          const int4 e = a.inj[ip2];
+          const int4 e = list_entry(a.inj, ip2);
           if (e.x != m2) break;
           if (e.y == j) {
-            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it + 1];
-            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it + 1];
+            dsx = dsx + uniform_value(a.val_x, int64_t(e.z) * a.nt_val + it + 1);
+            dsz = dsz + uniform_value(a.val_z, int64_t(e.z) * a.nt_val + it + 1);
           }
         }
         inj_row2 = next_row(a.inj, ip2, ie);
@@ -1158,7 +1170,7 @@ __device__ __forceinline__ void step_psv2(const StepArgs& a, int logical) {
       if constexpr (REC) {
         if (m2 == rec_row2) {
           for (; rp2 < re; ++rp2) {
-            const int4 e = a.rec[rp2];
+            const int4 e = list_entry(a.rec, rp2);
             if (e.x != m2) break;
             if (e.y == j) {
               float* u = uacc + e.z * N_UACC;
@@ -1396,11 +1408,11 @@ __device__ __forceinline__ void step_psv3(const StepArgs& a, int logical) {
       float dsz = qa[2] + qa[3];
       if (m == inj_row) {
         for (; ip < ie; ++ip) {
-          const int4 e = a.inj[ip];
+          const int4 e = list_entry(a.inj, ip);
           if (e.x != m) break;
           if (e.y == j) {
-            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it];
-            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it];
+            dsx = dsx + uniform_value(a.val_x, int64_t(e.z) * a.nt_val + it);
+            dsz = dsz + uniform_value(a.val_z, int64_t(e.z) * a.nt_val + it);
           }
         }
         inj_row = next_row(a.inj, ip, ie);
@@ -1436,7 +1448,7 @@ __device__ __forceinline__ void step_psv3(const StepArgs& a, int logical) {
       // receivers of step it (row m)
       if (m == rec_row) {
         for (; rp < re; ++rp) {
-          const int4 e = a.rec[rp];
+          const int4 e = list_entry(a.rec, rp);
           if (e.x != m) break;
           if (e.y == j) {
             float* u = uacc + e.z * N_UACC;
@@ -1488,11 +1500,11 @@ __device__ __forceinline__ void step_psv3(const StepArgs& a, int logical) {
       float dsz = qa[2] + qa[3];
       if (m2 == inj_row2) {
         for (; ip2 < ie; ++ip2) {
-          const int4 e = a.inj[ip2];
+          const int4 e = list_entry(a.inj, ip2);
           if (e.x != m2) break;
           if (e.y == j) {
-            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it + 1];
-            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it + 1];
+            dsx = dsx + uniform_value(a.val_x, int64_t(e.z) * a.nt_val + it + 1);
+            dsz = dsz + uniform_value(a.val_z, int64_t(e.z) * a.nt_val + it + 1);
           }
         }
         inj_row2 = next_row(a.inj, ip2, ie);
@@ -1508,7 +1520,7 @@ __device__ __forceinline__ void step_psv3(const StepArgs& a, int logical) {
       // receivers of step it+1 (row m-3)
       if (m2 == rec_row2) {
         for (; rp2 < re; ++rp2) {
-          const int4 e = a.rec[rp2];
+          const int4 e = list_entry(a.rec, rp2);
           if (e.x != m2) break;
           if (e.y == j) {
             float* u = uacc + e.z * N_UACC;
@@ -1569,11 +1581,11 @@ __device__ __forceinline__ void step_psv3(const StepArgs& a, int logical) {
       float dsz = qa[2] + qa[3];
       if (m3 == inj_row3) {
         for (; ip3 < ie; ++ip3) {
-          const int4 e = a.inj[ip3];
+          const int4 e = list_entry(a.inj, ip3);
           if (e.x != m3) break;
           if (e.y == j) {
-            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + it + 2];
-            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + it + 2];
+            dsx = dsx + uniform_value(a.val_x, int64_t(e.z) * a.nt_val + it + 2);
+            dsz = dsz + uniform_value(a.val_z, int64_t(e.z) * a.nt_val + it + 2);
           }
         }
         inj_row3 = next_row(a.inj, ip3, ie);
@@ -1594,7 +1606,7 @@ __device__ __forceinline__ void step_psv3(const StepArgs& a, int logical) {
       // receivers of step it+2 (row m-6)
       if (m3 == rec_row3) {
         for (; rp3 < re; ++rp3) {
-          const int4 e = a.rec[rp3];
+          const int4 e = list_entry(a.rec, rp3);
           if (e.x != m3) break;
           if (e.y == j) {
             float* u = uacc + e.z * N_UACC;
@@ -1795,9 +1807,9 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
       float dsy = qa[0] + qa[1];
       if (m == inj_row) {
         for (; ip < ie; ++ip) {
-          const int4 e = a.inj[ip];
+          const int4 e = list_entry(a.inj, ip);
           if (e.x != m) break;
-          if (e.y == j) dsy = dsy + a.val_y[int64_t(e.z) * a.nt_val + it];
+          if (e.y == j) dsy = dsy + uniform_value(a.val_y, int64_t(e.z) * a.nt_val + it);
         }
         inj_row = next_row(a.inj, ip, ie);
       }
@@ -1832,7 +1844,7 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
       if constexpr (REC) {
         if (m == rec_row) {
           for (; rp < re; ++rp) {
-            const int4 e = a.rec[rp];
+            const int4 e = list_entry(a.rec, rp);
             if (e.x != m) break;
             if (e.y == j) {
               float* u = uacc + e.z * N_UACC;
@@ -1869,9 +1881,9 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
       float dsy = qa[0] + qa[1];
       if (m2 == inj_row2) {
         for (; ip2 < ie; ++ip2) {
-          const int4 e = a.inj[ip2];
+          const int4 e = list_entry(a.inj, ip2);
           if (e.x != m2) break;
-          if (e.y == j) dsy = dsy + a.val_y[int64_t(e.z) * a.nt_val + it + 1];
+          if (e.y == j) dsy = dsy + uniform_value(a.val_y, int64_t(e.z) * a.nt_val + it + 1);
         }
         inj_row2 = next_row(a.inj, ip2, ie);
       }
@@ -1890,7 +1902,7 @@ __device__ __forceinline__ void step_sh2(const StepArgs& a, int logical) {
       if constexpr (REC) {
         if (m2 == rec_row2) {
           for (; rp2 < re; ++rp2) {
-            const int4 e = a.rec[rp2];
+            const int4 e = list_entry(a.rec, rp2);
             if (e.x != m2) break;
             if (e.y == j) {
               float* u = uacc + e.z * N_UACC;
@@ -2427,10 +2439,10 @@ __device__ __forceinline__ void adj_sh(const StepArgs& a, int logical) {
     float vb = vb_o, inj = 0.0f;
     if (m >= inj_row) {
       for (; ip < ie; ++ip) {
-        const int4 e = a.inj[ip];
+        const int4 e = list_entry(a.inj, ip);
         if (e.x > m) break;
         if (e.x == m && e.y == j) {
-          const float v = a.val_y[int64_t(e.z) * a.nt_val + it];
+          const float v = uniform_value(a.val_y, int64_t(e.z) * a.nt_val + it);
           vb = vb + v;
           inj = inj + v;
         }
@@ -2644,10 +2656,10 @@ __device__ __forceinline__ void adj_psv(const StepArgs& a, int logical) {
     float vbx = vbx_o, vbz = vbz_o, injx = 0.0f, injz = 0.0f;
     if (m >= inj_row) {
       for (; ip < ie; ++ip) {
-        const int4 e = a.inj[ip];
+        const int4 e = list_entry(a.inj, ip);
         if (e.x > m) break;
         if (e.x == m && e.y == j) {
-          const float vx = a.val_x[int64_t(e.z) * a.nt_val + it], vz = a.val_z[int64_t(e.z) * a.nt_val + it];
+          const float vx = uniform_value(a.val_x, int64_t(e.z) * a.nt_val + it), vz = uniform_value(a.val_z, int64_t(e.z) * a.nt_val + it);
           vbx = vbx + vx;
           vbz = vbz + vz;
           injx = injx + vx;

@@ -2108,8 +2108,8 @@ __device__ __forceinline__ void tile2(const StepArgs& a, int logical) {
         div_group(qa, xa, {Dx, Dz}, fast_c);
         float dsy = qa[0] + qa[1];
         for (int p = ip0; p < ie; ++p) {
-          const int4 e = a.inj[p];
-          if (e.x == i && e.y == j) dsy = dsy + a.val_y[int64_t(e.z) * a.nt_val + its];
+          const int4 e = list_entry(a.inj, p);
+          if (e.x == i && e.y == j) dsy = dsy + uniform_value(a.val_y, int64_t(e.z) * a.nt_val + its);
         }
         float qb[1];
         const float xb[1] = {dt * dsy};
@@ -2141,10 +2141,10 @@ __device__ __forceinline__ void tile2(const StepArgs& a, int logical) {
           dsyc = dsyc + 2.0f * L[L_SYY][r + 2][cc + 2];
         }
         for (int p = ip0; p < ie; ++p) {
-          const int4 e = a.inj[p];
+          const int4 e = list_entry(a.inj, p);
           if (e.x == i && e.y == j) {
-            dsx = dsx + a.val_x[int64_t(e.z) * a.nt_val + its];
-            dsz = dsz + a.val_z[int64_t(e.z) * a.nt_val + its];
+            dsx = dsx + uniform_value(a.val_x, int64_t(e.z) * a.nt_val + its);
+            dsz = dsz + uniform_value(a.val_z, int64_t(e.z) * a.nt_val + its);
           }
         }
         constexpr int NB = SPIN ? 3 : 2;
@@ -2167,7 +2167,7 @@ __device__ __forceinline__ void tile2(const StepArgs& a, int logical) {
     // (each receiver belongs to one tile; its owner thread reads the new
     // velocities of the neighbouring cells from LDS)
     for (int p = rp0; p < re; ++p) {
-      const int4 e = a.rec[p];
+      const int4 e = list_entry(a.rec, p);
 #pragma unroll
       for (int q = 0; q < NC; ++q) {
         if (!(in[q] && e.x == ii[q] && e.y == j)) continue;

@@ -233,9 +233,9 @@ struct sw_ctx {
 
   // captured blocks of time steps (hipGraph), keyed by everything the
   // captured launches depend on (kernel choice, grid and the bytes of their
-  // argument blocks), so a block is replayed only with the arguments it was
-  // captured with; the graph is kept with its executable until both go
-  // a captured block: the hash and the full bytes of what its launches depend on
+  // argument blocks: their hash and the bytes themselves), so a block is
+  // replayed only with the arguments it was captured with; the graph is
+  // kept with its executable until both go
   struct GraphEntry { uint64_t key; std::string bytes; hipGraph_t graph; hipGraphExec_t exec; uint64_t used; };
   std::vector<GraphEntry> graphs;
   uint64_t graph_clock = 0;

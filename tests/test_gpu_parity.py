@@ -276,6 +276,59 @@ def test_receiver_kernel_equals_recording_in_the_march(seiswg, monkeypatch, thre
     compare(out["kernel"][0][2][0], obs[comps[0]], "receiver-kernel shot 2 trace")
 
 
+def test_random_shapes_two_step_receiver_paths_equal_one_step(seiswg, monkeypatch):
+    """40 seeded random problems (SH or P-SV, grids 6-260 x 6-700, 1-6 shots,
+    5-39 steps, random two-step tile widths, segment lengths and shots per
+    workgroup, sources and receivers anywhere, receivers also on source cells
+    and the grid corners): two-step launches with the receiver kernel and
+    with recording in the march against one step per launch, traces and
+    state bit for bit."""
+    rng = np.random.default_rng(2024)
+    shapes2 = [(64, 1), (128, 1), (128, 2), (128, 4), (256, 1), (256, 2), (256, 4), (320, 1), (320, 2), (512, 1),
+               (512, 2)]
+    for case in range(40):
+        sh = bool(rng.integers(2))
+        phys = "sh" if sh else "psv"
+        nx, nz = int(rng.integers(6, 260)), int(rng.integers(6, 700))
+        nshots, nt, npt = int(rng.integers(1, 7)), int(rng.integers(5, 40)), nx * nz
+        model = {"rho": rng.uniform(1.0, 2.0, npt).astype(np.float32),
+                 "vp": rng.uniform(1.5, 2.5, npt).astype(np.float32),
+                 "vs": rng.uniform(0.5, 1.2, npt).astype(np.float32)}
+        nsrc = int(rng.integers(1, 6))
+        src_idx = [int(rng.integers(nx)) * nz + int(rng.integers(nz)) for _ in range(nsrc)]
+        rec_idx = [int(rng.integers(nx)) * nz + int(rng.integers(nz)) for _ in range(int(rng.integers(1, 12)))]
+        rec_idx += src_idx[:2] + [npt - 1, 0]
+        p = O.Problem(nx=nx, nz=nz, nt=nt, dx=float(np.float32(1.7)), dz=float(np.float32(2.3)),
+                      dt=float(np.float32(0.05)), sh=sh, psv=not sh, spin=False, abs_left=True, abs_right=True,
+                      abs_bottom=True, abs_top=bool(rng.integers(2)), abs_width=int(rng.integers(0, 8)),
+                      abs_alpha=float(np.float32(0.1)), model=model, src_rows=[], src_idx=src_idx, rec_idx=rec_idx)
+        stf = rng.standard_normal((3, nsrc, nt)).astype(np.float32)
+        nt2, group = shapes2[rng.integers(len(shapes2))]
+        seglen = int(rng.integers(2, 120))
+        runs = [int(rng.integers(nsrc)) for _ in range(nshots)]
+        comps = [O.OBS_Y] if sh else [O.OBS_X, O.OBS_Z]
+        print(f"case {case} {phys} {nx}x{nz} shots={nshots} nt={nt} tiling2=({nt2},{seglen},{group})", flush=True)
+        out = {}
+        for mode, spl, env in (("one-step", 1, None), ("march", 2, "0"), ("kernel", 2, "1")):
+            if env is None:
+                monkeypatch.delenv("SEISWG_RECORD_KERNEL", raising=False)
+            else:
+                monkeypatch.setenv("SEISWG_RECORD_KERNEL", env)
+            c = gpu_context(p, stf, seiswg)
+            c.set_steps_per_launch(spl)
+            if spl == 2:
+                c.set_tiling2(nt2, seglen, group)
+            c.forward_batch(runs)
+            out[mode] = [c.obs(q, k).copy() for q in range(nshots) for k in comps] + \
+                        [c.field(getattr(seiswg.abi, "F_" + n), shot=q).copy() for q in range(nshots)
+                         for n in STATE[phys]]
+            c.close()
+        monkeypatch.delenv("SEISWG_RECORD_KERNEL", raising=False)
+        for mode in ("march", "kernel"):
+            for q, (a, b) in enumerate(zip(out["one-step"], out[mode])):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (case, mode, q)
+
+
 @pytest.mark.parametrize("tiling2", [(512, 7, 1), (512, 13, 2), (256, 20, 4), (128, 5, 2), (64, 9, 1)])
 def test_two_step_shape_set_apart_from_one_step_shape(seiswg, tiling2):
     """Two-step launches with a tile shape of their own (sw_set_tiling2),
